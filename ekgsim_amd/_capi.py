@@ -1,0 +1,323 @@
+"""ctypes binding of include/ekg.h (lib/libekgsim.so).
+
+The library is loaded from the package's lib/ folder.  A missing library is
+an error: there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libekgsim.so")
+
+EKG_OK = 0
+ERRORS = {
+    1: "EINVAL", 2: "ENOMEM", 3: "EDEVICE", 4: "ENOSEED", 5: "ETRANSFER", 6: "ESTATE",
+    7: "ENOTFOUND", 8: "EIO",
+}
+NHOOD = {"2D4": 0, "2D8": 1, "3D4": 2, "3D8": 3, "cube": 3}
+MAX_LEADS = 16
+
+# Every symbol include/ekg.h declares.
+EXPORTS = (
+    "ekg_last_error", "ekg_version", "ekg_ctx_create", "ekg_ctx_destroy", "ekg_ctx_set_profiling",
+    "ekg_ctx_kernel_time", "ekg_ctx_reset_profiling", "ekg_heart_create", "ekg_heart_destroy",
+    "ekg_heart_num_myocardial", "ekg_heart_max_layer", "ekg_heart_excite_transfer",
+    "ekg_heart_excite_delay", "ekg_heart_set_activation", "ekg_heart_get_activation",
+    "ekg_num_steps", "ekg_simulate", "ekg_simulate_batch", "ekg_lead_basis", "ekg_move_lead",
+    "ekg_server_create", "ekg_server_destroy", "ekg_server_add_heart", "ekg_server_submit",
+    "ekg_server_poll", "ekg_server_wait", "ekg_server_stats",
+)
+
+
+class EkgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class SimDesc(C.Structure):
+    _fields_ = [
+        ("n_layers", C.c_int),
+        ("layer_k", C.POINTER(C.c_double)),
+        ("n_leads", C.c_int),
+        ("lead_pos", C.POINTER(C.c_double)),
+        ("nhood", C.c_int),
+        ("start", C.c_double),
+        ("length", C.c_double),
+        ("dt", C.c_double),
+    ]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", C.c_size_t),
+        ("max_inflight", C.c_int),
+        ("max_batch", C.c_int),
+        ("n_streams", C.c_int),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run `python -m ekgsim_amd.build` (hipcc, gfx950)")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    pd = C.POINTER(C.c_double)
+    L.ekg_last_error.restype = C.c_char_p
+    L.ekg_version.restype = C.c_char_p
+    L.ekg_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.ekg_ctx_destroy.argtypes = [vp]
+    L.ekg_ctx_destroy.restype = None
+    L.ekg_ctx_set_profiling.argtypes = [vp, C.c_int]
+    L.ekg_ctx_kernel_time.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64), pd]
+    L.ekg_ctx_reset_profiling.argtypes = [vp]
+    L.ekg_heart_create.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, C.POINTER(vp)]
+    L.ekg_heart_destroy.argtypes = [vp]
+    L.ekg_heart_destroy.restype = None
+    L.ekg_heart_num_myocardial.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.ekg_heart_max_layer.argtypes = [vp, C.POINTER(C.c_int)]
+    L.ekg_heart_excite_transfer.argtypes = [vp, vp, C.c_int, C.c_int]
+    L.ekg_heart_excite_delay.argtypes = [vp, vp]
+    L.ekg_heart_set_activation.argtypes = [vp, vp]
+    L.ekg_heart_get_activation.argtypes = [vp, vp]
+    L.ekg_num_steps.argtypes = [C.c_double, C.c_double]
+    L.ekg_num_steps.restype = C.c_int64
+    L.ekg_simulate.argtypes = [vp, C.POINTER(SimDesc), vp]
+    L.ekg_simulate_batch.argtypes = [vp, C.c_int, C.POINTER(SimDesc), vp]
+    L.ekg_lead_basis.argtypes = [vp, vp, vp]
+    L.ekg_lead_basis.restype = None
+    L.ekg_move_lead.argtypes = [vp, C.c_double, C.c_double, vp]
+    L.ekg_move_lead.restype = None
+    L.ekg_server_create.argtypes = [vp, C.POINTER(ServerOpts), C.POINTER(vp)]
+    L.ekg_server_destroy.argtypes = [vp]
+    L.ekg_server_destroy.restype = None
+    L.ekg_server_add_heart.argtypes = [vp, vp, C.POINTER(C.c_int)]
+    L.ekg_server_submit.argtypes = [vp, C.c_int, C.POINTER(SimDesc), C.POINTER(C.c_uint64)]
+    L.ekg_server_poll.argtypes = [vp, C.c_uint64, C.POINTER(C.c_int)]
+    L.ekg_server_wait.argtypes = [vp, C.c_uint64, vp, C.c_size_t, pd]
+    L.ekg_server_stats.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                   C.POINTER(C.c_int), C.POINTER(C.c_size_t)]
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != EKG_OK:
+        raise EkgError(rc, lib().ekg_last_error().decode())
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _pd(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def num_steps(length, dt):
+    return int(lib().ekg_num_steps(float(length), float(dt)))
+
+
+def lead_basis(p):
+    p = np.ascontiguousarray(p, np.float64)
+    u = np.zeros(3)
+    v = np.zeros(3)
+    lib().ekg_lead_basis(_p(p), _p(u), _p(v))
+    return u, v
+
+
+def move_leads(pos, disp):
+    pos = np.ascontiguousarray(pos, np.float64)
+    out = np.zeros_like(pos)
+    for i in range(pos.shape[0]):
+        o = np.zeros(3)
+        lib().ekg_move_lead(_p(np.ascontiguousarray(pos[i])), float(disp[i][0]), float(disp[i][1]), _p(o))
+        out[i] = o
+    return out
+
+
+class _Desc:
+    """Keeps the arrays behind one SimDesc alive."""
+
+    def __init__(self, layer_k, lead_pos, nhood, start, length, dt):
+        self.k = np.ascontiguousarray(layer_k, np.float64).reshape(-1, 9)
+        self.p = np.ascontiguousarray(lead_pos, np.float64).reshape(-1, 3)
+        self.s = SimDesc(self.k.shape[0], _pd(self.k), self.p.shape[0], _pd(self.p),
+                         NHOOD[nhood] if isinstance(nhood, str) else int(nhood),
+                         float(start), float(length), float(dt))
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.h = C.c_void_p()
+        check(lib().ekg_ctx_create(int(device), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().ekg_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_profiling(self, on: bool):
+        check(lib().ekg_ctx_set_profiling(self.h, 1 if on else 0))
+
+    def kernel_time(self, kernel="ecg_fused"):
+        n = C.c_int64()
+        ms = C.c_double()
+        check(lib().ekg_ctx_kernel_time(self.h, kernel.encode(), C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def reset_profiling(self):
+        check(lib().ekg_ctx_reset_profiling(self.h))
+
+    def heart(self, layer, seed=None):
+        return Heart(self, layer, seed)
+
+
+class Heart:
+    def __init__(self, ctx: Context, layer, seed=None):
+        self.ctx = ctx
+        self.layer = np.ascontiguousarray(layer, np.uint8)
+        if self.layer.ndim != 3:
+            raise ValueError("layer must be a 3-D (z, y, x) array")
+        self.seed = None if seed is None else np.ascontiguousarray(seed, np.uint8)
+        if self.seed is not None and self.seed.shape != self.layer.shape:
+            raise ValueError("seed grid shape differs from the layer grid")
+        self.h = C.c_void_p()
+        n0, n1, n2 = self.layer.shape
+        check(lib().ekg_heart_create(ctx.h, n0, n1, n2, _p(self.layer),
+                                     None if self.seed is None else _p(self.seed), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().ekg_heart_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def n_myo(self):
+        n = C.c_int64()
+        check(lib().ekg_heart_num_myocardial(self.h, C.byref(n)))
+        return n.value
+
+    @property
+    def max_layer(self):
+        n = C.c_int()
+        check(lib().ekg_heart_max_layer(self.h, C.byref(n)))
+        return n.value
+
+    def excite_transfer(self, transfer):
+        t = np.ascontiguousarray(transfer, np.float64)
+        check(lib().ekg_heart_excite_transfer(self.h, _p(t), t.shape[0], t.shape[1]))
+
+    def excite_delay(self, delay):
+        d = np.ascontiguousarray(delay, np.uint8)
+        if d.shape != self.layer.shape:
+            raise ValueError("delay grid shape differs from the layer grid")
+        check(lib().ekg_heart_excite_delay(self.h, _p(d)))
+
+    def set_activation(self, act):
+        a = np.ascontiguousarray(act, np.float64)
+        if a.shape != self.layer.shape:
+            raise ValueError("activation grid shape differs from the layer grid")
+        check(lib().ekg_heart_set_activation(self.h, _p(a)))
+
+    def activation(self):
+        a = np.zeros(self.layer.shape, np.float64)
+        check(lib().ekg_heart_get_activation(self.h, _p(a)))
+        return a
+
+    def simulate(self, layer_k, lead_pos, nhood="3D4", start=0.0, length=600.0, dt=1.0):
+        d = _Desc(layer_k, lead_pos, nhood, start, length, dt)
+        T = num_steps(length, dt)
+        out = np.zeros((d.p.shape[0], T), np.float64)
+        check(lib().ekg_simulate(self.h, C.byref(d.s), _p(out)))
+        return out
+
+    def simulate_batch(self, layer_k, lead_pos, nhood="3D4", start=0.0, length=600.0, dt=1.0):
+        """layer_k: (B, n_layers, 9); lead_pos: (B, L, 3) -> (B, L, T)."""
+        layer_k = np.asarray(layer_k, np.float64)
+        lead_pos = np.asarray(lead_pos, np.float64)
+        B = layer_k.shape[0]
+        ds = [_Desc(layer_k[b], lead_pos[b], nhood, start, length, dt) for b in range(B)]
+        arr = (SimDesc * B)(*[d.s for d in ds])
+        T = num_steps(length, dt)
+        out = np.zeros((B, lead_pos.shape[1], T), np.float64)
+        check(lib().ekg_simulate_batch(self.h, B, arr, _p(out)))
+        return out
+
+
+class Server:
+    def __init__(self, ctx: Context, hbm_budget_bytes=0, max_inflight=0, max_batch=0, n_streams=0):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        o = ServerOpts(int(hbm_budget_bytes), int(max_inflight), int(max_batch), int(n_streams))
+        check(lib().ekg_server_create(ctx.h, C.byref(o), C.byref(self.h)))
+        self._hearts = []
+        self._shapes = {}
+
+    def close(self):
+        if self.h:
+            lib().ekg_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_heart(self, heart: Heart) -> int:
+        i = C.c_int()
+        check(lib().ekg_server_add_heart(self.h, heart.h, C.byref(i)))
+        self._hearts.append(heart)
+        return i.value
+
+    def submit(self, heart_id, layer_k, lead_pos, nhood="3D4", start=0.0, length=600.0, dt=1.0):
+        d = _Desc(layer_k, lead_pos, nhood, start, length, dt)
+        t = C.c_uint64()
+        check(lib().ekg_server_submit(self.h, int(heart_id), C.byref(d.s), C.byref(t)))
+        self._shapes[t.value] = (d.p.shape[0], num_steps(length, dt))
+        return t.value
+
+    def poll(self, ticket) -> bool:
+        d = C.c_int()
+        check(lib().ekg_server_poll(self.h, C.c_uint64(ticket), C.byref(d)))
+        return bool(d.value)
+
+    def wait(self, ticket):
+        L, T = self._shapes.pop(ticket)
+        out = np.zeros((L, T), np.float64)
+        lat = C.c_double()
+        check(lib().ekg_server_wait(self.h, C.c_uint64(ticket), _p(out), out.size, C.byref(lat)))
+        return out, lat.value
+
+    def stats(self):
+        c = C.c_uint64()
+        g = C.c_uint64()
+        p = C.c_int()
+        b = C.c_size_t()
+        check(lib().ekg_server_stats(self.h, C.byref(c), C.byref(g), C.byref(p), C.byref(b)))
+        return {"completed": c.value, "groups": g.value, "peak_inflight": p.value,
+                "peak_bytes": b.value}

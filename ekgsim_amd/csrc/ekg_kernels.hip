@@ -1,0 +1,548 @@
+// ekg_kernels.hip -- CDNA4 (gfx950) kernels of the ekgsim simulation core.
+//
+//   ca_*            excitation sequence: min-plus relaxation over LDS-staged
+//                   8^3 voxel bricks with a 1-voxel halo, bit-exact with the
+//                   reference's priority-queue flood (simulator.cpp:212-286)
+//   lead_field      per-voxel lead weights G_i(u) in FP64 (regrouped
+//                   simulator.cpp:509-537) plus per-chunk sums of G
+//   ap_tables       per (request, layer, t) repolarisation factor and
+//                   per-layer FP32 constants of the Wohlfart AP (Wohlfart.h:195-203)
+//   ecg_fused       AP evaluation fused with the lead contraction: one
+//                   wave = 64 consecutive time steps of one voxel chunk,
+//                   voxel data in SGPRs, FP32 AP, FP64 FMA accumulation
+//   ecg_reduce      fixed-order FP64 sum of the per-chunk partial ECGs
+//
+// Every launch is checked on the host against the buffer extents it assumes
+// (ekg_engine.cpp) before it is issued.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ekg_kernels.hpp"
+
+namespace ekgk {
+
+// ---------------------------------------------------------------------------
+// Excitation sequence
+// ---------------------------------------------------------------------------
+
+constexpr int CA_B = 8;                      // brick edge (voxels)
+constexpr int CA_H = CA_B + 2;               // brick edge with halo
+constexpr int CA_N = CA_H * CA_H * CA_H;     // 1000 halo cells
+constexpr int CA_THREADS = CA_B * CA_B * CA_B;
+
+template <class T> __device__ __forceinline__ T ca_inf();
+template <> __device__ __forceinline__ double ca_inf<double>() { return __builtin_inf(); }
+template <> __device__ __forceinline__ int ca_inf<int>() { return 0x7fffffff; }
+
+// dist := 1 at seeds, +inf elsewhere; flag the bricks holding seeds.
+template <class T>
+__global__ void ca_init_kernel(const uint8_t* __restrict__ layer, const uint8_t* __restrict__ seed,
+                               T* __restrict__ dist, uint8_t* __restrict__ active, int n0, int n1,
+                               int n2, int nb1, int nb2)
+{
+    int64_t n = (int64_t)n0 * n1 * n2;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        bool s = seed[i] != 0 && layer[i] != 0;
+        dist[i] = s ? (T)1 : ca_inf<T>();
+        if (s) {
+            int64_t x = i % n2, r = i / n2;
+            int64_t y = r % n1, z = r / n1;
+            active[((z / CA_B) * nb1 + y / CA_B) * nb2 + x / CA_B] = 1;
+        }
+    }
+}
+
+// One relaxation sweep over the active bricks.  Within a brick the 512
+// threads (one per voxel) iterate Jacobi min-plus updates on the LDS copy
+// until no voxel improves (__syncthreads_or is the block-wide ballot); the
+// improved values are written back and, when a brick-boundary voxel
+// improved, the neighbouring bricks are flagged for the next sweep.
+//
+// kInt = false: FP64, lag(u->v) = wtab[(layer(u)*wcols + layer(v))*4 + |d|^2]
+//               with wtab = transfer * sqrt(|d|^2) computed on the host in
+//               FP64, so each candidate is one IEEE add, the same rounding as
+//               `active.time + lag` in simulator.cpp:241.
+// kInt = true:  int32, lag(u->v) = delay[v].
+// Values only decrease and each is the value of a real path, so any order
+// of relaxation reaches the least fixpoint, which is what the reference's
+// Dijkstra computes when every lag is >= 0 (checked on the host).
+template <bool kInt>
+__global__ __launch_bounds__(CA_THREADS) void ca_sweep_kernel(
+    const uint8_t* __restrict__ layer, const uint8_t* __restrict__ delay,
+    const double* __restrict__ wtab, int wcols,
+    typename CaT<kInt>::type* __restrict__ dist, int n0, int n1, int n2, int nb0, int nb1,
+    int nb2, const uint8_t* __restrict__ act_in, uint8_t* __restrict__ act_out,
+    int* __restrict__ any_marked)
+{
+    using T = typename CaT<kInt>::type;
+    __shared__ T s_d[CA_N];
+    __shared__ uint8_t s_l[CA_N];
+    __shared__ int s_mark[27];
+
+    const int brick = blockIdx.x;
+    if (!act_in[brick]) return;
+    const int bz = brick / (nb1 * nb2);
+    const int br = brick - bz * nb1 * nb2;
+    const int by = br / nb2, bx = br - by * nb2;
+    const int z0 = bz * CA_B, y0 = by * CA_B, x0 = bx * CA_B;
+
+    for (int i = threadIdx.x; i < CA_N; i += blockDim.x) {
+        int hz = i / (CA_H * CA_H), hy = (i / CA_H) % CA_H, hx = i % CA_H;
+        int gz = z0 - 1 + hz, gy = y0 - 1 + hy, gx = x0 - 1 + hx;
+        uint8_t lh = 0;
+        T dh = ca_inf<T>();
+        if (gz >= 0 && gz < n0 && gy >= 0 && gy < n1 && gx >= 0 && gx < n2) {
+            const int64_t g = ((int64_t)gz * n1 + gy) * n2 + gx;
+            lh = layer[g];
+            dh = dist[g];
+        }
+        s_l[i] = lh;
+        s_d[i] = dh;
+    }
+    if (threadIdx.x < 27) s_mark[threadIdx.x] = 0;
+    __syncthreads();
+
+    const int tz = threadIdx.x / (CA_B * CA_B);
+    const int ty = (threadIdx.x / CA_B) % CA_B;
+    const int tx = threadIdx.x % CA_B;
+    const int gz = z0 + tz, gy = y0 + ty, gx = x0 + tx;
+    const bool inside = gz < n0 && gy < n1 && gx < n2;
+    const int me = ((tz + 1) * CA_H + (ty + 1)) * CA_H + (tx + 1);
+    const int lv = inside ? s_l[me] : 0;
+    const int64_t g = ((int64_t)gz * n1 + gy) * n2 + gx;
+    const T orig = s_d[me];
+    T cur = orig;
+    int dly = 0;
+    if (kInt && lv) dly = delay[g];
+
+    for (;;) {
+        T cand = cur;
+        if (lv) {
+#pragma unroll
+            for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+                for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        if (dz == 0 && dy == 0 && dx == 0) continue;
+                        const int nb = me + (dz * CA_H + dy) * CA_H + dx;
+                        const int lu = s_l[nb];
+                        if (lu == 0) continue;
+                        const T du = s_d[nb];
+                        if (du == ca_inf<T>()) continue;
+                        T c;
+                        if constexpr (kInt) {
+                            c = du + dly;
+                        } else {
+                            const int k = dz * dz + dy * dy + dx * dx;
+                            c = du + wtab[(lu * wcols + lv) * 4 + k];
+                        }
+                        cand = c < cand ? c : cand;
+                    }
+        }
+        __syncthreads();
+        const int ch = cand < cur;
+        if (ch) {
+            cur = cand;
+            s_d[me] = cur;
+        }
+        if (!__syncthreads_or(ch)) break;
+    }
+
+    if (lv && cur < orig) {
+        dist[g] = cur;
+        const int oz0 = tz == 0 ? -1 : 0, oz1 = tz == CA_B - 1 ? 1 : 0;
+        const int oy0 = ty == 0 ? -1 : 0, oy1 = ty == CA_B - 1 ? 1 : 0;
+        const int ox0 = tx == 0 ? -1 : 0, ox1 = tx == CA_B - 1 ? 1 : 0;
+        for (int oz = oz0; oz <= oz1; ++oz)
+            for (int oy = oy0; oy <= oy1; ++oy)
+                for (int ox = ox0; ox <= ox1; ++ox)
+                    if (oz | oy | ox) s_mark[((oz + 1) * 3 + (oy + 1)) * 3 + (ox + 1)] = 1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 27 && s_mark[threadIdx.x]) {
+        const int oz = threadIdx.x / 9 - 1, oy = (threadIdx.x / 3) % 3 - 1, ox = threadIdx.x % 3 - 1;
+        const int nz = bz + oz, ny = by + oy, nx = bx + ox;
+        if (nz >= 0 && nz < nb0 && ny >= 0 && ny < nb1 && nx >= 0 && nx < nb2) {
+            act_out[(nz * nb1 + ny) * nb2 + nx] = 1;
+            *any_marked = 1;
+        }
+    }
+}
+
+// Unreached (still +inf) and empty voxels -> 0.0 (simulator.cpp:219 marks
+// "not excited" with 0); the result is the FP64 activation-time grid.
+template <class T>
+__global__ void ca_finalize_kernel(const T* __restrict__ dist, double* __restrict__ act, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        T d = dist[i];
+        act[i] = (d == ca_inf<T>()) ? 0.0 : (double)d;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Lead field
+// ---------------------------------------------------------------------------
+
+// m = (p - (u+1)) * (1/(l*sqrt(l))), l = |p - (u+1)|^2  (simulator.cpp:531-533)
+__device__ __forceinline__ void lead_m(const double* p, int i0, int i1, int i2, double& m0,
+                                       double& m1, double& m2)
+{
+    m0 = p[0] - (double)(i0 + 1);
+    m1 = p[1] - (double)(i1 + 1);
+    m2 = p[2] - (double)(i2 + 1);
+    double l = m0 * m0;
+    l += m1 * m1;
+    l += m2 * m2;
+    const double s = 1.0 / (l * sqrt(l));
+    m0 *= s;
+    m1 *= s;
+    m2 *= s;
+}
+
+// One block per (chunk, request).  G_i(u) = sum_{d in N, u+d myocardial}
+// (m_i(u+d) + m_i(u)) . d for the chunk's voxels, zero-padded to LP leads,
+// and the chunk sum of G per lead (fixed-order block reduction).
+template <int LP>
+__global__ __launch_bounds__(256) void lead_field_kernel(
+    const uint8_t* __restrict__ grid_layer, int n0, int n1, int n2,
+    const int32_t* __restrict__ vox_lin, const int4* __restrict__ chunks, int n_chunks,
+    const int8_t* __restrict__ nb_dif, int n_nb, const double* __restrict__ lead_pos, int n_leads,
+    int64_t V, double* __restrict__ G, double* __restrict__ gsum)
+{
+    __shared__ double s_sum[256][LP];
+    const int c = blockIdx.x;
+    const int b = blockIdx.y;
+    const int4 ch = chunks[c];
+    const double* pos = lead_pos + (int64_t)b * n_leads * 3;
+    double part[LP];
+#pragma unroll
+    for (int i = 0; i < LP; ++i) part[i] = 0.0;
+    for (int v = ch.y + threadIdx.x; v < ch.z; v += blockDim.x) {
+        const int64_t x = vox_lin[v];
+        const int i2 = (int)(x % n2);
+        const int64_t r = x / n2;
+        const int i1 = (int)(r % n1), i0 = (int)(r / n1);
+        double* gv = G + ((int64_t)b * V + v) * LP;
+#pragma unroll
+        for (int i = 0; i < LP; ++i) {
+            double gi = 0.0;
+            if (i < n_leads) {
+                double mx0, mx1, mx2;
+                lead_m(pos + 3 * i, i0, i1, i2, mx0, mx1, mx2);
+                for (int k = 0; k < n_nb; ++k) {
+                    const int d0 = nb_dif[3 * k], d1 = nb_dif[3 * k + 1], d2 = nb_dif[3 * k + 2];
+                    const int j0 = i0 + d0, j1 = i1 + d1, j2 = i2 + d2;
+                    if (j0 < 0 || j0 >= n0 || j1 < 0 || j1 >= n1 || j2 < 0 || j2 >= n2) continue;
+                    if (!grid_layer[((int64_t)j0 * n1 + j1) * n2 + j2]) continue;
+                    double my0, my1, my2;
+                    lead_m(pos + 3 * i, j0, j1, j2, my0, my1, my2);
+                    gi += (my0 + mx0) * d0 + (my1 + mx1) * d1 + (my2 + mx2) * d2;
+                }
+            }
+            gv[i] = gi;
+            part[i] += gi;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < LP; ++i) s_sum[threadIdx.x][i] = part[i];
+    __syncthreads();
+    if (threadIdx.x < LP) {
+        double s = 0.0;
+        for (int t = 0; t < (int)blockDim.x; ++t) s += s_sum[t][threadIdx.x];
+        gsum[((int64_t)b * n_chunks + c) * LP + threadIdx.x] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// AP tables
+// ---------------------------------------------------------------------------
+
+// Per (request b, layer l, padded step j): the repolarisation factor
+//   R(t_j) = 1 - (1 + exp(-k7 (t_j - k8) + log(2^(k7/k6) - 1)))^(-k6/k7)
+// of Wohlfart.h:200-201.  For a cell activated at a the reference evaluates
+// it at tau = t - a with k8 - a (simulator.cpp:154-158), and
+// tau - (k8 - a) = t - k8, so the factor depends on the layer and t only.
+// Thread j == 0 also writes the per-layer FP32 constants.
+__global__ void ap_tables_kernel(const double* __restrict__ layer_k, int n_layers,
+                                 const double* __restrict__ tgrid, int Tp, int n_req,
+                                 float* __restrict__ rtab, LayerConst* __restrict__ lc)
+{
+    const int64_t n = (int64_t)n_req * n_layers * Tp;
+    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int j = (int)(idx % Tp);
+        const int64_t bl = idx / Tp;
+        const double* k = layer_k + bl * 9;
+        const double t = tgrid[j];
+        const double r =
+            1.0 - pow(1.0 + exp(-k[7] * (t - k[8]) + log(pow(2.0, k[7] / k[6]) - 1.0)), -(k[6] / k[7]));
+        rtab[idx] = (float)r;
+        if (j == 0) {
+            const double LOG2E = 1.4426950408889634074;
+            LayerConst c;
+            c.c1 = (float)(-k[1] * LOG2E);
+            c.c4 = (float)(-k[4] * LOG2E);
+            c.c5 = (float)(-k[5] * LOG2E);
+            c.alpha = (float)(k[2] * (1.0 - k[3]));
+            c.beta = (float)(k[2] * k[3]);
+            c.pad0 = 0.f;
+            c.k0 = k[0];
+            lc[bl] = c;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused AP evaluation + lead contraction
+// ---------------------------------------------------------------------------
+
+// Sigmoid bounds in the exp2 domain.  With x = c1*tau:
+//   x >= SKIP_X for every lane  -> exp2(x) = +inf, S = 1/(1+inf) = 0 exactly:
+//                                  the voxel adds exactly 0 to this tile;
+//   x <= SAT_X for every lane   -> exp2(x) < 2^-24, 1 + exp2(x) rounds to 1,
+//                                  S = rcp(1) = 1 exactly: the sigmoid is
+//                                  skipped with bit-identical FP32 results.
+constexpr float SKIP_X = 130.0f;
+constexpr float SAT_X = -24.5f;
+
+__device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// tau = t - a from hi/lo FP32 pairs.
+__device__ __forceinline__ float tau_of(float th, float tl, float ah, float al)
+{
+    return (th - ah) + (tl - al);
+}
+
+// First voxel index in [lo, hi) whose predicate is false, for a predicate
+// that is true on a prefix of the a-sorted range.
+template <class P>
+__device__ __forceinline__ int first_false(int lo, int hi, P pred)
+{
+    while (lo < hi) {
+        const int mid = lo + ((hi - lo) >> 1);
+        if (pred(mid)) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Block = 4 waves; wave w of block (g, c, b) takes time tile 4g + w of
+// chunk c of request b.  Lane = time step.  Voxels of a chunk are one layer,
+// sorted by activation time, so for a tile they split into
+//   [v0, s1)  sigmoid saturated in every lane (no sigmoid evaluation),
+//   [s1, s2)  general case,
+//   [s2, v1)  exactly zero contribution (skipped),
+// found by binary search on the tile's first and last lane (tau is monotone
+// in t and in a).  When c1 >= 0 (no decaying sigmoid) the whole chunk takes
+// the general path.
+template <int LP>
+__global__ __launch_bounds__(256) void ecg_fused_kernel(
+    const float2* __restrict__ vox_a, const double* __restrict__ G,
+    const double* __restrict__ gsum, const int4* __restrict__ chunks,
+    const LayerConst* __restrict__ lc, const float* __restrict__ rtab,
+    const float2* __restrict__ thl, double* __restrict__ partial, int64_t V, int n_layers,
+    int n_chunks, int Tp, int n_tiles, int n_groups)
+{
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int c = blockIdx.x / n_groups;
+    const int tile = (blockIdx.x - c * n_groups) * 4 + w;
+    const int b = blockIdx.y;
+    if (tile >= n_tiles) return;
+
+    const int4 ch = chunks[c];
+    const int l = ch.x;  // layer, 1-based
+    const LayerConst k = lc[(int64_t)b * n_layers + (l - 1)];
+    const int j = tile * 64 + lane;
+    const float2 tt = thl[j];
+    const float R = rtab[((int64_t)b * n_layers + (l - 1)) * Tp + j];
+    const float2 tf = thl[tile * 64];
+    const float2 tl = thl[tile * 64 + 63];
+    const double* Gb = G + (int64_t)b * V * LP;
+
+    double acc[LP];
+#pragma unroll
+    for (int i = 0; i < LP; ++i) acc[i] = 0.0;
+
+    int s1 = ch.y, s2 = ch.z;
+    if (k.c1 < 0.0f) {
+        // saturated prefix: max over lanes of c1*tau <= SAT_X (max at the first lane)
+        s1 = first_false(ch.y, ch.z, [&](int v) {
+            const float2 a = vox_a[v];
+            return k.c1 * tau_of(tf.x, tf.y, a.x, a.y) <= SAT_X;
+        });
+        // non-zero range ends where min over lanes of c1*tau >= SKIP_X (min at the last lane)
+        s2 = first_false(s1, ch.z, [&](int v) {
+            const float2 a = vox_a[v];
+            return k.c1 * tau_of(tl.x, tl.y, a.x, a.y) < SKIP_X;
+        });
+    } else {
+        s1 = ch.y;
+    }
+
+    for (int v = ch.y; v < s1; ++v) {
+        const float2 a = vox_a[v];
+        const double* g = Gb + (int64_t)v * LP;
+        const float tau = tau_of(tt.x, tt.y, a.x, a.y);
+        const float e4 = fast_exp2(fminf(k.c4 * tau, 126.0f));
+        const float e5 = fast_exp2(fminf(k.c5 * tau, 126.0f));
+        const float ap = __builtin_fmaf(k.alpha, e4, k.beta) * e5 * R;
+        const double apd = (double)ap;
+#pragma unroll
+        for (int i = 0; i < LP; ++i) acc[i] = __builtin_fma(g[i], apd, acc[i]);
+    }
+    for (int v = s1; v < s2; ++v) {
+        const float2 a = vox_a[v];
+        const double* g = Gb + (int64_t)v * LP;
+        const float tau = tau_of(tt.x, tt.y, a.x, a.y);
+        const float e4 = fast_exp2(fminf(k.c4 * tau, 126.0f));
+        const float e5 = fast_exp2(fminf(k.c5 * tau, 126.0f));
+        const float e1 = fast_exp2(k.c1 * tau);
+        const float ap = __builtin_fmaf(k.alpha, e4, k.beta) * e5 * R * fast_rcp(1.0f + e1);
+        const double apd = (double)ap;
+#pragma unroll
+        for (int i = 0; i < LP; ++i) acc[i] = __builtin_fma(g[i], apd, acc[i]);
+    }
+
+    // k0 is constant in time: sum_u G_u (ap_u + k0) = sum_u G_u ap_u + k0 * sum_u G_u.
+    const double* gs = gsum + ((int64_t)b * n_chunks + c) * LP;
+    double* out = partial + (((int64_t)b * n_chunks + c) * LP) * Tp + j;
+#pragma unroll
+    for (int i = 0; i < LP; ++i) out[(int64_t)i * Tp] = __builtin_fma(k.k0, gs[i], acc[i]);
+}
+
+// ecg[b][i][t] = sum over chunks c (in order) of partial[b][c][i][t].
+__global__ void ecg_reduce_kernel(const double* __restrict__ partial, double* __restrict__ ecg,
+                                  int n_req, int n_chunks, int LP, int n_leads, int T, int Tp)
+{
+    const int64_t n = (int64_t)n_req * n_leads * T;
+    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int t = (int)(idx % T);
+        const int64_t bi = idx / T;
+        const int i = (int)(bi % n_leads);
+        const int b = (int)(bi / n_leads);
+        const double* p = partial + ((int64_t)b * n_chunks * LP + i) * Tp + t;
+        double s = 0.0;
+        for (int c = 0; c < n_chunks; ++c) s += p[(int64_t)c * LP * Tp];
+        ecg[idx] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+
+static int grid_for(int64_t n, int threads, int cap = 65536)
+{
+    int64_t g = (n + threads - 1) / threads;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+hipError_t launch_ca_init(bool is_int, const uint8_t* layer, const uint8_t* seed, void* dist,
+                          uint8_t* active, int n0, int n1, int n2, int nb1, int nb2, hipStream_t s)
+{
+    const int64_t n = (int64_t)n0 * n1 * n2;
+    if (is_int)
+        hipLaunchKernelGGL(ca_init_kernel<int>, dim3(grid_for(n, 256)), dim3(256), 0, s, layer, seed,
+                           (int*)dist, active, n0, n1, n2, nb1, nb2);
+    else
+        hipLaunchKernelGGL(ca_init_kernel<double>, dim3(grid_for(n, 256)), dim3(256), 0, s, layer,
+                           seed, (double*)dist, active, n0, n1, n2, nb1, nb2);
+    return hipGetLastError();
+}
+
+hipError_t launch_ca_sweep(bool is_int, const uint8_t* layer, const uint8_t* delay,
+                           const double* wtab, int wcols, void* dist, int n0, int n1, int n2,
+                           int nb0, int nb1, int nb2, const uint8_t* act_in, uint8_t* act_out,
+                           int* any_marked, hipStream_t s)
+{
+    const int nbricks = nb0 * nb1 * nb2;
+    if (is_int)
+        hipLaunchKernelGGL(ca_sweep_kernel<true>, dim3(nbricks), dim3(CA_THREADS), 0, s, layer,
+                           delay, wtab, wcols, (int*)dist, n0, n1, n2, nb0, nb1, nb2, act_in,
+                           act_out, any_marked);
+    else
+        hipLaunchKernelGGL(ca_sweep_kernel<false>, dim3(nbricks), dim3(CA_THREADS), 0, s, layer,
+                           delay, wtab, wcols, (double*)dist, n0, n1, n2, nb0, nb1, nb2, act_in,
+                           act_out, any_marked);
+    return hipGetLastError();
+}
+
+hipError_t launch_ca_finalize(bool is_int, const void* dist, double* act, int64_t n, hipStream_t s)
+{
+    if (is_int)
+        hipLaunchKernelGGL(ca_finalize_kernel<int>, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                           (const int*)dist, act, n);
+    else
+        hipLaunchKernelGGL(ca_finalize_kernel<double>, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                           (const double*)dist, act, n);
+    return hipGetLastError();
+}
+
+template <int LP>
+static void lf_launch(const LeadFieldArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(lead_field_kernel<LP>, dim3(a.n_chunks, a.n_req), dim3(256), 0, s,
+                       a.grid_layer, a.n0, a.n1, a.n2, a.vox_lin, a.chunks, a.n_chunks, a.nb_dif,
+                       a.n_nb, a.lead_pos, a.n_leads, a.V, a.G, a.gsum);
+}
+
+hipError_t launch_lead_field(const LeadFieldArgs& a, hipStream_t s)
+{
+    switch (a.LP) {
+    case 4: lf_launch<4>(a, s); break;
+    case 8: lf_launch<8>(a, s); break;
+    case 12: lf_launch<12>(a, s); break;
+    case 16: lf_launch<16>(a, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_ap_tables(const double* layer_k, int n_layers, const double* tgrid, int Tp,
+                            int n_req, float* rtab, LayerConst* lc, hipStream_t s)
+{
+    const int64_t n = (int64_t)n_req * n_layers * Tp;
+    hipLaunchKernelGGL(ap_tables_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, layer_k,
+                       n_layers, tgrid, Tp, n_req, rtab, lc);
+    return hipGetLastError();
+}
+
+template <int LP>
+static void fused_launch(const FusedArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(ecg_fused_kernel<LP>, dim3(a.n_chunks * a.n_groups, a.n_req), dim3(256), 0,
+                       s, a.vox_a, a.G, a.gsum, a.chunks, a.lc, a.rtab, a.thl, a.partial, a.V,
+                       a.n_layers, a.n_chunks, a.Tp, a.n_tiles, a.n_groups);
+}
+
+hipError_t launch_ecg_fused(const FusedArgs& a, hipStream_t s)
+{
+    switch (a.LP) {
+    case 4: fused_launch<4>(a, s); break;
+    case 8: fused_launch<8>(a, s); break;
+    case 12: fused_launch<12>(a, s); break;
+    case 16: fused_launch<16>(a, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_ecg_reduce(const double* partial, double* ecg, int n_req, int n_chunks, int LP,
+                             int n_leads, int T, int Tp, hipStream_t s)
+{
+    const int64_t n = (int64_t)n_req * n_leads * T;
+    hipLaunchKernelGGL(ecg_reduce_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, partial, ecg,
+                       n_req, n_chunks, LP, n_leads, T, Tp);
+    return hipGetLastError();
+}
+
+}  // namespace ekgk

@@ -1,0 +1,66 @@
+// ekg_kernels.hpp -- launch interface of the gfx950 kernels (ekg_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <hip/hip_vector_types.h>
+#include <stdint.h>
+
+namespace ekgk {
+
+template <bool kInt> struct CaT { using type = double; };
+template <> struct CaT<true> { using type = int; };
+
+// Per (request, layer) constants of the Wohlfart AP in the exp2 domain:
+//   V(tau) = S * (alpha*exp2(c4*tau) + beta) * exp2(c5*tau) * R(t) + k0,
+//   S = 1/(1 + exp2(c1*tau)),  c_i = -k_i*log2(e),  alpha = k2(1-k3), beta = k2*k3.
+struct LayerConst {
+    float c1, c4, c5, alpha, beta, pad0;
+    double k0;
+};
+static_assert(sizeof(LayerConst) == 32, "LayerConst layout");
+
+struct LeadFieldArgs {
+    const uint8_t* grid_layer;   // [n0*n1*n2]
+    int n0, n1, n2;
+    const int32_t* vox_lin;      // [V] linear grid index of each sorted myocardial voxel
+    const int4* chunks;          // [n_chunks] {layer, v_begin, v_end, 0}
+    int n_chunks;
+    const int8_t* nb_dif;        // [n_nb][3]
+    int n_nb;
+    const double* lead_pos;      // [n_req][n_leads][3]
+    int n_leads;
+    int64_t V;
+    int LP;                      // padded lead count (4, 8, 12, 16)
+    int n_req;
+    double* G;                   // [n_req][V][LP]
+    double* gsum;                // [n_req][n_chunks][LP]
+};
+
+struct FusedArgs {
+    const float2* vox_a;         // [V] activation time (hi, lo)
+    const double* G;             // [n_req][V][LP]
+    const double* gsum;          // [n_req][n_chunks][LP]
+    const int4* chunks;          // [n_chunks]
+    const LayerConst* lc;        // [n_req][n_layers]
+    const float* rtab;           // [n_req][n_layers][Tp]
+    const float2* thl;           // [Tp] time (hi, lo)
+    double* partial;             // [n_req][n_chunks][LP][Tp]
+    int64_t V;
+    int n_layers, n_chunks, Tp, n_tiles, n_groups, LP, n_req;
+};
+
+hipError_t launch_ca_init(bool is_int, const uint8_t* layer, const uint8_t* seed, void* dist,
+                          uint8_t* active, int n0, int n1, int n2, int nb1, int nb2, hipStream_t s);
+hipError_t launch_ca_sweep(bool is_int, const uint8_t* layer, const uint8_t* delay,
+                           const double* wtab, int wcols, void* dist, int n0, int n1, int n2,
+                           int nb0, int nb1, int nb2, const uint8_t* act_in, uint8_t* act_out,
+                           int* any_marked, hipStream_t s);
+hipError_t launch_ca_finalize(bool is_int, const void* dist, double* act, int64_t n, hipStream_t s);
+hipError_t launch_lead_field(const LeadFieldArgs& a, hipStream_t s);
+hipError_t launch_ap_tables(const double* layer_k, int n_layers, const double* tgrid, int Tp,
+                            int n_req, float* rtab, LayerConst* lc, hipStream_t s);
+hipError_t launch_ecg_fused(const FusedArgs& a, hipStream_t s);
+hipError_t launch_ecg_reduce(const double* partial, double* ecg, int n_req, int n_chunks, int LP,
+                             int n_leads, int T, int Tp, hipStream_t s);
+
+}  // namespace ekgk

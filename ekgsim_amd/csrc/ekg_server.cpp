@@ -1,0 +1,383 @@
+// ekg_server.cpp -- continuous batching on one GPU.
+//
+// The reference evaluates one simulation per MPI worker and keeps per-worker
+// job queues topped up by the master (ParallelFramework.h:286-430); an
+// external evaluator is one process per simulation (ExternalEvaluation.h:95-151).
+// Here one worker thread owns n_streams launch slots.  Requests queue FIFO;
+// whenever a slot is free the worker takes the oldest request, adds every
+// queued request with the same heart and shape (leads, layers,
+// neighbourhood, start, length, time step) up to max_batch, admits the group
+// if its device bytes fit the budget and the in-flight cap, and enqueues it
+// on the slot's stream.  Groups on different slots run concurrently, so a
+// request that arrives while others run is admitted as soon as a slot and
+// memory free up, and its kernels overlap with the running ones.  Results
+// are bitwise independent of the grouping: every request's partial sums
+// come from its own block rows over the heart's fixed chunking.
+#include "ekg_server.hpp"
+
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <thread>
+#include <unordered_map>
+
+namespace ekg {
+
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+struct Request {
+    uint64_t ticket = 0;
+    int heart_id = 0;
+    std::vector<double> layer_k, lead_pos;
+    ekg_sim_desc desc{};
+    int64_t T = 0;
+    enum State { Pending, Running, Done, Failed } state = Pending;
+    int err = EKG_OK;
+    std::string msg;
+    std::vector<double> ecg;
+    Clock::time_point t_submit, t_done;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    Workspace ws;
+    bool busy = false;
+    std::vector<Request*> reqs;
+    BatchPlan plan;
+    double* h_out = nullptr;
+    size_t h_out_cap = 0;
+    size_t bytes = 0;
+};
+
+bool same_shape(const Request& a, const Request& b)
+{
+    return a.heart_id == b.heart_id && a.desc.n_leads == b.desc.n_leads &&
+           a.desc.n_layers == b.desc.n_layers && a.desc.nhood == b.desc.nhood &&
+           a.desc.start == b.desc.start && a.desc.length == b.desc.length && a.desc.dt == b.desc.dt;
+}
+
+}  // namespace
+
+struct Server {
+    Ctx& ctx;
+    ekg_server_opts opts{};
+    std::vector<Heart*> hearts;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request*> pending;
+    std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
+    std::vector<std::unique_ptr<Slot>> slots;
+    std::thread worker;
+    bool stop = false;
+    uint64_t next_ticket = 1, completed = 0, groups = 0;
+    int inflight = 0, peak_inflight = 0;
+    size_t bytes_in_use = 0, peak_bytes = 0;
+
+    explicit Server(Ctx& c) : ctx(c) {}
+    void run();
+    void retire(Slot& s);
+    void fail_group(std::vector<Request*>& g, int code, const std::string& msg);
+};
+
+void Server::fail_group(std::vector<Request*>& g, int code, const std::string& msg)
+{
+    for (Request* r : g) {
+        r->state = Request::Failed;
+        r->err = code;
+        r->msg = msg;
+        r->t_done = Clock::now();
+    }
+    cv_done.notify_all();
+}
+
+// Called with mu held once the slot's completion event has fired.
+void Server::retire(Slot& s)
+{
+    const BatchPlan& p = s.plan;
+    for (size_t b = 0; b < s.reqs.size(); ++b) {
+        Request* r = s.reqs[b];
+        r->ecg.assign(s.h_out + b * p.L * p.T, s.h_out + (b + 1) * p.L * p.T);
+        if (p.Tn < p.T)
+            for (int i = 0; i < p.L; ++i)
+                std::fill(r->ecg.begin() + i * p.T + p.Tn, r->ecg.begin() + (i + 1) * p.T, 0.0);
+        r->state = Request::Done;
+        r->t_done = Clock::now();
+        ++completed;
+    }
+    inflight -= (int)s.reqs.size();
+    bytes_in_use -= s.bytes;
+    s.reqs.clear();
+    s.busy = false;
+    s.bytes = 0;
+    cv_done.notify_all();
+}
+
+void Server::run()
+{
+    (void)hipSetDevice(ctx.device);
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+        bool any_busy = false;
+        for (auto& sp : slots) {
+            Slot& s = *sp;
+            if (!s.busy) continue;
+            const hipError_t q = hipEventQuery(s.done);
+            if (q == hipSuccess) {
+                retire(s);
+            } else if (q != hipErrorNotReady) {
+                fail_group(s.reqs, EKG_EDEVICE, std::string("device error: ") + hipGetErrorString(q));
+                inflight -= (int)s.reqs.size();
+                bytes_in_use -= s.bytes;
+                s.reqs.clear();
+                s.busy = false;
+            } else {
+                any_busy = true;
+            }
+        }
+        if (stop) {
+            std::vector<Request*> rest(pending.begin(), pending.end());
+            pending.clear();
+            fail_group(rest, EKG_ESTATE, "server stopped before the request was admitted");
+            if (!any_busy) break;
+        }
+        Slot* free_slot = nullptr;
+        for (auto& sp : slots)
+            if (!sp->busy) {
+                free_slot = sp.get();
+                break;
+            }
+        bool launched = false;
+        if (free_slot && !pending.empty() && !stop) {
+            Request* head = pending.front();
+            std::vector<Request*> group;
+            const int cap_inflight = opts.max_inflight - inflight;
+            for (Request* r : pending) {
+                if ((int)group.size() >= opts.max_batch || (int)group.size() >= cap_inflight) break;
+                if (same_shape(*head, *r)) group.push_back(r);
+            }
+            Heart* h = hearts[head->heart_id];
+            BatchPlan plan;
+            size_t bytes = 0;
+            bool ok = !group.empty();
+            if (ok) {
+                try {
+                    std::lock_guard<std::mutex> hg(h->mu);
+                    ensure_layout(*h);
+                    std::vector<ekg_sim_desc> ds;
+                    for (Request* r : group) ds.push_back(r->desc);
+                    plan = plan_batch(*h, (int)ds.size(), ds.data());
+                    bytes = batch_device_bytes(*h, plan);
+                    // shrink the group until it fits the budget
+                    while (group.size() > 1 && bytes_in_use + bytes > opts.hbm_budget_bytes) {
+                        group.pop_back();
+                        ds.pop_back();
+                        plan = plan_batch(*h, (int)ds.size(), ds.data());
+                        bytes = batch_device_bytes(*h, plan);
+                    }
+                } catch (const Error& e) {
+                    std::vector<Request*> one{head};
+                    pending.pop_front();
+                    fail_group(one, e.code, e.msg);
+                    ok = false;
+                    launched = true;
+                }
+            }
+            if (ok && bytes_in_use + bytes > opts.hbm_budget_bytes) {
+                if (inflight == 0) {
+                    std::vector<Request*> one{head};
+                    pending.pop_front();
+                    fail_group(one, EKG_ENOMEM, "request does not fit the server's device budget");
+                    launched = true;
+                }
+                ok = false;
+            }
+            if (ok) {
+                for (Request* r : group) {
+                    r->state = Request::Running;
+                    for (auto it = pending.begin(); it != pending.end(); ++it)
+                        if (*it == r) {
+                            pending.erase(it);
+                            break;
+                        }
+                }
+                inflight += (int)group.size();
+                bytes_in_use += bytes;
+                peak_inflight = std::max(peak_inflight, inflight);
+                peak_bytes = std::max(peak_bytes, bytes_in_use);
+                free_slot->busy = true;
+                free_slot->reqs = group;
+                free_slot->plan = plan;
+                free_slot->bytes = bytes;
+                ++groups;
+                std::vector<ekg_sim_desc> ds;
+                for (Request* r : group) ds.push_back(r->desc);
+                lk.unlock();
+                std::string err;
+                int code = EKG_OK;
+                try {
+                    std::lock_guard<std::mutex> hg(h->mu);
+                    Slot& s = *free_slot;
+                    const size_t n_out = (size_t)plan.B * plan.L * plan.T;
+                    if (s.h_out_cap < n_out) {
+                        if (s.h_out) (void)hipHostFree(s.h_out);
+                        s.h_out = nullptr;
+                        s.h_out_cap = 0;
+                        hip_check(hipHostMalloc((void**)&s.h_out, n_out * 8, hipHostMallocDefault),
+                                  "hipHostMalloc");
+                        s.h_out_cap = n_out;
+                    }
+                    double* d = enqueue_batch(ctx, *h, plan, ds.data(), s.ws, s.stream, false);
+                    hip_check(hipMemcpyAsync(s.h_out, d, n_out * 8, hipMemcpyDeviceToHost, s.stream),
+                              "download ECGs");
+                    hip_check(hipEventRecord(s.done, s.stream), "hipEventRecord");
+                } catch (const Error& e) {
+                    code = e.code;
+                    err = e.msg;
+                }
+                lk.lock();
+                if (code != EKG_OK) {
+                    fail_group(free_slot->reqs, code, err);
+                    inflight -= (int)free_slot->reqs.size();
+                    bytes_in_use -= free_slot->bytes;
+                    free_slot->reqs.clear();
+                    free_slot->busy = false;
+                    free_slot->bytes = 0;
+                }
+                launched = true;
+            }
+        }
+        if (!launched) {
+            if (any_busy)
+                cv_work.wait_for(lk, std::chrono::microseconds(50));
+            else if (!stop)
+                cv_work.wait(lk, [&] { return stop || !pending.empty(); });
+        }
+    }
+}
+
+Server* server_create(Ctx& ctx, const ekg_server_opts& o)
+{
+    auto s = std::make_unique<Server>(ctx);
+    s->opts = o;
+    if (s->opts.hbm_budget_bytes == 0) s->opts.hbm_budget_bytes = (size_t)16 << 30;
+    if (s->opts.max_inflight <= 0) s->opts.max_inflight = 1024;
+    if (s->opts.max_batch <= 0) s->opts.max_batch = 256;
+    if (s->opts.n_streams <= 0) s->opts.n_streams = 2;
+    if (s->opts.n_streams > 8) fail(EKG_EINVAL, "at most 8 streams");
+    ctx.bind();
+    for (int i = 0; i < s->opts.n_streams; ++i) {
+        s->slots.push_back(std::make_unique<Slot>());
+        Slot& sl = *s->slots.back();
+        hip_check(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming), "hipEventCreate");
+    }
+    Server* raw = s.release();
+    raw->worker = std::thread([raw] { raw->run(); });
+    return raw;
+}
+
+void server_destroy(Server* s)
+{
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    (void)hipSetDevice(s->ctx.device);
+    for (auto& sp : s->slots) {
+        Slot& sl = *sp;
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        if (sl.h_out) (void)hipHostFree(sl.h_out);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    }
+    s->slots.clear();
+    delete s;
+}
+
+int server_add_heart(Server& s, Heart& h)
+{
+    if (h.ctx != &s.ctx) fail(EKG_EINVAL, "heart belongs to another context");
+    std::lock_guard<std::mutex> g(s.mu);
+    {
+        std::lock_guard<std::mutex> hg(h.mu);
+        ensure_layout(h);
+    }
+    s.hearts.push_back(&h);
+    return (int)s.hearts.size() - 1;
+}
+
+uint64_t server_submit(Server& s, int heart_id, const ekg_sim_desc& d)
+{
+    auto r = std::make_unique<Request>();
+    {
+        std::lock_guard<std::mutex> g(s.mu);
+        if (heart_id < 0 || heart_id >= (int)s.hearts.size())
+            fail(EKG_ENOTFOUND, "unknown heart id " + std::to_string(heart_id));
+        if (s.stop) fail(EKG_ESTATE, "server is stopping");
+    }
+    if (!d.layer_k || !d.lead_pos || d.n_layers < 1 || d.n_leads < 1 || d.n_leads > EKG_MAX_LEADS)
+        fail(EKG_EINVAL, "invalid simulation descriptor");
+    r->heart_id = heart_id;
+    r->layer_k.assign(d.layer_k, d.layer_k + 9 * d.n_layers);
+    r->lead_pos.assign(d.lead_pos, d.lead_pos + 3 * d.n_leads);
+    r->desc = d;
+    r->desc.layer_k = r->layer_k.data();
+    r->desc.lead_pos = r->lead_pos.data();
+    r->T = num_steps(d.length, d.dt);
+    if (r->T < 0) fail(EKG_EINVAL, "too many time steps");
+    r->t_submit = Clock::now();
+    std::lock_guard<std::mutex> g(s.mu);
+    r->ticket = s.next_ticket++;
+    const uint64_t t = r->ticket;
+    s.pending.push_back(r.get());
+    s.reqs.emplace(t, std::move(r));
+    s.cv_work.notify_one();
+    return t;
+}
+
+bool server_poll(Server& s, uint64_t ticket)
+{
+    std::lock_guard<std::mutex> g(s.mu);
+    auto it = s.reqs.find(ticket);
+    if (it == s.reqs.end()) fail(EKG_ENOTFOUND, "unknown ticket");
+    return it->second->state == Request::Done || it->second->state == Request::Failed;
+}
+
+void server_wait(Server& s, uint64_t ticket, double* ecg, size_t cap, double* latency_ms)
+{
+    std::unique_lock<std::mutex> lk(s.mu);
+    auto it = s.reqs.find(ticket);
+    if (it == s.reqs.end()) fail(EKG_ENOTFOUND, "unknown ticket");
+    Request* r = it->second.get();
+    s.cv_done.wait(lk, [&] { return r->state == Request::Done || r->state == Request::Failed; });
+    std::unique_ptr<Request> own = std::move(it->second);
+    s.reqs.erase(it);
+    if (latency_ms)
+        *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+    if (r->state == Request::Failed) fail(r->err, r->msg);
+    if (ecg) {
+        if (cap < r->ecg.size())
+            fail(EKG_EINVAL, "output buffer holds " + std::to_string(cap) + " doubles, " +
+                                 std::to_string(r->ecg.size()) + " needed");
+        std::memcpy(ecg, r->ecg.data(), r->ecg.size() * sizeof(double));
+    }
+}
+
+void server_stats(Server& s, uint64_t* completed, uint64_t* groups, int* peak_inflight,
+                  size_t* peak_bytes)
+{
+    std::lock_guard<std::mutex> g(s.mu);
+    if (completed) *completed = s.completed;
+    if (groups) *groups = s.groups;
+    if (peak_inflight) *peak_inflight = s.peak_inflight;
+    if (peak_bytes) *peak_bytes = s.peak_bytes;
+}
+
+}  // namespace ekg

@@ -1,0 +1,176 @@
+/*
+ * ekg.h -- C ABI of the MI355X-native ekgsim simulation core.
+ *
+ * The reference (synergy-twinning/ekgsim) exposes the simulation core as the
+ * C++ class SimLib::EkgSim (simlib/sim_lib.h:93-287) over SimLib::Simulation
+ * (simlib/simulator.h:508-586).  This header is the thin C layer a binding
+ * (ctypes, cgo, JNI, N-API) or the C++ facade in ekgsim_amd/csrc/ekgsim.hpp
+ * calls.  Conventions:
+ *   - every call returns EKG_OK (0) or an error code; the message of the last
+ *     failure on the calling thread is ekg_last_error();
+ *   - no exceptions cross the ABI; all buffers are caller-owned host memory;
+ *   - voxel arrays are row-major (i0, i1, i2) = (z, y, x) with i2 fastest,
+ *     the reference's storage order (copyOfLibs/Pattern/Hypermatrix.h:652-661);
+ *   - lead positions are (z, y, x) in voxel units relative to the model
+ *     corner, as SimLib::MeasuringPoint stores them (simulator.cpp:369-390);
+ *   - an ECG is n_leads rows of T = ceil(length/dt) FP64 samples
+ *     (simulator.cpp:470), lead-major.
+ * A context owns one GPU and one HIP stream; calls on one context must not
+ * overlap (the reference is single-threaded too, sim.cpp:444).  Servers have
+ * their own internal locking.
+ */
+#ifndef EKG_H_INCLUDED
+#define EKG_H_INCLUDED
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EKG_OK 0
+#define EKG_EINVAL 1      /* bad argument or shape */
+#define EKG_ENOMEM 2      /* host or device allocation failed / over budget */
+#define EKG_EDEVICE 3     /* HIP runtime error */
+#define EKG_ENOSEED 4     /* no excitation starting point (simulator.cpp:276-279) */
+#define EKG_ETRANSFER 5   /* conduction matrix too small / negative (simulator.cpp:201-205, 233-237) */
+#define EKG_ESTATE 6      /* call order violated (e.g. simulate before excitation) */
+#define EKG_ENOTFOUND 7   /* unknown ticket / heart id */
+#define EKG_EIO 8         /* file read/parse failure */
+
+/* Dipole neighbourhoods, `[simulation] neighbourhood type` (sim_lib.h:131-143).
+ * 3D4 is the 8 corner neighbours (+-1,+-1,+-1) (simulator.h:349-357). */
+#define EKG_NHOOD_2D4 0
+#define EKG_NHOOD_2D8 1
+#define EKG_NHOOD_3D4 2
+#define EKG_NHOOD_3D8 3
+
+typedef struct ekg_ctx ekg_ctx;
+typedef struct ekg_heart ekg_heart;
+typedef struct ekg_server ekg_server;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *ekg_last_error(void);
+
+/* Library build string (kernel target, features). */
+const char *ekg_version(void);
+
+/* ---- context ------------------------------------------------------------ */
+/* One GPU, one stream.  No reference counterpart (the reference is CPU-only);
+ * it replaces the process-wide state of SimLib::EkgSim. */
+int ekg_ctx_create(int device, ekg_ctx **out);
+void ekg_ctx_destroy(ekg_ctx *ctx);
+/* Kernel timing on the context stream: when enabled, every launch of the
+ * fused AP+lead kernel is bracketed by HIP events; ekg_ctx_kernel_time
+ * returns the number of launches and their summed duration since the last
+ * reset. */
+int ekg_ctx_set_profiling(ekg_ctx *ctx, int enable);
+int ekg_ctx_kernel_time(ekg_ctx *ctx, const char *kernel, int64_t *launches, double *total_ms);
+int ekg_ctx_reset_profiling(ekg_ctx *ctx);
+
+/* ---- heart (device-resident geometry) ----------------------------------- */
+/* Replaces EkgSim::loadShape (sim_lib.h:211-216) -> Simulation::loadShape
+ * (simulator.cpp:181-199).  layer[i] = 0 empty, 1..255 myocardial layer;
+ * seed[i] != 0 marks an excitation start (the 0x1000 flag of matrix.h:87,
+ * a negative value in the .matrix file, matrix.h:124-129).  seed may be
+ * NULL when activation times are supplied with ekg_heart_set_activation. */
+int ekg_heart_create(ekg_ctx *ctx, int n0, int n1, int n2, const uint8_t *layer,
+                     const uint8_t *seed, ekg_heart **out);
+void ekg_heart_destroy(ekg_heart *h);
+int ekg_heart_num_myocardial(const ekg_heart *h, int64_t *n);
+int ekg_heart_max_layer(const ekg_heart *h, int *max_layer);
+
+/* Excitation sequence on the GPU, FP64, bit-exact with
+ * Simulation::calculateExcitationSequence (simulator.cpp:248-286) /
+ * exciteElement (simulator.cpp:212-246): seeds at 1.0, lag
+ * transfer[layer(u)][layer(v)] * sqrt(|d|^2) over the 26-neighbourhood (8 in
+ * 2D), 0.0 for unreached myocardium.  transfer is row-major rows x cols;
+ * every myocardial layer must be < rows and < cols and every used entry must
+ * be >= 0 (EKG_ETRANSFER otherwise). */
+int ekg_heart_excite_transfer(ekg_heart *h, const double *transfer, int rows, int cols);
+
+/* Excitation sequence with an integer per-voxel conduction delay (the
+ * synthetic-heart configuration; no reference counterpart): seeds at 1, lag
+ * delay[v] steps into voxel v from any of its 26 (8 in 2D) neighbours. */
+int ekg_heart_excite_delay(ekg_heart *h, const uint8_t *delay);
+
+/* Activation times supplied by the caller, one per voxel of the grid
+ * (Simulation::loadExcitationSequence, simulator.cpp:288-367). */
+int ekg_heart_set_activation(ekg_heart *h, const double *act);
+/* Activation times of the whole grid (excitationDelay of every
+ * ShapeElement, matrix.h:84). */
+int ekg_heart_get_activation(const ekg_heart *h, double *act);
+
+/* ---- plain simulation (the drop-in) ------------------------------------- */
+typedef struct ekg_sim_desc {
+    int n_layers;           /* rows of layer_k; must be >= the heart's max layer */
+    const double *layer_k;  /* [n_layers][9] WohlfartPlus coefficients; layer l uses row l-1
+                               (Simulation::setApIndices, simulator.cpp:561-621) */
+    int n_leads;            /* 1..EKG_MAX_LEADS */
+    const double *lead_pos; /* [n_leads][3] (z,y,x) measuring points, already displaced */
+    int nhood;              /* EKG_NHOOD_* (dipole neighbourhood) */
+    double start;           /* [simulation] start, ms */
+    double length;          /* [simulation] length, ms */
+    double dt;              /* [simulation] time step, ms */
+} ekg_sim_desc;
+
+#define EKG_MAX_LEADS 16
+
+/* Samples per lead: ceil(length / dt) (simulator.cpp:470). */
+int64_t ekg_num_steps(double length, double dt);
+
+/* One ECG, replacing Simulation::run (simulator.cpp:452-550) and the
+ * measurements EkgSim::getMeasurement returns (sim_lib.h:259-262).  AP
+ * evaluation in FP32, lead sums and the cross-block reduction in FP64.
+ * ecg: [n_leads][T] FP64. */
+int ekg_simulate(ekg_heart *h, const ekg_sim_desc *d, double *ecg);
+
+/* B ECGs of one heart in one set of launches.  All descs must share n_leads,
+ * nhood, start, length and dt; layer_k and lead_pos differ per entry.
+ * ecg: [batch][n_leads][T]. */
+int ekg_simulate_batch(ekg_heart *h, int batch, const ekg_sim_desc *descs, double *ecg);
+
+/* ---- lead geometry helpers (host, FP64) --------------------------------- */
+/* U/V displacement basis of a measuring point (EkgSim::loadMeasuringPoints,
+ * sim_lib.h:167-193) and the displaced position p + du*U + dv*V
+ * (EkgSim::moveMeasuringPoints, sim_lib.h:197-208). */
+void ekg_lead_basis(const double p[3], double u[3], double v[3]);
+void ekg_move_lead(const double p[3], double du, double dv, double out[3]);
+
+/* ---- serving (continuous batching) -------------------------------------- */
+/* Device-resident request loop standing in for AMS-DEMO's
+ * ExternalEvaluation / PFramework job queues (ExternalEvaluation.h:95-151,
+ * ParallelFramework.h:286-430): requests are admitted while their device
+ * memory fits hbm_budget_bytes and a slot is free, grouped per heart into
+ * fused launches, and overlap on the GPU with requests already running. */
+typedef struct ekg_server_opts {
+    size_t hbm_budget_bytes;  /* device memory the server may use for requests (0 = 16 GiB) */
+    int max_inflight;         /* requests on the GPU at once (0 = 1024) */
+    int max_batch;            /* requests per launch group (0 = 256) */
+    int n_streams;            /* concurrent launch groups (0 = 2) */
+} ekg_server_opts;
+
+int ekg_server_create(ekg_ctx *ctx, const ekg_server_opts *opts, ekg_server **out);
+/* Stops the worker after finishing admitted requests; pending ones fail. */
+void ekg_server_destroy(ekg_server *s);
+/* Caches a heart (geometry + activation times) for requests; the heart must
+ * stay alive while the server uses it. */
+int ekg_server_add_heart(ekg_server *s, ekg_heart *h, int *heart_id);
+/* Queue one simulation; returns at once with a ticket. */
+int ekg_server_submit(ekg_server *s, int heart_id, const ekg_sim_desc *d, uint64_t *ticket);
+/* done = 1 when the result is ready (or failed). */
+int ekg_server_poll(ekg_server *s, uint64_t ticket, int *done);
+/* Block until the ticket is done, copy its ECG ([n_leads][T] FP64, cap
+ * doubles available) and release it.  latency_ms (optional) receives
+ * submit-to-completion time. */
+int ekg_server_wait(ekg_server *s, uint64_t ticket, double *ecg, size_t cap, double *latency_ms);
+/* Counters: completed requests, launch groups, peak in-flight requests,
+ * peak device bytes. */
+int ekg_server_stats(ekg_server *s, uint64_t *completed, uint64_t *groups, int *peak_inflight,
+                     size_t *peak_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EKG_H_INCLUDED */

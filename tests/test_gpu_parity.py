@@ -1,0 +1,182 @@
+"""HIP path vs the FP64 oracle on the same seeded inputs.
+
+Bars: activation times bit-exact (FP64 reference semantics and the integer
+per-voxel-delay mode); every ECG within rel-L2 <= 1e-5 of the FP64 oracle
+(the tolerance the simulation core is held to), plus max-error reporting.
+"""
+import numpy as np
+import pytest
+
+from ekgsim_amd import synth as S
+from oracle import oracle as O
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _heart(ctx, h):
+    from ekgsim_amd import Heart
+    return Heart(ctx, h.layer, h.seed)
+
+
+@pytest.mark.parametrize("n,seed", [(24, 3), (33, 7), (48, 1)])
+def test_ca_fp64_bit_exact(ctx, n, seed):
+    h = S.make_heart(n, seed=seed)
+    T = S.layered_transfer(3, 12, 1.0, 1.5)
+    ref = O.excite_fp64(h.layer, h.seed, T)
+    g = _heart(ctx, h)
+    g.excite_transfer(T)
+    got = g.activation()
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+
+
+def test_ca_fp64_irregular_weights_bit_exact(ctx):
+    """Random conduction entries make path sums rounding-sensitive."""
+    rng = np.random.default_rng(11)
+    h = S.make_heart(40, seed=5)
+    T = np.full((12, 12), -1.0)
+    T[1:, 1:] = rng.uniform(0.1, 3.0, size=(11, 11))
+    ref = O.excite_fp64(h.layer, h.seed, T)
+    g = _heart(ctx, h)
+    g.excite_transfer(T)
+    assert np.array_equal(g.activation().view(np.uint64), ref.view(np.uint64))
+
+
+def test_ca_2d_bit_exact(ctx):
+    """A one-plane model floods over the 8-neighbourhood (simulator.cpp:251-254)."""
+    h = S.make_heart(64, seed=2, dims=(1, 64, 64))
+    if h.seed.sum() == 0:
+        lay = h.layer.ravel()
+        h.seed.ravel()[np.flatnonzero(lay)[0]] = 1
+    T = S.layered_transfer(3, 12, 1.0, 2.0)
+    ref = O.excite_fp64(h.layer, h.seed, T)
+    g = _heart(ctx, h)
+    g.excite_transfer(T)
+    assert np.array_equal(g.activation(), ref)
+
+
+@pytest.mark.parametrize("n", [48, 128])
+def test_ca_int_bit_exact(ctx, n):
+    h = S.make_heart(n, seed=1)
+    ref = O.excite_int(h.layer, h.seed, h.delay).astype(np.float64)
+    g = _heart(ctx, h)
+    g.excite_delay(h.delay)
+    assert np.array_equal(g.activation(), ref)
+
+
+def _case(n, L, seed=1, pseed=1000):
+    h = S.make_heart(n, seed=seed)
+    act = O.excite_int(h.layer, h.seed, h.delay).astype(np.float64)
+    x = S.random_params(3, L, pseed)
+    k = S.layer_k_from_params(x, 3)
+    pos = S.move_leads(S.lead_positions(n, L), S.displacements(x, 3, L))
+    return h, act, k, pos
+
+
+@pytest.mark.parametrize("nhood", ["3D4", "3D8", "2D8", "2D4"])
+def test_ecg_parity_small(ctx, nhood):
+    h, act, k, pos = _case(24, 2)
+    ref = O.run(h.layer, act, k, pos, nhood, 0, 600, 1.0)
+    g = _heart(ctx, h)
+    g.set_activation(act)
+    got = g.simulate(k, pos, nhood, 0, 600, 1.0)
+    err = rel_l2(got, ref)
+    print(f"24^3 {nhood}: rel-L2 {err:.3e} max|err| {np.abs(got - ref).max():.3e}")
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("start,length,dt", [(100, 400, 1.0), (0, 600, 0.25), (0, 37, 1.0), (5, 64, 1.0)])
+def test_ecg_parity_time_grids(ctx, start, length, dt):
+    h, act, k, pos = _case(24, 3)
+    ref = O.run(h.layer, act, k, pos, "3D4", start, length, dt)
+    g = _heart(ctx, h)
+    g.set_activation(act)
+    got = g.simulate(k, pos, "3D4", start, length, dt)
+    assert got.shape == ref.shape
+    assert rel_l2(got, ref) <= TOL
+
+
+def test_ecg_parity_fp64_activation(ctx):
+    """Non-integer activation times from the FP64 flood (sqrt(2), sqrt(3) lags)."""
+    h = S.make_heart(32, seed=4)
+    T = S.layered_transfer(3, 12, 1.0, 1.5)
+    act = O.excite_fp64(h.layer, h.seed, T)
+    x = S.random_params(3, 4, 77)
+    k = S.layer_k_from_params(x, 3)
+    pos = S.move_leads(S.lead_positions(32, 4), S.displacements(x, 3, 4))
+    ref = O.run(h.layer, act, k, pos, "3D4", 0, 600, 1.0)
+    g = _heart(ctx, h)
+    g.excite_transfer(T)
+    got = g.simulate(k, pos, "3D4", 0, 600, 1.0)
+    assert rel_l2(got, ref) <= TOL
+
+
+def test_ecg_parity_resting_offset(ctx):
+    """k0 != 0 (a resting potential) enters through the per-chunk sum of G."""
+    h, act, k, pos = _case(24, 2)
+    k = k.copy()
+    k[:, 0] = [-86.0, -84.0, -80.0]
+    ref = O.run(h.layer, act, k, pos, "3D4", 0, 600, 1.0)
+    g = _heart(ctx, h)
+    g.set_activation(act)
+    got = g.simulate(k, pos, "3D4", 0, 600, 1.0)
+    assert rel_l2(got, ref) <= TOL
+
+
+def test_ecg_parity_48_config(ctx):
+    """48^3 heart, 2 leads, 600 ms at 1 ms, 3D4 (the CPU-reference case)."""
+    h, act, k, pos = _case(48, 2)
+    ref = O.run(h.layer, act, k, pos, "3D4", 0, 600, 1.0)
+    g = _heart(ctx, h)
+    g.excite_delay(h.delay)
+    got = g.simulate(k, pos, "3D4", 0, 600, 1.0)
+    err = rel_l2(got, ref)
+    print(f"48^3: rel-L2 {err:.3e} max|err| {np.abs(got - ref).max():.3e}")
+    assert err <= TOL
+
+
+def test_ecg_parity_128_12_leads(ctx):
+    """Full size: 128^3 heart, 12 leads, 600 ms at 1 ms."""
+    h, act, k, pos = _case(128, 12)
+    ref = O.run(h.layer, act, k, pos, "3D4", 0, 600, 1.0)
+    g = _heart(ctx, h)
+    g.excite_delay(h.delay)
+    got = g.simulate(k, pos, "3D4", 0, 600, 1.0)
+    errs = [rel_l2(got[i], ref[i]) for i in range(12)]
+    print(f"128^3: rel-L2 {rel_l2(got, ref):.3e} worst lead {max(errs):.3e}")
+    assert rel_l2(got, ref) <= TOL
+    assert max(errs) <= TOL
+
+
+def test_batch_equals_plain_bitwise(ctx):
+    h = S.make_heart(40, seed=9)
+    g = _heart(ctx, h)
+    g.excite_delay(h.delay)
+    ks, ps = [], []
+    for b in range(5):
+        x = S.random_params(3, 6, 500 + b)
+        ks.append(S.layer_k_from_params(x, 3))
+        ps.append(S.move_leads(S.lead_positions(40, 6), S.displacements(x, 3, 6)))
+    batch = g.simulate_batch(np.stack(ks), np.stack(ps), "3D4", 0, 600, 1.0)
+    for b in range(5):
+        one = g.simulate(ks[b], ps[b], "3D4", 0, 600, 1.0)
+        assert np.array_equal(batch[b], one)
+
+
+def test_heart_without_seed_rejected(ctx):
+    from ekgsim_amd import EkgError
+    h = S.make_heart(16, seed=1)
+    g = _heart(ctx, h)
+    g2 = type(g)(ctx, h.layer, np.zeros_like(h.seed))
+    with pytest.raises(EkgError):
+        g2.excite_delay(h.delay)
+
+
+def test_simulate_before_excitation_rejected(ctx):
+    from ekgsim_amd import EkgError
+    h, act, k, pos = _case(16, 2)
+    g = _heart(ctx, h)
+    with pytest.raises(EkgError):
+        g.simulate(k, pos)
