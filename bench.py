@@ -1,0 +1,312 @@
+#!/usr/bin/env python3
+"""Simulated ECGs/s at 128^3, 600 ms, 12 leads -- batched vs plain, on one MI355X.
+
+One step = one batch of 256 parameter vectors (4 free AP coefficients per
+endo/mid/epi layer + 2 electrode displacements per lead) simulated on the
+cached 128^3 ellipsoid heart: lead field, AP tables, fused AP+lead kernel and
+the FP64 reduction, with the per-request inputs already in HBM.  Beside it:
+the plain drop-in (one ekg_simulate call per simulation, host buffers in and
+out), a short continuous-batching burst (p50/p99 latency), accuracy against
+the FP64 oracle and the compiled reference, and the reference's CPU path
+timed on the host cores.
+
+Prints one JSON line on stdout; details go to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import multiprocessing as mp
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from ekgsim_amd import synth as S  # noqa: E402  (no GPU use)
+
+METRIC = ("simulated ECGs/sec at 128^3, 600 ms, 12 leads (batched vs plain) at "
+          "≤1e-5 rel-L2; p99")
+# gfx950 VALU issue rate: 256 CU x 4 SIMD x 16 lanes x 2.4 GHz (lane-cycles/s)
+VALU_PEAK = 256 * 4 * 16 * 2.4e9
+# Algorithmic VALU work per voxel-step at L leads: 4 transcendentals at
+# quarter rate (16 lane-cycles) + 10 FP32 ops + L FMAs.
+def lane_cycles_per_voxel_step(L):
+    return 16 + 10 + L
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(n, L, B, pseed0=1000):
+    h = S.make_heart(n, seed=1)
+    ks, ps = [], []
+    base = S.lead_positions(n, L)
+    for r in range(B):
+        x = S.random_params(3, L, pseed0 + r)
+        ks.append(S.layer_k_from_params(x, 3))
+        ps.append((x, base))
+    return h, ks, ps
+
+
+# ---------------------------------------------------------------------------
+# CPU baseline leg (runs before the GPU is touched; forked workers)
+# ---------------------------------------------------------------------------
+_CPU = {}
+
+
+def _cpu_worker(_i):
+    from oracle import oracle as O
+    c = _CPU
+    out = []
+    t0 = time.perf_counter()
+    fn = O.ref_run if c["kind"] == "reference" else (lambda *a, **k: O.run(*a, literal=True, **k))
+    e1 = fn(c["layer"], c["act"], c["k"], c["pos"], "3D4", c["fixed_start"], 1.0, 1.0)
+    t_fixed = time.perf_counter() - t0
+    wins = []
+    for s in c["windows"]:
+        t0 = time.perf_counter()
+        e = fn(c["layer"], c["act"], c["k"], c["pos"], "3D4", float(s), float(c["w"]), 1.0)
+        wins.append(time.perf_counter() - t0)
+        out.append(e)
+    return t_fixed, wins, out, e1
+
+
+def cpu_baseline(h, act, k0, pos0, T, cores, w, windows):
+    from oracle import oracle as O
+    kind = "reference" if O.have_ref() else "port"
+    _CPU.update(kind=kind, layer=h.layer, act=act, k=k0, pos=pos0, w=w, windows=windows,
+                fixed_start=float(windows[0]))
+    ctx = mp.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(cores) as pool:
+        res = pool.map(_cpu_worker, range(cores))
+    wall = time.perf_counter() - t0
+    fixed = float(np.median([r[0] for r in res]))
+    per_step = float(np.median([(sum(r[1]) - len(windows) * fixed) / (len(windows) * w) for r in res]))
+    per_ecg = fixed + T * per_step
+    ecg_per_s = cores / per_ecg
+    return {
+        "kind": kind, "cores": cores, "per_ecg_s": per_ecg, "ecgs_per_s": ecg_per_s,
+        "wall_s": wall, "windows": [(int(s), int(s + w)) for s in windows],
+        "window_ecg": res[0][2], "fixed_s": fixed, "per_step_s": per_step,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--grid", type=int, default=128)
+    ap.add_argument("--leads", type=int, default=12)
+    ap.add_argument("--length", type=float, default=600.0)
+    ap.add_argument("--plain-calls", type=int, default=40)
+    ap.add_argument("--serve-requests", type=int, default=384)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-cores", type=int, default=0)
+    ap.add_argument("--cpu-window", type=int, default=5)
+    ap.add_argument("--check", type=int, default=3, help="ECGs checked against the FP64 oracle")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="only the batched steps (for rocprofv3 runs)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(requests would shard across GPUs with no exchange; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    n, L, B = args.grid, args.leads, args.batch
+    T = int(math.ceil(args.length / 1.0))
+    h, ks, xs = workload(n, L, B)
+    V = h.n_myo
+    log(f"heart {n}^3: {V} myocardial voxels; batch {B} x {L} leads x {T} steps")
+
+    # ---- CPU leg first: the FP64 oracle / compiled reference on the host ----
+    cpu = None
+    ref_full = {}
+    if not args.no_cpu and not args.profile_only:
+        from oracle import oracle as O
+        act = O.excite_int(h.layer, h.seed, h.delay).astype(np.float64)
+        pos_r = [S.move_leads(base, S.displacements(x, 3, L)) for (x, base) in xs]
+        cores = args.cpu_cores or min(16, os.cpu_count() or 1)
+        windows = [0, 100, 200, 300, 400, 500]
+        cpu = cpu_baseline(h, act, ks[0], pos_r[0], T, cores, args.cpu_window, windows)
+        log(f"cpu baseline ({cpu['kind']}, {cores} cores): {cpu['per_ecg_s']:.1f} s per ECG, "
+            f"{cpu['ecgs_per_s']:.4f} ECGs/s (wall {cpu['wall_s']:.1f} s)")
+        for r in sorted({0, B // 2, B - 1})[:max(0, args.check)]:
+            t0 = time.perf_counter()
+            ref_full[r] = O.run(h.layer, act, ks[r], pos_r[r], "3D4", 0.0, args.length, 1.0)
+            log(f"oracle FP64 ECG {r}: {time.perf_counter() - t0:.1f} s")
+
+    # ---- GPU ----
+    import torch
+    import ekgsim_amd as E
+
+    ctx = E.Context(0)
+    heart = E.Heart(ctx, h.layer, h.seed)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    heart.excite_delay(h.delay)
+    ca_ms = (time.perf_counter() - t0) * 1e3
+    pos = np.stack([E.move_leads(base, S.displacements(x, 3, L)) for (x, base) in xs])
+    kk = np.stack(ks)
+    dev = torch.device("cuda", 0)
+    d_k = torch.from_numpy(kk).to(dev)
+    d_p = torch.from_numpy(pos).to(dev)
+    d_e = torch.empty((B, L, T), dtype=torch.float64, device=dev)
+
+    def step():
+        heart.simulate_batch_device(d_k.data_ptr(), d_p.data_ptr(), d_e.data_ptr(), B, 3, L,
+                                    "3D4", 0.0, args.length, 1.0)
+
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    ctx.set_profiling(True)
+    ctx.reset_profiling()
+    torch.cuda.synchronize()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    nl, kms = ctx.kernel_time("ecg_fused")
+    ctx.set_profiling(False)
+    ms_step = el * 1e3 / args.steps
+    value = B * args.steps / el
+    avg_kernel_s = kms / 1e3 / max(nl, 1)
+    vsteps = B * V * T
+    achieved = vsteps * lane_cycles_per_voxel_step(L) / avg_kernel_s
+    log(f"batched: {ms_step:.2f} ms/step, {value:.1f} ECGs/s; fused kernel {avg_kernel_s * 1e3:.2f} ms "
+        f"({vsteps / avg_kernel_s:.3e} voxel-steps/s)")
+    if args.profile_only:
+        print(json.dumps({"ms_per_step": ms_step, "fused_ms": avg_kernel_s * 1e3}))
+        return
+
+    ecg_gpu = d_e.cpu().numpy()
+
+    # accuracy vs FP64 oracle (full ECGs) and vs the compiled reference (windows)
+    errs = {}
+    for r, ref in ref_full.items():
+        errs[r] = float(np.linalg.norm(ecg_gpu[r] - ref) / np.linalg.norm(ref))
+    rel_max = max(errs.values()) if errs else None
+    ref_win_err = None
+    if cpu is not None:
+        num = den = 0.0
+        for (s, e), we in zip(cpu["windows"], cpu["window_ecg"]):
+            num += float(np.sum((ecg_gpu[0][:, s:e] - we) ** 2))
+            den += float(np.sum(we ** 2))
+        ref_win_err = math.sqrt(num / den)
+        log(f"rel-L2 vs FP64 oracle {errs}; vs {cpu['kind']} on windows {ref_win_err:.2e}")
+
+    # ---- plain drop-in: one ekg_simulate per simulation, host in/out ----
+    lat = []
+    for i in range(args.plain_calls + 3):
+        t1 = time.perf_counter()
+        heart.simulate(ks[i % B], pos[i % B], "3D4", 0.0, args.length, 1.0)
+        if i >= 3:
+            lat.append(time.perf_counter() - t1)
+    plain = len(lat) / sum(lat)
+    log(f"plain drop-in: {plain:.1f} ECGs/s (p50 {np.percentile(lat, 50) * 1e3:.2f} ms)")
+
+    # ---- serving burst: 4 cached hearts, Poisson arrivals, mixed shapes ----
+    serve = None
+    if args.serve_requests > 0:
+        srv = E.Server(ctx, hbm_budget_bytes=48 << 30, max_batch=64, n_streams=3)
+        hs = []
+        for gi, g in enumerate((64, 96, 128, 160)):
+            hh = S.make_heart(g, seed=gi + 1)
+            H = E.Heart(ctx, hh.layer, hh.seed)
+            H.excite_delay(hh.delay)
+            hs.append((srv.add_heart(H), g, H))
+        rng = np.random.default_rng(7)
+        reqs = []
+        for r in range(args.serve_requests):
+            hid, g, _ = hs[rng.integers(0, 4)]
+            nl = int(rng.integers(2, 13))
+            dur = float(rng.integers(400, 801))
+            x = S.random_params(3, nl, 5000 + r)
+            reqs.append((hid, S.layer_k_from_params(x, 3),
+                         E.move_leads(S.lead_positions(g, nl), S.displacements(x, 3, nl)), dur))
+        # Poisson arrivals at a rate above the service rate (saturating)
+        rate = 4.0 * value
+        gaps = rng.exponential(1.0 / rate, size=len(reqs))
+        tickets = []
+        t0 = time.perf_counter()
+        nxt = t0
+        for (hid, k, p, dur), gap in zip(reqs, gaps):
+            nxt += gap
+            while time.perf_counter() < nxt:
+                pass
+            tickets.append(srv.submit(hid, k, p, "3D4", 0.0, dur, 1.0))
+        lats = [srv.wait(t)[1] for t in tickets]
+        el_s = time.perf_counter() - t0
+        st = srv.stats()
+        srv.close()
+        serve = {"requests": len(reqs), "ecgs_per_s": len(reqs) / el_s,
+                 "p50_ms": float(np.percentile(lats, 50)), "p99_ms": float(np.percentile(lats, 99)),
+                 "groups": st["groups"], "peak_inflight": st["peak_inflight"]}
+        log(f"serving: {serve}")
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "ECGs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 AP / f64 lead sums",
+        "data": "synthetic (seeded ellipsoid heart, random parameter vectors)",
+        "config": {
+            "workload": (f"batched: {B} parameter vectors per step on one cached {n}^3 ellipsoid "
+                         f"heart ({V} myocardial voxels), {L} leads, {T} steps of 1 ms, 3D4 dipoles"),
+            "grid": n, "myocardial_voxels": V, "leads": L, "steps_per_ecg": T, "batch": B,
+            "neighbourhood": "3D4", "excitation": "int32 per-voxel delay 1-3, 5 endocardial seeds",
+        },
+        "plain_ecgs_per_s": round(plain, 2),
+        "plain_p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+        "plain_p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+        "batched_vs_plain": round(value / plain, 2),
+        "serving": serve,
+        "p99_ms": None if serve is None else round(serve["p99_ms"], 3),
+        "rel_l2_max_vs_fp64": rel_max,
+        "rel_l2_vs_reference_windows": ref_win_err,
+        "excitation_ms": round(ca_ms, 2),
+        "roofline": {
+            "bound": "valu",
+            "kernel": "ecg_fused",
+            "achieved": round(achieved / 1e12, 3),
+            "peak": round(VALU_PEAK / 1e12, 3),
+            "unit": "T lane-cycles/s",
+            "frac": round(achieved / VALU_PEAK, 4),
+            "traffic": None,
+            "voxel_steps_per_s": vsteps / avg_kernel_s,
+            "kernel_ms": round(avg_kernel_s * 1e3, 4),
+            "launches": nl,
+        },
+        "cpu_baseline": None if cpu is None else {
+            "value": cpu["ecgs_per_s"], "unit": "ECGs/s", "cores": cpu["cores"], "kind": cpu["kind"],
+            "sample": (f"{n}^3 heart, {L} leads, request 0: steps {cpu['windows']} of {T} "
+                       f"plus a 1-step run for the per-call setup, one process per core; "
+                       f"{cpu['per_ecg_s']:.1f} s per ECG per core"),
+        },
+        "vs_cpu": None if cpu is None else round(value / cpu["ecgs_per_s"], 1),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -29,7 +29,8 @@ EXPORTS = (
     "ekg_heart_excite_delay", "ekg_heart_set_activation", "ekg_heart_get_activation",
     "ekg_num_steps", "ekg_simulate", "ekg_simulate_batch", "ekg_lead_basis", "ekg_move_lead",
     "ekg_server_create", "ekg_server_destroy", "ekg_server_add_heart", "ekg_server_submit",
-    "ekg_server_poll", "ekg_server_wait", "ekg_server_stats",
+    "ekg_server_poll", "ekg_server_wait", "ekg_server_stats", "ekg_simulate_batch_device",
+    "ekg_ctx_synchronize",
 )
 
 
@@ -95,6 +96,8 @@ def lib():
     L.ekg_num_steps.restype = C.c_int64
     L.ekg_simulate.argtypes = [vp, C.POINTER(SimDesc), vp]
     L.ekg_simulate_batch.argtypes = [vp, C.c_int, C.POINTER(SimDesc), vp]
+    L.ekg_simulate_batch_device.argtypes = [vp, C.POINTER(BatchDev)]
+    L.ekg_ctx_synchronize.argtypes = [vp]
     L.ekg_lead_basis.argtypes = [vp, vp, vp]
     L.ekg_lead_basis.restype = None
     L.ekg_move_lead.argtypes = [vp, C.c_double, C.c_double, vp]
@@ -147,6 +150,21 @@ def move_leads(pos, disp):
     return out
 
 
+class BatchDev(C.Structure):
+    _fields_ = [
+        ("batch", C.c_int),
+        ("n_layers", C.c_int),
+        ("layer_k_dev", C.c_void_p),
+        ("n_leads", C.c_int),
+        ("lead_pos_dev", C.c_void_p),
+        ("nhood", C.c_int),
+        ("start", C.c_double),
+        ("length", C.c_double),
+        ("dt", C.c_double),
+        ("ecg_dev", C.c_void_p),
+    ]
+
+
 class _Desc:
     """Keeps the arrays behind one SimDesc alive."""
 
@@ -185,6 +203,9 @@ class Context:
 
     def reset_profiling(self):
         check(lib().ekg_ctx_reset_profiling(self.h))
+
+    def synchronize(self):
+        check(lib().ekg_ctx_synchronize(self.h))
 
     def heart(self, layer, seed=None):
         return Heart(self, layer, seed)
@@ -266,6 +287,17 @@ class Heart:
         out = np.zeros((B, lead_pos.shape[1], T), np.float64)
         check(lib().ekg_simulate_batch(self.h, B, arr, _p(out)))
         return out
+
+
+    def simulate_batch_device(self, layer_k_ptr, lead_pos_ptr, ecg_ptr, batch, n_layers, n_leads,
+                              nhood="3D4", start=0.0, length=600.0, dt=1.0):
+        """Enqueue a batch whose arrays are device pointers (ints), e.g. the
+        data_ptr() of torch tensors on the GPU; returns without waiting."""
+        d = BatchDev(int(batch), int(n_layers), C.c_void_p(int(layer_k_ptr)), int(n_leads),
+                     C.c_void_p(int(lead_pos_ptr)),
+                     NHOOD[nhood] if isinstance(nhood, str) else int(nhood),
+                     float(start), float(length), float(dt), C.c_void_p(int(ecg_ptr)))
+        check(lib().ekg_simulate_batch_device(self.h, C.byref(d)))
 
 
 class Server:

@@ -288,6 +288,24 @@ int ekg_simulate_batch(ekg_heart* h, int batch, const ekg_sim_desc* descs, doubl
     });
 }
 
+int ekg_simulate_batch_device(ekg_heart* h, const ekg_batch_dev* d)
+{
+    return guard([&] {
+        if (!h || !d) ekg::fail(EKG_EINVAL, "NULL argument");
+        ekg::simulate_device(h->h, *d);
+    });
+}
+
+int ekg_ctx_synchronize(ekg_ctx* ctx)
+{
+    return guard([&] {
+        if (!ctx) ekg::fail(EKG_EINVAL, "ctx is NULL");
+        std::lock_guard<std::mutex> g(ctx->c.mu);
+        ctx->c.bind();
+        ekg::hip_check(hipStreamSynchronize(ctx->c.stream), "hipStreamSynchronize");
+    });
+}
+
 void ekg_lead_basis(const double p[3], double u[3], double v[3]) { basis(p, u, v); }
 
 void ekg_move_lead(const double p[3], double du, double dv, double out[3])

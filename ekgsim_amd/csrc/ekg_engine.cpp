@@ -124,6 +124,32 @@ static int64_t neighbour_steps(double start, double length, double dt, int64_t c
     return n;
 }
 
+// Shape checks and derived sizes shared by the host and device entry points.
+BatchPlan plan_shape(const Heart& h, BatchPlan p)
+{
+    if (p.B < 1 || p.B > 65535) fail(EKG_EINVAL, "batch must hold 1..65535 simulations");
+    if (p.L < 1 || p.L > EKG_MAX_LEADS)
+        fail(EKG_EINVAL, "n_leads must be in 1.." + std::to_string(EKG_MAX_LEADS));
+    if (p.nhood < EKG_NHOOD_2D4 || p.nhood > EKG_NHOOD_3D8)
+        fail(EKG_EINVAL, "unknown neighbourhood (only know of these: 2D4, 3D4, 2D8, 3D8 (cube))");
+    if (p.n_layers < h.max_layer || p.n_layers < 1)
+        fail(EKG_EINVAL, "the heart has " + std::to_string(h.max_layer) + " layers but only " +
+                             std::to_string(p.n_layers) + " layer APs were given");
+    if (!(p.dt > 0) || !std::isfinite(p.dt) || !std::isfinite(p.start) || !std::isfinite(p.length))
+        fail(EKG_EINVAL, "time step must be positive and times finite");
+    p.T = num_steps(p.length, p.dt);
+    if (p.T < 0 || p.T > (1 << 24)) fail(EKG_EINVAL, "too many time steps");
+    p.LP = p.L <= 4 ? 4 : p.L <= 8 ? 8 : p.L <= 12 ? 12 : 16;
+    p.Tp = ((p.T + 63) / 64) * 64;
+    p.n_tiles = (int)(p.Tp / 64);
+    p.n_groups = (p.n_tiles + 3) / 4;
+    p.Tn = neighbour_steps(p.start, p.length, p.dt, p.T + 1);
+    if (p.Tn > p.T)
+        fail(EKG_EINVAL, "time step makes the neighbour loop of the reference run past the "
+                         "sample buffer (simulator.cpp:519); choose another time step");
+    return p;
+}
+
 BatchPlan plan_batch(const Heart& h, int batch, const ekg_sim_desc* descs)
 {
     if (batch < 1 || !descs) fail(EKG_EINVAL, "batch must hold at least one descriptor");
@@ -139,36 +165,18 @@ BatchPlan plan_batch(const Heart& h, int batch, const ekg_sim_desc* descs)
     for (int b = 0; b < batch; ++b) {
         const ekg_sim_desc& d = descs[b];
         if (!d.layer_k || !d.lead_pos) fail(EKG_EINVAL, "layer_k and lead_pos are required");
-        if (d.n_leads < 1 || d.n_leads > EKG_MAX_LEADS)
-            fail(EKG_EINVAL, "n_leads must be in 1.." + std::to_string(EKG_MAX_LEADS));
-        if (d.nhood < EKG_NHOOD_2D4 || d.nhood > EKG_NHOOD_3D8)
-            fail(EKG_EINVAL, "unknown neighbourhood (only know of these: 2D4, 3D4, 2D8, 3D8 (cube))");
-        if (d.n_layers < h.max_layer || d.n_layers < 1)
-            fail(EKG_EINVAL, "the heart has " + std::to_string(h.max_layer) +
-                                 " layers but only " + std::to_string(d.n_layers) +
-                                 " layer APs were given");
         if (d.n_leads != p.L || d.nhood != p.nhood || d.start != p.start || d.length != p.length ||
             d.dt != p.dt || d.n_layers != p.n_layers)
             fail(EKG_EINVAL, "all descriptors of a batch must share leads, neighbourhood, "
                              "layers, start, length and time step");
+        if (d.n_layers < 1 || d.n_leads < 1 || d.n_leads > EKG_MAX_LEADS)
+            fail(EKG_EINVAL, "invalid layer or lead count");
         for (int i = 0; i < 9 * d.n_layers; ++i)
             if (!std::isfinite(d.layer_k[i])) fail(EKG_EINVAL, "non-finite AP coefficient");
         for (int i = 0; i < 3 * d.n_leads; ++i)
             if (!std::isfinite(d.lead_pos[i])) fail(EKG_EINVAL, "non-finite lead position");
     }
-    if (!(p.dt > 0) || !std::isfinite(p.dt) || !std::isfinite(p.start) || !std::isfinite(p.length))
-        fail(EKG_EINVAL, "time step must be positive and times finite");
-    p.T = num_steps(p.length, p.dt);
-    if (p.T < 0) fail(EKG_EINVAL, "too many time steps");
-    p.LP = p.L <= 4 ? 4 : p.L <= 8 ? 8 : p.L <= 12 ? 12 : 16;
-    p.Tp = ((p.T + 63) / 64) * 64;
-    p.n_tiles = (int)(p.Tp / 64);
-    p.n_groups = (p.n_tiles + 3) / 4;
-    p.Tn = neighbour_steps(p.start, p.length, p.dt, p.T + 1);
-    if (p.Tn > p.T)
-        fail(EKG_EINVAL, "time step makes the neighbour loop of the reference run past the "
-                         "sample buffer (simulator.cpp:519); choose another time step");
-    return p;
+    return plan_shape(h, p);
 }
 
 void ensure_layout(Heart& h)
@@ -245,24 +253,39 @@ size_t batch_device_bytes(const Heart& h, const BatchPlan& p)
     return s;
 }
 
-double* enqueue_batch(Ctx& ctx, Heart& h, const BatchPlan& p, const ekg_sim_desc* descs,
-                      Workspace& ws, hipStream_t stream, bool time_fused)
+void upload_batch(const BatchPlan& p, const ekg_sim_desc* descs, Workspace& ws, hipStream_t stream)
+{
+    const int B = p.B;
+    ws.h_k.resize((size_t)B * p.n_layers * 9);
+    ws.h_pos.resize((size_t)B * p.L * 3);
+    for (int b = 0; b < B; ++b) {
+        std::memcpy(&ws.h_k[(size_t)b * p.n_layers * 9], descs[b].layer_k,
+                    sizeof(double) * 9 * p.n_layers);
+        std::memcpy(&ws.h_pos[(size_t)b * p.L * 3], descs[b].lead_pos, sizeof(double) * 3 * p.L);
+    }
+    ws.layer_k.reserve(ws.h_k.size() * 8);
+    ws.lead_pos.reserve(ws.h_pos.size() * 8);
+    ws.ecg.reserve((size_t)B * p.L * std::max<int64_t>(p.T, 1) * 8);
+    hip_check(hipMemcpyAsync(ws.layer_k.p, ws.h_k.data(), ws.h_k.size() * 8, hipMemcpyHostToDevice,
+                             stream),
+              "upload AP coefficients");
+    hip_check(hipMemcpyAsync(ws.lead_pos.p, ws.h_pos.data(), ws.h_pos.size() * 8,
+                             hipMemcpyHostToDevice, stream),
+              "upload lead positions");
+}
+
+void enqueue_batch(Ctx& ctx, Heart& h, const BatchPlan& p, const double* d_layer_k,
+                   const double* d_lead_pos, double* d_ecg, Workspace& ws, hipStream_t stream,
+                   bool time_fused)
 {
     const EcgLayout& Lo = h.layout;
     const int B = p.B;
     const int64_t V = Lo.V;
     const int nch = Lo.n_chunks;
-    // host staging
-    std::vector<double> hk((size_t)B * p.n_layers * 9), hp((size_t)B * p.L * 3);
-    for (int b = 0; b < B; ++b) {
-        std::memcpy(&hk[(size_t)b * p.n_layers * 9], descs[b].layer_k,
-                    sizeof(double) * 9 * p.n_layers);
-        std::memcpy(&hp[(size_t)b * p.L * 3], descs[b].lead_pos, sizeof(double) * 3 * p.L);
-    }
     // time grid t_j = start + j*dt by repeated addition (simulator.cpp:496-500),
     // continued over the padding so the last tile stays monotone
     if ((int64_t)ws.h_tgrid.size() != p.Tp || ws.h_tgrid.empty() || ws.h_tgrid[0] != p.start ||
-        (p.Tp > 1 && ws.h_tgrid[1] != p.start + p.dt)) {
+        ws.grid_dt != p.dt) {
         ws.h_tgrid.resize(p.Tp);
         ws.h_thl.resize(2 * p.Tp);
         double st = p.start;
@@ -273,6 +296,7 @@ double* enqueue_batch(Ctx& ctx, Heart& h, const BatchPlan& p, const ekg_sim_desc
             ws.h_thl[2 * j + 1] = (float)(st - (double)hi);
             st += p.dt;
         }
+        ws.grid_dt = p.dt;
         ws.tgrid.reserve(p.Tp * 8);
         ws.thl.reserve(p.Tp * 8);
         hip_check(hipMemcpyAsync(ws.tgrid.p, ws.h_tgrid.data(), p.Tp * 8, hipMemcpyHostToDevice, stream),
@@ -280,18 +304,11 @@ double* enqueue_batch(Ctx& ctx, Heart& h, const BatchPlan& p, const ekg_sim_desc
         hip_check(hipMemcpyAsync(ws.thl.p, ws.h_thl.data(), p.Tp * 8, hipMemcpyHostToDevice, stream),
                   "upload time grid");
     }
-    ws.layer_k.reserve(hk.size() * 8);
-    ws.lead_pos.reserve(hp.size() * 8);
     ws.rtab.reserve((size_t)B * p.n_layers * p.Tp * 4);
     ws.lc.reserve((size_t)B * p.n_layers * sizeof(ekgk::LayerConst));
     ws.G.reserve((size_t)B * std::max<int64_t>(V, 1) * p.LP * 8);
     ws.gsum.reserve((size_t)B * std::max(nch, 1) * p.LP * 8);
     ws.partial.reserve((size_t)B * std::max(nch, 1) * p.LP * p.Tp * 8);
-    ws.ecg.reserve((size_t)B * p.L * std::max<int64_t>(p.T, 1) * 8);
-    hip_check(hipMemcpyAsync(ws.layer_k.p, hk.data(), hk.size() * 8, hipMemcpyHostToDevice, stream),
-              "upload AP coefficients");
-    hip_check(hipMemcpyAsync(ws.lead_pos.p, hp.data(), hp.size() * 8, hipMemcpyHostToDevice, stream),
-              "upload lead positions");
 
     ekgk::LeadFieldArgs la;
     la.grid_layer = h.d_layer.as<uint8_t>();
@@ -304,7 +321,7 @@ double* enqueue_batch(Ctx& ctx, Heart& h, const BatchPlan& p, const ekg_sim_desc
     la.nb_dif = static_cast<const int8_t*>(ctx_nb_table(ctx, p.nhood));
     int8_t dif[26][3];
     la.n_nb = nhood_offsets(p.nhood, dif);
-    la.lead_pos = ws.lead_pos.as<double>();
+    la.lead_pos = d_lead_pos;
     la.n_leads = p.L;
     la.V = V;
     la.LP = p.LP;
@@ -313,7 +330,7 @@ double* enqueue_batch(Ctx& ctx, Heart& h, const BatchPlan& p, const ekg_sim_desc
     la.gsum = ws.gsum.as<double>();
     hip_check(ekgk::launch_lead_field(la, stream), "lead_field launch");
 
-    hip_check(ekgk::launch_ap_tables(ws.layer_k.as<double>(), p.n_layers, ws.tgrid.as<double>(),
+    hip_check(ekgk::launch_ap_tables(d_layer_k, p.n_layers, ws.tgrid.as<double>(),
                                      (int)p.Tp, B, ws.rtab.as<float>(),
                                      ws.lc.as<ekgk::LayerConst>(), stream),
               "ap_tables launch");
@@ -346,10 +363,9 @@ double* enqueue_batch(Ctx& ctx, Heart& h, const BatchPlan& p, const ekg_sim_desc
         hip_check(hipEventRecord(e1, stream), "hipEventRecord");
         ctx.pending.emplace_back(e0, e1);
     }
-    hip_check(ekgk::launch_ecg_reduce(ws.partial.as<double>(), ws.ecg.as<double>(), B, nch, p.LP,
-                                      p.L, (int)p.T, (int)p.Tp, stream),
+    hip_check(ekgk::launch_ecg_reduce(ws.partial.as<double>(), d_ecg, B, nch, p.LP, p.L, (int)p.T,
+                                      (int)p.Tn, (int)p.Tp, stream),
               "ecg_reduce launch");
-    return ws.ecg.as<double>();
 }
 
 void simulate(Heart& h, int batch, const ekg_sim_desc* descs, double* ecg)
@@ -367,15 +383,41 @@ void simulate(Heart& h, int batch, const ekg_sim_desc* descs, double* ecg)
         std::fill(ecg, ecg + n_out, 0.0);
         return;
     }
-    double* d = enqueue_batch(ctx, h, p, descs, ctx.ws, ctx.stream, ctx.profiling);
-    hip_check(hipMemcpyAsync(ecg, d, n_out * 8, hipMemcpyDeviceToHost, ctx.stream), "download ECG");
+    upload_batch(p, descs, ctx.ws, ctx.stream);
+    enqueue_batch(ctx, h, p, ctx.ws.layer_k.as<double>(), ctx.ws.lead_pos.as<double>(),
+                  ctx.ws.ecg.as<double>(), ctx.ws, ctx.stream, ctx.profiling);
+    hip_check(hipMemcpyAsync(ecg, ctx.ws.ecg.p, n_out * 8, hipMemcpyDeviceToHost, ctx.stream),
+              "download ECG");
     hip_check(hipStreamSynchronize(ctx.stream), "hipStreamSynchronize");
     if (ctx.profiling) ctx.collect_timings();
-    if (p.Tn < p.T)
-        for (int b = 0; b < p.B; ++b)
-            for (int i = 0; i < p.L; ++i)
-                std::fill(ecg + ((size_t)b * p.L + i) * p.T + p.Tn, ecg + ((size_t)b * p.L + i + 1) * p.T,
-                          0.0);
+}
+
+void simulate_device(Heart& h, const ekg_batch_dev& d)
+{
+    Ctx& ctx = *h.ctx;
+    std::lock_guard<std::mutex> g1(ctx.mu);
+    std::lock_guard<std::mutex> g2(h.mu);
+    if (!d.layer_k_dev || !d.lead_pos_dev || !d.ecg_dev)
+        fail(EKG_EINVAL, "device pointers are required");
+    BatchPlan p;
+    p.B = d.batch;
+    p.L = d.n_leads;
+    p.n_layers = d.n_layers;
+    p.nhood = d.nhood;
+    p.start = d.start;
+    p.length = d.length;
+    p.dt = d.dt;
+    p = plan_shape(h, p);
+    if (p.T == 0) return;
+    ctx.bind();
+    ensure_layout(h);
+    if (h.layout.V == 0) {
+        hip_check(hipMemsetAsync(d.ecg_dev, 0, (size_t)p.B * p.L * p.T * 8, ctx.stream),
+                  "hipMemsetAsync");
+        return;
+    }
+    enqueue_batch(ctx, h, p, d.layer_k_dev, d.lead_pos_dev, d.ecg_dev, ctx.ws, ctx.stream,
+                  ctx.profiling);
 }
 
 void excite(Heart& h, bool is_int, const double* transfer, int rows, int cols, const uint8_t* delay)

@@ -39,8 +39,9 @@ struct DevBuf {
 // Device workspace of one batch of simulations on one stream.
 struct Workspace {
     DevBuf layer_k, lead_pos, tgrid, thl, rtab, lc, G, gsum, partial, ecg;
-    std::vector<double> h_tgrid;
+    std::vector<double> h_tgrid, h_k, h_pos;
     std::vector<float> h_thl;
+    double grid_dt = 0.0;
 };
 
 struct Ctx {
@@ -94,14 +95,20 @@ struct BatchPlan {
 int nhood_offsets(int type, int8_t dif[26][3]);
 const void* ctx_nb_table(const Ctx& ctx, int nhood);
 int64_t num_steps(double length, double dt);
+BatchPlan plan_shape(const Heart& h, BatchPlan p);
 BatchPlan plan_batch(const Heart& h, int batch, const ekg_sim_desc* descs);
 void ensure_layout(Heart& h);
-// Enqueue the whole batch on `stream` into workspace `ws`; the ECGs end in
-// ws.ecg ([B][L][T] FP64).  Returns the device pointer of the result.
-double* enqueue_batch(Ctx& ctx, Heart& h, const BatchPlan& p, const ekg_sim_desc* descs,
-                      Workspace& ws, hipStream_t stream, bool time_fused);
+// Copy the per-request coefficients and lead positions of host descriptors
+// into ws.layer_k / ws.lead_pos (and size ws.ecg).
+void upload_batch(const BatchPlan& p, const ekg_sim_desc* descs, Workspace& ws, hipStream_t stream);
+// Enqueue the whole batch on `stream`: d_layer_k [B][n_layers][9] and
+// d_lead_pos [B][L][3] in device memory, ECGs to d_ecg [B][L][T] FP64.
+void enqueue_batch(Ctx& ctx, Heart& h, const BatchPlan& p, const double* d_layer_k,
+                   const double* d_lead_pos, double* d_ecg, Workspace& ws, hipStream_t stream,
+                   bool time_fused);
 size_t batch_device_bytes(const Heart& h, const BatchPlan& p);
 void simulate(Heart& h, int batch, const ekg_sim_desc* descs, double* ecg);
+void simulate_device(Heart& h, const ekg_batch_dev& d);
 
 void excite(Heart& h, bool is_int, const double* transfer, int rows, int cols,
             const uint8_t* delay);

@@ -416,9 +416,11 @@ __global__ __launch_bounds__(256) void ecg_fused_kernel(
     for (int i = 0; i < LP; ++i) out[(int64_t)i * Tp] = __builtin_fma(k.k0, gs[i], acc[i]);
 }
 
-// ecg[b][i][t] = sum over chunks c (in order) of partial[b][c][i][t].
+// ecg[b][i][t] = sum over chunks c (in order) of partial[b][c][i][t] for
+// t < Tn, and 0 for Tn <= t < T: samples past the last iteration of the
+// reference's neighbour loop (simulator.cpp:519) receive no dipole.
 __global__ void ecg_reduce_kernel(const double* __restrict__ partial, double* __restrict__ ecg,
-                                  int n_req, int n_chunks, int LP, int n_leads, int T, int Tp)
+                                  int n_req, int n_chunks, int LP, int n_leads, int T, int Tn, int Tp)
 {
     const int64_t n = (int64_t)n_req * n_leads * T;
     for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n;
@@ -427,9 +429,11 @@ __global__ void ecg_reduce_kernel(const double* __restrict__ partial, double* __
         const int64_t bi = idx / T;
         const int i = (int)(bi % n_leads);
         const int b = (int)(bi / n_leads);
-        const double* p = partial + ((int64_t)b * n_chunks * LP + i) * Tp + t;
         double s = 0.0;
-        for (int c = 0; c < n_chunks; ++c) s += p[(int64_t)c * LP * Tp];
+        if (t < Tn) {
+            const double* p = partial + ((int64_t)b * n_chunks * LP + i) * Tp + t;
+            for (int c = 0; c < n_chunks; ++c) s += p[(int64_t)c * LP * Tp];
+        }
         ecg[idx] = s;
     }
 }
@@ -537,11 +541,11 @@ hipError_t launch_ecg_fused(const FusedArgs& a, hipStream_t s)
 }
 
 hipError_t launch_ecg_reduce(const double* partial, double* ecg, int n_req, int n_chunks, int LP,
-                             int n_leads, int T, int Tp, hipStream_t s)
+                             int n_leads, int T, int Tn, int Tp, hipStream_t s)
 {
     const int64_t n = (int64_t)n_req * n_leads * T;
     hipLaunchKernelGGL(ecg_reduce_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, partial, ecg,
-                       n_req, n_chunks, LP, n_leads, T, Tp);
+                       n_req, n_chunks, LP, n_leads, T, Tn, Tp);
     return hipGetLastError();
 }
 

@@ -61,6 +61,6 @@ hipError_t launch_ap_tables(const double* layer_k, int n_layers, const double* t
                             int n_req, float* rtab, LayerConst* lc, hipStream_t s);
 hipError_t launch_ecg_fused(const FusedArgs& a, hipStream_t s);
 hipError_t launch_ecg_reduce(const double* partial, double* ecg, int n_req, int n_chunks, int LP,
-                             int n_leads, int T, int Tp, hipStream_t s);
+                             int n_leads, int T, int Tn, int Tp, hipStream_t s);
 
 }  // namespace ekgk

@@ -102,9 +102,6 @@ void Server::retire(Slot& s)
     for (size_t b = 0; b < s.reqs.size(); ++b) {
         Request* r = s.reqs[b];
         r->ecg.assign(s.h_out + b * p.L * p.T, s.h_out + (b + 1) * p.L * p.T);
-        if (p.Tn < p.T)
-            for (int i = 0; i < p.L; ++i)
-                std::fill(r->ecg.begin() + i * p.T + p.Tn, r->ecg.begin() + (i + 1) * p.T, 0.0);
         r->state = Request::Done;
         r->t_done = Clock::now();
         ++completed;
@@ -231,8 +228,11 @@ void Server::run()
                                   "hipHostMalloc");
                         s.h_out_cap = n_out;
                     }
-                    double* d = enqueue_batch(ctx, *h, plan, ds.data(), s.ws, s.stream, false);
-                    hip_check(hipMemcpyAsync(s.h_out, d, n_out * 8, hipMemcpyDeviceToHost, s.stream),
+                    upload_batch(plan, ds.data(), s.ws, s.stream);
+                    enqueue_batch(ctx, *h, plan, s.ws.layer_k.as<double>(), s.ws.lead_pos.as<double>(),
+                                  s.ws.ecg.as<double>(), s.ws, s.stream, false);
+                    hip_check(hipMemcpyAsync(s.h_out, s.ws.ecg.p, n_out * 8, hipMemcpyDeviceToHost,
+                                             s.stream),
                               "download ECGs");
                     hip_check(hipEventRecord(s.done, s.stream), "hipEventRecord");
                 } catch (const Error& e) {

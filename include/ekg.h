@@ -131,6 +131,28 @@ int ekg_simulate(ekg_heart *h, const ekg_sim_desc *d, double *ecg);
  * ecg: [batch][n_leads][T]. */
 int ekg_simulate_batch(ekg_heart *h, int batch, const ekg_sim_desc *descs, double *ecg);
 
+/* B ECGs whose per-request arrays already live in device memory (HBM):
+ * layer_k_dev [batch][n_layers][9], lead_pos_dev [batch][n_leads][3], output
+ * ecg_dev [batch][n_leads][T], all FP64.  Same computation as
+ * ekg_simulate_batch (bitwise equal results); the work is enqueued on the
+ * context stream and the call returns without waiting (ekg_ctx_synchronize
+ * waits).  Values are not inspected on the host: non-finite inputs give
+ * non-finite ECGs.  No reference counterpart (the reference has no device);
+ * it is the form a device-resident caller such as a batched optimiser uses. */
+typedef struct ekg_batch_dev {
+    int batch;
+    int n_layers;
+    const double *layer_k_dev;
+    int n_leads;
+    const double *lead_pos_dev;
+    int nhood;
+    double start, length, dt;
+    double *ecg_dev;
+} ekg_batch_dev;
+
+int ekg_simulate_batch_device(ekg_heart *h, const ekg_batch_dev *d);
+int ekg_ctx_synchronize(ekg_ctx *ctx);
+
 /* ---- lead geometry helpers (host, FP64) --------------------------------- */
 /* U/V displacement basis of a measuring point (EkgSim::loadMeasuringPoints,
  * sim_lib.h:167-193) and the displaced position p + du*U + dv*V
