@@ -113,6 +113,8 @@ def main():
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--cpu-window", type=int, default=5)
     ap.add_argument("--check", type=int, default=3, help="ECGs checked against the FP64 oracle")
+    ap.add_argument("--kernel", default="auto", choices=["auto", "valu", "mfma"],
+                    help="lead-contraction path of the fused kernel")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the batched steps (for rocprofv3 runs)")
     args = ap.parse_args()
@@ -150,6 +152,7 @@ def main():
     import ekgsim_amd as E
 
     ctx = E.Context(0)
+    ctx.set_kernel(args.kernel)
     heart = E.Heart(ctx, h.layer, h.seed)
     torch.cuda.synchronize()
     t0 = time.perf_counter()

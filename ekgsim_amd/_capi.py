@@ -30,8 +30,9 @@ EXPORTS = (
     "ekg_num_steps", "ekg_simulate", "ekg_simulate_batch", "ekg_lead_basis", "ekg_move_lead",
     "ekg_server_create", "ekg_server_destroy", "ekg_server_add_heart", "ekg_server_submit",
     "ekg_server_poll", "ekg_server_wait", "ekg_server_stats", "ekg_simulate_batch_device",
-    "ekg_ctx_synchronize",
+    "ekg_ctx_synchronize", "ekg_ctx_set_kernel",
 )
+KERNELS = {"auto": 0, "valu": 1, "mfma": 2}
 
 
 class EkgError(RuntimeError):
@@ -98,6 +99,7 @@ def lib():
     L.ekg_simulate_batch.argtypes = [vp, C.c_int, C.POINTER(SimDesc), vp]
     L.ekg_simulate_batch_device.argtypes = [vp, C.POINTER(BatchDev)]
     L.ekg_ctx_synchronize.argtypes = [vp]
+    L.ekg_ctx_set_kernel.argtypes = [vp, C.c_int]
     L.ekg_lead_basis.argtypes = [vp, vp, vp]
     L.ekg_lead_basis.restype = None
     L.ekg_move_lead.argtypes = [vp, C.c_double, C.c_double, vp]
@@ -206,6 +208,10 @@ class Context:
 
     def synchronize(self):
         check(lib().ekg_ctx_synchronize(self.h))
+
+    def set_kernel(self, kind: str):
+        """'auto', 'valu' or 'mfma': lead-contraction path of the fused kernel."""
+        check(lib().ekg_ctx_set_kernel(self.h, KERNELS[kind]))
 
     def heart(self, layer, seed=None):
         return Heart(self, layer, seed)

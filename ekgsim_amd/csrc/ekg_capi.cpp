@@ -100,8 +100,23 @@ int ekg_ctx_create(int device, ekg_ctx** out)
             c->c.bind();
             ekg::hip_check(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking),
                            "hipStreamCreate");
-            int8_t tab[4][26][3] = {};
-            for (int t = 0; t < 4; ++t) ekg::nhood_offsets(t, tab[t]);
+            // per neighbourhood: {d0, d1, d2, index of d among the 26 cube offsets}
+            int8_t tab[4][26][4] = {};
+            int8_t cube[26][3];
+            ekg::nhood_offsets(EKG_NHOOD_3D8, cube);
+            for (int t = 0; t < 4; ++t) {
+                int8_t d[26][3];
+                const int n = ekg::nhood_offsets(t, d);
+                for (int q = 0; q < n; ++q) {
+                    int bit = 0;
+                    while (cube[bit][0] != d[q][0] || cube[bit][1] != d[q][1] || cube[bit][2] != d[q][2])
+                        ++bit;
+                    tab[t][q][0] = d[q][0];
+                    tab[t][q][1] = d[q][1];
+                    tab[t][q][2] = d[q][2];
+                    tab[t][q][3] = (int8_t)bit;
+                }
+            }
             c->c.nb_table.reserve(sizeof tab);
             ekg::hip_check(hipMemcpy(c->c.nb_table.p, tab, sizeof tab, hipMemcpyHostToDevice),
                            "upload neighbourhoods");
@@ -148,6 +163,16 @@ int ekg_ctx_kernel_time(ekg_ctx* ctx, const char* kernel, int64_t* launches, dou
         ctx->c.collect_timings();
         *launches = ctx->c.fused_launches;
         *total_ms = ctx->c.fused_ms;
+    });
+}
+
+int ekg_ctx_set_kernel(ekg_ctx* ctx, int kind)
+{
+    return guard([&] {
+        if (!ctx) ekg::fail(EKG_EINVAL, "ctx is NULL");
+        if (kind < EKG_KERNEL_AUTO || kind > EKG_KERNEL_MFMA) ekg::fail(EKG_EINVAL, "unknown kernel kind");
+        std::lock_guard<std::mutex> g(ctx->c.mu);
+        ctx->c.kernel = kind;
     });
 }
 

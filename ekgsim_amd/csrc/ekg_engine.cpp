@@ -100,7 +100,7 @@ int nhood_offsets(int type, int8_t dif[26][3])
 
 const void* ctx_nb_table(const Ctx& ctx, int nhood)
 {
-    return static_cast<const int8_t*>(ctx.nb_table.p) + nhood * 26 * 3;
+    return static_cast<const int8_t*>(ctx.nb_table.p) + nhood * 26 * 4;
 }
 
 int64_t num_steps(double length, double dt)
@@ -150,6 +150,13 @@ BatchPlan plan_shape(const Heart& h, BatchPlan p)
     return p;
 }
 
+// Matrix-core contraction unless the caller pinned the VALU kernel.
+bool use_mfma(const Ctx& ctx, int n_leads)
+{
+    (void)n_leads;
+    return ctx.kernel != EKG_KERNEL_VALU;
+}
+
 BatchPlan plan_batch(const Heart& h, int batch, const ekg_sim_desc* descs)
 {
     if (batch < 1 || !descs) fail(EKG_EINVAL, "batch must hold at least one descriptor");
@@ -179,6 +186,11 @@ BatchPlan plan_batch(const Heart& h, int batch, const ekg_sim_desc* descs)
     return plan_shape(h, p);
 }
 
+// Largest activation-time span inside one chunk (ms).  The saturated AP path
+// scales exponentials by e^{(k4+k5)(a - t_ref)} per voxel; a bounded span
+// keeps that factor far from FP32 overflow for any realistic k4, k5.
+constexpr double CHUNK_SPAN_MS = 16.0;
+
 void ensure_layout(Heart& h)
 {
     if (!h.excited) fail(EKG_ESTATE, "excitation sequence not computed or loaded");
@@ -197,7 +209,8 @@ void ensure_layout(Heart& h)
     L.max_layer = h.max_layer;
     L.h_chunks.clear();
     // Chunk size: enough waves for one simulation to fill the GPU
-    // (~400 chunks x time tiles), bounded so that per-wave setup stays small.
+    // (~400 chunks x time tiles), bounded so that per-wave setup stays small;
+    // chunks are layer-pure and span at most CHUNK_SPAN_MS of activation time.
     const int64_t target = std::min<int64_t>(4096, std::max<int64_t>(256, V / 400));
     int64_t v = 0;
     while (v < V) {
@@ -207,26 +220,54 @@ void ensure_layout(Heart& h)
         const int64_t n = e - v;
         const int64_t k = (n + target - 1) / target;
         for (int64_t c = 0; c < k; ++c) {
-            const int64_t b0 = v + n * c / k, b1 = v + n * (c + 1) / k;
-            L.h_chunks.push_back(int4{lay, (int)b0, (int)b1, 0});
+            int64_t b0 = v + n * c / k;
+            const int64_t b1 = v + n * (c + 1) / k;
+            while (b0 < b1) {
+                const float tref = (float)h.h_act[idx[b0]];
+                int64_t b2 = b0 + 1;
+                while (b2 < b1 && h.h_act[idx[b2]] - (double)tref <= CHUNK_SPAN_MS) ++b2;
+                int tbits;
+                std::memcpy(&tbits, &tref, 4);
+                L.h_chunks.push_back(int4{lay, (int)b0, (int)b2, tbits});
+                b0 = b2;
+            }
         }
         v = e;
     }
     L.n_chunks = (int)L.h_chunks.size();
-    std::vector<int32_t> lin(V ? V : 1);
+    // the 26 cube offsets in Neighbourhood::create order; bit q of the mask is
+    // set when the neighbour at offset q is inside the grid and myocardial
+    int8_t cube[26][3];
+    nhood_offsets(EKG_NHOOD_3D8, cube);
+    std::vector<int4> geo(V ? V : 1);
+    std::vector<double> a64(V ? V : 1);
     std::vector<float2> af(V ? V : 1);
     for (int64_t i = 0; i < V; ++i) {
-        lin[i] = (int32_t)idx[i];
-        const double a = h.h_act[idx[i]];
+        const int64_t x = idx[i];
+        const int i2 = (int)(x % h.n2);
+        const int64_t r = x / h.n2;
+        const int i1 = (int)(r % h.n1), i0 = (int)(r / h.n1);
+        int mask = 0;
+        for (int q = 0; q < 26; ++q) {
+            const int j0 = i0 + cube[q][0], j1 = i1 + cube[q][1], j2 = i2 + cube[q][2];
+            if (j0 < 0 || j0 >= h.n0 || j1 < 0 || j1 >= h.n1 || j2 < 0 || j2 >= h.n2) continue;
+            if (h.h_layer[((int64_t)j0 * h.n1 + j1) * h.n2 + j2]) mask |= 1 << q;
+        }
+        geo[i] = int4{i0, i1, i2, mask};
+        const double a = h.h_act[x];
+        a64[i] = a;
         const float hi = (float)a;
         af[i] = float2{hi, (float)(a - (double)hi)};
     }
     h.ctx->bind();
-    L.vox_lin.reserve(lin.size() * sizeof(int32_t));
+    L.vox_geo.reserve(geo.size() * sizeof(int4));
+    L.vox_a64.reserve(a64.size() * sizeof(double));
     L.vox_a.reserve(af.size() * sizeof(float2));
     L.chunks.reserve(std::max<size_t>(1, L.h_chunks.size()) * sizeof(int4));
-    hip_check(hipMemcpy(L.vox_lin.p, lin.data(), lin.size() * sizeof(int32_t), hipMemcpyHostToDevice),
-              "upload voxel list");
+    hip_check(hipMemcpy(L.vox_geo.p, geo.data(), geo.size() * sizeof(int4), hipMemcpyHostToDevice),
+              "upload voxel geometry");
+    hip_check(hipMemcpy(L.vox_a64.p, a64.data(), a64.size() * sizeof(double), hipMemcpyHostToDevice),
+              "upload activation times");
     hip_check(hipMemcpy(L.vox_a.p, af.data(), af.size() * sizeof(float2), hipMemcpyHostToDevice),
               "upload activation times");
     if (!L.h_chunks.empty())
@@ -244,9 +285,10 @@ size_t batch_device_bytes(const Heart& h, const BatchPlan& p)
     s += (size_t)p.B * p.n_layers * 9 * 8;
     s += (size_t)p.B * p.L * 3 * 8;
     s += (size_t)p.Tp * (8 + 8);
-    s += (size_t)p.B * p.n_layers * p.Tp * 4;
+    s += (size_t)p.B * p.n_layers * p.Tp * 8;
     s += (size_t)p.B * p.n_layers * sizeof(ekgk::LayerConst);
-    s += (size_t)p.B * V * p.LP * 8;
+    s += p.mfma ? (size_t)p.B * (V + ekgk::MFMA_PAD) * (16 * 8 + 16)
+                : (size_t)p.B * (V + 1) * (p.LP + 2) * 8;
     s += (size_t)p.B * nch * p.LP * 8;
     s += (size_t)p.B * nch * p.LP * p.Tp * 8;
     s += (size_t)p.B * p.L * p.T * 8;
@@ -304,44 +346,56 @@ void enqueue_batch(Ctx& ctx, Heart& h, const BatchPlan& p, const double* d_layer
         hip_check(hipMemcpyAsync(ws.thl.p, ws.h_thl.data(), p.Tp * 8, hipMemcpyHostToDevice, stream),
                   "upload time grid");
     }
-    ws.rtab.reserve((size_t)B * p.n_layers * p.Tp * 4);
+    ws.rtab.reserve((size_t)B * p.n_layers * p.Tp * 8);
     ws.lc.reserve((size_t)B * p.n_layers * sizeof(ekgk::LayerConst));
-    ws.G.reserve((size_t)B * std::max<int64_t>(V, 1) * p.LP * 8);
+    if (p.mfma) {
+        ws.G.reserve((size_t)B * (V + ekgk::MFMA_PAD) * 16 * 8);
+        ws.aux.reserve((size_t)B * (V + ekgk::MFMA_PAD) * sizeof(float4));
+    } else {
+        ws.G.reserve((size_t)B * (V + 1) * (p.LP + 2) * 8);
+    }
     ws.gsum.reserve((size_t)B * std::max(nch, 1) * p.LP * 8);
     ws.partial.reserve((size_t)B * std::max(nch, 1) * p.LP * p.Tp * 8);
 
     ekgk::LeadFieldArgs la;
-    la.grid_layer = h.d_layer.as<uint8_t>();
-    la.n0 = h.n0;
-    la.n1 = h.n1;
-    la.n2 = h.n2;
-    la.vox_lin = Lo.vox_lin.as<int32_t>();
+    la.vox_geo = Lo.vox_geo.as<int4>();
+    la.vox_a64 = Lo.vox_a64.as<double>();
+    la.vox_a = Lo.vox_a.as<float2>();
     la.chunks = Lo.chunks.as<int4>();
     la.n_chunks = nch;
-    la.nb_dif = static_cast<const int8_t*>(ctx_nb_table(ctx, p.nhood));
+    la.nb = static_cast<const int8_t*>(ctx_nb_table(ctx, p.nhood));
     int8_t dif[26][3];
     la.n_nb = nhood_offsets(p.nhood, dif);
     la.lead_pos = d_lead_pos;
     la.n_leads = p.L;
+    la.layer_k = d_layer_k;
+    la.n_layers = p.n_layers;
     la.V = V;
     la.LP = p.LP;
     la.n_req = B;
-    la.G = ws.G.as<double>();
+    la.rec = ws.G.as<double>();
+    la.aux = ws.aux.as<float4>();
+    la.mfma = p.mfma;
     la.gsum = ws.gsum.as<double>();
     hip_check(ekgk::launch_lead_field(la, stream), "lead_field launch");
 
     hip_check(ekgk::launch_ap_tables(d_layer_k, p.n_layers, ws.tgrid.as<double>(),
-                                     (int)p.Tp, B, ws.rtab.as<float>(),
+                                     (int)p.Tp, B, ws.rtab.as<double>(),
                                      ws.lc.as<ekgk::LayerConst>(), stream),
               "ap_tables launch");
 
     ekgk::FusedArgs fa;
     fa.vox_a = Lo.vox_a.as<float2>();
-    fa.G = ws.G.as<double>();
+    fa.rec = ws.G.as<double>();
+    fa.aux = ws.aux.as<float4>();
+    fa.mfma = p.mfma;
+    fa.n_leads = p.L;
     fa.gsum = ws.gsum.as<double>();
     fa.chunks = Lo.chunks.as<int4>();
     fa.lc = ws.lc.as<ekgk::LayerConst>();
-    fa.rtab = ws.rtab.as<float>();
+    fa.layer_k = d_layer_k;
+    fa.rtab = ws.rtab.as<double>();
+    fa.tgrid = ws.tgrid.as<double>();
     fa.thl = ws.thl.as<float2>();
     fa.partial = ws.partial.as<double>();
     fa.V = V;
@@ -374,8 +428,9 @@ void simulate(Heart& h, int batch, const ekg_sim_desc* descs, double* ecg)
     std::lock_guard<std::mutex> g1(ctx.mu);
     std::lock_guard<std::mutex> g2(h.mu);
     if (!ecg) fail(EKG_EINVAL, "ecg output buffer is required");
-    const BatchPlan p = plan_batch(h, batch, descs);
+    BatchPlan p = plan_batch(h, batch, descs);
     if (p.T == 0) return;
+    p.mfma = use_mfma(ctx, p.L);
     ctx.bind();
     ensure_layout(h);
     const size_t n_out = (size_t)p.B * p.L * p.T;
@@ -409,6 +464,7 @@ void simulate_device(Heart& h, const ekg_batch_dev& d)
     p.dt = d.dt;
     p = plan_shape(h, p);
     if (p.T == 0) return;
+    p.mfma = use_mfma(ctx, p.L);
     ctx.bind();
     ensure_layout(h);
     if (h.layout.V == 0) {

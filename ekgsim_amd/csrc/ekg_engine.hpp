@@ -38,7 +38,7 @@ struct DevBuf {
 
 // Device workspace of one batch of simulations on one stream.
 struct Workspace {
-    DevBuf layer_k, lead_pos, tgrid, thl, rtab, lc, G, gsum, partial, ecg;
+    DevBuf layer_k, lead_pos, tgrid, thl, rtab, lc, G, aux, gsum, partial, ecg;  // G: voxel records
     std::vector<double> h_tgrid, h_k, h_pos;
     std::vector<float> h_thl;
     double grid_dt = 0.0;
@@ -49,11 +49,12 @@ struct Ctx {
     hipStream_t stream = nullptr;
     Workspace ws;
     bool profiling = false;
+    int kernel = EKG_KERNEL_AUTO;  // contraction path of the fused kernel
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;  // fused launches not yet timed
     std::vector<hipEvent_t> free_events;
     int64_t fused_launches = 0;
     double fused_ms = 0.0;
-    DevBuf nb_table;  // [4][26][3] int8 neighbour offsets of the four dipole neighbourhoods
+    DevBuf nb_table;  // [4][26][4] int8 {d0, d1, d2, cube bit} of the four dipole neighbourhoods
     std::mutex mu;    // serialises ABI calls on this context
     void bind() const;
     hipEvent_t get_event();
@@ -68,7 +69,7 @@ struct EcgLayout {
     int n_chunks = 0;
     int max_layer = 0;
     std::vector<int4> h_chunks;
-    DevBuf vox_lin, vox_a, chunks;
+    DevBuf vox_geo, vox_a64, vox_a, chunks;
 };
 
 struct Heart {
@@ -87,6 +88,7 @@ struct Heart {
 // Host-side description of one batch, validated.
 struct BatchPlan {
     int B = 0, L = 0, LP = 0, n_layers = 0, nhood = 0;
+    bool mfma = true;
     int64_t T = 0, Tn = 0, Tp = 0;
     int n_tiles = 0, n_groups = 0;
     double start = 0, length = 0, dt = 0;
@@ -96,6 +98,7 @@ int nhood_offsets(int type, int8_t dif[26][3]);
 const void* ctx_nb_table(const Ctx& ctx, int nhood);
 int64_t num_steps(double length, double dt);
 BatchPlan plan_shape(const Heart& h, BatchPlan p);
+bool use_mfma(const Ctx& ctx, int n_leads);
 BatchPlan plan_batch(const Heart& h, int batch, const ekg_sim_desc* descs);
 void ensure_layout(Heart& h);
 // Copy the per-request coefficients and lead positions of host descriptors

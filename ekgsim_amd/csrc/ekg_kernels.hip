@@ -187,72 +187,122 @@ __global__ void ca_finalize_kernel(const T* __restrict__ dist, double* __restric
 // Lead field
 // ---------------------------------------------------------------------------
 
-// m = (p - (u+1)) * (1/(l*sqrt(l))), l = |p - (u+1)|^2  (simulator.cpp:531-533)
-__device__ __forceinline__ void lead_m(const double* p, int i0, int i1, int i2, double& m0,
-                                       double& m1, double& m2)
+// f(e) = e * (1/(l*sqrt(l))), l = |e|^2: the lead weight m of simulator.cpp:531-533
+// for e = p - (u+1).  1/(l*sqrt(l)) = rsq(l)^3 with two Newton steps on the
+// hardware FP64 reciprocal square root (quadratic convergence: a few ulp).
+__device__ __forceinline__ void lead_f(double e0, double e1, double e2, double& m0, double& m1,
+                                       double& m2)
 {
-    m0 = p[0] - (double)(i0 + 1);
-    m1 = p[1] - (double)(i1 + 1);
-    m2 = p[2] - (double)(i2 + 1);
-    double l = m0 * m0;
-    l += m1 * m1;
-    l += m2 * m2;
-    const double s = 1.0 / (l * sqrt(l));
-    m0 *= s;
-    m1 *= s;
-    m2 *= s;
+    double l = e0 * e0;
+    l = __builtin_fma(e1, e1, l);
+    l = __builtin_fma(e2, e2, l);
+    double r = __builtin_amdgcn_rsq(l);
+    const double h = 0.5 * l;
+    r = r * __builtin_fma(-h * r, r, 1.5);
+    r = r * __builtin_fma(-h * r, r, 1.5);
+    const double s = r * r * r;
+    m0 = e0 * s;
+    m1 = e1 * s;
+    m2 = e2 * s;
 }
 
-// One block per (chunk, request).  G_i(u) = sum_{d in N, u+d myocardial}
-// (m_i(u+d) + m_i(u)) . d for the chunk's voxels, zero-padded to LP leads,
-// and the chunk sum of G per lead (fixed-order block reduction).
-template <int LP>
+// One block per (chunk, request); one voxel per thread and all leads.
+// Writes the voxel record consumed by ecg_fused (RS = LP + 2 doubles):
+//   [0, LP)  G_i(u) = sum_{d in N, u+d myocardial} (m_i(u+d) + m_i(u)) . d
+//                   = sum_d m_i(u+d) . d + m_i(u) . s_u,  s_u = sum of those d,
+//            zero-padded to LP leads;
+//   LP       (P_u, Q_u) as two floats: the per-voxel AP factors of the
+//            saturated path, P_u = k2(1-k3) e^{(k4+k5)(a_u - t_ref)},
+//            Q_u = k2 k3 e^{k5 (a_u - t_ref)}, t_ref = the chunk's reference time;
+//   LP + 1   the activation time as (hi, lo) floats;
+// and the chunk sum of G per lead, reduced in a fixed order (shuffle tree
+// within waves, then wave order).  Request b's records start at
+// b * (V + 1) * RS; the extra record per request is never written and only
+// keeps the fused kernel's one-ahead load in bounds.
+//
+// kMfma = true writes the layout of ecg_fused_mfma instead: G rows of 16
+// doubles (zero-padded leads) at rec[(b*(V+MFMA_PAD) + v)*16], and the float4
+// {P, Q, a_hi, a_lo} at aux[b*(V+MFMA_PAD) + v]; the spare rows per request
+// keep the 4-voxel groups and their one-ahead loads in bounds.
+template <int LP, bool kMfma>
 __global__ __launch_bounds__(256) void lead_field_kernel(
-    const uint8_t* __restrict__ grid_layer, int n0, int n1, int n2,
-    const int32_t* __restrict__ vox_lin, const int4* __restrict__ chunks, int n_chunks,
-    const int8_t* __restrict__ nb_dif, int n_nb, const double* __restrict__ lead_pos, int n_leads,
-    int64_t V, double* __restrict__ G, double* __restrict__ gsum)
+    const int4* __restrict__ vox_geo, const double* __restrict__ vox_a64,
+    const float2* __restrict__ vox_a, const int4* __restrict__ chunks, int n_chunks,
+    const int8_t* __restrict__ nb, int n_nb, const double* __restrict__ lead_pos, int n_leads,
+    const double* __restrict__ layer_k, int n_layers, int64_t V, double* __restrict__ rec,
+    float4* __restrict__ aux, double* __restrict__ gsum)
 {
-    __shared__ double s_sum[256][LP];
+    constexpr int RS = kMfma ? 16 : LP + 2;
+    constexpr int NG = kMfma ? 16 : LP;
+    __shared__ double s_wave[4][LP];
     const int c = blockIdx.x;
     const int b = blockIdx.y;
     const int4 ch = chunks[c];
     const double* pos = lead_pos + (int64_t)b * n_leads * 3;
+    const double* k = layer_k + ((int64_t)b * n_layers + (ch.x - 1)) * 9;
+    const double alpha = k[2] * (1.0 - k[3]), beta = k[2] * k[3];
+    const double k45 = k[4] + k[5], k5 = k[5];
+    const double tref = (double)__int_as_float(ch.w);
     double part[LP];
 #pragma unroll
     for (int i = 0; i < LP; ++i) part[i] = 0.0;
     for (int v = ch.y + threadIdx.x; v < ch.z; v += blockDim.x) {
-        const int64_t x = vox_lin[v];
-        const int i2 = (int)(x % n2);
-        const int64_t r = x / n2;
-        const int i1 = (int)(r % n1), i0 = (int)(r / n1);
-        double* gv = G + ((int64_t)b * V + v) * LP;
+        const int4 g = vox_geo[v];
+        double* gv = rec + ((int64_t)b * (V + (kMfma ? MFMA_PAD : 1)) + v) * RS;
+        double sd0 = 0.0, sd1 = 0.0, sd2 = 0.0;
+        for (int q = 0; q < n_nb; ++q)
+            if ((g.w >> nb[4 * q + 3]) & 1) {
+                sd0 += nb[4 * q];
+                sd1 += nb[4 * q + 1];
+                sd2 += nb[4 * q + 2];
+            }
 #pragma unroll
         for (int i = 0; i < LP; ++i) {
             double gi = 0.0;
             if (i < n_leads) {
-                double mx0, mx1, mx2;
-                lead_m(pos + 3 * i, i0, i1, i2, mx0, mx1, mx2);
-                for (int k = 0; k < n_nb; ++k) {
-                    const int d0 = nb_dif[3 * k], d1 = nb_dif[3 * k + 1], d2 = nb_dif[3 * k + 2];
-                    const int j0 = i0 + d0, j1 = i1 + d1, j2 = i2 + d2;
-                    if (j0 < 0 || j0 >= n0 || j1 < 0 || j1 >= n1 || j2 < 0 || j2 >= n2) continue;
-                    if (!grid_layer[((int64_t)j0 * n1 + j1) * n2 + j2]) continue;
-                    double my0, my1, my2;
-                    lead_m(pos + 3 * i, j0, j1, j2, my0, my1, my2);
-                    gi += (my0 + mx0) * d0 + (my1 + mx1) * d1 + (my2 + mx2) * d2;
+                const double e0 = pos[3 * i] - (double)(g.x + 1);
+                const double e1 = pos[3 * i + 1] - (double)(g.y + 1);
+                const double e2 = pos[3 * i + 2] - (double)(g.z + 1);
+                double m0, m1, m2;
+                lead_f(e0, e1, e2, m0, m1, m2);
+                gi = m0 * sd0 + m1 * sd1 + m2 * sd2;
+                for (int q = 0; q < n_nb; ++q) {
+                    if (!((g.w >> nb[4 * q + 3]) & 1)) continue;
+                    const double d0 = nb[4 * q], d1 = nb[4 * q + 1], d2 = nb[4 * q + 2];
+                    double n0, n1, n2;
+                    lead_f(e0 - d0, e1 - d1, e2 - d2, n0, n1, n2);
+                    gi += n0 * d0 + n1 * d1 + n2 * d2;
                 }
             }
             gv[i] = gi;
             part[i] += gi;
         }
-    }
+        if constexpr (kMfma) {
 #pragma unroll
-    for (int i = 0; i < LP; ++i) s_sum[threadIdx.x][i] = part[i];
+            for (int i = LP; i < NG; ++i) gv[i] = 0.0;
+        }
+        const double da = vox_a64[v] - tref;
+        const float2 pq{(float)(alpha * exp(k45 * da)), (float)(beta * exp(k5 * da))};
+        if constexpr (kMfma) {
+            const float2 av = vox_a[v];
+            aux[(int64_t)b * (V + MFMA_PAD) + v] = float4{pq.x, pq.y, av.x, av.y};
+        } else {
+            float2* tail = reinterpret_cast<float2*>(gv + LP);
+            tail[0] = pq;
+            tail[1] = vox_a[v];
+        }
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < LP; ++i) {
+        double x = part[i];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+        if (lane == 0) s_wave[w][i] = x;
+    }
     __syncthreads();
     if (threadIdx.x < LP) {
         double s = 0.0;
-        for (int t = 0; t < (int)blockDim.x; ++t) s += s_sum[t][threadIdx.x];
+        for (int ww = 0; ww < (int)(blockDim.x >> 6); ++ww) s += s_wave[ww][threadIdx.x];
         gsum[((int64_t)b * n_chunks + c) * LP + threadIdx.x] = s;
     }
 }
@@ -263,13 +313,13 @@ __global__ __launch_bounds__(256) void lead_field_kernel(
 
 // Per (request b, layer l, padded step j): the repolarisation factor
 //   R(t_j) = 1 - (1 + exp(-k7 (t_j - k8) + log(2^(k7/k6) - 1)))^(-k6/k7)
-// of Wohlfart.h:200-201.  For a cell activated at a the reference evaluates
-// it at tau = t - a with k8 - a (simulator.cpp:154-158), and
+// of Wohlfart.h:200-201, in FP64.  For a cell activated at a the reference
+// evaluates it at tau = t - a with k8 - a (simulator.cpp:154-158), and
 // tau - (k8 - a) = t - k8, so the factor depends on the layer and t only.
 // Thread j == 0 also writes the per-layer FP32 constants.
 __global__ void ap_tables_kernel(const double* __restrict__ layer_k, int n_layers,
                                  const double* __restrict__ tgrid, int Tp, int n_req,
-                                 float* __restrict__ rtab, LayerConst* __restrict__ lc)
+                                 double* __restrict__ rtab, LayerConst* __restrict__ lc)
 {
     const int64_t n = (int64_t)n_req * n_layers * Tp;
     for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n;
@@ -278,9 +328,8 @@ __global__ void ap_tables_kernel(const double* __restrict__ layer_k, int n_layer
         const int64_t bl = idx / Tp;
         const double* k = layer_k + bl * 9;
         const double t = tgrid[j];
-        const double r =
+        rtab[idx] =
             1.0 - pow(1.0 + exp(-k[7] * (t - k[8]) + log(pow(2.0, k[7] / k[6]) - 1.0)), -(k[6] / k[7]));
-        rtab[idx] = (float)r;
         if (j == 0) {
             const double LOG2E = 1.4426950408889634074;
             LayerConst c;
@@ -304,10 +353,12 @@ __global__ void ap_tables_kernel(const double* __restrict__ layer_k, int n_layer
 //   x >= SKIP_X for every lane  -> exp2(x) = +inf, S = 1/(1+inf) = 0 exactly:
 //                                  the voxel adds exactly 0 to this tile;
 //   x <= SAT_X for every lane   -> exp2(x) < 2^-24, 1 + exp2(x) rounds to 1,
-//                                  S = rcp(1) = 1 exactly: the sigmoid is
-//                                  skipped with bit-identical FP32 results.
+//                                  S = rcp(1) = 1 exactly: no sigmoid.
 constexpr float SKIP_X = 130.0f;
 constexpr float SAT_X = -24.5f;
+// Largest lane factor kept; with S = 0 exactly in such lanes the product
+// stays 0 instead of 0 * inf.
+constexpr double XMAX = 1e30;
 
 __device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -332,21 +383,28 @@ __device__ __forceinline__ int first_false(int lo, int hi, P pred)
 }
 
 // Block = 4 waves; wave w of block (g, c, b) takes time tile 4g + w of
-// chunk c of request b.  Lane = time step.  Voxels of a chunk are one layer,
-// sorted by activation time, so for a tile they split into
-//   [v0, s1)  sigmoid saturated in every lane (no sigmoid evaluation),
+// chunk c of request b.  Lane = time step.  The cell AP of voxel u at t is
+//   V = S(tau) * (P_u X(t) + Q_u Y(t)) + k0,  tau = t - a_u,
+//   X(t) = e^{-(k4+k5)(t - t_ref)} R(t),  Y(t) = e^{-k5 (t - t_ref)} R(t),
+// which is Wohlfart.h:197-202 with the exponentials split into a per-voxel
+// factor (P, Q from lead_field) and a per-lane factor computed once per wave
+// in FP64.  Voxels of a chunk are one layer, sorted by activation time, so
+// for a tile they split into
+//   [v0, s1)  sigmoid saturated in every lane: V = P X + Q Y (2 VALU ops),
 //   [s1, s2)  general case,
 //   [s2, v1)  exactly zero contribution (skipped),
 // found by binary search on the tile's first and last lane (tau is monotone
 // in t and in a).  When c1 >= 0 (no decaying sigmoid) the whole chunk takes
-// the general path.
+// the general path.  Lead sums are FP64 FMAs with the G row in SGPRs.
 template <int LP>
 __global__ __launch_bounds__(256) void ecg_fused_kernel(
-    const float2* __restrict__ vox_a, const double* __restrict__ G,
-    const double* __restrict__ gsum, const int4* __restrict__ chunks,
-    const LayerConst* __restrict__ lc, const float* __restrict__ rtab,
-    const float2* __restrict__ thl, double* __restrict__ partial, int64_t V, int n_layers,
-    int n_chunks, int Tp, int n_tiles, int n_groups)
+    const float2* __restrict__ vox_a, const double* __restrict__ rec,
+    const double* __restrict__ gsum,
+    const int4* __restrict__ chunks, const LayerConst* __restrict__ lc,
+    const double* __restrict__ layer_k, const double* __restrict__ rtab,
+    const double* __restrict__ tgrid, const float2* __restrict__ thl,
+    double* __restrict__ partial, int64_t V, int n_layers, int n_chunks, int Tp, int n_tiles,
+    int n_groups)
 {
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -357,63 +415,218 @@ __global__ __launch_bounds__(256) void ecg_fused_kernel(
 
     const int4 ch = chunks[c];
     const int l = ch.x;  // layer, 1-based
-    const LayerConst k = lc[(int64_t)b * n_layers + (l - 1)];
+    const int64_t bl = (int64_t)b * n_layers + (l - 1);
+    const LayerConst kc = lc[bl];
     const int j = tile * 64 + lane;
     const float2 tt = thl[j];
-    const float R = rtab[((int64_t)b * n_layers + (l - 1)) * Tp + j];
     const float2 tf = thl[tile * 64];
     const float2 tl = thl[tile * 64 + 63];
-    const double* Gb = G + (int64_t)b * V * LP;
+    // per-lane time factors, FP64 then rounded once
+    float X, Y;
+    {
+        const double* k = layer_k + bl * 9;
+        const double R = rtab[bl * Tp + j];
+        const double dtr = tgrid[j] - (double)__int_as_float(ch.w);
+        X = (float)fmin(exp(-(k[4] + k[5]) * dtr) * R, XMAX);
+        Y = (float)fmin(exp(-k[5] * dtr) * R, XMAX);
+    }
+    constexpr int RS = LP + 2;
+    const double* Rb = rec + (int64_t)b * (V + 1) * RS;
 
     double acc[LP];
 #pragma unroll
     for (int i = 0; i < LP; ++i) acc[i] = 0.0;
 
     int s1 = ch.y, s2 = ch.z;
-    if (k.c1 < 0.0f) {
+    if (kc.c1 < 0.0f) {
         // saturated prefix: max over lanes of c1*tau <= SAT_X (max at the first lane)
         s1 = first_false(ch.y, ch.z, [&](int v) {
             const float2 a = vox_a[v];
-            return k.c1 * tau_of(tf.x, tf.y, a.x, a.y) <= SAT_X;
+            return kc.c1 * tau_of(tf.x, tf.y, a.x, a.y) <= SAT_X;
         });
         // non-zero range ends where min over lanes of c1*tau >= SKIP_X (min at the last lane)
         s2 = first_false(s1, ch.z, [&](int v) {
             const float2 a = vox_a[v];
-            return k.c1 * tau_of(tl.x, tl.y, a.x, a.y) < SKIP_X;
+            return kc.c1 * tau_of(tl.x, tl.y, a.x, a.y) < SKIP_X;
         });
-    } else {
-        s1 = ch.y;
     }
 
-    for (int v = ch.y; v < s1; ++v) {
-        const float2 a = vox_a[v];
-        const double* g = Gb + (int64_t)v * LP;
-        const float tau = tau_of(tt.x, tt.y, a.x, a.y);
-        const float e4 = fast_exp2(fminf(k.c4 * tau, 126.0f));
-        const float e5 = fast_exp2(fminf(k.c5 * tau, 126.0f));
-        const float ap = __builtin_fmaf(k.alpha, e4, k.beta) * e5 * R;
-        const double apd = (double)ap;
+    // Both loops keep the next voxel's record in flight while the current
+    // one is consumed: one group of wave-uniform scalar loads per voxel,
+    // issued before the FP64 FMAs of the previous voxel (the one-ahead load
+    // of the last voxel reads the next record, which always exists).
+    if (ch.y < s1) {
+        const double* rp = Rb + (int64_t)ch.y * RS;
+        double rn[RS];
 #pragma unroll
-        for (int i = 0; i < LP; ++i) acc[i] = __builtin_fma(g[i], apd, acc[i]);
+        for (int i = 0; i < RS; ++i) rn[i] = rp[i];
+        for (int v = ch.y; v < s1; ++v) {
+            double r[RS];
+#pragma unroll
+            for (int i = 0; i < RS; ++i) r[i] = rn[i];
+            rp += RS;
+#pragma unroll
+            for (int i = 0; i < RS; ++i) rn[i] = rp[i];
+            const float2 pq = __builtin_bit_cast(float2, r[LP]);
+            const float ap = __builtin_fmaf(pq.x, X, pq.y * Y);
+            const double apd = (double)ap;
+#pragma unroll
+            for (int i = 0; i < LP; ++i) acc[i] = __builtin_fma(r[i], apd, acc[i]);
+        }
     }
-    for (int v = s1; v < s2; ++v) {
-        const float2 a = vox_a[v];
-        const double* g = Gb + (int64_t)v * LP;
-        const float tau = tau_of(tt.x, tt.y, a.x, a.y);
-        const float e4 = fast_exp2(fminf(k.c4 * tau, 126.0f));
-        const float e5 = fast_exp2(fminf(k.c5 * tau, 126.0f));
-        const float e1 = fast_exp2(k.c1 * tau);
-        const float ap = __builtin_fmaf(k.alpha, e4, k.beta) * e5 * R * fast_rcp(1.0f + e1);
-        const double apd = (double)ap;
+    if (s1 < s2) {
+        const double* rp = Rb + (int64_t)s1 * RS;
+        double rn[RS];
 #pragma unroll
-        for (int i = 0; i < LP; ++i) acc[i] = __builtin_fma(g[i], apd, acc[i]);
+        for (int i = 0; i < RS; ++i) rn[i] = rp[i];
+        for (int v = s1; v < s2; ++v) {
+            double r[RS];
+#pragma unroll
+            for (int i = 0; i < RS; ++i) r[i] = rn[i];
+            rp += RS;
+#pragma unroll
+            for (int i = 0; i < RS; ++i) rn[i] = rp[i];
+            const float2 pq = __builtin_bit_cast(float2, r[LP]);
+            const float2 a = __builtin_bit_cast(float2, r[LP + 1]);
+            const float tau = tau_of(tt.x, tt.y, a.x, a.y);
+            const float e1 = fast_exp2(kc.c1 * tau);
+            const float ap = __builtin_fmaf(pq.x, X, pq.y * Y) * fast_rcp(1.0f + e1);
+            const double apd = (double)ap;
+#pragma unroll
+            for (int i = 0; i < LP; ++i) acc[i] = __builtin_fma(r[i], apd, acc[i]);
+        }
     }
 
     // k0 is constant in time: sum_u G_u (ap_u + k0) = sum_u G_u ap_u + k0 * sum_u G_u.
     const double* gs = gsum + ((int64_t)b * n_chunks + c) * LP;
     double* out = partial + (((int64_t)b * n_chunks + c) * LP) * Tp + j;
 #pragma unroll
-    for (int i = 0; i < LP; ++i) out[(int64_t)i * Tp] = __builtin_fma(k.k0, gs[i], acc[i]);
+    for (int i = 0; i < LP; ++i) out[(int64_t)i * Tp] = __builtin_fma(kc.k0, gs[i], acc[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Fused AP evaluation + lead contraction on the matrix cores
+// ---------------------------------------------------------------------------
+
+typedef double mfma_f64x4 __attribute__((ext_vector_type(4)));
+
+// Same computation as ecg_fused_kernel with the lead contraction on
+// v_mfma_f64_16x16x4_f64: D[lead i][time j] += sum_k G[voxel k][lead i] * V[voxel k][time j].
+// gfx950 lane layout (checked by a one-hot probe): A(i,k) in lane 16k+i,
+// B(k,j) in lane 16k+j, D(i,j) in lane 16(i%4)+j, register i/4.
+// Block = up to 16 waves of one (chunk, request); a wave takes 64-step time
+// tiles w, w+nw, ... and, per tile, splits the a-sorted chunk like the VALU
+// kernel.  Per group of 4 voxels: one coalesced 512-byte load of the four G
+// rows (A), one float4 {P, Q, a} per lane (voxel k = lane/16), 4 AP values per
+// lane (time blocks m = 0..3, j = lane%16) and 4 MFMAs into 4 FP64
+// accumulator tiles.  Groups that reach past the chunk end are masked with
+// selects; the 4 spare rows per request keep their loads in bounds.
+template <int LP>
+__global__ __launch_bounds__(1024) void ecg_fused_mfma_kernel(
+    const float2* __restrict__ vox_a, const double* __restrict__ G16,
+    const float4* __restrict__ aux, const double* __restrict__ gsum,
+    const int4* __restrict__ chunks, const LayerConst* __restrict__ lc,
+    const double* __restrict__ layer_k, const double* __restrict__ rtab,
+    const double* __restrict__ tgrid, const float2* __restrict__ thl,
+    double* __restrict__ partial, int64_t V, int n_layers, int n_chunks, int Tp, int n_tiles,
+    int n_leads)
+{
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nw = blockDim.x >> 6;
+    const int c = blockIdx.x;
+    const int b = blockIdx.y;
+    const int4 ch = chunks[c];
+    const int l = ch.x;
+    const int64_t bl = (int64_t)b * n_layers + (l - 1);
+    const LayerConst kc = lc[bl];
+    const double* kk = layer_k + bl * 9;
+    const double k45 = kk[4] + kk[5], k5 = kk[5];
+    const double tref = (double)__int_as_float(ch.w);
+    const int jl = lane & 15;   // time column
+    const int kv = lane >> 4;   // voxel slot of the B operand / lead group of D
+    const double* Gb = G16 + (int64_t)b * (V + MFMA_PAD) * 16;
+    const float4* Ab = aux + (int64_t)b * (V + MFMA_PAD);
+
+    for (int tile = w; tile < n_tiles; tile += nw) {
+        float X[4], Y[4], TH[4], TL[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int j = tile * 64 + 16 * m + jl;
+            const double R = rtab[bl * Tp + j];
+            const double dtr = tgrid[j] - tref;
+            X[m] = (float)fmin(exp(-k45 * dtr) * R, XMAX);
+            Y[m] = (float)fmin(exp(-k5 * dtr) * R, XMAX);
+            const float2 t2 = thl[j];
+            TH[m] = t2.x;
+            TL[m] = t2.y;
+        }
+        const float2 tf = thl[tile * 64];
+        const float2 tl = thl[tile * 64 + 63];
+        int s1 = ch.y, s2 = ch.z;
+        if (kc.c1 < 0.0f) {
+            s1 = first_false(ch.y, ch.z, [&](int v) {
+                const float2 a = vox_a[v];
+                return kc.c1 * tau_of(tf.x, tf.y, a.x, a.y) <= SAT_X;
+            });
+            s2 = first_false(s1, ch.z, [&](int v) {
+                const float2 a = vox_a[v];
+                return kc.c1 * tau_of(tl.x, tl.y, a.x, a.y) < SKIP_X;
+            });
+        }
+        mfma_f64x4 acc[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) acc[m] = mfma_f64x4{0.0, 0.0, 0.0, 0.0};
+
+        // Groups of 4 voxels; the next group's A rows and {P, Q, a} are loaded
+        // while the current group's MFMAs run (vector loads complete in order).
+        // Every group starts below ch.z <= V, so the look-ahead group starts at
+        // most at V + 3 and its rows stay below V + 7 < V + MFMA_PAD.
+        int vg = ch.y;
+        double a_n = Gb[(int64_t)vg * 16 + lane];
+        float4 q_n = Ab[vg + kv];
+        // groups of 4 voxels that are saturated in every lane of the tile
+        for (; vg + 4 <= s1; vg += 4) {
+            const double a_op = a_n;
+            const float4 q = q_n;
+            a_n = Gb[(int64_t)(vg + 4) * 16 + lane];
+            q_n = Ab[vg + 4 + kv];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const double bv = (double)__builtin_fmaf(q.x, X[m], q.y * Y[m]);
+                acc[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_op, bv, acc[m], 0, 0, 0);
+            }
+        }
+        // remaining groups up to the last non-zero voxel: general path, masked at the chunk end
+        for (; vg < s2; vg += 4) {
+            const bool ok = vg + kv < ch.z;
+            const double a_op = ok ? a_n : 0.0;
+            const float4 q = q_n;
+            a_n = Gb[(int64_t)(vg + 4) * 16 + lane];
+            q_n = Ab[vg + 4 + kv];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const float tau = tau_of(TH[m], TL[m], q.z, q.w);
+                const float e1 = fast_exp2(kc.c1 * tau);
+                const float ap = __builtin_fmaf(q.x, X[m], q.y * Y[m]) * fast_rcp(1.0f + e1);
+                const double bv = ok ? (double)ap : 0.0;
+                acc[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_op, bv, acc[m], 0, 0, 0);
+            }
+        }
+        // D(i, j) sits in lane 16(i%4)+j, register i/4: lead i = 4r + lane/16
+        const double* gs = gsum + ((int64_t)b * n_chunks + c) * LP;
+        double* out = partial + ((int64_t)b * n_chunks + c) * LP * Tp;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = 4 * r + kv;
+            if (i < n_leads && i < LP) {
+                const double g = gs[i];
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+                    out[(int64_t)i * Tp + tile * 64 + 16 * m + jl] = __builtin_fma(kc.k0, g, acc[m][r]);
+            }
+        }
+    }
 }
 
 // ecg[b][i][t] = sum over chunks c (in order) of partial[b][c][i][t] for
@@ -494,9 +707,14 @@ hipError_t launch_ca_finalize(bool is_int, const void* dist, double* act, int64_
 template <int LP>
 static void lf_launch(const LeadFieldArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL(lead_field_kernel<LP>, dim3(a.n_chunks, a.n_req), dim3(256), 0, s,
-                       a.grid_layer, a.n0, a.n1, a.n2, a.vox_lin, a.chunks, a.n_chunks, a.nb_dif,
-                       a.n_nb, a.lead_pos, a.n_leads, a.V, a.G, a.gsum);
+    if (a.mfma)
+        hipLaunchKernelGGL((lead_field_kernel<LP, true>), dim3(a.n_chunks, a.n_req), dim3(256), 0,
+                           s, a.vox_geo, a.vox_a64, a.vox_a, a.chunks, a.n_chunks, a.nb, a.n_nb,
+                           a.lead_pos, a.n_leads, a.layer_k, a.n_layers, a.V, a.rec, a.aux, a.gsum);
+    else
+        hipLaunchKernelGGL((lead_field_kernel<LP, false>), dim3(a.n_chunks, a.n_req), dim3(256), 0,
+                           s, a.vox_geo, a.vox_a64, a.vox_a, a.chunks, a.n_chunks, a.nb, a.n_nb,
+                           a.lead_pos, a.n_leads, a.layer_k, a.n_layers, a.V, a.rec, a.aux, a.gsum);
 }
 
 hipError_t launch_lead_field(const LeadFieldArgs& a, hipStream_t s)
@@ -512,7 +730,7 @@ hipError_t launch_lead_field(const LeadFieldArgs& a, hipStream_t s)
 }
 
 hipError_t launch_ap_tables(const double* layer_k, int n_layers, const double* tgrid, int Tp,
-                            int n_req, float* rtab, LayerConst* lc, hipStream_t s)
+                            int n_req, double* rtab, LayerConst* lc, hipStream_t s)
 {
     const int64_t n = (int64_t)n_req * n_layers * Tp;
     hipLaunchKernelGGL(ap_tables_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, layer_k,
@@ -524,12 +742,31 @@ template <int LP>
 static void fused_launch(const FusedArgs& a, hipStream_t s)
 {
     hipLaunchKernelGGL(ecg_fused_kernel<LP>, dim3(a.n_chunks * a.n_groups, a.n_req), dim3(256), 0,
-                       s, a.vox_a, a.G, a.gsum, a.chunks, a.lc, a.rtab, a.thl, a.partial, a.V,
-                       a.n_layers, a.n_chunks, a.Tp, a.n_tiles, a.n_groups);
+                       s, a.vox_a, a.rec, a.gsum, a.chunks, a.lc, a.layer_k, a.rtab, a.tgrid,
+                       a.thl, a.partial, a.V, a.n_layers, a.n_chunks, a.Tp, a.n_tiles, a.n_groups);
+}
+
+template <int LP>
+static void fused_mfma_launch(const FusedArgs& a, hipStream_t s)
+{
+    const int nw = a.n_tiles < 16 ? a.n_tiles : 16;
+    hipLaunchKernelGGL(ecg_fused_mfma_kernel<LP>, dim3(a.n_chunks, a.n_req), dim3(64 * nw), 0, s,
+                       a.vox_a, a.rec, a.aux, a.gsum, a.chunks, a.lc, a.layer_k, a.rtab, a.tgrid,
+                       a.thl, a.partial, a.V, a.n_layers, a.n_chunks, a.Tp, a.n_tiles, a.n_leads);
 }
 
 hipError_t launch_ecg_fused(const FusedArgs& a, hipStream_t s)
 {
+    if (a.mfma) {
+        switch (a.LP) {
+        case 4: fused_mfma_launch<4>(a, s); break;
+        case 8: fused_mfma_launch<8>(a, s); break;
+        case 12: fused_mfma_launch<12>(a, s); break;
+        case 16: fused_mfma_launch<16>(a, s); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (a.LP) {
     case 4: fused_launch<4>(a, s); break;
     case 8: fused_launch<8>(a, s); break;

@@ -7,6 +7,10 @@
 
 namespace ekgk {
 
+// Spare voxel rows per request in the matrix-core layout (groups of 4 voxels
+// plus a one-group look-ahead past the last chunk).
+constexpr int MFMA_PAD = 8;
+
 template <bool kInt> struct CaT { using type = double; };
 template <> struct CaT<true> { using type = int; };
 
@@ -20,33 +24,42 @@ struct LayerConst {
 static_assert(sizeof(LayerConst) == 32, "LayerConst layout");
 
 struct LeadFieldArgs {
-    const uint8_t* grid_layer;   // [n0*n1*n2]
-    int n0, n1, n2;
-    const int32_t* vox_lin;      // [V] linear grid index of each sorted myocardial voxel
-    const int4* chunks;          // [n_chunks] {layer, v_begin, v_end, 0}
+    const int4* vox_geo;         // [V] {i0, i1, i2, 26-bit myocardial-neighbour mask}
+    const double* vox_a64;       // [V] activation time
+    const float2* vox_a;         // [V] activation time (hi, lo)
+    const int4* chunks;          // [n_chunks] {layer, v_begin, v_end, t_ref as float bits}
     int n_chunks;
-    const int8_t* nb_dif;        // [n_nb][3]
+    const int8_t* nb;            // [n_nb][4] {d0, d1, d2, cube bit}
     int n_nb;
     const double* lead_pos;      // [n_req][n_leads][3]
     int n_leads;
+    const double* layer_k;       // [n_req][n_layers][9]
+    int n_layers;
     int64_t V;
     int LP;                      // padded lead count (4, 8, 12, 16)
     int n_req;
-    double* G;                   // [n_req][V][LP]
+    double* rec;                 // VALU: [n_req][V + 1][LP + 2] voxel records (G, P/Q, a)
+                                 // MFMA: [n_req][V + MFMA_PAD][16] G rows
+    float4* aux;                 // MFMA: [n_req][V + MFMA_PAD] {P, Q, a_hi, a_lo}
     double* gsum;                // [n_req][n_chunks][LP]
+    bool mfma;
 };
 
 struct FusedArgs {
     const float2* vox_a;         // [V] activation time (hi, lo)
-    const double* G;             // [n_req][V][LP]
+    const double* rec;           // see LeadFieldArgs::rec
+    const float4* aux;           // MFMA only
     const double* gsum;          // [n_req][n_chunks][LP]
     const int4* chunks;          // [n_chunks]
     const LayerConst* lc;        // [n_req][n_layers]
-    const float* rtab;           // [n_req][n_layers][Tp]
+    const double* layer_k;       // [n_req][n_layers][9]
+    const double* rtab;          // [n_req][n_layers][Tp]
+    const double* tgrid;         // [Tp] time
     const float2* thl;           // [Tp] time (hi, lo)
     double* partial;             // [n_req][n_chunks][LP][Tp]
     int64_t V;
-    int n_layers, n_chunks, Tp, n_tiles, n_groups, LP, n_req;
+    int n_layers, n_chunks, Tp, n_tiles, n_groups, LP, n_req, n_leads;
+    bool mfma;
 };
 
 hipError_t launch_ca_init(bool is_int, const uint8_t* layer, const uint8_t* seed, void* dist,
@@ -58,7 +71,7 @@ hipError_t launch_ca_sweep(bool is_int, const uint8_t* layer, const uint8_t* del
 hipError_t launch_ca_finalize(bool is_int, const void* dist, double* act, int64_t n, hipStream_t s);
 hipError_t launch_lead_field(const LeadFieldArgs& a, hipStream_t s);
 hipError_t launch_ap_tables(const double* layer_k, int n_layers, const double* tgrid, int Tp,
-                            int n_req, float* rtab, LayerConst* lc, hipStream_t s);
+                            int n_req, double* rtab, LayerConst* lc, hipStream_t s);
 hipError_t launch_ecg_fused(const FusedArgs& a, hipStream_t s);
 hipError_t launch_ecg_reduce(const double* partial, double* ecg, int n_req, int n_chunks, int LP,
                              int n_leads, int T, int Tn, int Tp, hipStream_t s);

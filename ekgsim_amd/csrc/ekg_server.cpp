@@ -168,12 +168,14 @@ void Server::run()
                     std::vector<ekg_sim_desc> ds;
                     for (Request* r : group) ds.push_back(r->desc);
                     plan = plan_batch(*h, (int)ds.size(), ds.data());
+                    plan.mfma = use_mfma(ctx, plan.L);
                     bytes = batch_device_bytes(*h, plan);
                     // shrink the group until it fits the budget
                     while (group.size() > 1 && bytes_in_use + bytes > opts.hbm_budget_bytes) {
                         group.pop_back();
                         ds.pop_back();
                         plan = plan_batch(*h, (int)ds.size(), ds.data());
+                        plan.mfma = use_mfma(ctx, plan.L);
                         bytes = batch_device_bytes(*h, plan);
                     }
                 } catch (const Error& e) {

@@ -69,6 +69,15 @@ int ekg_ctx_set_profiling(ekg_ctx *ctx, int enable);
 int ekg_ctx_kernel_time(ekg_ctx *ctx, const char *kernel, int64_t *launches, double *total_ms);
 int ekg_ctx_reset_profiling(ekg_ctx *ctx);
 
+/* Lead-contraction path of the fused AP+lead kernel: the matrix cores
+ * (v_mfma_f64_16x16x4_f64, FP64 accumulate) or the vector ALUs (FP64 FMA
+ * with the lead weights in scalar registers).  Both give the same ECGs to
+ * FP64 rounding; AUTO picks the matrix cores. */
+#define EKG_KERNEL_AUTO 0
+#define EKG_KERNEL_VALU 1
+#define EKG_KERNEL_MFMA 2
+int ekg_ctx_set_kernel(ekg_ctx *ctx, int kind);
+
 /* ---- heart (device-resident geometry) ----------------------------------- */
 /* Replaces EkgSim::loadShape (sim_lib.h:211-216) -> Simulation::loadShape
  * (simulator.cpp:181-199).  layer[i] = 0 empty, 1..255 myocardial layer;

@@ -16,10 +16,13 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "slow: takes more than a few seconds on the CPU")
 
 
-@pytest.fixture(scope="session")
-def ctx():
+@pytest.fixture(scope="session", params=["mfma", "valu"])
+def ctx(request):
+    """A GPU context per lead-contraction path: every GPU test runs on both
+    the matrix-core and the vector-ALU kernel."""
     from ekgsim_amd import Context
     c = Context(0)
+    c.set_kernel(request.param)
     yield c
     c.close()
 
