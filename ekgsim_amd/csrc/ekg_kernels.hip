@@ -370,14 +370,27 @@ __device__ __forceinline__ float tau_of(float th, float tl, float ah, float al)
 }
 
 // First voxel index in [lo, hi) whose predicate is false, for a predicate
-// that is true on a prefix of the a-sorted range.
+// that is true on a prefix of the a-sorted range.  Wave-cooperative 64-ary
+// search: each lane tests one probe, the ballot's count of true probes
+// narrows the range to one stride; ranges up to 4096 take two rounds.  The
+// result is wave-uniform.
 template <class P>
 __device__ __forceinline__ int first_false(int lo, int hi, P pred)
 {
-    while (lo < hi) {
-        const int mid = lo + ((hi - lo) >> 1);
-        if (pred(mid)) lo = mid + 1;
-        else hi = mid;
+    const int lane = threadIdx.x & 63;
+    while (hi - lo > 0) {
+        const int n = hi - lo;
+        const int stride = (n + 63) >> 6;
+        const int v = lo + lane * stride;
+        const bool t = v < hi && pred(v);
+        const int c = __popcll(__ballot(t));  // probes lo, lo+stride, ... true for the first c
+        if (stride == 1) return lo + c;
+        // the first false lies in (lo + (c-1)*stride, lo + c*stride]
+        const int nlo = c == 0 ? lo : lo + (c - 1) * stride + 1;
+        const int nhi = lo + c * stride < hi ? lo + c * stride + 1 : hi;
+        if (c == 0) return lo;
+        lo = nlo;
+        hi = nhi;
     }
     return lo;
 }

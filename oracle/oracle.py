@@ -30,9 +30,9 @@ _i64 = C.c_int64
 def build(ref: bool = False) -> None:
     """Compile liboracle.so (and the reference harness when ref=True and the
     reference sources exist)."""
-    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+    subprocess.run(["make", "-s", "-B", "-C", HERE, "liboracle.so"], check=True)
     if ref and os.path.isdir(os.environ.get("REF", "/root/reference")):
-        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+        subprocess.run(["make", "-s", "-B", "-C", HERE, "ref"], check=True)
 
 
 _lib = None
