@@ -235,37 +235,96 @@ void ensure_layout(Heart& h)
         v = e;
     }
     L.n_chunks = (int)L.h_chunks.size();
-    // the 26 cube offsets in Neighbourhood::create order; bit q of the mask is
-    // set when the neighbour at offset q is inside the grid and myocardial
-    int8_t cube[26][3];
-    nhood_offsets(EKG_NHOOD_3D8, cube);
-    std::vector<int4> geo(V ? V : 1);
+    // per sorted voxel: chunk index and activation time (FP64 and hi/lo FP32)
+    std::vector<int> vchunk(V ? V : 1);
+    for (int c = 0; c < L.n_chunks; ++c)
+        for (int v2 = L.h_chunks[c].y; v2 < L.h_chunks[c].z; ++v2) vchunk[v2] = c;
     std::vector<double> a64(V ? V : 1);
     std::vector<float2> af(V ? V : 1);
+    for (int64_t i = 0; i < V; ++i) {
+        const double a = h.h_act[idx[i]];
+        a64[i] = a;
+        const float hi = (float)a;
+        af[i] = float2{hi, (float)(a - (double)hi)};
+    }
+    // 8^3 bricks of the grid that hold myocardium, with their voxels
+    // {sorted index, local cell} grouped by brick (counting sort)
+    const int nb0 = (h.n0 + 7) / 8, nb1 = (h.n1 + 7) / 8, nb2 = (h.n2 + 7) / 8;
+    std::vector<int> cnt((size_t)nb0 * nb1 * nb2 + 1, 0);
+    std::vector<int> vbrick(V ? V : 1);
     for (int64_t i = 0; i < V; ++i) {
         const int64_t x = idx[i];
         const int i2 = (int)(x % h.n2);
         const int64_t r = x / h.n2;
         const int i1 = (int)(r % h.n1), i0 = (int)(r / h.n1);
-        int mask = 0;
-        for (int q = 0; q < 26; ++q) {
-            const int j0 = i0 + cube[q][0], j1 = i1 + cube[q][1], j2 = i2 + cube[q][2];
-            if (j0 < 0 || j0 >= h.n0 || j1 < 0 || j1 >= h.n1 || j2 < 0 || j2 >= h.n2) continue;
-            if (h.h_layer[((int64_t)j0 * h.n1 + j1) * h.n2 + j2]) mask |= 1 << q;
-        }
-        geo[i] = int4{i0, i1, i2, mask};
-        const double a = h.h_act[x];
-        a64[i] = a;
-        const float hi = (float)a;
-        af[i] = float2{hi, (float)(a - (double)hi)};
+        vbrick[i] = ((i0 / 8) * nb1 + i1 / 8) * nb2 + i2 / 8;
+        ++cnt[vbrick[i]];
     }
+    std::vector<int4> bricks;
+    std::vector<int> boff;
+    std::vector<int> bslot(cnt.size(), -1);
+    for (int bq = 0; bq < nb0 * nb1 * nb2; ++bq)
+        if (cnt[bq]) {
+            bslot[bq] = (int)bricks.size();
+            bricks.push_back(int4{(bq / (nb1 * nb2)) * 8, ((bq / nb2) % nb1) * 8, (bq % nb2) * 8, 0});
+        }
+    boff.assign(bricks.size() + 1, 0);
+    for (int bq = 0; bq < nb0 * nb1 * nb2; ++bq)
+        if (cnt[bq]) boff[bslot[bq] + 1] = cnt[bq];
+    for (size_t q = 1; q < boff.size(); ++q) boff[q] += boff[q - 1];
+    std::vector<int2> bvox(V ? V : 1);
+    std::vector<int> fill(boff.begin(), boff.end() - 1);
+    for (int64_t i = 0; i < V; ++i) {
+        const int64_t x = idx[i];
+        const int i2 = (int)(x % h.n2);
+        const int64_t r = x / h.n2;
+        const int i1 = (int)(r % h.n1), i0 = (int)(r / h.n1);
+        const int cell = ((i0 % 8) * 8 + (i1 % 8)) * 8 + (i2 % 8);
+        bvox[fill[bslot[vbrick[i]]]++] = int2{(int)i, cell};
+    }
+    // cell order inside each brick: a wave's stencil gathers then touch
+    // neighbouring LDS words (few bank conflicts)
+    for (size_t q = 0; q + 1 < boff.size(); ++q)
+        std::sort(bvox.begin() + boff[q], bvox.begin() + boff[q + 1],
+                  [](const int2& x, const int2& y) { return x.y < y.y; });
+    L.n_bricks = (int)bricks.size();
+    // myocardial cells of each brick's 10^3 box (brick + 1-voxel halo)
+    std::vector<int> hoff(bricks.size() + 1, 0);
+    std::vector<uint16_t> hcells;
+    for (size_t q = 0; q < bricks.size(); ++q) {
+        const int4 bq = bricks[q];
+        for (int c = 0; c < 1000; ++c) {
+            const int gz = bq.x - 1 + c / 100, gy = bq.y - 1 + (c / 10) % 10, gx = bq.z - 1 + c % 10;
+            if (gz < 0 || gz >= h.n0 || gy < 0 || gy >= h.n1 || gx < 0 || gx >= h.n2) continue;
+            if (h.h_layer[((int64_t)gz * h.n1 + gy) * h.n2 + gx]) hcells.push_back((uint16_t)c);
+        }
+        hoff[q + 1] = (int)hcells.size();
+    }
+    if (hcells.empty()) hcells.push_back(0);
     h.ctx->bind();
-    L.vox_geo.reserve(geo.size() * sizeof(int4));
+    L.vox_chunk.reserve(vchunk.size() * sizeof(int));
+    L.bricks.reserve(std::max<size_t>(1, bricks.size()) * sizeof(int4));
+    L.brick_off.reserve(boff.size() * sizeof(int));
+    L.brick_vox.reserve(bvox.size() * sizeof(int2));
     L.vox_a64.reserve(a64.size() * sizeof(double));
     L.vox_a.reserve(af.size() * sizeof(float2));
     L.chunks.reserve(std::max<size_t>(1, L.h_chunks.size()) * sizeof(int4));
-    hip_check(hipMemcpy(L.vox_geo.p, geo.data(), geo.size() * sizeof(int4), hipMemcpyHostToDevice),
-              "upload voxel geometry");
+    hip_check(hipMemcpy(L.vox_chunk.p, vchunk.data(), vchunk.size() * sizeof(int), hipMemcpyHostToDevice),
+              "upload voxel chunks");
+    if (!bricks.empty())
+        hip_check(hipMemcpy(L.bricks.p, bricks.data(), bricks.size() * sizeof(int4), hipMemcpyHostToDevice),
+                  "upload bricks");
+    hip_check(hipMemcpy(L.brick_off.p, boff.data(), boff.size() * sizeof(int), hipMemcpyHostToDevice),
+              "upload brick offsets");
+    hip_check(hipMemcpy(L.brick_vox.p, bvox.data(), bvox.size() * sizeof(int2), hipMemcpyHostToDevice),
+              "upload brick voxels");
+    L.halo_off.reserve(hoff.size() * sizeof(int));
+    L.halo_cells.reserve(hcells.size() * sizeof(uint16_t));
+    hip_check(hipMemcpy(L.halo_off.p, hoff.data(), hoff.size() * sizeof(int), hipMemcpyHostToDevice),
+              "upload halo offsets");
+    hip_check(hipMemcpy(L.halo_cells.p, hcells.data(), hcells.size() * sizeof(uint16_t),
+                        hipMemcpyHostToDevice),
+              "upload halo cells");
     hip_check(hipMemcpy(L.vox_a64.p, a64.data(), a64.size() * sizeof(double), hipMemcpyHostToDevice),
               "upload activation times");
     hip_check(hipMemcpy(L.vox_a.p, af.data(), af.size() * sizeof(float2), hipMemcpyHostToDevice),
@@ -358,14 +417,22 @@ void enqueue_batch(Ctx& ctx, Heart& h, const BatchPlan& p, const double* d_layer
     ws.partial.reserve((size_t)B * std::max(nch, 1) * p.LP * p.Tp * 8);
 
     ekgk::LeadFieldArgs la;
-    la.vox_geo = Lo.vox_geo.as<int4>();
+    la.grid_layer = h.d_layer.as<uint8_t>();
+    la.n0 = h.n0;
+    la.n1 = h.n1;
+    la.n2 = h.n2;
+    la.bricks = Lo.bricks.as<int4>();
+    la.brick_off = Lo.brick_off.as<int>();
+    la.brick_vox = Lo.brick_vox.as<int2>();
+    la.halo_off = Lo.halo_off.as<int>();
+    la.halo_cells = Lo.halo_cells.as<uint16_t>();
+    la.n_bricks = Lo.n_bricks;
+    la.vox_chunk = Lo.vox_chunk.as<int>();
     la.vox_a64 = Lo.vox_a64.as<double>();
     la.vox_a = Lo.vox_a.as<float2>();
     la.chunks = Lo.chunks.as<int4>();
     la.n_chunks = nch;
-    la.nb = static_cast<const int8_t*>(ctx_nb_table(ctx, p.nhood));
-    int8_t dif[26][3];
-    la.n_nb = nhood_offsets(p.nhood, dif);
+    la.nhood = p.nhood;
     la.lead_pos = d_lead_pos;
     la.n_leads = p.L;
     la.layer_k = d_layer_k;
@@ -375,8 +442,8 @@ void enqueue_batch(Ctx& ctx, Heart& h, const BatchPlan& p, const double* d_layer
     la.n_req = B;
     la.rec = ws.G.as<double>();
     la.aux = ws.aux.as<float4>();
-    la.mfma = p.mfma;
     la.gsum = ws.gsum.as<double>();
+    la.mfma = p.mfma;
     hip_check(ekgk::launch_lead_field(la, stream), "lead_field launch");
 
     hip_check(ekgk::launch_ap_tables(d_layer_k, p.n_layers, ws.tgrid.as<double>(),

@@ -67,9 +67,10 @@ struct EcgLayout {
     bool valid = false;
     int64_t V = 0;
     int n_chunks = 0;
+    int n_bricks = 0;
     int max_layer = 0;
     std::vector<int4> h_chunks;
-    DevBuf vox_geo, vox_a64, vox_a, chunks;
+    DevBuf vox_chunk, vox_a64, vox_a, chunks, bricks, brick_off, brick_vox, halo_off, halo_cells;
 };
 
 struct Heart {

@@ -206,91 +206,205 @@ __device__ __forceinline__ void lead_f(double e0, double e1, double e2, double& 
     m2 = e2 * s;
 }
 
-// One block per (chunk, request); one voxel per thread and all leads.
-// Writes the voxel record consumed by ecg_fused (RS = LP + 2 doubles):
-//   [0, LP)  G_i(u) = sum_{d in N, u+d myocardial} (m_i(u+d) + m_i(u)) . d
-//                   = sum_d m_i(u+d) . d + m_i(u) . s_u,  s_u = sum of those d,
-//            zero-padded to LP leads;
-//   LP       (P_u, Q_u) as two floats: the per-voxel AP factors of the
-//            saturated path, P_u = k2(1-k3) e^{(k4+k5)(a_u - t_ref)},
-//            Q_u = k2 k3 e^{k5 (a_u - t_ref)}, t_ref = the chunk's reference time;
-//   LP + 1   the activation time as (hi, lo) floats;
-// and the chunk sum of G per lead, reduced in a fixed order (shuffle tree
-// within waves, then wave order).  Request b's records start at
-// b * (V + 1) * RS; the extra record per request is never written and only
-// keeps the fused kernel's one-ahead load in bounds.
-//
-// kMfma = true writes the layout of ecg_fused_mfma instead: G rows of 16
-// doubles (zero-padded leads) at rec[(b*(V+MFMA_PAD) + v)*16], and the float4
-// {P, Q, a_hi, a_lo} at aux[b*(V+MFMA_PAD) + v]; the spare rows per request
-// keep the 4-voxel groups and their one-ahead loads in bounds.
-template <int LP, bool kMfma>
-__global__ __launch_bounds__(256) void lead_field_kernel(
-    const int4* __restrict__ vox_geo, const double* __restrict__ vox_a64,
-    const float2* __restrict__ vox_a, const int4* __restrict__ chunks, int n_chunks,
-    const int8_t* __restrict__ nb, int n_nb, const double* __restrict__ lead_pos, int n_leads,
+constexpr int LB = 8;                        // lead-field brick edge
+constexpr int LH = LB + 2;                   // with halo
+constexpr int LN = LH * LH * LH;             // halo cells
+
+// One block per (8^3 brick holding myocardium, request).  For each lead i
+// the lead weight m_i of every myocardial cell of the brick and its 1-voxel
+// halo is computed once into LDS; each myocardial voxel u of the brick then
+// gathers
+//   G_i(u) = sum_{d in N, u+d myocardial} (m_i(u+d) + m_i(u)) . d
+//          = sum_d m_i(u+d) . d + m_i(u) . s_u,   s_u = sum of those d,
+// the coefficient of V_u(t) in the reference's dipole sum regrouped by cell
+// (simulator.cpp:509-537).  The voxel's record (sorted index v) receives
+//   VALU layout (RS = LP + 2 doubles at rec[(b*(V+1) + v)*RS]):
+//     [0, LP) G zero-padded, LP: (P_u, Q_u) floats, LP+1: (a_hi, a_lo) floats;
+//   MFMA layout: G rows of 16 doubles at rec[(b*(V+MFMA_PAD) + v)*16] and
+//     float4 {P_u, Q_u, a_hi, a_lo} at aux[b*(V+MFMA_PAD) + v];
+// with P_u = k2(1-k3) e^{(k4+k5)(a_u - t_ref)}, Q_u = k2 k3 e^{k5 (a_u - t_ref)},
+// t_ref the reference time of the voxel's chunk (saturated-path AP factors).
+// Dipole neighbourhoods at compile time, in Neighbourhood::create order
+// (simulator.h:325-390): NH = 0 2D4, 1 2D8, 2 3D4, 3 3D8.
+template <int NH> struct Nhood;
+template <> struct Nhood<0> {
+    static constexpr int n = 4;
+    static constexpr int8_t d[4][3] = {{0, -1, -1}, {0, -1, 1}, {0, 1, -1}, {0, 1, 1}};
+};
+template <> struct Nhood<1> {
+    static constexpr int n = 8;
+    static constexpr int8_t d[8][3] = {{0, -1, -1}, {0, -1, 0}, {0, -1, 1}, {0, 0, -1},
+                                       {0, 0, 1},   {0, 1, -1}, {0, 1, 0},  {0, 1, 1}};
+};
+template <> struct Nhood<2> {
+    static constexpr int n = 8;
+    static constexpr int8_t d[8][3] = {{-1, -1, -1}, {-1, -1, 1}, {-1, 1, -1}, {-1, 1, 1},
+                                       {1, -1, -1},  {1, -1, 1},  {1, 1, -1},  {1, 1, 1}};
+};
+template <> struct Nhood<3> {
+    static constexpr int n = 26;
+    static constexpr int8_t d[26][3] = {
+        {-1, -1, -1}, {-1, -1, 0}, {-1, -1, 1}, {-1, 0, -1}, {-1, 0, 0}, {-1, 0, 1}, {-1, 1, -1},
+        {-1, 1, 0},   {-1, 1, 1},  {0, -1, -1}, {0, -1, 0},  {0, -1, 1}, {0, 0, -1}, {0, 0, 1},
+        {0, 1, -1},   {0, 1, 0},   {0, 1, 1},   {1, -1, -1}, {1, -1, 0}, {1, -1, 1}, {1, 0, -1},
+        {1, 0, 0},    {1, 0, 1},   {1, 1, -1},  {1, 1, 0},   {1, 1, 1}};
+};
+
+// One block of 512 threads per (8^3 brick holding myocardium, request).
+// For two leads at a time the lead weight m_i of every myocardial cell of the
+// brick and its 1-voxel halo (a compact per-brick list) is computed once into
+// LDS (component arrays, zero outside the myocardium); thread t < n then
+// gathers for the brick's t-th myocardial voxel u
+//   G_i(u) = sum_{d in N} M_i(u+d) . d + m_i(u) . s_u,
+//   M_i = m_i on myocardium and 0 elsewhere,  s_u = sum of d with u+d myocardial,
+// which is sum_{d in N, u+d myo} (m_i(u+d) + m_i(u)) . d -- the coefficient of
+// V_u(t) in the reference's dipole sum regrouped by cell (simulator.cpp:509-537).
+// The voxel's record (sorted index v) receives
+//   VALU layout (RS = LP + 2 doubles at rec[(b*(V+1) + v)*RS]):
+//     [0, LP) G zero-padded, LP: (P_u, Q_u) floats, LP+1: (a_hi, a_lo) floats;
+//   MFMA layout: G rows of 16 doubles at rec[(b*(V+MFMA_PAD) + v)*16] and
+//     float4 {P_u, Q_u, a_hi, a_lo} at aux[b*(V+MFMA_PAD) + v];
+// with P_u = k2(1-k3) e^{(k4+k5)(a_u - t_ref)}, Q_u = k2 k3 e^{k5 (a_u - t_ref)},
+// t_ref the reference time of the voxel's chunk (saturated-path AP factors).
+template <int LP, bool kMfma, int NH>
+__global__ __launch_bounds__(512) void lead_field_kernel(
+    const uint8_t* __restrict__ grid_layer, int n0, int n1, int n2,
+    const int4* __restrict__ bricks, const int* __restrict__ brick_off,
+    const int2* __restrict__ brick_vox, const int* __restrict__ halo_off,
+    const uint16_t* __restrict__ halo_cells, const int* __restrict__ vox_chunk,
+    const double* __restrict__ vox_a64, const float2* __restrict__ vox_a,
+    const int4* __restrict__ chunks, const double* __restrict__ lead_pos, int n_leads,
     const double* __restrict__ layer_k, int n_layers, int64_t V, double* __restrict__ rec,
-    float4* __restrict__ aux, double* __restrict__ gsum)
+    float4* __restrict__ aux)
 {
+    using N = Nhood<NH>;
     constexpr int RS = kMfma ? 16 : LP + 2;
     constexpr int NG = kMfma ? 16 : LP;
+    __shared__ uint8_t s_myo[LN];
+    __shared__ double s_m[2][3][LN];
+    const int br = blockIdx.x;
+    const int b = blockIdx.y;
+    const int t = threadIdx.x;
+    const int4 bi = bricks[br];
+    const int e0 = brick_off[br], nv = brick_off[br + 1] - e0;
+    const int c0 = halo_off[br], nc = halo_off[br + 1] - c0;
+    for (int q = t; q < LN; q += blockDim.x) {
+        s_myo[q] = 0;
+#pragma unroll
+        for (int w = 0; w < 2; ++w) s_m[w][0][q] = s_m[w][1][q] = s_m[w][2][q] = 0.0;
+    }
+    __syncthreads();
+    for (int q = t; q < nc; q += blockDim.x) s_myo[halo_cells[c0 + q]] = 1;
+    __syncthreads();
+    int h = 0, v = -1;
+    double sd0 = 0.0, sd1 = 0.0, sd2 = 0.0;
+    if (t < nv) {
+        const int2 bv = brick_vox[e0 + t];
+        v = bv.x;
+        const int lz = bv.y >> 6, ly = (bv.y >> 3) & 7, lx = bv.y & 7;
+        h = ((lz + 1) * LH + (ly + 1)) * LH + (lx + 1);
+#pragma unroll
+        for (int q = 0; q < N::n; ++q) {
+            const double f = s_myo[h + (N::d[q][0] * LH + N::d[q][1]) * LH + N::d[q][2]];
+            sd0 += f * N::d[q][0];
+            sd1 += f * N::d[q][1];
+            sd2 += f * N::d[q][2];
+        }
+    }
+    const double* pos = lead_pos + (int64_t)b * n_leads * 3;
+    const int64_t rows = V + (kMfma ? MFMA_PAD : 1);
+    // the voxel's whole record is assembled in registers and stored once
+    double row[RS];
+#pragma unroll
+    for (int i = 0; i < RS; ++i) row[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < LP; i += 2) {
+        if (i >= n_leads) break;
+        const bool two = i + 1 < n_leads;
+        for (int q = t; q < nc; q += blockDim.x) {
+            const int cell = halo_cells[c0 + q];
+            const int gz = bi.x - 1 + cell / (LH * LH), gy = bi.y - 1 + (cell / LH) % LH,
+                      gx = bi.z - 1 + cell % LH;
+            const double z = (double)(gz + 1), y = (double)(gy + 1), x = (double)(gx + 1);
+            double m0, m1, m2;
+            lead_f(pos[3 * i] - z, pos[3 * i + 1] - y, pos[3 * i + 2] - x, m0, m1, m2);
+            s_m[0][0][cell] = m0;
+            s_m[0][1][cell] = m1;
+            s_m[0][2][cell] = m2;
+            if (two) {
+                lead_f(pos[3 * i + 3] - z, pos[3 * i + 4] - y, pos[3 * i + 5] - x, m0, m1, m2);
+                s_m[1][0][cell] = m0;
+                s_m[1][1][cell] = m1;
+                s_m[1][2][cell] = m2;
+            }
+        }
+        __syncthreads();
+        if (t < nv) {
+#pragma unroll
+            for (int w = 0; w < 2; ++w) {
+                double g = s_m[w][0][h] * sd0 + s_m[w][1][h] * sd1 + s_m[w][2][h] * sd2;
+#pragma unroll
+                for (int q = 0; q < N::n; ++q) {
+                    const int hn = h + (N::d[q][0] * LH + N::d[q][1]) * LH + N::d[q][2];
+                    if (N::d[q][0]) g += s_m[w][0][hn] * N::d[q][0];
+                    if (N::d[q][1]) g += s_m[w][1][hn] * N::d[q][1];
+                    if (N::d[q][2]) g += s_m[w][2][hn] * N::d[q][2];
+                }
+                if (w == 0 || two) row[i + w] = g;
+            }
+        }
+        __syncthreads();
+    }
+    if (t < nv) {
+        const int4 ch = chunks[vox_chunk[v]];
+        const double* k = layer_k + ((int64_t)b * n_layers + (ch.x - 1)) * 9;
+        const double da = vox_a64[v] - (double)__int_as_float(ch.w);
+        const float2 pq{(float)(k[2] * (1.0 - k[3]) * exp((k[4] + k[5]) * da)),
+                        (float)(k[2] * k[3] * exp(k[5] * da))};
+        const float2 av = vox_a[v];
+        double* gv = rec + ((int64_t)b * rows + v) * RS;
+        if constexpr (kMfma) {
+            aux[(int64_t)b * rows + v] = float4{pq.x, pq.y, av.x, av.y};
+        } else {
+            row[LP] = __builtin_bit_cast(double, pq);
+            row[LP + 1] = __builtin_bit_cast(double, av);
+        }
+#pragma unroll
+        for (int i = 0; i < RS; i += 2)
+            *reinterpret_cast<double2*>(gv + i) = double2{row[i], row[i + 1]};
+    }
+}
+
+// Per (chunk, request): sum over the chunk's voxels of G (k0 enters the ECG
+// as k0 * sum_u G_u, constant in time), in a fixed order -- shuffle tree in
+// each wave, then wave order.  Requests whose every layer has k0 == 0 get
+// zeros without reading G.
+template <int LP, bool kMfma>
+__global__ __launch_bounds__(256) void chunk_gsum_kernel(
+    const int4* __restrict__ chunks, int n_chunks, const double* __restrict__ layer_k,
+    int n_layers, int n_leads, int64_t V, const double* __restrict__ rec,
+    double* __restrict__ gsum)
+{
+    constexpr int RS = kMfma ? 16 : LP + 2;
     __shared__ double s_wave[4][LP];
     const int c = blockIdx.x;
     const int b = blockIdx.y;
+    double* out = gsum + ((int64_t)b * n_chunks + c) * LP;
+    bool any = false;
+    for (int l = 0; l < n_layers; ++l) any |= layer_k[((int64_t)b * n_layers + l) * 9] != 0.0;
+    if (!any) {
+        if (threadIdx.x < LP) out[threadIdx.x] = 0.0;
+        return;
+    }
     const int4 ch = chunks[c];
-    const double* pos = lead_pos + (int64_t)b * n_leads * 3;
-    const double* k = layer_k + ((int64_t)b * n_layers + (ch.x - 1)) * 9;
-    const double alpha = k[2] * (1.0 - k[3]), beta = k[2] * k[3];
-    const double k45 = k[4] + k[5], k5 = k[5];
-    const double tref = (double)__int_as_float(ch.w);
+    const double* recb = rec + (int64_t)b * (V + (kMfma ? MFMA_PAD : 1)) * RS;
     double part[LP];
 #pragma unroll
     for (int i = 0; i < LP; ++i) part[i] = 0.0;
     for (int v = ch.y + threadIdx.x; v < ch.z; v += blockDim.x) {
-        const int4 g = vox_geo[v];
-        double* gv = rec + ((int64_t)b * (V + (kMfma ? MFMA_PAD : 1)) + v) * RS;
-        double sd0 = 0.0, sd1 = 0.0, sd2 = 0.0;
-        for (int q = 0; q < n_nb; ++q)
-            if ((g.w >> nb[4 * q + 3]) & 1) {
-                sd0 += nb[4 * q];
-                sd1 += nb[4 * q + 1];
-                sd2 += nb[4 * q + 2];
-            }
 #pragma unroll
-        for (int i = 0; i < LP; ++i) {
-            double gi = 0.0;
-            if (i < n_leads) {
-                const double e0 = pos[3 * i] - (double)(g.x + 1);
-                const double e1 = pos[3 * i + 1] - (double)(g.y + 1);
-                const double e2 = pos[3 * i + 2] - (double)(g.z + 1);
-                double m0, m1, m2;
-                lead_f(e0, e1, e2, m0, m1, m2);
-                gi = m0 * sd0 + m1 * sd1 + m2 * sd2;
-                for (int q = 0; q < n_nb; ++q) {
-                    if (!((g.w >> nb[4 * q + 3]) & 1)) continue;
-                    const double d0 = nb[4 * q], d1 = nb[4 * q + 1], d2 = nb[4 * q + 2];
-                    double n0, n1, n2;
-                    lead_f(e0 - d0, e1 - d1, e2 - d2, n0, n1, n2);
-                    gi += n0 * d0 + n1 * d1 + n2 * d2;
-                }
-            }
-            gv[i] = gi;
-            part[i] += gi;
-        }
-        if constexpr (kMfma) {
-#pragma unroll
-            for (int i = LP; i < NG; ++i) gv[i] = 0.0;
-        }
-        const double da = vox_a64[v] - tref;
-        const float2 pq{(float)(alpha * exp(k45 * da)), (float)(beta * exp(k5 * da))};
-        if constexpr (kMfma) {
-            const float2 av = vox_a[v];
-            aux[(int64_t)b * (V + MFMA_PAD) + v] = float4{pq.x, pq.y, av.x, av.y};
-        } else {
-            float2* tail = reinterpret_cast<float2*>(gv + LP);
-            tail[0] = pq;
-            tail[1] = vox_a[v];
-        }
+        for (int i = 0; i < LP; ++i)
+            if (i < n_leads) part[i] += recb[(int64_t)v * RS + i];
     }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
@@ -301,9 +415,9 @@ __global__ __launch_bounds__(256) void lead_field_kernel(
     }
     __syncthreads();
     if (threadIdx.x < LP) {
-        double s = 0.0;
-        for (int ww = 0; ww < (int)(blockDim.x >> 6); ++ww) s += s_wave[ww][threadIdx.x];
-        gsum[((int64_t)b * n_chunks + c) * LP + threadIdx.x] = s;
+        double sum = 0.0;
+        for (int ww = 0; ww < (int)(blockDim.x >> 6); ++ww) sum += s_wave[ww][threadIdx.x];
+        out[threadIdx.x] = sum;
     }
 }
 
@@ -717,17 +831,33 @@ hipError_t launch_ca_finalize(bool is_int, const void* dist, double* act, int64_
     return hipGetLastError();
 }
 
+template <int LP, bool M, int NH>
+static void lf_launch_nh(const LeadFieldArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL((lead_field_kernel<LP, M, NH>), dim3(a.n_bricks, a.n_req), dim3(512), 0, s,
+                       a.grid_layer, a.n0, a.n1, a.n2, a.bricks, a.brick_off, a.brick_vox,
+                       a.halo_off, a.halo_cells, a.vox_chunk, a.vox_a64, a.vox_a, a.chunks,
+                       a.lead_pos, a.n_leads, a.layer_k, a.n_layers, a.V, a.rec, a.aux);
+}
+
+template <int LP, bool M>
+static void lf_launch_t(const LeadFieldArgs& a, hipStream_t s)
+{
+    switch (a.nhood) {
+    case 0: lf_launch_nh<LP, M, 0>(a, s); break;
+    case 1: lf_launch_nh<LP, M, 1>(a, s); break;
+    case 2: lf_launch_nh<LP, M, 2>(a, s); break;
+    default: lf_launch_nh<LP, M, 3>(a, s); break;
+    }
+    hipLaunchKernelGGL((chunk_gsum_kernel<LP, M>), dim3(a.n_chunks, a.n_req), dim3(256), 0, s,
+                       a.chunks, a.n_chunks, a.layer_k, a.n_layers, a.n_leads, a.V, a.rec, a.gsum);
+}
+
 template <int LP>
 static void lf_launch(const LeadFieldArgs& a, hipStream_t s)
 {
-    if (a.mfma)
-        hipLaunchKernelGGL((lead_field_kernel<LP, true>), dim3(a.n_chunks, a.n_req), dim3(256), 0,
-                           s, a.vox_geo, a.vox_a64, a.vox_a, a.chunks, a.n_chunks, a.nb, a.n_nb,
-                           a.lead_pos, a.n_leads, a.layer_k, a.n_layers, a.V, a.rec, a.aux, a.gsum);
-    else
-        hipLaunchKernelGGL((lead_field_kernel<LP, false>), dim3(a.n_chunks, a.n_req), dim3(256), 0,
-                           s, a.vox_geo, a.vox_a64, a.vox_a, a.chunks, a.n_chunks, a.nb, a.n_nb,
-                           a.lead_pos, a.n_leads, a.layer_k, a.n_layers, a.V, a.rec, a.aux, a.gsum);
+    if (a.mfma) lf_launch_t<LP, true>(a, s);
+    else lf_launch_t<LP, false>(a, s);
 }
 
 hipError_t launch_lead_field(const LeadFieldArgs& a, hipStream_t s)

@@ -24,13 +24,20 @@ struct LayerConst {
 static_assert(sizeof(LayerConst) == 32, "LayerConst layout");
 
 struct LeadFieldArgs {
-    const int4* vox_geo;         // [V] {i0, i1, i2, 26-bit myocardial-neighbour mask}
+    const uint8_t* grid_layer;   // [n0*n1*n2]
+    int n0, n1, n2;
+    const int4* bricks;          // [n_bricks] {z0, y0, x0, 0} of bricks holding myocardium
+    const int* brick_off;        // [n_bricks + 1] offsets into brick_vox
+    const int2* brick_vox;       // {sorted voxel index, local cell lz*64 + ly*8 + lx}
+    const int* halo_off;         // [n_bricks + 1] offsets into halo_cells
+    const uint16_t* halo_cells;  // myocardial cells of each brick + halo (index in the 10^3 box)
+    int n_bricks;
+    const int* vox_chunk;        // [V] chunk of each sorted voxel
     const double* vox_a64;       // [V] activation time
     const float2* vox_a;         // [V] activation time (hi, lo)
     const int4* chunks;          // [n_chunks] {layer, v_begin, v_end, t_ref as float bits}
     int n_chunks;
-    const int8_t* nb;            // [n_nb][4] {d0, d1, d2, cube bit}
-    int n_nb;
+    int nhood;                   // EKG_NHOOD_* (dipole neighbourhood)
     const double* lead_pos;      // [n_req][n_leads][3]
     int n_leads;
     const double* layer_k;       // [n_req][n_layers][9]

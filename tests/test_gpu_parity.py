@@ -4,12 +4,15 @@ Bars: activation times bit-exact (FP64 reference semantics and the integer
 per-voxel-delay mode); every ECG within rel-L2 <= 1e-5 of the FP64 oracle
 (the tolerance the simulation core is held to), plus max-error reporting.
 """
+import glob
+import os
+
 import numpy as np
 import pytest
 
 from ekgsim_amd import synth as S
 from oracle import oracle as O
-from tests.conftest import rel_l2
+from tests.conftest import GOLDEN, load_golden, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -180,3 +183,41 @@ def test_simulate_before_excitation_rejected(ctx):
     g = _heart(ctx, h)
     with pytest.raises(EkgError):
         g.simulate(k, pos)
+
+
+# --- against the reference's own outputs (golden vectors) ---------------------
+
+
+@pytest.mark.parametrize("name", ["h16", "h22_irregular", "h2d"])
+def test_ca_matches_reference_golden_bitwise(ctx, name):
+    g = load_golden(f"ref_ca_{name}.npz")
+    from ekgsim_amd import Heart
+    h = Heart(ctx, g["layer"], g["seed"])
+    h.excite_transfer(g["transfer"])
+    assert np.array_equal(h.activation().view(np.uint64), g["act"].view(np.uint64))
+
+
+@pytest.mark.parametrize("name", sorted(os.path.basename(p) for p in glob.glob(os.path.join(GOLDEN, "ref_run_*.npz"))))
+def test_ecg_matches_reference_golden(ctx, name):
+    g = load_golden(name)
+    from ekgsim_amd import Heart
+    h = Heart(ctx, g["layer"])
+    h.set_activation(g["act"])
+    got = h.simulate(g["layer_k"], g["pos"], str(g["nhood"]), float(g["start"]), float(g["length"]),
+                     float(g["dt"]))
+    assert got.shape == g["ecg"].shape
+    assert rel_l2(got, g["ecg"]) <= TOL
+
+
+def test_pipeline_matches_reference_golden(ctx):
+    """Shape + conduction -> excitation on the GPU, file lead positions moved by
+    the displacement genes through the C ABI, ECG on the GPU; against the
+    reference EkgSim facade run on the same files."""
+    import ekgsim_amd as E
+    g = load_golden("ref_pipeline_16.npz")
+    h = E.Heart(ctx, g["layer"], g["seed"])
+    h.excite_transfer(g["transfer"])
+    assert np.array_equal(h.activation(), g["act"])
+    pos = E.move_leads(g["pos_file"], g["disp"])
+    got = h.simulate(g["layer_k"], pos, "3D4", 0.0, 500.0, 1.0)
+    assert rel_l2(got, g["ecg"]) <= TOL
