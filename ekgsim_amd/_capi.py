@@ -30,7 +30,8 @@ EXPORTS = (
     "ekg_num_steps", "ekg_simulate", "ekg_simulate_batch", "ekg_lead_basis", "ekg_move_lead",
     "ekg_server_create", "ekg_server_destroy", "ekg_server_add_heart", "ekg_server_submit",
     "ekg_server_poll", "ekg_server_wait", "ekg_server_stats", "ekg_simulate_batch_device",
-    "ekg_ctx_synchronize", "ekg_ctx_set_kernel",
+    "ekg_ctx_synchronize", "ekg_ctx_set_kernel", "ekg_ap_value", "ekg_ap_num_genes", "ekg_layer_aps",
+    "ekg_criterion", "ekg_resample",
 )
 KERNELS = {"auto": 0, "valu": 1, "mfma": 2}
 
@@ -113,6 +114,12 @@ def lib():
     L.ekg_server_wait.argtypes = [vp, C.c_uint64, vp, C.c_size_t, pd]
     L.ekg_server_stats.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                    C.POINTER(C.c_int), C.POINTER(C.c_size_t)]
+    L.ekg_ap_value.argtypes = [vp, C.c_double]
+    L.ekg_ap_value.restype = C.c_double
+    L.ekg_ap_num_genes.argtypes = [C.POINTER(ApModel), C.POINTER(C.c_int)]
+    L.ekg_layer_aps.argtypes = [C.POINTER(ApModel), C.c_int, vp, C.c_int, vp, pd]
+    L.ekg_criterion.argtypes = [C.c_int, vp, C.c_int64, vp, C.c_int64, C.c_double, pd]
+    L.ekg_resample.argtypes = [vp, C.c_int64, C.c_double, C.c_int, vp, C.c_int64, C.POINTER(C.c_int64)]
     _lib = L
     return L
 
@@ -149,6 +156,78 @@ def move_leads(pos, disp):
         o = np.zeros(3)
         lib().ekg_move_lead(_p(np.ascontiguousarray(pos[i])), float(disp[i][0]), float(disp[i][1]), _p(o))
         out[i] = o
+    return out
+
+
+class ApModel(C.Structure):
+    _fields_ = [
+        ("n_base", C.c_int),
+        ("base_k", C.c_void_p),
+        ("k_min", C.c_void_p),
+        ("k_max", C.c_void_p),
+        ("n_free", C.c_int),
+        ("free_k", C.c_void_p),
+        ("interpolation", C.c_int),
+        ("mid_position", C.c_double),
+    ]
+
+
+INTERP = {"endo-epi": 1, "endo-mid-epi": 2}
+COMPARE = {"rms": 1, "correlation": 2, "dev-from-linear": 3, "vector-correlation": 4}
+
+
+def ap_value(k, t):
+    """WohlfartPlus W(t) in FP64 (host)."""
+    k = np.ascontiguousarray(k, np.float64)
+    return float(lib().ekg_ap_value(_p(k), float(t)))
+
+
+def _ap_model(base_k, k_min, k_max, free_k, interpolation, mid_position):
+    keep = dict(base=np.ascontiguousarray(base_k, np.float64).reshape(-1, 9),
+                kmin=np.ascontiguousarray(k_min, np.float64), kmax=np.ascontiguousarray(k_max, np.float64),
+                free=np.ascontiguousarray(free_k, np.int32))
+    m = ApModel(keep["base"].shape[0], keep["base"].ctypes.data, keep["kmin"].ctypes.data,
+                keep["kmax"].ctypes.data, keep["free"].size, keep["free"].ctypes.data if keep["free"].size else None,
+                INTERP.get(interpolation, interpolation), float(mid_position))
+    return m, keep
+
+
+def ap_num_genes(free_k, interpolation):
+    m, keep = _ap_model(np.zeros(9), np.zeros(9), np.zeros(9), free_k, interpolation, 0.5)
+    n = C.c_int()
+    check(lib().ekg_ap_num_genes(C.byref(m), C.byref(n)))
+    return n.value
+
+
+def layer_aps(base_k, k_min, k_max, free_k, interpolation, mid_position, n_layers, genes):
+    """Layer APs [n_layers][9] and violation from the leading parameter-vector
+    entries (the `[wohlfart ap]` model of simulator.ini)."""
+    m, keep = _ap_model(base_k, k_min, k_max, free_k, interpolation, mid_position)
+    g = np.ascontiguousarray(genes, np.float64)
+    out = np.zeros((n_layers, 9))
+    viol = C.c_double()
+    check(lib().ekg_layer_aps(C.byref(m), int(n_layers), _p(g) if g.size else None, g.size, _p(out),
+                              C.byref(viol)))
+    return out, viol.value
+
+
+def criterion(mode, sim, target, target_offset=0.0):
+    """Comparison value of one simulated lead against a target (offset 0)."""
+    a = np.ascontiguousarray(sim, np.float64)
+    b = np.ascontiguousarray(target, np.float64)
+    v = C.c_double()
+    check(lib().ekg_criterion(COMPARE.get(mode, mode), _p(a), a.size, _p(b), b.size, float(target_offset),
+                              C.byref(v)))
+    return v.value
+
+
+def resample(src, factor, start):
+    """Target resampling of the reference's target loader."""
+    s = np.ascontiguousarray(src, np.float64)
+    n = C.c_int64()
+    check(lib().ekg_resample(_p(s), s.size, float(factor), int(start), None, 0, C.byref(n)))
+    out = np.zeros(n.value)
+    check(lib().ekg_resample(_p(s), s.size, float(factor), int(start), _p(out), out.size, C.byref(n)))
     return out
 
 

@@ -8,6 +8,7 @@
 
 #include "ekg.h"
 #include "ekg_engine.hpp"
+#include "ekg_model.hpp"
 #include "ekg_server.hpp"
 
 struct ekg_ctx {
@@ -338,6 +339,49 @@ void ekg_move_lead(const double p[3], double du, double dv, double out[3])
     double u[3], v[3];
     basis(p, u, v);
     for (int k = 0; k < 3; ++k) out[k] = (p[k] + u[k] * du) + v[k] * dv;
+}
+
+// ---- parameter vector -> layer APs, comparison ----------------------------
+
+double ekg_ap_value(const double k[9], double t) { return ekg::ap_value(k, t); }
+
+int ekg_ap_num_genes(const ekg_ap_model* m, int* n_genes)
+{
+    return guard([&] {
+        if (!m || !n_genes) ekg::fail(EKG_EINVAL, "null argument");
+        if (m->interpolation != EKG_INTERP_ENDO_EPI && m->interpolation != EKG_INTERP_ENDO_MID_EPI)
+            ekg::fail(EKG_EINVAL, "unknown interpolation type");
+        if (m->n_free < 0 || (m->n_free > 0 && !m->free_k)) ekg::fail(EKG_EINVAL, "bad free-k list");
+        *n_genes = ekg::ap_num_genes(*m);
+    });
+}
+
+int ekg_layer_aps(const ekg_ap_model* m, int n_layers, const double* genes, int n_genes, double* layer_k,
+                  double* violation)
+{
+    return guard([&] {
+        if (!m) ekg::fail(EKG_EINVAL, "null model");
+        ekg::layer_aps(*m, n_layers, genes, n_genes, layer_k, violation);
+    });
+}
+
+int ekg_resample(const double* src, int64_t n, double factor, int start, double* dest, int64_t cap,
+                 int64_t* n_out)
+{
+    return guard([&] {
+        if (!src || n <= 0 || !n_out || cap < 0 || (cap > 0 && !dest)) ekg::fail(EKG_EINVAL, "bad resample arguments");
+        *n_out = (int64_t)ekg::resample(src, (size_t)n, factor, start, dest, (size_t)cap);
+    });
+}
+
+int ekg_criterion(int mode, const double* sim, int64_t n_sim, const double* target, int64_t n_target,
+                  double target_offset, double* value)
+{
+    return guard([&] {
+        if (!value || n_sim < 0 || n_target < 0 || (n_sim > 0 && !sim) || (n_target > 0 && !target))
+            ekg::fail(EKG_EINVAL, "bad criterion arguments");
+        *value = ekg::criterion(mode, sim, (size_t)n_sim, target, (size_t)n_target, target_offset);
+    });
 }
 
 // ---- server ---------------------------------------------------------------

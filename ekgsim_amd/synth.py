@@ -64,8 +64,9 @@ def ellipsoid_axes(n: int):
     return tuple(v * s for v in o), tuple(v * s for v in i)
 
 
-def make_heart(n: int, seed: int = 0, n_seeds: int = 5, dims=None) -> Heart:
-    """Ellipsoid-shell heart on an n^3 grid (or dims = (n0, n1, n2))."""
+def make_heart(n: int, seed: int = 0, n_seeds: int = 5, dims=None, n_layers: int = 3) -> Heart:
+    """Ellipsoid-shell heart on an n^3 grid (or dims = (n0, n1, n2)) whose
+    myocardium is split into n_layers transmural layers of equal thickness."""
     if dims is None:
         dims = (n, n, n)
     n0, n1, n2 = dims
@@ -82,7 +83,7 @@ def make_heart(n: int, seed: int = 0, n_seeds: int = 5, dims=None) -> Heart:
     with np.errstate(divide="ignore", invalid="ignore"):
         s = (r_in - 1.0) / ((r_in - 1.0) + (1.0 - r_out))
     lay = np.zeros(dims, np.uint8)
-    lay[myo] = (1 + np.minimum(2, np.floor(3.0 * s[myo]))).astype(np.uint8)
+    lay[myo] = (1 + np.minimum(n_layers - 1, np.floor(n_layers * s[myo]))).astype(np.uint8)
     rng = np.random.default_rng(seed)
     delay = np.zeros(dims, np.uint8)
     delay[myo] = rng.integers(1, 4, size=int(myo.sum()), dtype=np.uint8)
@@ -101,7 +102,7 @@ def make_heart(n: int, seed: int = 0, n_seeds: int = 5, dims=None) -> Heart:
     if len(endo):
         pick = rng.choice(endo, size=min(n_seeds, len(endo)), replace=False)
         seeds.ravel()[pick] = 1
-    return Heart(layer=lay, seed=seeds, delay=delay, n_layers=3)
+    return Heart(layer=lay, seed=seeds, delay=delay, n_layers=n_layers)
 
 
 def lead_positions(n: int, n_leads: int, dims=None) -> np.ndarray:

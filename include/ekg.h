@@ -169,6 +169,64 @@ int ekg_ctx_synchronize(ekg_ctx *ctx);
 void ekg_lead_basis(const double p[3], double u[3], double v[3]);
 void ekg_move_lead(const double p[3], double du, double dv, double out[3]);
 
+/* ---- parameter vector -> layer APs (host, FP64) ------------------------- */
+/* The `[wohlfart ap]` section of simulator.ini (SimSettings.h:176-237). */
+#define EKG_INTERP_ENDO_EPI 1     /* `interpolation = endo-epi` */
+#define EKG_INTERP_ENDO_MID_EPI 2 /* `interpolation = endo-mid-epi` */
+
+typedef struct ekg_ap_model {
+    int n_base;             /* rows of base_k, `base ap 1..n` */
+    const double *base_k;   /* [n_base][9] */
+    const double *k_min;    /* [9] `k min` */
+    const double *k_max;    /* [9] `k max` */
+    int n_free;             /* entries of free_k */
+    const int *free_k;      /* `free k`: 0-based coefficient indices */
+    int interpolation;      /* EKG_INTERP_* */
+    double mid_position;    /* `mid AP position`, in [0, 1] */
+} ekg_ap_model;
+
+/* WohlfartPlus W(t) for coefficients k[0..8] (WohlfartPlus::operator[],
+ * Wohlfart.h:195-203), FP64 in the reference's operation order. */
+double ekg_ap_value(const double k[9], double t);
+
+/* Number of parameter-vector entries the layer APs take: distinct free
+ * coefficients x 2 (endo-epi) or x 3 (endo-mid-epi) (sim.cpp:492-509). */
+int ekg_ap_num_genes(const ekg_ap_model *m, int *n_genes);
+
+/* Layer APs of an n_layers heart from the leading entries of a parameter
+ * vector (SimImplementation::simUsingBorderAps / simUsingBorderAndMidAps,
+ * sim.cpp:751-916): border (and mid) layers take the free coefficients from
+ * the vector, the layers between are fitted by the reference's transmural
+ * shape interpolation (sim.cpp:91-313, nonlinearFit.h:92-168).  layer_k:
+ * [n_layers][9]; violation: summed distance of free values outside
+ * [k min, k max] (sim.cpp:582-596). */
+int ekg_layer_aps(const ekg_ap_model *m, int n_layers, const double *genes, int n_genes,
+                  double *layer_k, double *violation);
+
+/* ---- comparison with a target ECG (host, FP64) --------------------------- */
+/* `[optimization] comparison mode` (SimSettings.h:77-82). */
+#define EKG_CMP_RMS 1
+#define EKG_CMP_CORRELATION 2        /* 1 - Pearson r */
+#define EKG_CMP_DEV_FROM_LINEAR 3
+#define EKG_CMP_VECTOR_CORRELATION 4 /* 1 - a.b / (|a| |b|) */
+
+/* Comparison value of one simulated lead against one target at offset 0,
+ * over the first min(n_sim, n_target) samples (sim.cpp:640-660,
+ * vectorMath.h:277-401).  target_offset is used by EKG_CMP_DEV_FROM_LINEAR
+ * only.  The value is returned as computed: the caller's criterion maps a
+ * negative, zero, subnormal or non-finite value to 2e10 (sim.cpp:679-685). */
+int ekg_criterion(int mode, const double *sim, int64_t n_sim, const double *target, int64_t n_target,
+                  double target_offset, double *value);
+
+/* Target resampling of SimImplementation::loadTargets (sim.cpp:1013-1022,
+ * vectorMath.h:209-227): dest[i] = linear interpolation of src at
+ * i / factor + start for 1 + ceil((n - 1 - start) * factor) samples; a factor
+ * of exactly 1 copies src unshifted (the reference ignores start then).
+ * *n_out receives the full length; at most cap samples are written (dest
+ * may be NULL with cap 0 to query the length). */
+int ekg_resample(const double *src, int64_t n, double factor, int start, double *dest, int64_t cap,
+                 int64_t *n_out);
+
 /* ---- serving (continuous batching) -------------------------------------- */
 /* Device-resident request loop standing in for AMS-DEMO's
  * ExternalEvaluation / PFramework job queues (ExternalEvaluation.h:95-151,

@@ -15,6 +15,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(HERE, "liboracle.so")
 _REF = os.path.join(HERE, "_ref", "libekgref.so")
+_REFAPP = os.path.join(HERE, "_ref", "libekgrefapp.so")
 REF_CLI = os.path.join(HERE, "_ref", "ekgSim")
 
 NHOOD = {"2D4": 0, "2D8": 1, "3D4": 2, "3D8": 3, "cube": 3}
@@ -263,3 +264,52 @@ def ref_pipeline(ini_path, disp, layer_k, n_leads, n_steps, act_size=0):
     if rc:
         raise OracleError(ref().ref_last_error().decode())
     return ecg, (act if act_size else None)
+
+
+# -- compiled reference application code (oracle/_ref/libekgrefapp.so) --------
+_refapp = None
+
+
+def have_refapp() -> bool:
+    return os.path.exists(_REFAPP)
+
+
+def refapp():
+    global _refapp
+    if _refapp is None:
+        L = C.CDLL(_REFAPP)
+        L.ref_interp_layers.restype = _i
+        L.ref_interp_layers.argtypes = [_i, _i, _dbl, _d, _d, _d, _d]
+        L.ref_apd90.restype = _dbl
+        L.ref_apd90.argtypes = [_d]
+        L.ref_compare.restype = _dbl
+        L.ref_compare.argtypes = [_i, _d, _i64, _d, _i64, _dbl]
+        L.ref_resample.restype = _i64
+        L.ref_resample.argtypes = [_d, _i64, _dbl, _i, _d, _i64]
+        _refapp = L
+    return _refapp
+
+
+def ref_interp_layers(interp, n_layers, mid_position, front_k, mid_k, back_k):
+    out = np.zeros((n_layers, 9))
+    refapp().ref_interp_layers(int(interp), int(n_layers), float(mid_position), _c(front_k, np.float64),
+                               _c(mid_k, np.float64), _c(back_k, np.float64), out)
+    return out
+
+
+def ref_apd90(k):
+    return refapp().ref_apd90(_c(k, np.float64))
+
+
+def ref_compare(mode, a, b, bofs=0.0):
+    a = _c(a, np.float64)
+    b = _c(b, np.float64)
+    return refapp().ref_compare(int(mode), a, a.size, b, b.size, float(bofs))
+
+
+def ref_resample(src, factor, start):
+    src = _c(src, np.float64)
+    n = refapp().ref_resample(src, src.size, float(factor), int(start), np.zeros(1), 0)
+    out = np.zeros(max(n, 1))
+    refapp().ref_resample(src, src.size, float(factor), int(start), out, n)
+    return out[:n]

@@ -113,6 +113,63 @@ def main():
     save("ref_pipeline_16.npz", layer=h.layer, seed=h.seed, transfer=S.layered_transfer(3, 12, 1.0, 1.5),
          pos_file=pts_read, disp=disp, layer_k=k, act=act.reshape(h.layer.shape), ecg=ecg)
 
+    # 5. application model code (sim.cpp:72-313, 751-916; vectorMath.h:179-404)
+    if O.have_refapp():
+        model_goldens(rng)
+
+
+def model_goldens(rng):
+    """Layer APs of the transmural interpolation, comparison values and
+    target resampling, from the reference's own application code."""
+    base = np.array(S.BASE_AP, float)
+    rows = []
+    for interp in (1, 2):
+        for n_layers in (1, 2, 3, 4, 7, 24):
+            for mid in (0.6, 0.0, 1.0, 0.35):
+                if interp == 1 and mid != 0.6:
+                    continue
+                g = rng.uniform([3e-4, 0.01, 0.01, 300] * 3, [1e-3, 0.1, 0.1, 450] * 3)
+                fk, mk, bk = base.copy(), base.copy(), base.copy()
+                fk[5:9] = g[0:4]
+                mk[5:9] = g[4:8]
+                bk[5:9] = g[8:12] if interp == 2 else g[4:8]
+                out = O.ref_interp_layers(interp, n_layers, mid, fk, mk, bk)
+                rows.append((interp, n_layers, mid, g, out))
+    save("ref_layer_aps.npz", interp=np.array([r[0] for r in rows]), n_layers=np.array([r[1] for r in rows]),
+         mid=np.array([r[2] for r in rows]), genes=np.array([r[3] for r in rows]),
+         layer_k=np.concatenate([r[4] for r in rows]))
+
+    # comparisons of a simulated lead against a target, every mode, equal
+    # and unequal lengths, a constant curve (zero variance)
+    a_list, b_list, modes, offs, vals = [], [], [], [], []
+    t = np.arange(400.0)
+    for mode in (1, 2, 3, 4):
+        for na, nb in ((400, 400), (400, 335), (250, 400)):
+            a = 30 * np.exp(-((t[:na] - 200) / 40) ** 2) + rng.normal(0, 1, na)
+            b = np.exp(-((t[:nb] - 210) / 45) ** 2) + 0.01 * rng.normal(0, 1, nb)
+            off = 1.0 - b.min() / (b.max() - b.min())
+            a_list.append(np.pad(a, (0, 400 - na), constant_values=np.nan))
+            b_list.append(np.pad(b, (0, 400 - nb), constant_values=np.nan))
+            modes.append((mode, na, nb))
+            offs.append(off)
+            vals.append(O.ref_compare(mode, a, b, off))
+        c = np.full(300, 2.5)
+        a_list.append(np.pad(c, (0, 100), constant_values=np.nan))
+        b_list.append(np.pad(c * 0.5 + 1, (0, 100), constant_values=np.nan))
+        modes.append((mode, 300, 300))
+        offs.append(0.5)
+        vals.append(O.ref_compare(mode, c, c * 0.5 + 1, 0.5))
+    save("ref_compare.npz", a=np.array(a_list), b=np.array(b_list), shape=np.array(modes), offset=np.array(offs),
+         value=np.array(vals))
+
+    # target resampling (vectorMath.h:209-227)
+    src = np.sin(np.arange(120) / 7.0) + 0.1 * rng.normal(size=120)
+    res = {}
+    for i, (factor, start) in enumerate(((1.0, 20), (2.0, 0), (2.0, 15), (0.5, 3), (4.0, 100), (1.0 / 3.0, 0))):
+        res[f"factor_{i}"] = np.array([factor, start])
+        res[f"out_{i}"] = O.ref_resample(src, factor, start)
+    save("ref_resample.npz", src=src, **res)
+
 
 if __name__ == "__main__":
     main()
