@@ -348,7 +348,7 @@ size_t batch_device_bytes(const Heart& h, const BatchPlan& p)
     s += (size_t)p.B * p.n_layers * sizeof(ekgk::LayerConst);
     s += p.mfma ? (size_t)p.B * (V + ekgk::MFMA_PAD) * (16 * 8 + 16)
                 : (size_t)p.B * (V + 1) * (p.LP + 2) * 8;
-    s += (size_t)p.B * nch * p.LP * 8;
+    s += (size_t)p.B * nch * 3 * p.LP * 8;
     s += (size_t)p.B * nch * p.LP * p.Tp * 8;
     s += (size_t)p.B * p.L * p.T * 8;
     return s;
@@ -413,7 +413,7 @@ void enqueue_batch(Ctx& ctx, Heart& h, const BatchPlan& p, const double* d_layer
     } else {
         ws.G.reserve((size_t)B * (V + 1) * (p.LP + 2) * 8);
     }
-    ws.gsum.reserve((size_t)B * std::max(nch, 1) * p.LP * 8);
+    ws.gsum.reserve((size_t)B * std::max(nch, 1) * 3 * p.LP * 8);
     ws.partial.reserve((size_t)B * std::max(nch, 1) * p.LP * p.Tp * 8);
 
     ekgk::LeadFieldArgs la;
@@ -459,6 +459,7 @@ void enqueue_batch(Ctx& ctx, Heart& h, const BatchPlan& p, const double* d_layer
     fa.n_leads = p.L;
     fa.gsum = ws.gsum.as<double>();
     fa.chunks = Lo.chunks.as<int4>();
+    fa.vox_a64 = Lo.vox_a64.as<double>();
     fa.lc = ws.lc.as<ekgk::LayerConst>();
     fa.layer_k = d_layer_k;
     fa.rtab = ws.rtab.as<double>();

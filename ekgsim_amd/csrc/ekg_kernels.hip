@@ -375,49 +375,56 @@ __global__ __launch_bounds__(512) void lead_field_kernel(
     }
 }
 
-// Per (chunk, request): sum over the chunk's voxels of G (k0 enters the ECG
-// as k0 * sum_u G_u, constant in time), in a fixed order -- shuffle tree in
-// each wave, then wave order.  Requests whose every layer has k0 == 0 get
-// zeros without reading G.
+// Per (chunk, request) and lead, over the chunk's voxels u:
+//   sum G_u,   sum G_u P_u,   sum G_u Q_u     (FP64, P and Q as in lead_field)
+// k0 enters the ECG as k0 * sum G (constant in time); once every voxel of the
+// chunk is past its upstroke (sigmoid 1), the chunk adds exactly
+//   X(t) * sum G P + Y(t) * sum G Q
+// (the fully saturated tiles of the fused kernels).  Fixed summation order:
+// a strided pass per thread, a shuffle tree per wave, then wave order.
 template <int LP, bool kMfma>
-__global__ __launch_bounds__(256) void chunk_gsum_kernel(
-    const int4* __restrict__ chunks, int n_chunks, const double* __restrict__ layer_k,
-    int n_layers, int n_leads, int64_t V, const double* __restrict__ rec,
-    double* __restrict__ gsum)
+__global__ __launch_bounds__(256) void chunk_sums_kernel(
+    const int4* __restrict__ chunks, const double* __restrict__ vox_a64,
+    const double* __restrict__ layer_k, int n_layers, int n_leads, int64_t V,
+    const double* __restrict__ rec, int n_chunks, double* __restrict__ sums)
 {
     constexpr int RS = kMfma ? 16 : LP + 2;
-    __shared__ double s_wave[4][LP];
+    __shared__ double s_wave[4][3 * LP];
     const int c = blockIdx.x;
     const int b = blockIdx.y;
-    double* out = gsum + ((int64_t)b * n_chunks + c) * LP;
-    bool any = false;
-    for (int l = 0; l < n_layers; ++l) any |= layer_k[((int64_t)b * n_layers + l) * 9] != 0.0;
-    if (!any) {
-        if (threadIdx.x < LP) out[threadIdx.x] = 0.0;
-        return;
-    }
     const int4 ch = chunks[c];
+    const double* k = layer_k + ((int64_t)b * n_layers + (ch.x - 1)) * 9;
+    const double tref = (double)__int_as_float(ch.w);
+    const double alpha = k[2] * (1.0 - k[3]), beta = k[2] * k[3], k45 = k[4] + k[5], k5 = k[5];
     const double* recb = rec + (int64_t)b * (V + (kMfma ? MFMA_PAD : 1)) * RS;
-    double part[LP];
+    double part[3 * LP];
 #pragma unroll
-    for (int i = 0; i < LP; ++i) part[i] = 0.0;
+    for (int i = 0; i < 3 * LP; ++i) part[i] = 0.0;
     for (int v = ch.y + threadIdx.x; v < ch.z; v += blockDim.x) {
+        const double da = vox_a64[v] - tref;
+        const double P = alpha * exp(k45 * da), Q = beta * exp(k5 * da);
 #pragma unroll
         for (int i = 0; i < LP; ++i)
-            if (i < n_leads) part[i] += recb[(int64_t)v * RS + i];
+            if (i < n_leads) {
+                const double g = recb[(int64_t)v * RS + i];
+                part[i] += g;
+                part[LP + i] = __builtin_fma(g, P, part[LP + i]);
+                part[2 * LP + i] = __builtin_fma(g, Q, part[2 * LP + i]);
+            }
     }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-    for (int i = 0; i < LP; ++i) {
+    for (int i = 0; i < 3 * LP; ++i) {
         double x = part[i];
         for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
         if (lane == 0) s_wave[w][i] = x;
     }
     __syncthreads();
-    if (threadIdx.x < LP) {
+    double* out = sums + ((int64_t)b * n_chunks + c) * 3 * LP;
+    for (int i = threadIdx.x; i < 3 * LP; i += blockDim.x) {
         double sum = 0.0;
-        for (int ww = 0; ww < (int)(blockDim.x >> 6); ++ww) sum += s_wave[ww][threadIdx.x];
-        out[threadIdx.x] = sum;
+        for (int ww = 0; ww < (int)(blockDim.x >> 6); ++ww) sum += s_wave[ww][i];
+        out[i] = sum;
     }
 }
 
@@ -452,7 +459,7 @@ __global__ void ap_tables_kernel(const double* __restrict__ layer_k, int n_layer
             c.c5 = (float)(-k[5] * LOG2E);
             c.alpha = (float)(k[2] * (1.0 - k[3]));
             c.beta = (float)(k[2] * k[3]);
-            c.pad0 = 0.f;
+            c.skip = (k[1] > 0.0 && k[4] + k[5] <= 0.25 * k[1]) ? 1.f : 0.f;
             c.k0 = k[0];
             lc[bl] = c;
         }
@@ -463,13 +470,18 @@ __global__ void ap_tables_kernel(const double* __restrict__ layer_k, int n_layer
 // Fused AP evaluation + lead contraction
 // ---------------------------------------------------------------------------
 
-// Sigmoid bounds in the exp2 domain.  With x = c1*tau:
-//   x >= SKIP_X for every lane  -> exp2(x) = +inf, S = 1/(1+inf) = 0 exactly:
-//                                  the voxel adds exactly 0 to this tile;
+// Sigmoid bounds in the exp2 domain.  With x = c1*tau, c1 = -k1 log2(e):
+//   x >= SKIP_X for every lane  -> S < 2^-64: with k4 + k5 <= k1/4 (the
+//                                  LayerConst skip flag) the cell's AP is below
+//                                  k2 e^-33 and the voxel is skipped;
 //   x <= SAT_X for every lane   -> exp2(x) < 2^-24, 1 + exp2(x) rounds to 1,
-//                                  S = rcp(1) = 1 exactly: no sigmoid.
-constexpr float SKIP_X = 130.0f;
+//                                  S = rcp(1) = 1 exactly in FP32: no sigmoid;
+//   x <= SAT64_X                -> 1 + 2^x rounds to 1 in FP64 as well: the
+//                                  reference's sigmoid is exactly 1.
+constexpr float SKIP_X = 64.0f;
 constexpr float SAT_X = -24.5f;
+constexpr double SAT64_X = -54.0;
+constexpr double LOG2E_D = 1.4426950408889634074;
 // Largest lane factor kept; with S = 0 exactly in such lanes the product
 // stays 0 instead of 0 * inf.
 constexpr double XMAX = 1e30;
@@ -527,7 +539,7 @@ template <int LP>
 __global__ __launch_bounds__(256) void ecg_fused_kernel(
     const float2* __restrict__ vox_a, const double* __restrict__ rec,
     const double* __restrict__ gsum,
-    const int4* __restrict__ chunks, const LayerConst* __restrict__ lc,
+    const int4* __restrict__ chunks, const double* __restrict__ vox_a64, const LayerConst* __restrict__ lc,
     const double* __restrict__ layer_k, const double* __restrict__ rtab,
     const double* __restrict__ tgrid, const float2* __restrict__ thl,
     double* __restrict__ partial, int64_t V, int n_layers, int n_chunks, int Tp, int n_tiles,
@@ -545,18 +557,27 @@ __global__ __launch_bounds__(256) void ecg_fused_kernel(
     const int64_t bl = (int64_t)b * n_layers + (l - 1);
     const LayerConst kc = lc[bl];
     const int j = tile * 64 + lane;
+    const double* sums = gsum + ((int64_t)b * n_chunks + c) * 3 * LP;
+    double* out = partial + (((int64_t)b * n_chunks + c) * LP) * Tp + j;
+    const double* k = layer_k + bl * 9;
+    // per-lane time factors in FP64
+    const double R = rtab[bl * Tp + j];
+    const double dtr = tgrid[j] - (double)__int_as_float(ch.w);
+    const double X64 = exp(-(k[4] + k[5]) * dtr) * R, Y64 = exp(-k[5] * dtr) * R;
+    if (-k[1] * LOG2E_D * (tgrid[tile * 64] - vox_a64[ch.z - 1]) <= SAT64_X) {
+        // every voxel of the chunk has a sigmoid of exactly 1 in every lane:
+        // the chunk adds X(t) sum G P + Y(t) sum G Q (+ k0 sum G)
+#pragma unroll
+        for (int i = 0; i < LP; ++i)
+            out[(int64_t)i * Tp] = __builtin_fma(X64, sums[LP + i], __builtin_fma(Y64, sums[2 * LP + i], kc.k0 * sums[i]));
+        return;
+    }
     const float2 tt = thl[j];
     const float2 tf = thl[tile * 64];
     const float2 tl = thl[tile * 64 + 63];
-    // per-lane time factors, FP64 then rounded once
-    float X, Y;
-    {
-        const double* k = layer_k + bl * 9;
-        const double R = rtab[bl * Tp + j];
-        const double dtr = tgrid[j] - (double)__int_as_float(ch.w);
-        X = (float)fmin(exp(-(k[4] + k[5]) * dtr) * R, XMAX);
-        Y = (float)fmin(exp(-k[5] * dtr) * R, XMAX);
-    }
+    // FP32 copies for the voxel loops
+    const float X = (float)fmin(X64, XMAX);
+    const float Y = (float)fmin(Y64, XMAX);
     constexpr int RS = LP + 2;
     const double* Rb = rec + (int64_t)b * (V + 1) * RS;
 
@@ -572,10 +593,11 @@ __global__ __launch_bounds__(256) void ecg_fused_kernel(
             return kc.c1 * tau_of(tf.x, tf.y, a.x, a.y) <= SAT_X;
         });
         // non-zero range ends where min over lanes of c1*tau >= SKIP_X (min at the last lane)
-        s2 = first_false(s1, ch.z, [&](int v) {
-            const float2 a = vox_a[v];
-            return kc.c1 * tau_of(tl.x, tl.y, a.x, a.y) < SKIP_X;
-        });
+        if (kc.skip != 0.0f)
+            s2 = first_false(s1, ch.z, [&](int v) {
+                const float2 a = vox_a[v];
+                return kc.c1 * tau_of(tl.x, tl.y, a.x, a.y) < SKIP_X;
+            });
     }
 
     // Both loops keep the next voxel's record in flight while the current
@@ -625,10 +647,8 @@ __global__ __launch_bounds__(256) void ecg_fused_kernel(
     }
 
     // k0 is constant in time: sum_u G_u (ap_u + k0) = sum_u G_u ap_u + k0 * sum_u G_u.
-    const double* gs = gsum + ((int64_t)b * n_chunks + c) * LP;
-    double* out = partial + (((int64_t)b * n_chunks + c) * LP) * Tp + j;
 #pragma unroll
-    for (int i = 0; i < LP; ++i) out[(int64_t)i * Tp] = __builtin_fma(kc.k0, gs[i], acc[i]);
+    for (int i = 0; i < LP; ++i) out[(int64_t)i * Tp] = __builtin_fma(kc.k0, sums[i], acc[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -652,7 +672,7 @@ template <int LP>
 __global__ __launch_bounds__(1024) void ecg_fused_mfma_kernel(
     const float2* __restrict__ vox_a, const double* __restrict__ G16,
     const float4* __restrict__ aux, const double* __restrict__ gsum,
-    const int4* __restrict__ chunks, const LayerConst* __restrict__ lc,
+    const int4* __restrict__ chunks, const double* __restrict__ vox_a64, const LayerConst* __restrict__ lc,
     const double* __restrict__ layer_k, const double* __restrict__ rtab,
     const double* __restrict__ tgrid, const float2* __restrict__ thl,
     double* __restrict__ partial, int64_t V, int n_layers, int n_chunks, int Tp, int n_tiles,
@@ -674,8 +694,26 @@ __global__ __launch_bounds__(1024) void ecg_fused_mfma_kernel(
     const int kv = lane >> 4;   // voxel slot of the B operand / lead group of D
     const double* Gb = G16 + (int64_t)b * (V + MFMA_PAD) * 16;
     const float4* Ab = aux + (int64_t)b * (V + MFMA_PAD);
+    const double* sums = gsum + ((int64_t)b * n_chunks + c) * 3 * LP;
+    double* out = partial + ((int64_t)b * n_chunks + c) * LP * Tp;
+    const double c1d = -kk[1] * LOG2E_D;
+    const double a_last = vox_a64[ch.z - 1];
 
     for (int tile = w; tile < n_tiles; tile += nw) {
+        if (c1d * (tgrid[tile * 64] - a_last) <= SAT64_X) {
+            // every voxel's sigmoid is exactly 1 in every lane: the chunk adds
+            // X(t) sum G P + Y(t) sum G Q (+ k0 sum G), one time step per lane
+            const int j = tile * 64 + lane;
+            const double R = rtab[bl * Tp + j];
+            const double dtr = tgrid[j] - tref;
+            const double X64 = exp(-k45 * dtr) * R, Y64 = exp(-k5 * dtr) * R;
+#pragma unroll
+            for (int i = 0; i < LP; ++i)
+                if (i < n_leads)
+                    out[(int64_t)i * Tp + j] =
+                        __builtin_fma(X64, sums[LP + i], __builtin_fma(Y64, sums[2 * LP + i], kc.k0 * sums[i]));
+            continue;
+        }
         float X[4], Y[4], TH[4], TL[4];
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
@@ -696,10 +734,11 @@ __global__ __launch_bounds__(1024) void ecg_fused_mfma_kernel(
                 const float2 a = vox_a[v];
                 return kc.c1 * tau_of(tf.x, tf.y, a.x, a.y) <= SAT_X;
             });
-            s2 = first_false(s1, ch.z, [&](int v) {
-                const float2 a = vox_a[v];
-                return kc.c1 * tau_of(tl.x, tl.y, a.x, a.y) < SKIP_X;
-            });
+            if (kc.skip != 0.0f)
+                s2 = first_false(s1, ch.z, [&](int v) {
+                    const float2 a = vox_a[v];
+                    return kc.c1 * tau_of(tl.x, tl.y, a.x, a.y) < SKIP_X;
+                });
         }
         mfma_f64x4 acc[4];
 #pragma unroll
@@ -741,13 +780,11 @@ __global__ __launch_bounds__(1024) void ecg_fused_mfma_kernel(
             }
         }
         // D(i, j) sits in lane 16(i%4)+j, register i/4: lead i = 4r + lane/16
-        const double* gs = gsum + ((int64_t)b * n_chunks + c) * LP;
-        double* out = partial + ((int64_t)b * n_chunks + c) * LP * Tp;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int i = 4 * r + kv;
             if (i < n_leads && i < LP) {
-                const double g = gs[i];
+                const double g = sums[i];
 #pragma unroll
                 for (int m = 0; m < 4; ++m)
                     out[(int64_t)i * Tp + tile * 64 + 16 * m + jl] = __builtin_fma(kc.k0, g, acc[m][r]);
@@ -849,8 +886,9 @@ static void lf_launch_t(const LeadFieldArgs& a, hipStream_t s)
     case 2: lf_launch_nh<LP, M, 2>(a, s); break;
     default: lf_launch_nh<LP, M, 3>(a, s); break;
     }
-    hipLaunchKernelGGL((chunk_gsum_kernel<LP, M>), dim3(a.n_chunks, a.n_req), dim3(256), 0, s,
-                       a.chunks, a.n_chunks, a.layer_k, a.n_layers, a.n_leads, a.V, a.rec, a.gsum);
+    hipLaunchKernelGGL((chunk_sums_kernel<LP, M>), dim3(a.n_chunks, a.n_req), dim3(256), 0, s,
+                       a.chunks, a.vox_a64, a.layer_k, a.n_layers, a.n_leads, a.V, a.rec, a.n_chunks,
+                       a.gsum);
 }
 
 template <int LP>
@@ -885,7 +923,7 @@ template <int LP>
 static void fused_launch(const FusedArgs& a, hipStream_t s)
 {
     hipLaunchKernelGGL(ecg_fused_kernel<LP>, dim3(a.n_chunks * a.n_groups, a.n_req), dim3(256), 0,
-                       s, a.vox_a, a.rec, a.gsum, a.chunks, a.lc, a.layer_k, a.rtab, a.tgrid,
+                       s, a.vox_a, a.rec, a.gsum, a.chunks, a.vox_a64, a.lc, a.layer_k, a.rtab, a.tgrid,
                        a.thl, a.partial, a.V, a.n_layers, a.n_chunks, a.Tp, a.n_tiles, a.n_groups);
 }
 
@@ -894,7 +932,7 @@ static void fused_mfma_launch(const FusedArgs& a, hipStream_t s)
 {
     const int nw = a.n_tiles < 16 ? a.n_tiles : 16;
     hipLaunchKernelGGL(ecg_fused_mfma_kernel<LP>, dim3(a.n_chunks, a.n_req), dim3(64 * nw), 0, s,
-                       a.vox_a, a.rec, a.aux, a.gsum, a.chunks, a.lc, a.layer_k, a.rtab, a.tgrid,
+                       a.vox_a, a.rec, a.aux, a.gsum, a.chunks, a.vox_a64, a.lc, a.layer_k, a.rtab, a.tgrid,
                        a.thl, a.partial, a.V, a.n_layers, a.n_chunks, a.Tp, a.n_tiles, a.n_leads);
 }
 

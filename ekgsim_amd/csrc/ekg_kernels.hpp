@@ -18,7 +18,8 @@ template <> struct CaT<true> { using type = int; };
 //   V(tau) = S * (alpha*exp2(c4*tau) + beta) * exp2(c5*tau) * R(t) + k0,
 //   S = 1/(1 + exp2(c1*tau)),  c_i = -k_i*log2(e),  alpha = k2(1-k3), beta = k2*k3.
 struct LayerConst {
-    float c1, c4, c5, alpha, beta, pad0;
+    float c1, c4, c5, alpha, beta;
+    float skip;  // 1: voxels far before their upstroke may be skipped (k4 + k5 <= k1 / 4)
     double k0;
 };
 static_assert(sizeof(LayerConst) == 32, "LayerConst layout");
@@ -48,7 +49,7 @@ struct LeadFieldArgs {
     double* rec;                 // VALU: [n_req][V + 1][LP + 2] voxel records (G, P/Q, a)
                                  // MFMA: [n_req][V + MFMA_PAD][16] G rows
     float4* aux;                 // MFMA: [n_req][V + MFMA_PAD] {P, Q, a_hi, a_lo}
-    double* gsum;                // [n_req][n_chunks][LP]
+    double* gsum;                // [n_req][n_chunks][3][LP]: sum G, sum G P, sum G Q
     bool mfma;
 };
 
@@ -56,8 +57,9 @@ struct FusedArgs {
     const float2* vox_a;         // [V] activation time (hi, lo)
     const double* rec;           // see LeadFieldArgs::rec
     const float4* aux;           // MFMA only
-    const double* gsum;          // [n_req][n_chunks][LP]
+    const double* gsum;          // [n_req][n_chunks][3][LP]: sum G, sum G P, sum G Q
     const int4* chunks;          // [n_chunks]
+    const double* vox_a64;       // [V] activation time, FP64
     const LayerConst* lc;        // [n_req][n_layers]
     const double* layer_k;       // [n_req][n_layers][9]
     const double* rtab;          // [n_req][n_layers][Tp]

@@ -186,6 +186,8 @@ def test_cli_matches_reference_program(case, tmp_path):
             h0, r0 = read_column(os.path.join(exp_dir, fn))
             assert h1 == h0
             assert r1.shape == r0.shape
+            if r0.size == 0:  # simulation skipped: header only
+                continue
             assert np.array_equal(r1[:, 0], r0[:, 0])
             for i in range(1, r0.shape[1]):
                 e = rel_l2(r1[:, i], r0[:, i])
