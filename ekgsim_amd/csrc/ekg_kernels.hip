@@ -397,30 +397,67 @@ __global__ __launch_bounds__(256) void chunk_sums_kernel(
     const double tref = (double)__int_as_float(ch.w);
     const double alpha = k[2] * (1.0 - k[3]), beta = k[2] * k[3], k45 = k[4] + k[5], k5 = k[5];
     const double* recb = rec + (int64_t)b * (V + (kMfma ? MFMA_PAD : 1)) * RS;
-    double part[3 * LP];
-#pragma unroll
-    for (int i = 0; i < 3 * LP; ++i) part[i] = 0.0;
-    for (int v = ch.y + threadIdx.x; v < ch.z; v += blockDim.x) {
-        const double da = vox_a64[v] - tref;
-        const double P = alpha * exp(k45 * da), Q = beta * exp(k5 * da);
-#pragma unroll
-        for (int i = 0; i < LP; ++i)
-            if (i < n_leads) {
-                const double g = recb[(int64_t)v * RS + i];
-                part[i] += g;
-                part[LP + i] = __builtin_fma(g, P, part[LP + i]);
-                part[2 * LP + i] = __builtin_fma(g, Q, part[2 * LP + i]);
-            }
-    }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double* out = sums + ((int64_t)b * n_chunks + c) * 3 * LP;
+    if constexpr (kMfma) {
+        // 16-double rows: a wave loads 4 whole rows per instruction (lane =
+        // 16 * row slot + lead), so every load is one coalesced 512-byte
+        // read.  For the wave's 16 rows, lanes 0-15 compute P and lanes 16-31
+        // Q in one exp (lanes 32-63 repeat them), broadcast by shuffles.
+        const int col = lane & 15, slot = lane >> 4;
+        const bool is_q = (lane >> 4) & 1;
+        const double coef = is_q ? beta : alpha, rate = is_q ? k5 : k45;
+        double sg = 0.0, sp = 0.0, sq = 0.0;
+        for (int v0 = ch.y + 16 * w; v0 < ch.z; v0 += 64) {
+            const int ve = v0 + col;
+            const double e = ve < ch.z ? coef * exp(rate * (vox_a64[ve] - tref)) : 0.0;
 #pragma unroll
-    for (int i = 0; i < 3 * LP; ++i) {
-        double x = part[i];
-        for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
-        if (lane == 0) s_wave[w][i] = x;
+            for (int r = 0; r < 4; ++r) {
+                const int row = 4 * r + slot;
+                const int v = v0 + row;
+                const double g = v < ch.z ? recb[(int64_t)v * 16 + col] : 0.0;
+                const double P = __shfl(e, row, 64), Q = __shfl(e, 16 + row, 64);
+                sg += g;
+                sp = __builtin_fma(g, P, sp);
+                sq = __builtin_fma(g, Q, sq);
+            }
+        }
+        // combine the 4 row slots of the wave (fixed order), then the waves
+        sg += __shfl_down(sg, 16, 64);
+        sp += __shfl_down(sp, 16, 64);
+        sq += __shfl_down(sq, 16, 64);
+        sg += __shfl_down(sg, 32, 64);
+        sp += __shfl_down(sp, 32, 64);
+        sq += __shfl_down(sq, 32, 64);
+        if (lane < LP) {
+            s_wave[w][lane] = sg;
+            s_wave[w][LP + lane] = sp;
+            s_wave[w][2 * LP + lane] = sq;
+        }
+    } else {
+        double part[3 * LP];
+#pragma unroll
+        for (int i = 0; i < 3 * LP; ++i) part[i] = 0.0;
+        for (int v = ch.y + threadIdx.x; v < ch.z; v += blockDim.x) {
+            const double da = vox_a64[v] - tref;
+            const double P = alpha * exp(k45 * da), Q = beta * exp(k5 * da);
+#pragma unroll
+            for (int i = 0; i < LP; ++i)
+                if (i < n_leads) {
+                    const double g = recb[(int64_t)v * RS + i];
+                    part[i] += g;
+                    part[LP + i] = __builtin_fma(g, P, part[LP + i]);
+                    part[2 * LP + i] = __builtin_fma(g, Q, part[2 * LP + i]);
+                }
+        }
+#pragma unroll
+        for (int i = 0; i < 3 * LP; ++i) {
+            double x = part[i];
+            for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+            if (lane == 0) s_wave[w][i] = x;
+        }
     }
     __syncthreads();
-    double* out = sums + ((int64_t)b * n_chunks + c) * 3 * LP;
     for (int i = threadIdx.x; i < 3 * LP; i += blockDim.x) {
         double sum = 0.0;
         for (int ww = 0; ww < (int)(blockDim.x >> 6); ++ww) sum += s_wave[ww][i];
