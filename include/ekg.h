@@ -4,7 +4,7 @@
  * The reference (synergy-twinning/ekgsim) exposes the simulation core as the
  * C++ class SimLib::EkgSim (simlib/sim_lib.h:93-287) over SimLib::Simulation
  * (simlib/simulator.h:508-586).  This header is the thin C layer a binding
- * (ctypes, cgo, JNI, N-API) or the C++ facade in ekgsim_amd/csrc/ekgsim.hpp
+ * (ctypes, cgo, JNI, N-API) or the C++ facade in ekgsim_amd/csrc/sim_lib.hpp
  * calls.  Conventions:
  *   - every call returns EKG_OK (0) or an error code; the message of the last
  *     failure on the calling thread is ekg_last_error();
