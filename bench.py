@@ -31,12 +31,30 @@ from ekgsim_amd import synth as S  # noqa: E402  (no GPU use)
 
 METRIC = ("simulated ECGs/sec at 128^3, 600 ms, 12 leads (batched vs plain) at "
           "≤1e-5 rel-L2; p99")
-# gfx950 VALU issue rate: 256 CU x 4 SIMD x 16 lanes x 2.4 GHz (lane-cycles/s)
-VALU_PEAK = 256 * 4 * 16 * 2.4e9
-# Algorithmic VALU work per voxel-step at L leads: 4 transcendentals at
-# quarter rate (16 lane-cycles) + 10 FP32 ops + L FMAs.
-def lane_cycles_per_voxel_step(L):
-    return 16 + 10 + L
+# MI355X dense FP64 matrix-core peak (the lead contraction's precision), FLOP/s
+MFMA_FP64_PEAK = 78.6e12
+# Sigmoid window of the fused kernel in the exp2 domain x = c1 (t - a),
+# c1 = -k1 log2(e): outside (SAT64_X, SKIP_X) a voxel's AP is an exact closed
+# form (sigmoid 1) or below 2^-64 of its plateau, so only window voxel-steps
+# need an individually evaluated AP.
+SAT64_X, SKIP_X = -54.0, 64.0
+
+
+def window_voxel_steps(act, k1, start, dt, T):
+    """Voxel-steps (u, j) with SAT64_X < c1 (t_j - a_u) < SKIP_X, t_j = start + j dt."""
+    c1 = -k1 * math.log2(math.e)
+    lo = act + SKIP_X / c1          # t > a - 64/|c1|
+    hi = act + SAT64_X / c1         # t < a + 54/|c1|
+    jlo = np.clip(np.floor((lo - start) / dt) + 1, 0, T)
+    jhi = np.clip(np.ceil((hi - start) / dt), 0, T)
+    return int(np.maximum(jhi - jlo, 0).sum())
+
+
+# HBM bytes per launch of the fused kernel from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE, each scaled by the counter's reading for a copy kernel of known
+# size (profiles/ecg_fused_mfma_128cubed_batch256_hbm_traffic.md); None until
+# measured.
+TRAFFIC_BYTES = None
 
 
 def log(*a):
@@ -187,10 +205,13 @@ def main():
     ms_step = el * 1e3 / args.steps
     value = B * args.steps / el
     avg_kernel_s = kms / 1e3 / max(nl, 1)
-    vsteps = B * V * T
-    achieved = vsteps * lane_cycles_per_voxel_step(L) / avg_kernel_s
-    log(f"batched: {ms_step:.2f} ms/step, {value:.1f} ECGs/s; fused kernel {avg_kernel_s * 1e3:.2f} ms "
-        f"({vsteps / avg_kernel_s:.3e} voxel-steps/s)")
+    act_grid = heart.activation()
+    act_myo = act_grid[h.layer > 0]
+    win = sum(window_voxel_steps(act_myo, float(k[0][1]), 0.0, 1.0, T) for k in ks)
+    flops = 2.0 * L * win
+    achieved = flops / avg_kernel_s
+    log(f"batched: {ms_step:.2f} ms/step, {value:.1f} ECGs/s; fused kernel {avg_kernel_s * 1e3:.2f} ms; "
+        f"{win / B / V:.1f} window steps per voxel; {achieved / 1e12:.2f} TFLOP/s FP64 contraction")
     if args.profile_only:
         print(json.dumps({"ms_per_step": ms_step, "fused_ms": avg_kernel_s * 1e3}))
         return
@@ -289,14 +310,16 @@ def main():
         "rel_l2_vs_reference_windows": ref_win_err,
         "excitation_ms": round(ca_ms, 2),
         "roofline": {
-            "bound": "valu",
-            "kernel": "ecg_fused",
-            "achieved": round(achieved / 1e12, 3),
-            "peak": round(VALU_PEAK / 1e12, 3),
-            "unit": "T lane-cycles/s",
-            "frac": round(achieved / VALU_PEAK, 4),
-            "traffic": None,
-            "voxel_steps_per_s": vsteps / avg_kernel_s,
+            "bound": "mfma",
+            "kernel": "ecg_fused_mfma_kernel" if args.kernel != "valu" else "ecg_fused_kernel",
+            "achieved": round(achieved / 1e12, 4),
+            "peak": MFMA_FP64_PEAK / 1e12,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / MFMA_FP64_PEAK, 5),
+            "traffic": TRAFFIC_BYTES,
+            "algorithmic_flops_per_launch": flops,
+            "window_voxel_steps_per_launch": win,
+            "flops_per_window_voxel_step": 2 * L,
             "kernel_ms": round(avg_kernel_s * 1e3, 4),
             "launches": nl,
         },

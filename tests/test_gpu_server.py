@@ -1,0 +1,138 @@
+"""Continuous batching: every request's ECG is the same alone and in the mix.
+
+Requests of several hearts and shapes are submitted together, while others
+run, and under an HBM budget that forces small groups; each result must equal
+the plain one-call simulation of the same request bit for bit (every
+request's partial sums come from its own rows over the heart's fixed
+chunking, so grouping cannot change them), and the plain call must be within
+rel-L2 1e-5 of the FP64 oracle.
+"""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from ekgsim_amd import synth as S
+from oracle import oracle as O
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _hearts(ctx, specs):
+    from ekgsim_amd import Heart
+    out = []
+    for n, seed in specs:
+        h = S.make_heart(n, seed=seed)
+        H = Heart(ctx, h.layer, h.seed)
+        H.excite_delay(h.delay)
+        out.append((h, H))
+    return out
+
+
+def _requests(hearts, n_req, seed):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for r in range(n_req):
+        hi = int(rng.integers(0, len(hearts)))
+        h, _ = hearts[hi]
+        L = int(rng.choice([2, 3, 6]))
+        length = float(rng.choice([200.0, 300.0]))
+        x = S.random_params(3, L, 7000 + seed * 100 + r)
+        k = S.layer_k_from_params(x, 3)
+        pos = S.move_leads(S.lead_positions(h.layer.shape[0], L), S.displacements(x, 3, L))
+        reqs.append((hi, k, pos, length))
+    return reqs
+
+
+def _alone(hearts, req):
+    hi, k, pos, length = req
+    return hearts[hi][1].simulate(k, pos, "3D4", 0.0, length, 1.0)
+
+
+def test_mixed_burst_equals_alone(ctx):
+    from ekgsim_amd import Server
+    hearts = _hearts(ctx, [(24, 1), (32, 2), (20, 3)])
+    reqs = _requests(hearts, 48, 1)
+    srv = Server(ctx, max_batch=8, n_streams=3)
+    ids = [srv.add_heart(H) for _, H in hearts]
+    tickets = [srv.submit(ids[hi], k, pos, "3D4", 0.0, length, 1.0) for hi, k, pos, length in reqs]
+    got = [srv.wait(t)[0] for t in tickets]
+    st = srv.stats()
+    srv.close()
+    for req, g in zip(reqs, got):
+        assert np.array_equal(g, _alone(hearts, req))
+    assert st["completed"] == len(reqs)
+    assert st["groups"] < len(reqs)      # requests were batched
+    assert st["peak_inflight"] > 1       # and ran concurrently
+    # the plain call itself is the checked drop-in
+    hi, k, pos, length = reqs[0]
+    h = hearts[hi][0]
+    act = O.excite_int(h.layer, h.seed, h.delay).astype(np.float64)
+    assert rel_l2(got[0], O.run(h.layer, act, k, pos, "3D4", 0.0, length, 1.0)) <= 1e-5
+
+
+def test_arrivals_while_running_equal_alone(ctx):
+    """A producer thread keeps submitting while earlier groups are on the
+    GPU; results are collected in completion order."""
+    from ekgsim_amd import Server
+    hearts = _hearts(ctx, [(28, 4), (24, 5)])
+    reqs = _requests(hearts, 40, 2)
+    srv = Server(ctx, max_batch=4, n_streams=2)
+    ids = [srv.add_heart(H) for _, H in hearts]
+    tickets = [None] * len(reqs)
+    done = threading.Event()
+
+    def produce():
+        for i, (hi, k, pos, length) in enumerate(reqs):
+            tickets[i] = srv.submit(ids[hi], k, pos, "3D4", 0.0, length, 1.0)
+            if i % 5 == 4:
+                time.sleep(0.002)
+        done.set()
+
+    th = threading.Thread(target=produce)
+    th.start()
+    got = {}
+    while len(got) < len(reqs):
+        for i, t in enumerate(tickets):
+            if t is not None and i not in got and srv.poll(t):
+                got[i] = srv.wait(t)[0]
+        time.sleep(0.0005)
+    th.join()
+    assert done.is_set()
+    srv.close()
+    for i, req in enumerate(reqs):
+        assert np.array_equal(got[i], _alone(hearts, req))
+
+
+def test_hbm_budget_limits_groups(ctx):
+    """A budget of about two requests' device memory: groups shrink to fit,
+    in-use bytes never exceed the budget, results are unchanged; a request
+    larger than the whole budget fails with ENOMEM instead of running."""
+    from ekgsim_amd import EkgError, Server
+    hearts = _hearts(ctx, [(24, 6)])
+    reqs = [(0, k, pos, 300.0) for _, k, pos, _ in _requests(hearts, 12, 3)]
+    probe = Server(ctx, max_batch=1, n_streams=1)
+    hid = probe.add_heart(hearts[0][1])
+    t = probe.submit(hid, reqs[0][1], reqs[0][2], "3D4", 0.0, 300.0, 1.0)
+    probe.wait(t)
+    one = probe.stats()["peak_bytes"]
+    probe.close()
+    assert one > 0
+    budget = int(2.5 * one)
+    srv = Server(ctx, hbm_budget_bytes=budget, max_batch=8, n_streams=3)
+    hid = srv.add_heart(hearts[0][1])
+    tickets = [srv.submit(hid, k, pos, "3D4", 0.0, length, 1.0) for _, k, pos, length in reqs]
+    got = [srv.wait(t)[0] for t in tickets]
+    st = srv.stats()
+    assert st["peak_bytes"] <= budget
+    assert st["groups"] >= len(reqs) / 2
+    for req, g in zip(reqs, got):
+        assert np.array_equal(g, _alone(hearts, req))
+    # one request alone larger than the budget
+    big = _requests(hearts, 1, 4)[0]
+    t = srv.submit(hid, big[1], big[2], "3D4", 0.0, 50000.0, 1.0)
+    with pytest.raises(EkgError):
+        srv.wait(t)
+    srv.close()
