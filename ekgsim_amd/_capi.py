@@ -374,6 +374,21 @@ class Heart:
         return out
 
 
+    def simulate_many(self, requests, nhood="3D4"):
+        """One launch group of requests that may differ in lead count, layer
+        count, start, length and time step: requests = [(layer_k, lead_pos,
+        start, length, dt), ...] -> list of (L, T) arrays."""
+        ds = [_Desc(k, p, nhood, st, ln, dt) for (k, p, st, ln, dt) in requests]
+        arr = (SimDesc * len(ds))(*[d.s for d in ds])
+        shapes = [(d.p.shape[0], num_steps(r[3], r[4])) for d, r in zip(ds, requests)]
+        out = np.zeros(sum(L * T for L, T in shapes), np.float64)
+        check(lib().ekg_simulate_batch(self.h, len(ds), arr, _p(out)))
+        res, o = [], 0
+        for L, T in shapes:
+            res.append(out[o:o + L * T].reshape(L, T).copy())
+            o += L * T
+        return res
+
     def simulate_batch_device(self, layer_k_ptr, lead_pos_ptr, ecg_ptr, batch, n_layers, n_leads,
                               nhood="3D4", start=0.0, length=600.0, dt=1.0):
         """Enqueue a batch whose arrays are device pointers (ints), e.g. the

@@ -1,0 +1,942 @@
+// ekg_ecg.hip -- CDNA4 (gfx950) kernels of the ECG path.
+//
+// A launch group holds requests of any cached hearts, lead counts and time
+// grids (ekg_kernels.hpp: GroupArgs).  Work items are enumerated per request
+// from exclusive prefix sums, so one launch of each kernel covers the group:
+//
+//   ap_tables     per (request, layer, step): X_l(t), Y_l(t) -- the
+//                 time factors of the Wohlfart AP (Wohlfart.h:195-203) with
+//                 the repolarisation factor R_l(t) -- in FP64, and the
+//                 per-layer FP32 constants
+//   lead_field    per (request, 8^3 brick): lead weights G_i(u) in FP64
+//                 (the reference's dipole sum regrouped by cell,
+//                 simulator.cpp:509-537), per-voxel AP factors P_u, Q_u
+//   chunk_sums    per (request, chunk): sum G, sum G P', sum G Q' in FP64
+//                 and the chunk's tile bounds [z, f): before z every voxel
+//                 is far before its upstroke, from f on every sigmoid is 1
+//   layer_prefix  per (request, layer, tile): prefix sums of the saturated
+//                 chunks and the range of chunks still in transition
+//   ecg_fused     per (request, chunk), transition tiles only: AP
+//                 evaluation fused with the lead contraction (FP32 AP, FP64
+//                 accumulation on the matrix cores or the vector ALUs)
+//   ecg_reduce    per (request, lead, step): closed form of the saturated
+//                 chunks + fixed-order FP64 sum of the transition partials
+//
+// Every launch is checked on the host against the buffer extents it assumes
+// (ekg_engine.cpp) before it is issued.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ekg_kernels.hpp"
+
+namespace ekgk {
+
+// Request owning work item x: the largest r with start[r] <= x (requests
+// without work items share their start with the next one).
+__device__ __forceinline__ int find_req(const int64_t* __restrict__ start, int B, int64_t x)
+{
+    int lo = 0, hi = B;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (start[mid] <= x)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Sigmoid bounds in the exp2 domain.  With x = c1*tau, c1 = -k1 log2(e):
+//   x >= SKIP_X for every lane  -> S < 2^-64: with k4 + k5 <= k1/4 (the
+//                                  LayerConst skip flag) the cell's AP is below
+//                                  k2 e^-33 and the voxel is skipped;
+//   x <= SAT_X for every lane   -> exp2(x) < 2^-24, 1 + exp2(x) rounds to 1,
+//                                  S = rcp(1) = 1 exactly in FP32: no sigmoid;
+//   x <= SAT64_X                -> 1 + 2^x rounds to 1 in FP64 as well: the
+//                                  reference's sigmoid is exactly 1.
+constexpr float SKIP_X = 64.0f;
+constexpr float SAT_X = -24.5f;
+constexpr double SAT64_X = -54.0;
+constexpr double LOG2E_D = 1.4426950408889634074;
+// Largest FP32 lane factor kept; with S = 0 exactly in such lanes the
+// product stays 0 instead of 0 * inf.
+constexpr double XMAX = 1e30;
+
+__device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// tau = t - a from hi/lo FP32 pairs.
+__device__ __forceinline__ float tau_of(float th, float tl, float ah, float al)
+{
+    return (th - ah) + (tl - al);
+}
+
+// First index in [lo, hi) whose predicate is false, for a predicate that is
+// true on a prefix of the range.  Wave-cooperative 64-ary search: each lane
+// tests one probe, the ballot's count of true probes narrows the range to one
+// stride; ranges up to 4096 take two rounds.  The result is wave-uniform.
+template <class P>
+__device__ __forceinline__ int first_false(int lo, int hi, P pred)
+{
+    const int lane = threadIdx.x & 63;
+    while (hi - lo > 0) {
+        const int n = hi - lo;
+        const int stride = (n + 63) >> 6;
+        const int v = lo + lane * stride;
+        const bool t = v < hi && pred(v);
+        const int c = __popcll(__ballot(t));  // probes lo, lo+stride, ... true for the first c
+        if (stride == 1) return lo + c;
+        if (c == 0) return lo;
+        const int nlo = lo + (c - 1) * stride + 1;
+        const int nhi = lo + c * stride < hi ? lo + c * stride + 1 : hi;
+        lo = nlo;
+        hi = nhi;
+    }
+    return lo;
+}
+
+// First index in [0, n) where a monotone (false...true) predicate holds, n if none.
+template <class P>
+__device__ __forceinline__ int first_true(int n, P pred)
+{
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (pred(mid))
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------------------
+// AP tables
+// ---------------------------------------------------------------------------
+
+// Per (request, layer l, padded step j), with t0 the layer's reference time:
+//   R(t) = 1 - (1 + exp(-k7 (t - k8) + log(2^(k7/k6) - 1)))^(-k6/k7)
+//   X_l(t) = e^{-(k4+k5)(t - t0)} R(t),   Y_l(t) = e^{-k5 (t - t0)} R(t)
+// R is the repolarisation factor of Wohlfart.h:200-201.  A cell activated at
+// a is evaluated at tau = t - a with k8 - a (simulator.cpp:154-158), and
+// tau - (k8 - a) = t - k8: R depends on the layer and t only.  Step j == 0
+// also writes the per-layer FP32 constants.
+__global__ void ap_tables_kernel(const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs,
+                                 int B, const int64_t* __restrict__ xy_start, int64_t n_xy,
+                                 const double* __restrict__ layer_k, const double* __restrict__ tgrid,
+                                 double2* __restrict__ xy, LayerConst* __restrict__ lc)
+{
+    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n_xy;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int r = find_req(xy_start, B, idx);
+        const ReqDev R = reqs[r];
+        const int64_t loc = idx - xy_start[r];
+        const int l = (int)(loc / R.Tp) + 1;
+        const int j = (int)(loc % R.Tp);
+        const HeartDev& H = hearts[R.heart];
+        const double* k = layer_k + R.k_off + 9 * (l - 1);
+        const double t = tgrid[R.t_off + j];
+        const double t0 = l <= H.max_layer ? H.layer_t0[l] : 0.0;
+        const double Rt =
+            1.0 - pow(1.0 + exp(-k[7] * (t - k[8]) + log(pow(2.0, k[7] / k[6]) - 1.0)), -(k[6] / k[7]));
+        xy[R.xy_off + loc] = double2{exp(-(k[4] + k[5]) * (t - t0)) * Rt, exp(-k[5] * (t - t0)) * Rt};
+        if (j == 0) {
+            LayerConst c;
+            c.c1 = (float)(-k[1] * LOG2E_D);
+            c.c4 = (float)(-k[4] * LOG2E_D);
+            c.c5 = (float)(-k[5] * LOG2E_D);
+            c.alpha = (float)(k[2] * (1.0 - k[3]));
+            c.beta = (float)(k[2] * k[3]);
+            c.skip = (k[1] > 0.0 && k[4] + k[5] <= 0.25 * k[1]) ? 1.f : 0.f;
+            c.k0 = k[0];
+            lc[R.lay_off + l - 1] = c;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Lead field
+// ---------------------------------------------------------------------------
+
+// f(e) = e * (1/(l*sqrt(l))), l = |e|^2: the lead weight m of simulator.cpp:531-533
+// for e = p - (u+1).  1/(l*sqrt(l)) = rsq(l)^3 with two Newton steps on the
+// hardware FP64 reciprocal square root (quadratic convergence: a few ulp).
+__device__ __forceinline__ void lead_f(double e0, double e1, double e2, double& m0, double& m1,
+                                       double& m2)
+{
+    double l = e0 * e0;
+    l = __builtin_fma(e1, e1, l);
+    l = __builtin_fma(e2, e2, l);
+    double r = __builtin_amdgcn_rsq(l);
+    const double h = 0.5 * l;
+    r = r * __builtin_fma(-h * r, r, 1.5);
+    r = r * __builtin_fma(-h * r, r, 1.5);
+    const double s = r * r * r;
+    m0 = e0 * s;
+    m1 = e1 * s;
+    m2 = e2 * s;
+}
+
+constexpr int LB = 8;                        // lead-field brick edge
+constexpr int LH = LB + 2;                   // with halo
+constexpr int LN = LH * LH * LH;             // halo cells
+
+// Dipole neighbourhoods at compile time, in Neighbourhood::create order
+// (simulator.h:325-390): NH = 0 2D4, 1 2D8, 2 3D4, 3 3D8.
+template <int NH> struct Nhood;
+template <> struct Nhood<0> {
+    static constexpr int n = 4;
+    static constexpr int8_t d[4][3] = {{0, -1, -1}, {0, -1, 1}, {0, 1, -1}, {0, 1, 1}};
+};
+template <> struct Nhood<1> {
+    static constexpr int n = 8;
+    static constexpr int8_t d[8][3] = {{0, -1, -1}, {0, -1, 0}, {0, -1, 1}, {0, 0, -1},
+                                       {0, 0, 1},   {0, 1, -1}, {0, 1, 0},  {0, 1, 1}};
+};
+template <> struct Nhood<2> {
+    static constexpr int n = 8;
+    static constexpr int8_t d[8][3] = {{-1, -1, -1}, {-1, -1, 1}, {-1, 1, -1}, {-1, 1, 1},
+                                       {1, -1, -1},  {1, -1, 1},  {1, 1, -1},  {1, 1, 1}};
+};
+template <> struct Nhood<3> {
+    static constexpr int n = 26;
+    static constexpr int8_t d[26][3] = {
+        {-1, -1, -1}, {-1, -1, 0}, {-1, -1, 1}, {-1, 0, -1}, {-1, 0, 0}, {-1, 0, 1}, {-1, 1, -1},
+        {-1, 1, 0},   {-1, 1, 1},  {0, -1, -1}, {0, -1, 0},  {0, -1, 1}, {0, 0, -1}, {0, 0, 1},
+        {0, 1, -1},   {0, 1, 0},   {0, 1, 1},   {1, -1, -1}, {1, -1, 0}, {1, -1, 1}, {1, 0, -1},
+        {1, 0, 0},    {1, 0, 1},   {1, 1, -1},  {1, 1, 0},   {1, 1, 1}};
+};
+
+// One block of 512 threads per (request, 8^3 brick holding myocardium).
+// For two leads at a time the lead weight m_i of every myocardial cell of the
+// brick and its 1-voxel halo (a compact per-brick list) is computed once into
+// LDS (component arrays, zero outside the myocardium); thread t < n then
+// gathers for the brick's t-th myocardial voxel u
+//   G_i(u) = sum_{d in N} M_i(u+d) . d + m_i(u) . s_u,
+//   M_i = m_i on myocardium and 0 elsewhere,  s_u = sum of d with u+d myocardial,
+// which is sum_{d in N, u+d myo} (m_i(u+d) + m_i(u)) . d -- the coefficient of
+// V_u(t) in the reference's dipole sum regrouped by cell (simulator.cpp:509-537).
+// The voxel's record (row g_off + v of the request) receives
+//   VALU layout (RS = LP + 2 doubles): [0, LP) G zero-padded,
+//     LP: (P_u, Q_u) floats, LP+1: (a_hi, a_lo) floats;
+//   MFMA layout: 16-double G rows and float4 {P_u, Q_u, a_hi, a_lo} in aux;
+// with P_u = k2(1-k3) e^{(k4+k5)(a_u - t_ref)}, Q_u = k2 k3 e^{k5 (a_u - t_ref)},
+// t_ref the reference time of the voxel's chunk.
+template <int LP, bool kMfma, int NH>
+__global__ __launch_bounds__(512) void lead_field_kernel(
+    const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
+    const int64_t* __restrict__ lf_start, const double* __restrict__ lead_pos,
+    const double* __restrict__ layer_k, double* __restrict__ rec, float4* __restrict__ aux)
+{
+    using N = Nhood<NH>;
+    constexpr int RS = kMfma ? 16 : LP + 2;
+    __shared__ uint8_t s_myo[LN];
+    __shared__ double s_m[2][3][LN];
+    const int r = find_req(lf_start, B, blockIdx.x);
+    const ReqDev R = reqs[r];
+    const HeartDev& H = hearts[R.heart];
+    const int br = (int)(blockIdx.x - lf_start[r]);
+    const int t = threadIdx.x;
+    const int4 bi = H.bricks[br];
+    const int e0 = H.brick_off[br], nv = H.brick_off[br + 1] - e0;
+    const int c0 = H.halo_off[br], nc = H.halo_off[br + 1] - c0;
+    for (int q = t; q < LN; q += blockDim.x) {
+        s_myo[q] = 0;
+#pragma unroll
+        for (int w = 0; w < 2; ++w) s_m[w][0][q] = s_m[w][1][q] = s_m[w][2][q] = 0.0;
+    }
+    __syncthreads();
+    for (int q = t; q < nc; q += blockDim.x) s_myo[H.halo_cells[c0 + q]] = 1;
+    __syncthreads();
+    int h = 0, v = -1;
+    double sd0 = 0.0, sd1 = 0.0, sd2 = 0.0;
+    if (t < nv) {
+        const int2 bv = H.brick_vox[e0 + t];
+        v = bv.x;
+        const int lz = bv.y >> 6, ly = (bv.y >> 3) & 7, lx = bv.y & 7;
+        h = ((lz + 1) * LH + (ly + 1)) * LH + (lx + 1);
+#pragma unroll
+        for (int q = 0; q < N::n; ++q) {
+            const double f = s_myo[h + (N::d[q][0] * LH + N::d[q][1]) * LH + N::d[q][2]];
+            sd0 += f * N::d[q][0];
+            sd1 += f * N::d[q][1];
+            sd2 += f * N::d[q][2];
+        }
+    }
+    const double* pos = lead_pos + R.pos_off;
+    const int n_leads = R.L;
+    // the voxel's whole record is assembled in registers and stored once
+    double row[RS];
+#pragma unroll
+    for (int i = 0; i < RS; ++i) row[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < LP; i += 2) {
+        if (i >= n_leads) break;
+        const bool two = i + 1 < n_leads;
+        for (int q = t; q < nc; q += blockDim.x) {
+            const int cell = H.halo_cells[c0 + q];
+            const int gz = bi.x - 1 + cell / (LH * LH), gy = bi.y - 1 + (cell / LH) % LH,
+                      gx = bi.z - 1 + cell % LH;
+            const double z = (double)(gz + 1), y = (double)(gy + 1), x = (double)(gx + 1);
+            double m0, m1, m2;
+            lead_f(pos[3 * i] - z, pos[3 * i + 1] - y, pos[3 * i + 2] - x, m0, m1, m2);
+            s_m[0][0][cell] = m0;
+            s_m[0][1][cell] = m1;
+            s_m[0][2][cell] = m2;
+            if (two) {
+                lead_f(pos[3 * i + 3] - z, pos[3 * i + 4] - y, pos[3 * i + 5] - x, m0, m1, m2);
+                s_m[1][0][cell] = m0;
+                s_m[1][1][cell] = m1;
+                s_m[1][2][cell] = m2;
+            }
+        }
+        __syncthreads();
+        if (t < nv) {
+#pragma unroll
+            for (int w = 0; w < 2; ++w) {
+                double g = s_m[w][0][h] * sd0 + s_m[w][1][h] * sd1 + s_m[w][2][h] * sd2;
+#pragma unroll
+                for (int q = 0; q < N::n; ++q) {
+                    const int hn = h + (N::d[q][0] * LH + N::d[q][1]) * LH + N::d[q][2];
+                    if (N::d[q][0]) g += s_m[w][0][hn] * N::d[q][0];
+                    if (N::d[q][1]) g += s_m[w][1][hn] * N::d[q][1];
+                    if (N::d[q][2]) g += s_m[w][2][hn] * N::d[q][2];
+                }
+                if (w == 0 || two) row[i + w] = g;
+            }
+        }
+        __syncthreads();
+    }
+    if (t < nv) {
+        const int4 ch = H.chunks[H.vox_chunk[v]];
+        const double* k = layer_k + R.k_off + 9 * (ch.x - 1);
+        const double da = H.vox_a64[v] - (double)__int_as_float(ch.w);
+        const float2 pq{(float)(k[2] * (1.0 - k[3]) * exp((k[4] + k[5]) * da)),
+                        (float)(k[2] * k[3] * exp(k[5] * da))};
+        const float2 av = H.vox_a[v];
+        double* gv = rec + (R.g_off + v) * RS;
+        if constexpr (kMfma) {
+            aux[R.g_off + v] = float4{pq.x, pq.y, av.x, av.y};
+        } else {
+            row[LP] = __builtin_bit_cast(double, pq);
+            row[LP + 1] = __builtin_bit_cast(double, av);
+        }
+#pragma unroll
+        for (int i = 0; i < RS; i += 2)
+            *reinterpret_cast<double2*>(gv + i) = double2{row[i], row[i + 1]};
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Chunk sums and tile bounds
+// ---------------------------------------------------------------------------
+
+// Per (request, chunk) and lead, over the chunk's voxels u, in FP64 with the
+// layer's reference time t0:
+//   sum G_u,   sum G_u P'_u,   sum G_u Q'_u,
+//   P'_u = k2(1-k3) e^{(k4+k5)(a_u - t0)},  Q'_u = k2 k3 e^{k5 (a_u - t0)}.
+// Once every voxel of the chunk is past its upstroke (sigmoid 1), the chunk
+// adds exactly X_l(t) sum G P' + Y_l(t) sum G Q' (+ k0 sum G).  Fixed
+// summation order: coalesced row passes, a shuffle tree per wave, then wave
+// order.  Thread 0 also finds the chunk's tile bounds {z, f}: tiles before z
+// are skipped for every voxel (the fused kernel's skip test on the earliest
+// voxel at the tile's last step), tiles from f on are saturated in FP64 for
+// every voxel (test on the latest voxel at the tile's first step).
+template <int LP, bool kMfma>
+__global__ __launch_bounds__(256) void chunk_sums_kernel(
+    const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
+    const int64_t* __restrict__ ch_start, const double* __restrict__ layer_k,
+    const LayerConst* __restrict__ lc, const double* __restrict__ tgrid,
+    const float2* __restrict__ thl, const double* __restrict__ rec, double* __restrict__ sums,
+    int2* __restrict__ bnd)
+{
+    constexpr int RS = kMfma ? 16 : LP + 2;
+    __shared__ double s_wave[4][3 * LP];
+    const int r = find_req(ch_start, B, blockIdx.x);
+    const ReqDev R = reqs[r];
+    const HeartDev& H = hearts[R.heart];
+    const int c = (int)(blockIdx.x - ch_start[r]);
+    const int4 ch = H.chunks[c];
+    const double* k = layer_k + R.k_off + 9 * (ch.x - 1);
+    const double t0 = H.layer_t0[ch.x];
+    const double alpha = k[2] * (1.0 - k[3]), beta = k[2] * k[3], k45 = k[4] + k[5], k5 = k[5];
+    const double* recb = rec + R.g_off * RS;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double* out = sums + (R.c_off + c) * 3 * LP;
+    if constexpr (kMfma) {
+        // 16-double rows: a wave loads 4 whole rows per instruction (lane =
+        // 16 * row slot + lead), one coalesced 512-byte read.  For the wave's
+        // 16 rows, lanes 0-15 compute P' and lanes 16-31 Q' in one exp (lanes
+        // 32-63 repeat them), broadcast by shuffles.
+        const int col = lane & 15, slot = lane >> 4;
+        const bool is_q = (lane >> 4) & 1;
+        const double coef = is_q ? beta : alpha, rate = is_q ? k5 : k45;
+        double sg = 0.0, sp = 0.0, sq = 0.0;
+        for (int v0 = ch.y + 16 * w; v0 < ch.z; v0 += 64) {
+            const int ve = v0 + col;
+            const double e = ve < ch.z ? coef * exp(rate * (H.vox_a64[ve] - t0)) : 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = 4 * q + slot;
+                const int v = v0 + row;
+                const double g = v < ch.z ? recb[(int64_t)v * 16 + col] : 0.0;
+                const double P = __shfl(e, row, 64), Q = __shfl(e, 16 + row, 64);
+                sg += g;
+                sp = __builtin_fma(g, P, sp);
+                sq = __builtin_fma(g, Q, sq);
+            }
+        }
+        sg += __shfl_down(sg, 16, 64);
+        sp += __shfl_down(sp, 16, 64);
+        sq += __shfl_down(sq, 16, 64);
+        sg += __shfl_down(sg, 32, 64);
+        sp += __shfl_down(sp, 32, 64);
+        sq += __shfl_down(sq, 32, 64);
+        if (lane < LP) {
+            s_wave[w][lane] = sg;
+            s_wave[w][LP + lane] = sp;
+            s_wave[w][2 * LP + lane] = sq;
+        }
+    } else {
+        double part[3 * LP];
+#pragma unroll
+        for (int i = 0; i < 3 * LP; ++i) part[i] = 0.0;
+        for (int v = ch.y + threadIdx.x; v < ch.z; v += blockDim.x) {
+            const double da = H.vox_a64[v] - t0;
+            const double P = alpha * exp(k45 * da), Q = beta * exp(k5 * da);
+#pragma unroll
+            for (int i = 0; i < LP; ++i)
+                if (i < R.L) {
+                    const double g = recb[(int64_t)v * RS + i];
+                    part[i] += g;
+                    part[LP + i] = __builtin_fma(g, P, part[LP + i]);
+                    part[2 * LP + i] = __builtin_fma(g, Q, part[2 * LP + i]);
+                }
+        }
+#pragma unroll
+        for (int i = 0; i < 3 * LP; ++i) {
+            double x = part[i];
+            for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+            if (lane == 0) s_wave[w][i] = x;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 3 * LP; i += blockDim.x) {
+        double sum = 0.0;
+        for (int ww = 0; ww < (int)(blockDim.x >> 6); ++ww) sum += s_wave[ww][i];
+        out[i] = sum;
+    }
+    if (threadIdx.x == 0) {
+        const LayerConst kc = lc[R.lay_off + ch.x - 1];
+        const double c1d = -k[1] * LOG2E_D;
+        const double a_max = H.vox_a64[ch.z - 1];
+        const float2 a_min = H.vox_a[ch.y];
+        const double* tg = tgrid + R.t_off;
+        const float2* th = thl + R.t_off;
+        const int f = c1d < 0.0 ? first_true(R.n_tiles, [&](int tile) {
+            return c1d * (tg[tile * 64] - a_max) <= SAT64_X;
+        }) : R.n_tiles;
+        const int z = (kc.c1 < 0.0f && kc.skip != 0.0f) ? first_true(R.n_tiles, [&](int tile) {
+            const float2 tl = th[tile * 64 + 63];
+            return kc.c1 * tau_of(tl.x, tl.y, a_min.x, a_min.y) < SKIP_X;
+        }) : 0;
+        bnd[R.c_off + c] = int2{z < f ? z : f, f};
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Layer prefix tables
+// ---------------------------------------------------------------------------
+
+// One wave per (request, layer); lane i < LP is lead i.  Within a layer the
+// chunks are ordered by activation, so both tile bounds are non-decreasing
+// along the chunk list: for tile tau the saturated chunks are a prefix
+// [begin, lo) and the chunks in transition a range [lo, hi).  Writes per tile
+// {lo, hi} and the prefix sums A = sum_{c < lo} sum G P', B = sum_{c < lo}
+// sum G Q'; slot n_tiles holds the layer's sum G (for k0).  Fixed order.
+template <int LP>
+__global__ __launch_bounds__(64) void layer_prefix_kernel(
+    const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
+    const int64_t* __restrict__ lay_start, const double* __restrict__ sums,
+    const int2* __restrict__ bnd, int2* __restrict__ pre_rng, double* __restrict__ pre_ab)
+{
+    const int r = find_req(lay_start, B, blockIdx.x);
+    const ReqDev R = reqs[r];
+    const HeartDev& H = hearts[R.heart];
+    const int l = (int)(blockIdx.x - lay_start[r]) + 1;
+    const int i = threadIdx.x;
+    const int cb = l <= H.max_layer ? H.layer_chunk[l] : 0;
+    const int ce = l <= H.max_layer ? H.layer_chunk[l + 1] : 0;
+    const int64_t slot = R.pre_off + (int64_t)(l - 1) * (R.n_tiles + 1);
+    double A = 0.0, Bq = 0.0;
+    int lo = cb, hi = cb;
+    for (int tile = 0; tile < R.n_tiles; ++tile) {
+        while (lo < ce && bnd[R.c_off + lo].y <= tile) {
+            if (i < LP) {
+                const double* s = sums + (R.c_off + lo) * 3 * LP;
+                A += s[LP + i];
+                Bq += s[2 * LP + i];
+            }
+            ++lo;
+        }
+        if (hi < lo) hi = lo;
+        while (hi < ce && bnd[R.c_off + hi].x <= tile) ++hi;
+        if (i == 0) pre_rng[slot + tile] = int2{lo, hi};
+        if (i < LP) {
+            pre_ab[(slot + tile) * 2 * LP + i] = A;
+            pre_ab[(slot + tile) * 2 * LP + LP + i] = Bq;
+        }
+    }
+    if (i < LP) {
+        double g = 0.0;
+        for (int c = cb; c < ce; ++c) g += sums[(R.c_off + c) * 3 * LP + i];
+        pre_ab[(slot + R.n_tiles) * 2 * LP + i] = g;
+        pre_ab[(slot + R.n_tiles) * 2 * LP + LP + i] = 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused AP evaluation + lead contraction (transition tiles)
+// ---------------------------------------------------------------------------
+
+// The cell AP of voxel u at t is
+//   V = S(tau) * (P_u X(t) + Q_u Y(t)) + k0,  tau = t - a_u,
+//   X(t) = e^{-(k4+k5)(t - t_ref)} R(t),  Y(t) = e^{-k5 (t - t_ref)} R(t),
+// Wohlfart.h:197-202 with the exponentials split into a per-voxel factor
+// (P, Q from lead_field) and a per-step factor (from the FP64 X_l, Y_l
+// tables, rescaled to the chunk's t_ref, rounded once to FP32).  Voxels of a
+// chunk are one layer, sorted by activation time, so for a tile they split
+// into [v0, s1) saturated (2 VALU ops), [s1, s2) general, [s2, v1) skipped,
+// found by ballot searches.  k0 is added by the reduction (k0 sum G).
+//
+// VALU variant: block = 4 waves of one (request, chunk); a wave takes tiles
+// z + w, z + w + 4, ... < f; lane = step; lead sums are FP64 FMAs with the
+// G row in SGPRs.
+template <int LP>
+__global__ __launch_bounds__(256) void ecg_fused_kernel(
+    const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
+    const int64_t* __restrict__ ch_start, const double* __restrict__ rec,
+    const LayerConst* __restrict__ lc, const double* __restrict__ layer_k,
+    const double2* __restrict__ xy, const float2* __restrict__ thl, const int2* __restrict__ bnd,
+    double* __restrict__ partial)
+{
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int r = find_req(ch_start, B, blockIdx.x);
+    const ReqDev R = reqs[r];
+    const HeartDev& H = hearts[R.heart];
+    const int c = (int)(blockIdx.x - ch_start[r]);
+    const int2 tb = bnd[R.c_off + c];
+    const int4 ch = H.chunks[c];
+    const int l = ch.x;
+    const LayerConst kc = lc[R.lay_off + l - 1];
+    const double* k = layer_k + R.k_off + 9 * (l - 1);
+    const double tref = (double)__int_as_float(ch.w);
+    const double dtr = tref - H.layer_t0[l];
+    const double ex = exp((k[4] + k[5]) * dtr), ey = exp(k[5] * dtr);
+    constexpr int RS = LP + 2;
+    const double* Rb = rec + R.g_off * RS;
+    const double2* XY = xy + R.xy_off + (int64_t)(l - 1) * R.Tp;
+    const float2* th = thl + R.t_off;
+
+    for (int tile = tb.x + w; tile < tb.y; tile += 4) {
+        const int j = tile * 64 + lane;
+        const double2 xyj = XY[j];
+        const float X = (float)fmin(xyj.x * ex, XMAX);
+        const float Y = (float)fmin(xyj.y * ey, XMAX);
+        const float2 tt = th[j];
+        const float2 tf = th[tile * 64];
+        const float2 tl = th[tile * 64 + 63];
+        double acc[LP];
+#pragma unroll
+        for (int i = 0; i < LP; ++i) acc[i] = 0.0;
+        int s1 = ch.y, s2 = ch.z;
+        if (kc.c1 < 0.0f) {
+            // saturated prefix: max over lanes of c1*tau <= SAT_X (max at the first lane)
+            s1 = first_false(ch.y, ch.z, [&](int v) {
+                const float2 a = H.vox_a[v];
+                return kc.c1 * tau_of(tf.x, tf.y, a.x, a.y) <= SAT_X;
+            });
+            // non-zero range ends where min over lanes of c1*tau >= SKIP_X (min at the last lane)
+            if (kc.skip != 0.0f)
+                s2 = first_false(s1, ch.z, [&](int v) {
+                    const float2 a = H.vox_a[v];
+                    return kc.c1 * tau_of(tl.x, tl.y, a.x, a.y) < SKIP_X;
+                });
+        }
+        // Both loops keep the next voxel's record in flight while the current
+        // one is consumed (the one-ahead load of the last voxel reads the next
+        // record, which always exists: rows run to V + 1).
+        if (ch.y < s1) {
+            const double* rp = Rb + (int64_t)ch.y * RS;
+            double rn[RS];
+#pragma unroll
+            for (int i = 0; i < RS; ++i) rn[i] = rp[i];
+            for (int v = ch.y; v < s1; ++v) {
+                double rr[RS];
+#pragma unroll
+                for (int i = 0; i < RS; ++i) rr[i] = rn[i];
+                rp += RS;
+#pragma unroll
+                for (int i = 0; i < RS; ++i) rn[i] = rp[i];
+                const float2 pq = __builtin_bit_cast(float2, rr[LP]);
+                const double apd = (double)__builtin_fmaf(pq.x, X, pq.y * Y);
+#pragma unroll
+                for (int i = 0; i < LP; ++i) acc[i] = __builtin_fma(rr[i], apd, acc[i]);
+            }
+        }
+        if (s1 < s2) {
+            const double* rp = Rb + (int64_t)s1 * RS;
+            double rn[RS];
+#pragma unroll
+            for (int i = 0; i < RS; ++i) rn[i] = rp[i];
+            for (int v = s1; v < s2; ++v) {
+                double rr[RS];
+#pragma unroll
+                for (int i = 0; i < RS; ++i) rr[i] = rn[i];
+                rp += RS;
+#pragma unroll
+                for (int i = 0; i < RS; ++i) rn[i] = rp[i];
+                const float2 pq = __builtin_bit_cast(float2, rr[LP]);
+                const float2 a = __builtin_bit_cast(float2, rr[LP + 1]);
+                const float tau = tau_of(tt.x, tt.y, a.x, a.y);
+                const float e1 = fast_exp2(kc.c1 * tau);
+                const double apd = (double)(__builtin_fmaf(pq.x, X, pq.y * Y) * fast_rcp(1.0f + e1));
+#pragma unroll
+                for (int i = 0; i < LP; ++i) acc[i] = __builtin_fma(rr[i], apd, acc[i]);
+            }
+        }
+        double* out = partial + R.p_off + ((int64_t)c * LP) * R.Tp + j;
+#pragma unroll
+        for (int i = 0; i < LP; ++i)
+            if (i < R.L) out[(int64_t)i * R.Tp] = acc[i];
+    }
+}
+
+typedef double mfma_f64x4 __attribute__((ext_vector_type(4)));
+
+// Matrix-core variant: the lead contraction on v_mfma_f64_16x16x4_f64,
+// D[lead i][step j] += sum_k G[voxel k][lead i] * V[voxel k][step j].  gfx950
+// lane layout (checked by a one-hot probe): A(i,k) in lane 16k+i, B(k,j) in
+// lane 16k+j, D(i,j) in lane 16(i%4)+j, register i/4.  Block = `nw` waves of
+// one (request, chunk); a wave takes tiles z + w, z + w + nw, ... < f.  A
+// tile is four 16-step sub-tiles m, each classified on its own: saturated in
+// FP64 for every voxel (exact closed form X_l(t) sum G P' + Y_l(t) sum G Q',
+// no voxel loop), or evaluated over the voxels before its skip point.  Per
+// group of 4 voxels: one coalesced 512-byte load of the four G rows (A), one
+// float4 {P, Q, a} per lane (voxel k = lane/16), an AP per (voxel, step) and
+// one MFMA per evaluated sub-tile.  Groups that reach past the chunk end are
+// masked; the spare rows per request keep their loads in bounds.
+template <int LP>
+__global__ __launch_bounds__(1024) void ecg_fused_mfma_kernel(
+    const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
+    const int64_t* __restrict__ ch_start, const double* __restrict__ G16,
+    const float4* __restrict__ aux, const double* __restrict__ sums,
+    const LayerConst* __restrict__ lc, const double* __restrict__ layer_k,
+    const double2* __restrict__ xy, const double* __restrict__ tgrid, const float2* __restrict__ thl,
+    const int2* __restrict__ bnd, double* __restrict__ partial)
+{
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nw = blockDim.x >> 6;
+    const int r = find_req(ch_start, B, blockIdx.x);
+    const ReqDev R = reqs[r];
+    const HeartDev& H = hearts[R.heart];
+    const int c = (int)(blockIdx.x - ch_start[r]);
+    const int2 tb = bnd[R.c_off + c];
+    if (tb.x + w >= tb.y) return;
+    const int4 ch = H.chunks[c];
+    const int l = ch.x;
+    const LayerConst kc = lc[R.lay_off + l - 1];
+    const double* kk = layer_k + R.k_off + 9 * (l - 1);
+    const double tref = (double)__int_as_float(ch.w);
+    const double dtr = tref - H.layer_t0[l];
+    const double ex = exp((kk[4] + kk[5]) * dtr), ey = exp(kk[5] * dtr);
+    const double c1d = -kk[1] * LOG2E_D;
+    const double a_last = H.vox_a64[ch.z - 1];
+    const int jl = lane & 15;   // step column
+    const int kv = lane >> 4;   // voxel slot of the B operand / lead group of D
+    const double* Gb = G16 + R.g_off * 16;
+    const float4* Ab = aux + R.g_off;
+    const double* sm = sums + (R.c_off + c) * 3 * LP;
+    const double2* XY = xy + R.xy_off + (int64_t)(l - 1) * R.Tp;
+    const double* tg = tgrid + R.t_off;
+    const float2* th = thl + R.t_off;
+    double* out = partial + R.p_off + (int64_t)c * LP * R.Tp;
+
+    for (int tile = tb.x + w; tile < tb.y; tile += nw) {
+        float X[4], Y[4], TH[4], TL[4];
+        bool sat[4];
+        int s1[4], s2[4];
+        int vend = ch.y;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int j0 = tile * 64 + 16 * m;
+            sat[m] = c1d * (tg[j0] - a_last) <= SAT64_X;
+            const double2 xyj = XY[j0 + jl];
+            X[m] = (float)fmin(xyj.x * ex, XMAX);
+            Y[m] = (float)fmin(xyj.y * ey, XMAX);
+            const float2 t2 = th[j0 + jl];
+            TH[m] = t2.x;
+            TL[m] = t2.y;
+            s1[m] = ch.y;
+            s2[m] = ch.z;
+            if (!sat[m] && kc.c1 < 0.0f) {
+                const float2 tf = th[j0], tl = th[j0 + 15];
+                s1[m] = first_false(ch.y, ch.z, [&](int v) {
+                    const float2 a = H.vox_a[v];
+                    return kc.c1 * tau_of(tf.x, tf.y, a.x, a.y) <= SAT_X;
+                });
+                if (kc.skip != 0.0f)
+                    s2[m] = first_false(s1[m], ch.z, [&](int v) {
+                        const float2 a = H.vox_a[v];
+                        return kc.c1 * tau_of(tl.x, tl.y, a.x, a.y) < SKIP_X;
+                    });
+            }
+            if (!sat[m] && s2[m] > vend) vend = s2[m];
+        }
+        mfma_f64x4 acc[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) acc[m] = mfma_f64x4{0.0, 0.0, 0.0, 0.0};
+
+        // Groups of 4 voxels; the next group's A rows and {P, Q, a} are loaded
+        // while the current group's MFMAs run (vector loads complete in order).
+        // Every group starts below ch.z <= V, so the look-ahead group starts at
+        // most at V + 3 and its rows stay below V + 7 < V + MFMA_PAD.
+        int vg = ch.y;
+        if (vg < vend) {
+            double a_n = Gb[(int64_t)vg * 16 + lane];
+            float4 q_n = Ab[vg + kv];
+            for (; vg < vend; vg += 4) {
+                const bool ok = vg + kv < ch.z;
+                const double a_op = ok ? a_n : 0.0;
+                const float4 q = q_n;
+                a_n = Gb[(int64_t)(vg + 4) * 16 + lane];
+                q_n = Ab[vg + 4 + kv];
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    if (sat[m] || vg >= s2[m]) continue;
+                    float ap = __builtin_fmaf(q.x, X[m], q.y * Y[m]);
+                    if (vg + 4 > s1[m]) {
+                        const float tau = tau_of(TH[m], TL[m], q.z, q.w);
+                        ap *= fast_rcp(1.0f + fast_exp2(kc.c1 * tau));
+                    }
+                    const double bv = ok ? (double)ap : 0.0;
+                    acc[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_op, bv, acc[m], 0, 0, 0);
+                }
+            }
+        }
+        // D(i, j) sits in lane 16(i%4)+j, register i/4: lead i = 4q + lane/16
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int j = tile * 64 + 16 * m + jl;
+            double2 xyj = double2{0.0, 0.0};
+            if (sat[m]) xyj = XY[j];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int i = 4 * q + kv;
+                if (i < R.L && i < LP) {
+                    double v = acc[m][q];
+                    if (sat[m]) v = __builtin_fma(xyj.x, sm[LP + i], xyj.y * sm[2 * LP + i]);
+                    out[(int64_t)i * R.Tp + j] = v;
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reduction
+// ---------------------------------------------------------------------------
+
+// ecg[r][i][t] for t < Tn: per layer (in order) the closed form of the
+// chunks saturated by t's tile, X_l(t) A + Y_l(t) B, then k0 * sum G, then
+// the partial rows of the chunks in transition in chunk order; 0 for
+// Tn <= t < T: samples past the last iteration of the reference's neighbour
+// loop (simulator.cpp:519) receive no dipole.
+template <int LP>
+__global__ void ecg_reduce_kernel(const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs,
+                                  int B, const int64_t* __restrict__ out_start, int64_t n_out,
+                                  const LayerConst* __restrict__ lc, const double2* __restrict__ xy,
+                                  const int2* __restrict__ pre_rng, const double* __restrict__ pre_ab,
+                                  const double* __restrict__ partial, double* __restrict__ ecg)
+{
+    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n_out;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int r = find_req(out_start, B, idx);
+        const ReqDev R = reqs[r];
+        const HeartDev& H = hearts[R.heart];
+        const int64_t loc = idx - out_start[r];
+        const int i = (int)(loc / R.T);
+        const int j = (int)(loc % R.T);
+        double s = 0.0;
+        if (j < R.Tn) {
+            const int tile = j >> 6;
+            const int nl = R.nl < H.max_layer ? R.nl : H.max_layer;
+            for (int l = 1; l <= nl; ++l) {
+                const int64_t slot = R.pre_off + (int64_t)(l - 1) * (R.n_tiles + 1);
+                const double2 xyj = xy[R.xy_off + (int64_t)(l - 1) * R.Tp + j];
+                const double* ab = pre_ab + (slot + tile) * 2 * LP;
+                s = __builtin_fma(xyj.x, ab[i], s);
+                s = __builtin_fma(xyj.y, ab[LP + i], s);
+                s = __builtin_fma(lc[R.lay_off + l - 1].k0, pre_ab[(slot + R.n_tiles) * 2 * LP + i], s);
+                const int2 rg = pre_rng[slot + tile];
+                const double* p = partial + R.p_off + (int64_t)i * R.Tp + j;
+                for (int c = rg.x; c < rg.y; ++c) s += p[(int64_t)c * LP * R.Tp];
+            }
+        }
+        ecg[R.ecg_off + loc] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+
+static int grid_for(int64_t n, int threads, int cap = 1 << 20)
+{
+    int64_t g = (n + threads - 1) / threads;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+hipError_t launch_ap_tables(const GroupArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(ap_tables_kernel, dim3(grid_for(a.n_xy, 256)), dim3(256), 0, s, a.hearts, a.reqs,
+                       a.B, a.xy_start, a.n_xy, a.layer_k, a.tgrid, a.xy, a.lc);
+    return hipGetLastError();
+}
+
+template <int LP, bool M, int NH>
+static void lf_launch(const GroupArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL((lead_field_kernel<LP, M, NH>), dim3((unsigned)a.n_lf), dim3(512), 0, s, a.hearts,
+                       a.reqs, a.B, a.lf_start, a.lead_pos, a.layer_k, a.rec, a.aux);
+}
+
+template <int LP, bool M>
+static void lf_launch_nh(const GroupArgs& a, hipStream_t s)
+{
+    switch (a.nhood) {
+    case 0: lf_launch<LP, M, 0>(a, s); break;
+    case 1: lf_launch<LP, M, 1>(a, s); break;
+    case 2: lf_launch<LP, M, 2>(a, s); break;
+    default: lf_launch<LP, M, 3>(a, s); break;
+    }
+}
+
+template <int LP>
+static void lf_launch_m(const GroupArgs& a, hipStream_t s)
+{
+    if (a.mfma)
+        lf_launch_nh<LP, true>(a, s);
+    else
+        lf_launch_nh<LP, false>(a, s);
+}
+
+hipError_t launch_lead_field(const GroupArgs& a, hipStream_t s)
+{
+    if (a.n_lf == 0) return hipSuccess;
+    switch (a.LP) {
+    case 4: lf_launch_m<4>(a, s); break;
+    case 8: lf_launch_m<8>(a, s); break;
+    case 12: lf_launch_m<12>(a, s); break;
+    case 16: lf_launch_m<16>(a, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int LP>
+static void sums_launch(const GroupArgs& a, hipStream_t s)
+{
+    if (a.mfma)
+        hipLaunchKernelGGL((chunk_sums_kernel<LP, true>), dim3((unsigned)a.n_ch), dim3(256), 0, s, a.hearts,
+                           a.reqs, a.B, a.ch_start, a.layer_k, a.lc, a.tgrid, a.thl, a.rec, a.sums, a.bnd);
+    else
+        hipLaunchKernelGGL((chunk_sums_kernel<LP, false>), dim3((unsigned)a.n_ch), dim3(256), 0, s, a.hearts,
+                           a.reqs, a.B, a.ch_start, a.layer_k, a.lc, a.tgrid, a.thl, a.rec, a.sums, a.bnd);
+}
+
+hipError_t launch_chunk_sums(const GroupArgs& a, hipStream_t s)
+{
+    if (a.n_ch == 0) return hipSuccess;
+    switch (a.LP) {
+    case 4: sums_launch<4>(a, s); break;
+    case 8: sums_launch<8>(a, s); break;
+    case 12: sums_launch<12>(a, s); break;
+    case 16: sums_launch<16>(a, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_layer_prefix(const GroupArgs& a, hipStream_t s)
+{
+    if (a.n_lay == 0) return hipSuccess;
+    switch (a.LP) {
+#define EKG_PREFIX(LPV)                                                                                  \
+    case LPV:                                                                                            \
+        hipLaunchKernelGGL(layer_prefix_kernel<LPV>, dim3((unsigned)a.n_lay), dim3(64), 0, s, a.hearts, \
+                           a.reqs, a.B, a.lay_start, a.sums, a.bnd, a.pre_rng, a.pre_ab);               \
+        break;
+        EKG_PREFIX(4)
+        EKG_PREFIX(8)
+        EKG_PREFIX(12)
+        EKG_PREFIX(16)
+#undef EKG_PREFIX
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int LP>
+static void fused_launch(const GroupArgs& a, hipStream_t s)
+{
+    if (a.mfma) {
+        const int nw = a.mfma_waves;
+        hipLaunchKernelGGL(ecg_fused_mfma_kernel<LP>, dim3((unsigned)a.n_ch), dim3(64 * nw), 0, s, a.hearts,
+                           a.reqs, a.B, a.ch_start, a.rec, a.aux, a.sums, a.lc, a.layer_k, a.xy, a.tgrid,
+                           a.thl, a.bnd, a.partial);
+    } else {
+        hipLaunchKernelGGL(ecg_fused_kernel<LP>, dim3((unsigned)a.n_ch), dim3(256), 0, s, a.hearts, a.reqs,
+                           a.B, a.ch_start, a.rec, a.lc, a.layer_k, a.xy, a.thl, a.bnd, a.partial);
+    }
+}
+
+hipError_t launch_ecg_fused(const GroupArgs& a, hipStream_t s)
+{
+    if (a.n_ch == 0) return hipSuccess;
+    if (a.mfma && (a.mfma_waves < 1 || a.mfma_waves > 16)) return hipErrorInvalidValue;
+    switch (a.LP) {
+    case 4: fused_launch<4>(a, s); break;
+    case 8: fused_launch<8>(a, s); break;
+    case 12: fused_launch<12>(a, s); break;
+    case 16: fused_launch<16>(a, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_ecg_reduce(const GroupArgs& a, hipStream_t s)
+{
+    if (a.n_out == 0) return hipSuccess;
+    switch (a.LP) {
+#define EKG_REDUCE(LPV)                                                                                 \
+    case LPV:                                                                                           \
+        hipLaunchKernelGGL(ecg_reduce_kernel<LPV>, dim3(grid_for(a.n_out, 256)), dim3(256), 0, s,      \
+                           a.hearts, a.reqs, a.B, a.out_start, a.n_out, a.lc, a.xy, a.pre_rng,          \
+                           a.pre_ab, a.partial, a.ecg);                                                 \
+        break;
+        EKG_REDUCE(4)
+        EKG_REDUCE(8)
+        EKG_REDUCE(12)
+        EKG_REDUCE(16)
+#undef EKG_REDUCE
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace ekgk

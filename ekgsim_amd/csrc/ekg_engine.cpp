@@ -98,11 +98,6 @@ int nhood_offsets(int type, int8_t dif[26][3])
     return n;
 }
 
-const void* ctx_nb_table(const Ctx& ctx, int nhood)
-{
-    return static_cast<const int8_t*>(ctx.nb_table.p) + nhood * 26 * 4;
-}
-
 int64_t num_steps(double length, double dt)
 {
     if (!(dt > 0) || !std::isfinite(dt) || !std::isfinite(length) || length <= 0) return 0;
@@ -124,66 +119,63 @@ static int64_t neighbour_steps(double start, double length, double dt, int64_t c
     return n;
 }
 
-// Shape checks and derived sizes shared by the host and device entry points.
-BatchPlan plan_shape(const Heart& h, BatchPlan p)
+ReqSpec make_spec(Heart& h, const ekg_sim_desc& d, bool host_values)
 {
-    if (p.B < 1 || p.B > 65535) fail(EKG_EINVAL, "batch must hold 1..65535 simulations");
-    if (p.L < 1 || p.L > EKG_MAX_LEADS)
+    ReqSpec r;
+    r.h = &h;
+    r.L = d.n_leads;
+    r.nl = d.n_layers;
+    r.nhood = d.nhood;
+    r.start = d.start;
+    r.length = d.length;
+    r.dt = d.dt;
+    if (r.L < 1 || r.L > EKG_MAX_LEADS)
         fail(EKG_EINVAL, "n_leads must be in 1.." + std::to_string(EKG_MAX_LEADS));
-    if (p.nhood < EKG_NHOOD_2D4 || p.nhood > EKG_NHOOD_3D8)
+    if (r.nhood < EKG_NHOOD_2D4 || r.nhood > EKG_NHOOD_3D8)
         fail(EKG_EINVAL, "unknown neighbourhood (only know of these: 2D4, 3D4, 2D8, 3D8 (cube))");
-    if (p.n_layers < h.max_layer || p.n_layers < 1)
-        fail(EKG_EINVAL, "the heart has " + std::to_string(h.max_layer) + " layers but only " +
-                             std::to_string(p.n_layers) + " layer APs were given");
-    if (!(p.dt > 0) || !std::isfinite(p.dt) || !std::isfinite(p.start) || !std::isfinite(p.length))
+    if (r.nl < h.max_layer || r.nl < 1 || r.nl > 255)
+        fail(EKG_EINVAL, "the heart has " + std::to_string(h.max_layer) + " layers but " +
+                             std::to_string(r.nl) + " layer APs were given");
+    if (!(r.dt > 0) || !std::isfinite(r.dt) || !std::isfinite(r.start) || !std::isfinite(r.length))
         fail(EKG_EINVAL, "time step must be positive and times finite");
-    p.T = num_steps(p.length, p.dt);
-    if (p.T < 0 || p.T > (1 << 24)) fail(EKG_EINVAL, "too many time steps");
-    p.LP = p.L <= 4 ? 4 : p.L <= 8 ? 8 : p.L <= 12 ? 12 : 16;
-    p.Tp = ((p.T + 63) / 64) * 64;
-    p.n_tiles = (int)(p.Tp / 64);
-    p.n_groups = (p.n_tiles + 3) / 4;
-    p.Tn = neighbour_steps(p.start, p.length, p.dt, p.T + 1);
-    if (p.Tn > p.T)
+    r.T = num_steps(r.length, r.dt);
+    if (r.T < 0 || r.T > (1 << 24)) fail(EKG_EINVAL, "too many time steps");
+    r.Tp = std::max<int64_t>(64, ((r.T + 63) / 64) * 64);
+    r.n_tiles = (int)(r.Tp / 64);
+    r.Tn = neighbour_steps(r.start, r.length, r.dt, r.T + 1);
+    if (r.Tn > r.T)
         fail(EKG_EINVAL, "time step makes the neighbour loop of the reference run past the "
                          "sample buffer (simulator.cpp:519); choose another time step");
-    return p;
-}
-
-// Matrix-core contraction unless the caller pinned the VALU kernel.
-bool use_mfma(const Ctx& ctx, int n_leads)
-{
-    (void)n_leads;
-    return ctx.kernel != EKG_KERNEL_VALU;
-}
-
-BatchPlan plan_batch(const Heart& h, int batch, const ekg_sim_desc* descs)
-{
-    if (batch < 1 || !descs) fail(EKG_EINVAL, "batch must hold at least one descriptor");
-    BatchPlan p;
-    const ekg_sim_desc& d0 = descs[0];
-    p.B = batch;
-    p.L = d0.n_leads;
-    p.n_layers = d0.n_layers;
-    p.nhood = d0.nhood;
-    p.start = d0.start;
-    p.length = d0.length;
-    p.dt = d0.dt;
-    for (int b = 0; b < batch; ++b) {
-        const ekg_sim_desc& d = descs[b];
+    if (host_values) {
         if (!d.layer_k || !d.lead_pos) fail(EKG_EINVAL, "layer_k and lead_pos are required");
-        if (d.n_leads != p.L || d.nhood != p.nhood || d.start != p.start || d.length != p.length ||
-            d.dt != p.dt || d.n_layers != p.n_layers)
-            fail(EKG_EINVAL, "all descriptors of a batch must share leads, neighbourhood, "
-                             "layers, start, length and time step");
-        if (d.n_layers < 1 || d.n_leads < 1 || d.n_leads > EKG_MAX_LEADS)
-            fail(EKG_EINVAL, "invalid layer or lead count");
-        for (int i = 0; i < 9 * d.n_layers; ++i)
+        for (int i = 0; i < 9 * r.nl; ++i)
             if (!std::isfinite(d.layer_k[i])) fail(EKG_EINVAL, "non-finite AP coefficient");
-        for (int i = 0; i < 3 * d.n_leads; ++i)
+        for (int i = 0; i < 3 * r.L; ++i)
             if (!std::isfinite(d.lead_pos[i])) fail(EKG_EINVAL, "non-finite lead position");
+        r.k = d.layer_k;
+        r.pos = d.lead_pos;
     }
-    return plan_shape(h, p);
+    return r;
+}
+
+GroupPlan plan_group(Ctx& ctx, std::vector<ReqSpec> reqs)
+{
+    if (reqs.empty()) fail(EKG_EINVAL, "empty group");
+    GroupPlan g;
+    g.nhood = reqs[0].nhood;
+    int L = 1, kernel = ctx.kernel, waves = 0;
+    for (const ReqSpec& r : reqs) {
+        if (r.nhood != g.nhood) fail(EKG_EINVAL, "all requests of a launch group must share the neighbourhood");
+        L = std::max(L, r.L);
+        if (r.h->tune_kernel) kernel = r.h->tune_kernel;
+        if (r.h->tune_waves) waves = std::max(waves, r.h->tune_waves);
+    }
+    if (ctx.kernel != EKG_KERNEL_AUTO) kernel = ctx.kernel;  // an explicit context choice wins
+    g.LP = L <= 4 ? 4 : L <= 8 ? 8 : L <= 12 ? 12 : 16;
+    g.mfma = kernel != EKG_KERNEL_VALU;
+    g.waves = waves ? waves : 4;
+    g.reqs = std::move(reqs);
+    return g;
 }
 
 // Largest activation-time span inside one chunk (ms).  The saturated AP path
@@ -195,7 +187,14 @@ void ensure_layout(Heart& h)
 {
     if (!h.excited) fail(EKG_ESTATE, "excitation sequence not computed or loaded");
     EcgLayout& L = h.layout;
-    if (L.valid) return;
+    // Chunk size: enough waves for one simulation to fill the GPU
+    // (~400 chunks), bounded so that per-wave setup stays small; an autotuned
+    // size replaces it.  Chunks are layer-pure and span at most
+    // CHUNK_SPAN_MS of activation time.
+    const int64_t target = h.tune_chunk ? h.tune_chunk
+                                        : std::min<int64_t>(4096, std::max<int64_t>(256, h.n_myo / 400));
+    if (L.valid && L.chunk_target == target) return;
+    L.valid = false;
     std::vector<int64_t> idx;
     idx.reserve(h.n_myo);
     for (int64_t i = 0; i < h.nvox; ++i)
@@ -208,10 +207,7 @@ void ensure_layout(Heart& h)
     L.V = V;
     L.max_layer = h.max_layer;
     L.h_chunks.clear();
-    // Chunk size: enough waves for one simulation to fill the GPU
-    // (~400 chunks x time tiles), bounded so that per-wave setup stays small;
-    // chunks are layer-pure and span at most CHUNK_SPAN_MS of activation time.
-    const int64_t target = std::min<int64_t>(4096, std::max<int64_t>(256, V / 400));
+    L.chunk_target = (int)target;
     int64_t v = 0;
     while (v < V) {
         const int lay = h.h_layer[idx[v]];
@@ -235,6 +231,20 @@ void ensure_layout(Heart& h)
         v = e;
     }
     L.n_chunks = (int)L.h_chunks.size();
+    // chunks of layer l: [layer_chunk[l], layer_chunk[l+1]); reference time of
+    // layer l: t_ref of its first chunk
+    L.h_layer_chunk.assign(h.max_layer + 2, L.n_chunks);
+    L.h_layer_t0.assign(h.max_layer + 1, 0.0);
+    for (int c = L.n_chunks - 1; c >= 0; --c) {
+        const int lay = L.h_chunks[c].x;
+        L.h_layer_chunk[lay] = c;
+        float t;
+        std::memcpy(&t, &L.h_chunks[c].w, 4);
+        L.h_layer_t0[lay] = (double)t;
+    }
+    L.h_layer_chunk[0] = 0;
+    for (int l = h.max_layer; l >= 1; --l)
+        if (L.h_layer_chunk[l] > L.h_layer_chunk[l + 1]) L.h_layer_chunk[l] = L.h_layer_chunk[l + 1];
     // per sorted voxel: chunk index and activation time (FP64 and hi/lo FP32)
     std::vector<int> vchunk(V ? V : 1);
     for (int c = 0; c < L.n_chunks; ++c)
@@ -333,161 +343,270 @@ void ensure_layout(Heart& h)
         hip_check(hipMemcpy(L.chunks.p, L.h_chunks.data(), L.h_chunks.size() * sizeof(int4),
                             hipMemcpyHostToDevice),
                   "upload chunks");
+    L.layer_chunk.reserve(L.h_layer_chunk.size() * sizeof(int));
+    L.layer_t0.reserve(L.h_layer_t0.size() * sizeof(double));
+    hip_check(hipMemcpy(L.layer_chunk.p, L.h_layer_chunk.data(), L.h_layer_chunk.size() * sizeof(int),
+                        hipMemcpyHostToDevice),
+              "upload layer chunks");
+    hip_check(hipMemcpy(L.layer_t0.p, L.h_layer_t0.data(), L.h_layer_t0.size() * sizeof(double),
+                        hipMemcpyHostToDevice),
+              "upload layer times");
+    ekgk::HeartDev& d = L.dev;
+    d.grid_layer = h.d_layer.as<uint8_t>();
+    d.bricks = L.bricks.as<int4>();
+    d.brick_off = L.brick_off.as<int>();
+    d.brick_vox = L.brick_vox.as<int2>();
+    d.halo_off = L.halo_off.as<int>();
+    d.halo_cells = L.halo_cells.as<uint16_t>();
+    d.vox_chunk = L.vox_chunk.as<int>();
+    d.vox_a64 = L.vox_a64.as<double>();
+    d.vox_a = L.vox_a.as<float2>();
+    d.chunks = L.chunks.as<int4>();
+    d.layer_chunk = L.layer_chunk.as<int>();
+    d.layer_t0 = L.layer_t0.as<double>();
+    d.n0 = h.n0;
+    d.n1 = h.n1;
+    d.n2 = h.n2;
+    d.V = (int)V;
+    d.n_chunks = L.n_chunks;
+    d.n_bricks = L.n_bricks;
+    d.max_layer = h.max_layer;
+    d.pad = 0;
     L.valid = true;
 }
 
-size_t batch_device_bytes(const Heart& h, const BatchPlan& p)
+namespace {
+
+// Per-request sizes of a group's packed arrays.
+struct ReqSizes {
+    int64_t rows, chunks, bricks;
+};
+
+ReqSizes req_sizes(const ReqSpec& r, bool mfma)
 {
-    const int64_t V = h.layout.valid ? h.layout.V : h.n_myo;
-    const int64_t nch = h.layout.valid ? h.layout.n_chunks : std::max<int64_t>(1, V / 256 + 3);
+    const EcgLayout& L = r.h->layout;
+    return ReqSizes{L.V + (mfma ? ekgk::MFMA_PAD : 1), L.n_chunks, L.n_bricks};
+}
+
+template <class T>
+void append(std::vector<char>& buf, const T* p, size_t n)
+{
+    const char* c = reinterpret_cast<const char*>(p);
+    buf.insert(buf.end(), c, c + n * sizeof(T));
+}
+
+}  // namespace
+
+size_t group_device_bytes(const GroupPlan& g)
+{
     size_t s = 0;
-    s += (size_t)p.B * p.n_layers * 9 * 8;
-    s += (size_t)p.B * p.L * 3 * 8;
-    s += (size_t)p.Tp * (8 + 8);
-    s += (size_t)p.B * p.n_layers * p.Tp * 8;
-    s += (size_t)p.B * p.n_layers * sizeof(ekgk::LayerConst);
-    s += p.mfma ? (size_t)p.B * (V + ekgk::MFMA_PAD) * (16 * 8 + 16)
-                : (size_t)p.B * (V + 1) * (p.LP + 2) * 8;
-    s += (size_t)p.B * nch * 3 * p.LP * 8;
-    s += (size_t)p.B * nch * p.LP * p.Tp * 8;
-    s += (size_t)p.B * p.L * p.T * 8;
+    const int LP = g.LP;
+    for (const ReqSpec& r : g.reqs) {
+        const EcgLayout& L = r.h->layout;
+        const int64_t V = L.valid ? L.V : r.h->n_myo;
+        const int64_t nch = L.valid ? L.n_chunks : std::max<int64_t>(1, V / 256 + r.h->max_layer + 4);
+        const int64_t rows = V + (g.mfma ? ekgk::MFMA_PAD : 1);
+        s += (size_t)r.nl * 9 * 8 + (size_t)r.L * 3 * 8 + (size_t)r.Tp * 16;
+        s += (size_t)r.nl * (sizeof(ekgk::LayerConst) + r.Tp * 16);
+        s += g.mfma ? (size_t)rows * (16 * 8 + 16) : (size_t)rows * (LP + 2) * 8;
+        s += (size_t)nch * (3 * LP * 8 + 8);
+        s += (size_t)r.nl * (r.n_tiles + 1) * (8 + 2 * LP * 8);
+        s += (size_t)nch * LP * r.Tp * 8;
+        s += (size_t)r.L * r.T * 8;
+        s += sizeof(ekgk::ReqDev) + sizeof(ekgk::HeartDev) + 5 * 8;
+    }
     return s;
 }
 
-void upload_batch(const BatchPlan& p, const ekg_sim_desc* descs, Workspace& ws, hipStream_t stream)
+void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, const double* d_k,
+                   const double* d_pos, double* d_ecg, bool time_fused)
 {
-    const int B = p.B;
-    ws.h_k.resize((size_t)B * p.n_layers * 9);
-    ws.h_pos.resize((size_t)B * p.L * 3);
-    for (int b = 0; b < B; ++b) {
-        std::memcpy(&ws.h_k[(size_t)b * p.n_layers * 9], descs[b].layer_k,
-                    sizeof(double) * 9 * p.n_layers);
-        std::memcpy(&ws.h_pos[(size_t)b * p.L * 3], descs[b].lead_pos, sizeof(double) * 3 * p.L);
-    }
-    ws.layer_k.reserve(ws.h_k.size() * 8);
-    ws.lead_pos.reserve(ws.h_pos.size() * 8);
-    ws.ecg.reserve((size_t)B * p.L * std::max<int64_t>(p.T, 1) * 8);
-    hip_check(hipMemcpyAsync(ws.layer_k.p, ws.h_k.data(), ws.h_k.size() * 8, hipMemcpyHostToDevice,
-                             stream),
-              "upload AP coefficients");
-    hip_check(hipMemcpyAsync(ws.lead_pos.p, ws.h_pos.data(), ws.h_pos.size() * 8,
-                             hipMemcpyHostToDevice, stream),
-              "upload lead positions");
-}
-
-void enqueue_batch(Ctx& ctx, Heart& h, const BatchPlan& p, const double* d_layer_k,
-                   const double* d_lead_pos, double* d_ecg, Workspace& ws, hipStream_t stream,
-                   bool time_fused)
-{
-    const EcgLayout& Lo = h.layout;
-    const int B = p.B;
-    const int64_t V = Lo.V;
-    const int nch = Lo.n_chunks;
-    // time grid t_j = start + j*dt by repeated addition (simulator.cpp:496-500),
+    const int B = (int)g.reqs.size();
+    const int LP = g.LP;
+    const bool dev_in = d_k != nullptr;
+    // hearts of the group
+    std::vector<Heart*> hs;
+    std::vector<ekgk::HeartDev> hdev;
+    std::vector<ekgk::ReqDev> rd(B);
+    // time grids shared by requests with the same start, step and length:
+    // t_j = start + j*dt by repeated addition (simulator.cpp:496-500),
     // continued over the padding so the last tile stays monotone
-    if ((int64_t)ws.h_tgrid.size() != p.Tp || ws.h_tgrid.empty() || ws.h_tgrid[0] != p.start ||
-        ws.grid_dt != p.dt) {
-        ws.h_tgrid.resize(p.Tp);
-        ws.h_thl.resize(2 * p.Tp);
-        double st = p.start;
-        for (int64_t j = 0; j < p.Tp; ++j) {
-            ws.h_tgrid[j] = st;
-            const float hi = (float)st;
-            ws.h_thl[2 * j] = hi;
-            ws.h_thl[2 * j + 1] = (float)(st - (double)hi);
-            st += p.dt;
+    ws.h_tgrid.clear();
+    ws.h_thl.clear();
+    std::vector<std::pair<std::pair<double, double>, std::pair<int64_t, int64_t>>> grids;  // (start, dt) -> (Tp, off)
+    std::vector<int64_t> st_lf(B + 1, 0), st_ch(B + 1, 0), st_xy(B + 1, 0), st_lay(B + 1, 0), st_out(B + 1, 0);
+    int64_t k_off = 0, pos_off = 0, rows = 0, c_off = 0, p_off = 0, pre_off = 0;
+    if (!dev_in) {
+        ws.h_k.clear();
+        ws.h_pos.clear();
+    }
+    for (int r = 0; r < B; ++r) {
+        ReqSpec& q = g.reqs[r];
+        Heart& h = *q.h;
+        ensure_layout(h);
+        int hi = -1;
+        for (size_t i = 0; i < hs.size(); ++i)
+            if (hs[i] == &h) hi = (int)i;
+        if (hi < 0) {
+            hi = (int)hs.size();
+            hs.push_back(&h);
+            hdev.push_back(h.layout.dev);
         }
-        ws.grid_dt = p.dt;
-        ws.tgrid.reserve(p.Tp * 8);
-        ws.thl.reserve(p.Tp * 8);
-        hip_check(hipMemcpyAsync(ws.tgrid.p, ws.h_tgrid.data(), p.Tp * 8, hipMemcpyHostToDevice, stream),
-                  "upload time grid");
-        hip_check(hipMemcpyAsync(ws.thl.p, ws.h_thl.data(), p.Tp * 8, hipMemcpyHostToDevice, stream),
-                  "upload time grid");
+        int64_t t_off = -1;
+        for (auto& gr : grids)
+            if (gr.first.first == q.start && gr.first.second == q.dt && gr.second.first >= q.Tp) t_off = gr.second.second;
+        if (t_off < 0) {
+            t_off = (int64_t)ws.h_tgrid.size();
+            grids.push_back({{q.start, q.dt}, {q.Tp, t_off}});
+            double st = q.start;
+            for (int64_t j = 0; j < q.Tp; ++j) {
+                ws.h_tgrid.push_back(st);
+                const float hi32 = (float)st;
+                ws.h_thl.push_back(hi32);
+                ws.h_thl.push_back((float)(st - (double)hi32));
+                st += q.dt;
+            }
+        }
+        const ReqSizes z = req_sizes(q, g.mfma);
+        ekgk::ReqDev& R = rd[r];
+        R.heart = hi;
+        R.L = q.L;
+        R.nl = q.nl;
+        R.T = (int)q.T;
+        R.Tn = (int)q.Tn;
+        R.Tp = (int)q.Tp;
+        R.n_tiles = q.n_tiles;
+        R.pad = 0;
+        R.k_off = dev_in ? (int64_t)r * q.nl * 9 : k_off;
+        R.pos_off = dev_in ? (int64_t)r * q.L * 3 : pos_off;
+        R.ecg_off = st_out[r];
+        R.t_off = t_off;
+        R.lay_off = st_lay[r];
+        R.xy_off = st_xy[r];
+        R.g_off = rows;
+        R.c_off = c_off;
+        R.p_off = p_off;
+        R.pre_off = pre_off;
+        if (!dev_in) {
+            ws.h_k.insert(ws.h_k.end(), q.k, q.k + 9 * q.nl);
+            ws.h_pos.insert(ws.h_pos.end(), q.pos, q.pos + 3 * q.L);
+            k_off += 9 * q.nl;
+            pos_off += 3 * q.L;
+        }
+        rows += z.rows;
+        c_off += z.chunks;
+        p_off += z.chunks * LP * q.Tp;
+        pre_off += (int64_t)q.nl * (q.n_tiles + 1);
+        st_lf[r + 1] = st_lf[r] + z.bricks;
+        st_ch[r + 1] = st_ch[r] + z.chunks;
+        st_xy[r + 1] = st_xy[r] + (int64_t)q.nl * q.Tp;
+        st_lay[r + 1] = st_lay[r] + q.nl;
+        st_out[r + 1] = st_out[r] + (int64_t)q.L * q.T;
     }
-    ws.rtab.reserve((size_t)B * p.n_layers * p.Tp * 8);
-    ws.lc.reserve((size_t)B * p.n_layers * sizeof(ekgk::LayerConst));
-    if (p.mfma) {
-        ws.G.reserve((size_t)B * (V + ekgk::MFMA_PAD) * 16 * 8);
-        ws.aux.reserve((size_t)B * (V + ekgk::MFMA_PAD) * sizeof(float4));
-    } else {
-        ws.G.reserve((size_t)B * (V + 1) * (p.LP + 2) * 8);
+    if (st_lf[B] > 0x7fffffffLL || st_ch[B] > 0x7fffffffLL || st_lay[B] > 0x7fffffffLL)
+        fail(EKG_EINVAL, "launch group too large");
+    // device buffers (grow-only)
+    ws.lc.reserve((size_t)st_lay[B] * sizeof(ekgk::LayerConst));
+    ws.xy.reserve((size_t)std::max<int64_t>(1, st_xy[B]) * 16);
+    ws.G.reserve((size_t)rows * (g.mfma ? 16 : LP + 2) * 8);
+    if (g.mfma) ws.aux.reserve((size_t)rows * sizeof(float4));
+    ws.sums.reserve((size_t)std::max<int64_t>(1, c_off) * 3 * LP * 8);
+    ws.bnd.reserve((size_t)std::max<int64_t>(1, c_off) * sizeof(int2));
+    ws.pre_rng.reserve((size_t)std::max<int64_t>(1, pre_off) * sizeof(int2));
+    ws.pre_ab.reserve((size_t)std::max<int64_t>(1, pre_off) * 2 * LP * 8);
+    ws.partial.reserve((size_t)std::max<int64_t>(1, p_off) * 8);
+    if (!d_ecg) {
+        ws.ecg.reserve((size_t)std::max<int64_t>(1, st_out[B]) * 8);
+        d_ecg = ws.ecg.as<double>();
     }
-    ws.gsum.reserve((size_t)B * std::max(nch, 1) * 3 * p.LP * 8);
-    ws.partial.reserve((size_t)B * std::max(nch, 1) * p.LP * p.Tp * 8);
-
-    ekgk::LeadFieldArgs la;
-    la.grid_layer = h.d_layer.as<uint8_t>();
-    la.n0 = h.n0;
-    la.n1 = h.n1;
-    la.n2 = h.n2;
-    la.bricks = Lo.bricks.as<int4>();
-    la.brick_off = Lo.brick_off.as<int>();
-    la.brick_vox = Lo.brick_vox.as<int2>();
-    la.halo_off = Lo.halo_off.as<int>();
-    la.halo_cells = Lo.halo_cells.as<uint16_t>();
-    la.n_bricks = Lo.n_bricks;
-    la.vox_chunk = Lo.vox_chunk.as<int>();
-    la.vox_a64 = Lo.vox_a64.as<double>();
-    la.vox_a = Lo.vox_a.as<float2>();
-    la.chunks = Lo.chunks.as<int4>();
-    la.n_chunks = nch;
-    la.nhood = p.nhood;
-    la.lead_pos = d_lead_pos;
-    la.n_leads = p.L;
-    la.layer_k = d_layer_k;
-    la.n_layers = p.n_layers;
-    la.V = V;
-    la.LP = p.LP;
-    la.n_req = B;
-    la.rec = ws.G.as<double>();
-    la.aux = ws.aux.as<float4>();
-    la.gsum = ws.gsum.as<double>();
-    la.mfma = p.mfma;
-    hip_check(ekgk::launch_lead_field(la, stream), "lead_field launch");
-
-    hip_check(ekgk::launch_ap_tables(d_layer_k, p.n_layers, ws.tgrid.as<double>(),
-                                     (int)p.Tp, B, ws.rtab.as<double>(),
-                                     ws.lc.as<ekgk::LayerConst>(), stream),
-              "ap_tables launch");
-
-    ekgk::FusedArgs fa;
-    fa.vox_a = Lo.vox_a.as<float2>();
-    fa.rec = ws.G.as<double>();
-    fa.aux = ws.aux.as<float4>();
-    fa.mfma = p.mfma;
-    fa.n_leads = p.L;
-    fa.gsum = ws.gsum.as<double>();
-    fa.chunks = Lo.chunks.as<int4>();
-    fa.vox_a64 = Lo.vox_a64.as<double>();
-    fa.lc = ws.lc.as<ekgk::LayerConst>();
-    fa.layer_k = d_layer_k;
-    fa.rtab = ws.rtab.as<double>();
-    fa.tgrid = ws.tgrid.as<double>();
-    fa.thl = ws.thl.as<float2>();
-    fa.partial = ws.partial.as<double>();
-    fa.V = V;
-    fa.n_layers = p.n_layers;
-    fa.n_chunks = nch;
-    fa.Tp = (int)p.Tp;
-    fa.n_tiles = p.n_tiles;
-    fa.n_groups = p.n_groups;
-    fa.LP = p.LP;
-    fa.n_req = B;
+    // tables: uploaded only when they changed since the last group on this
+    // workspace (a repeated group replays without host traffic)
+    std::vector<char> tables;
+    append(tables, hdev.data(), hdev.size());
+    const size_t o_req = tables.size();
+    append(tables, rd.data(), rd.size());
+    const size_t o_st = tables.size();
+    for (auto* st : {&st_lf, &st_ch, &st_xy, &st_lay, &st_out}) append(tables, st->data(), st->size());
+    void* before = ws.hearts.p;
+    ws.hearts.reserve(tables.size());
+    if (ws.hearts.p != before) ws.h_tables.clear();
+    if (tables != ws.h_tables) {
+        hip_check(hipMemcpyAsync(ws.hearts.p, tables.data(), tables.size(), hipMemcpyHostToDevice, stream),
+                  "upload group tables");
+        ws.h_tables.swap(tables);
+    }
+    std::vector<char> grid_bytes;
+    append(grid_bytes, ws.h_tgrid.data(), ws.h_tgrid.size());
+    append(grid_bytes, ws.h_thl.data(), ws.h_thl.size());
+    before = ws.tgrid.p;
+    ws.tgrid.reserve(grid_bytes.size());
+    if (ws.tgrid.p != before) ws.h_grids.clear();
+    if (grid_bytes != ws.h_grids) {
+        hip_check(hipMemcpyAsync(ws.tgrid.p, grid_bytes.data(), grid_bytes.size(), hipMemcpyHostToDevice, stream),
+                  "upload time grids");
+        ws.h_grids.swap(grid_bytes);
+    }
+    if (!dev_in) {
+        ws.layer_k.reserve(ws.h_k.size() * 8);
+        ws.lead_pos.reserve(ws.h_pos.size() * 8);
+        hip_check(hipMemcpyAsync(ws.layer_k.p, ws.h_k.data(), ws.h_k.size() * 8, hipMemcpyHostToDevice, stream),
+                  "upload AP coefficients");
+        hip_check(hipMemcpyAsync(ws.lead_pos.p, ws.h_pos.data(), ws.h_pos.size() * 8, hipMemcpyHostToDevice,
+                                 stream),
+                  "upload lead positions");
+        d_k = ws.layer_k.as<double>();
+        d_pos = ws.lead_pos.as<double>();
+    }
+    const char* tb = static_cast<const char*>(ws.hearts.p);
+    const int64_t* stp = reinterpret_cast<const int64_t*>(tb + o_st);
+    ekgk::GroupArgs a{};
+    a.hearts = reinterpret_cast<const ekgk::HeartDev*>(tb);
+    a.reqs = reinterpret_cast<const ekgk::ReqDev*>(tb + o_req);
+    a.B = B;
+    a.LP = LP;
+    a.nhood = g.nhood;
+    a.mfma = g.mfma ? 1 : 0;
+    a.mfma_waves = g.waves;
+    a.lf_start = stp;
+    a.ch_start = stp + (B + 1);
+    a.xy_start = stp + 2 * (B + 1);
+    a.lay_start = stp + 3 * (B + 1);
+    a.out_start = stp + 4 * (B + 1);
+    a.n_lf = st_lf[B];
+    a.n_ch = st_ch[B];
+    a.n_xy = st_xy[B];
+    a.n_lay = st_lay[B];
+    a.n_out = st_out[B];
+    a.layer_k = d_k;
+    a.lead_pos = d_pos;
+    a.tgrid = ws.tgrid.as<double>();
+    a.thl = reinterpret_cast<const float2*>(static_cast<const char*>(ws.tgrid.p) + ws.h_tgrid.size() * 8);
+    a.lc = ws.lc.as<ekgk::LayerConst>();
+    a.xy = ws.xy.as<double2>();
+    a.rec = ws.G.as<double>();
+    a.aux = ws.aux.as<float4>();
+    a.sums = ws.sums.as<double>();
+    a.bnd = ws.bnd.as<int2>();
+    a.pre_rng = ws.pre_rng.as<int2>();
+    a.pre_ab = ws.pre_ab.as<double>();
+    a.partial = ws.partial.as<double>();
+    a.ecg = d_ecg;
+    hip_check(ekgk::launch_ap_tables(a, stream), "ap_tables launch");
+    hip_check(ekgk::launch_lead_field(a, stream), "lead_field launch");
+    hip_check(ekgk::launch_chunk_sums(a, stream), "chunk_sums launch");
+    hip_check(ekgk::launch_layer_prefix(a, stream), "layer_prefix launch");
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (time_fused) {
         e0 = ctx.get_event();
         e1 = ctx.get_event();
         hip_check(hipEventRecord(e0, stream), "hipEventRecord");
     }
-    hip_check(ekgk::launch_ecg_fused(fa, stream), "ecg_fused launch");
+    hip_check(ekgk::launch_ecg_fused(a, stream), "ecg_fused launch");
     if (time_fused) {
         hip_check(hipEventRecord(e1, stream), "hipEventRecord");
         ctx.pending.emplace_back(e0, e1);
     }
-    hip_check(ekgk::launch_ecg_reduce(ws.partial.as<double>(), d_ecg, B, nch, p.LP, p.L, (int)p.T,
-                                      (int)p.Tn, (int)p.Tp, stream),
-              "ecg_reduce launch");
+    hip_check(ekgk::launch_ecg_reduce(a, stream), "ecg_reduce launch");
 }
 
 void simulate(Heart& h, int batch, const ekg_sim_desc* descs, double* ecg)
@@ -496,20 +615,16 @@ void simulate(Heart& h, int batch, const ekg_sim_desc* descs, double* ecg)
     std::lock_guard<std::mutex> g1(ctx.mu);
     std::lock_guard<std::mutex> g2(h.mu);
     if (!ecg) fail(EKG_EINVAL, "ecg output buffer is required");
-    BatchPlan p = plan_batch(h, batch, descs);
-    if (p.T == 0) return;
-    p.mfma = use_mfma(ctx, p.L);
+    if (batch < 1 || !descs) fail(EKG_EINVAL, "batch must hold at least one descriptor");
+    std::vector<ReqSpec> specs;
+    for (int b = 0; b < batch; ++b) specs.push_back(make_spec(h, descs[b], true));
+    int64_t n_out = 0;
+    for (const ReqSpec& r : specs) n_out += (int64_t)r.L * r.T;
+    if (n_out == 0) return;
     ctx.bind();
-    ensure_layout(h);
-    const size_t n_out = (size_t)p.B * p.L * p.T;
-    if (h.layout.V == 0) {
-        std::fill(ecg, ecg + n_out, 0.0);
-        return;
-    }
-    upload_batch(p, descs, ctx.ws, ctx.stream);
-    enqueue_batch(ctx, h, p, ctx.ws.layer_k.as<double>(), ctx.ws.lead_pos.as<double>(),
-                  ctx.ws.ecg.as<double>(), ctx.ws, ctx.stream, ctx.profiling);
-    hip_check(hipMemcpyAsync(ecg, ctx.ws.ecg.p, n_out * 8, hipMemcpyDeviceToHost, ctx.stream),
+    GroupPlan g = plan_group(ctx, std::move(specs));
+    enqueue_group(ctx, g, ctx.ws, ctx.stream, nullptr, nullptr, nullptr, ctx.profiling);
+    hip_check(hipMemcpyAsync(ecg, ctx.ws.ecg.p, (size_t)n_out * 8, hipMemcpyDeviceToHost, ctx.stream),
               "download ECG");
     hip_check(hipStreamSynchronize(ctx.stream), "hipStreamSynchronize");
     if (ctx.profiling) ctx.collect_timings();
@@ -520,28 +635,20 @@ void simulate_device(Heart& h, const ekg_batch_dev& d)
     Ctx& ctx = *h.ctx;
     std::lock_guard<std::mutex> g1(ctx.mu);
     std::lock_guard<std::mutex> g2(h.mu);
-    if (!d.layer_k_dev || !d.lead_pos_dev || !d.ecg_dev)
-        fail(EKG_EINVAL, "device pointers are required");
-    BatchPlan p;
-    p.B = d.batch;
-    p.L = d.n_leads;
-    p.n_layers = d.n_layers;
-    p.nhood = d.nhood;
-    p.start = d.start;
-    p.length = d.length;
-    p.dt = d.dt;
-    p = plan_shape(h, p);
-    if (p.T == 0) return;
-    p.mfma = use_mfma(ctx, p.L);
+    if (!d.layer_k_dev || !d.lead_pos_dev || !d.ecg_dev) fail(EKG_EINVAL, "device pointers are required");
+    if (d.batch < 1 || d.batch > (1 << 20)) fail(EKG_EINVAL, "batch must hold 1..2^20 simulations");
+    ekg_sim_desc s{};
+    s.n_layers = d.n_layers;
+    s.n_leads = d.n_leads;
+    s.nhood = d.nhood;
+    s.start = d.start;
+    s.length = d.length;
+    s.dt = d.dt;
+    const ReqSpec one = make_spec(h, s, false);
+    if (one.T == 0) return;
     ctx.bind();
-    ensure_layout(h);
-    if (h.layout.V == 0) {
-        hip_check(hipMemsetAsync(d.ecg_dev, 0, (size_t)p.B * p.L * p.T * 8, ctx.stream),
-                  "hipMemsetAsync");
-        return;
-    }
-    enqueue_batch(ctx, h, p, d.layer_k_dev, d.lead_pos_dev, d.ecg_dev, ctx.ws, ctx.stream,
-                  ctx.profiling);
+    GroupPlan g = plan_group(ctx, std::vector<ReqSpec>(d.batch, one));
+    enqueue_group(ctx, g, ctx.ws, ctx.stream, d.layer_k_dev, d.lead_pos_dev, d.ecg_dev, ctx.profiling);
 }
 
 void excite(Heart& h, bool is_int, const double* transfer, int rows, int cols, const uint8_t* delay)

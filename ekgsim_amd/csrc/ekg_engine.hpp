@@ -36,14 +36,15 @@ struct DevBuf {
     DevBuf& operator=(const DevBuf&) = delete;
 };
 
-// Device workspace of one batch of simulations on one stream.
+// Device workspace of one launch group on one stream (grow-only buffers).
 struct Workspace {
-    DevBuf layer_k, lead_pos, tgrid, thl, rtab, lc, G, aux, gsum, partial, ecg;  // G: voxel records
-    std::vector<double> h_tgrid, h_k, h_pos;
+    DevBuf hearts, reqs, starts, layer_k, lead_pos, tgrid, thl, lc, xy, G, aux, sums, bnd, pre_rng,
+        pre_ab, partial, ecg;
+    std::vector<double> h_k, h_pos, h_tgrid;
     std::vector<float> h_thl;
-    double grid_dt = 0.0;
+    std::vector<char> h_tables;    // hearts + reqs + starts as last uploaded
+    std::vector<char> h_grids;     // tgrid + thl as last uploaded
 };
-
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -69,10 +70,14 @@ struct EcgLayout {
     int n_chunks = 0;
     int n_bricks = 0;
     int max_layer = 0;
+    int chunk_target = 0;          // voxels per chunk the layout was cut with
     std::vector<int4> h_chunks;
-    DevBuf vox_chunk, vox_a64, vox_a, chunks, bricks, brick_off, brick_vox, halo_off, halo_cells;
+    std::vector<int> h_layer_chunk;
+    std::vector<double> h_layer_t0;
+    DevBuf vox_chunk, vox_a64, vox_a, chunks, bricks, brick_off, brick_vox, halo_off, halo_cells,
+        layer_chunk, layer_t0;
+    ekgk::HeartDev dev{};          // device pointers and sizes for the kernels
 };
-
 struct Heart {
     Ctx* ctx = nullptr;
     int n0 = 0, n1 = 0, n2 = 0;
@@ -83,34 +88,46 @@ struct Heart {
     std::vector<double> h_act;
     bool excited = false;
     EcgLayout layout;
+    // tuning chosen by ekg_heart_autotune (0 = defaults)
+    int tune_chunk = 0;   // target voxels per chunk
+    int tune_waves = 0;   // waves per block of the matrix-core fused kernel
+    int tune_kernel = 0;  // EKG_KERNEL_* override
     std::mutex mu;
 };
 
-// Host-side description of one batch, validated.
-struct BatchPlan {
-    int B = 0, L = 0, LP = 0, n_layers = 0, nhood = 0;
-    bool mfma = true;
-    int64_t T = 0, Tn = 0, Tp = 0;
-    int n_tiles = 0, n_groups = 0;
+// One validated request of a launch group.
+struct ReqSpec {
+    Heart* h = nullptr;
+    int L = 0, nl = 0, nhood = 0;
     double start = 0, length = 0, dt = 0;
+    int64_t T = 0, Tn = 0, Tp = 0;
+    int n_tiles = 0;
+    const double* k = nullptr;    // host coefficients [nl][9] (host groups)
+    const double* pos = nullptr;  // host lead positions [L][3] (host groups)
+};
+
+// A launch group: requests sharing the neighbourhood and the kernel path.
+struct GroupPlan {
+    std::vector<ReqSpec> reqs;
+    int LP = 4, nhood = 0, waves = 4;
+    bool mfma = true;
 };
 
 int nhood_offsets(int type, int8_t dif[26][3]);
-const void* ctx_nb_table(const Ctx& ctx, int nhood);
 int64_t num_steps(double length, double dt);
-BatchPlan plan_shape(const Heart& h, BatchPlan p);
-bool use_mfma(const Ctx& ctx, int n_leads);
-BatchPlan plan_batch(const Heart& h, int batch, const ekg_sim_desc* descs);
+// Validates one descriptor against its heart (shape only when k/pos are device data).
+ReqSpec make_spec(Heart& h, const ekg_sim_desc& d, bool host_values);
+// Builds a group from specs (all with the same neighbourhood).
+GroupPlan plan_group(Ctx& ctx, std::vector<ReqSpec> reqs);
 void ensure_layout(Heart& h);
-// Copy the per-request coefficients and lead positions of host descriptors
-// into ws.layer_k / ws.lead_pos (and size ws.ecg).
-void upload_batch(const BatchPlan& p, const ekg_sim_desc* descs, Workspace& ws, hipStream_t stream);
-// Enqueue the whole batch on `stream`: d_layer_k [B][n_layers][9] and
-// d_lead_pos [B][L][3] in device memory, ECGs to d_ecg [B][L][T] FP64.
-void enqueue_batch(Ctx& ctx, Heart& h, const BatchPlan& p, const double* d_layer_k,
-                   const double* d_lead_pos, double* d_ecg, Workspace& ws, hipStream_t stream,
-                   bool time_fused);
-size_t batch_device_bytes(const Heart& h, const BatchPlan& p);
+size_t group_device_bytes(const GroupPlan& g);
+// Enqueue a whole group on `stream`.  With d_k/d_pos null the host values of
+// the specs are packed and uploaded; otherwise request r reads
+// d_k[r*nl*9 ...] and d_pos[r*L*3 ...] (uniform device batches).  ECGs go to
+// d_ecg (null: the workspace's ecg buffer), request r at the packed offset
+// sum_{q<r} L_q T_q.
+void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, const double* d_k,
+                   const double* d_pos, double* d_ecg, bool time_fused);
 void simulate(Heart& h, int batch, const ekg_sim_desc* descs, double* ecg);
 void simulate_device(Heart& h, const ekg_batch_dev& d);
 

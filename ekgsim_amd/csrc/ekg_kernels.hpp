@@ -1,4 +1,5 @@
-// ekg_kernels.hpp -- launch interface of the gfx950 kernels (ekg_kernels.hip).
+// ekg_kernels.hpp -- launch interface of the gfx950 kernels (ekg_ca.hip,
+// ekg_ecg.hip).
 #pragma once
 
 #include <hip/hip_runtime_api.h>
@@ -11,8 +12,20 @@ namespace ekgk {
 // plus a one-group look-ahead past the last chunk).
 constexpr int MFMA_PAD = 8;
 
+// ---- excitation sequence ---------------------------------------------------
+
 template <bool kInt> struct CaT { using type = double; };
 template <> struct CaT<true> { using type = int; };
+
+hipError_t launch_ca_init(bool is_int, const uint8_t* layer, const uint8_t* seed, void* dist,
+                          uint8_t* active, int n0, int n1, int n2, int nb1, int nb2, hipStream_t s);
+hipError_t launch_ca_sweep(bool is_int, const uint8_t* layer, const uint8_t* delay,
+                           const double* wtab, int wcols, void* dist, int n0, int n1, int n2,
+                           int nb0, int nb1, int nb2, const uint8_t* act_in, uint8_t* act_out,
+                           int* any_marked, hipStream_t s);
+hipError_t launch_ca_finalize(bool is_int, const void* dist, double* act, int64_t n, hipStream_t s);
+
+// ---- ECG --------------------------------------------------------------------
 
 // Per (request, layer) constants of the Wohlfart AP in the exp2 domain:
 //   V(tau) = S * (alpha*exp2(c4*tau) + beta) * exp2(c5*tau) * R(t) + k0,
@@ -24,65 +37,73 @@ struct LayerConst {
 };
 static_assert(sizeof(LayerConst) == 32, "LayerConst layout");
 
-struct LeadFieldArgs {
-    const uint8_t* grid_layer;   // [n0*n1*n2]
-    int n0, n1, n2;
-    const int4* bricks;          // [n_bricks] {z0, y0, x0, 0} of bricks holding myocardium
+// One heart of a launch group, as the kernels see it.
+struct HeartDev {
+    const uint8_t* grid_layer;   // [n0*n1*n2] layer per grid voxel
+    const int4* bricks;          // [n_bricks] {z0, y0, x0, 0} of 8^3 bricks holding myocardium
     const int* brick_off;        // [n_bricks + 1] offsets into brick_vox
     const int2* brick_vox;       // {sorted voxel index, local cell lz*64 + ly*8 + lx}
     const int* halo_off;         // [n_bricks + 1] offsets into halo_cells
     const uint16_t* halo_cells;  // myocardial cells of each brick + halo (index in the 10^3 box)
-    int n_bricks;
     const int* vox_chunk;        // [V] chunk of each sorted voxel
     const double* vox_a64;       // [V] activation time
     const float2* vox_a;         // [V] activation time (hi, lo)
     const int4* chunks;          // [n_chunks] {layer, v_begin, v_end, t_ref as float bits}
-    int n_chunks;
-    int nhood;                   // EKG_NHOOD_* (dipole neighbourhood)
-    const double* lead_pos;      // [n_req][n_leads][3]
-    int n_leads;
-    const double* layer_k;       // [n_req][n_layers][9]
-    int n_layers;
-    int64_t V;
-    int LP;                      // padded lead count (4, 8, 12, 16)
-    int n_req;
-    double* rec;                 // VALU: [n_req][V + 1][LP + 2] voxel records (G, P/Q, a)
-                                 // MFMA: [n_req][V + MFMA_PAD][16] G rows
-    float4* aux;                 // MFMA: [n_req][V + MFMA_PAD] {P, Q, a_hi, a_lo}
-    double* gsum;                // [n_req][n_chunks][3][LP]: sum G, sum G P, sum G Q
-    bool mfma;
+    const int* layer_chunk;      // [max_layer + 2]: chunks of layer l are [layer_chunk[l], layer_chunk[l+1])
+    const double* layer_t0;      // [max_layer + 1]: reference time of layer l
+    int n0, n1, n2;
+    int V, n_chunks, n_bricks, max_layer, pad;
 };
 
-struct FusedArgs {
-    const float2* vox_a;         // [V] activation time (hi, lo)
-    const double* rec;           // see LeadFieldArgs::rec
-    const float4* aux;           // MFMA only
-    const double* gsum;          // [n_req][n_chunks][3][LP]: sum G, sum G P, sum G Q
-    const int4* chunks;          // [n_chunks]
-    const double* vox_a64;       // [V] activation time, FP64
-    const LayerConst* lc;        // [n_req][n_layers]
-    const double* layer_k;       // [n_req][n_layers][9]
-    const double* rtab;          // [n_req][n_layers][Tp]
-    const double* tgrid;         // [Tp] time
-    const float2* thl;           // [Tp] time (hi, lo)
-    double* partial;             // [n_req][n_chunks][LP][Tp]
-    int64_t V;
-    int n_layers, n_chunks, Tp, n_tiles, n_groups, LP, n_req, n_leads;
-    bool mfma;
+// One request of a launch group.  Offsets index the group's packed arrays.
+struct ReqDev {
+    int heart, L, nl, T, Tn, Tp, n_tiles, pad;
+    int64_t k_off;    // layer_k[k_off + 9*(l-1) + q]
+    int64_t pos_off;  // lead_pos[pos_off + 3*i + c]
+    int64_t ecg_off;  // ecg[ecg_off + i*T + j]
+    int64_t t_off;    // tgrid/thl[t_off + j], j < Tp
+    int64_t lay_off;  // lc[lay_off + l-1]
+    int64_t xy_off;   // xy[xy_off + (l-1)*Tp + j] = {X_l(t_j), Y_l(t_j)}
+    int64_t g_off;    // record row of sorted voxel v: g_off + v
+    int64_t c_off;    // per-chunk arrays: sums[(c_off + c)*3*LP ...], bnd[c_off + c]
+    int64_t p_off;    // partial[p_off + (c*LP + i)*Tp + j]
+    int64_t pre_off;  // prefix tables: [(pre_off + (l-1)*(n_tiles+1) + tile)]
 };
 
-hipError_t launch_ca_init(bool is_int, const uint8_t* layer, const uint8_t* seed, void* dist,
-                          uint8_t* active, int n0, int n1, int n2, int nb1, int nb2, hipStream_t s);
-hipError_t launch_ca_sweep(bool is_int, const uint8_t* layer, const uint8_t* delay,
-                           const double* wtab, int wcols, void* dist, int n0, int n1, int n2,
-                           int nb0, int nb1, int nb2, const uint8_t* act_in, uint8_t* act_out,
-                           int* any_marked, hipStream_t s);
-hipError_t launch_ca_finalize(bool is_int, const void* dist, double* act, int64_t n, hipStream_t s);
-hipError_t launch_lead_field(const LeadFieldArgs& a, hipStream_t s);
-hipError_t launch_ap_tables(const double* layer_k, int n_layers, const double* tgrid, int Tp,
-                            int n_req, double* rtab, LayerConst* lc, hipStream_t s);
-hipError_t launch_ecg_fused(const FusedArgs& a, hipStream_t s);
-hipError_t launch_ecg_reduce(const double* partial, double* ecg, int n_req, int n_chunks, int LP,
-                             int n_leads, int T, int Tn, int Tp, hipStream_t s);
+// A launch group: requests of any hearts, lead counts and time grids that
+// share the dipole neighbourhood, the padded lead count LP and the kernel
+// path.  Work-item starts are exclusive prefix sums over requests (B + 1).
+struct GroupArgs {
+    const HeartDev* hearts;
+    const ReqDev* reqs;
+    int B, LP, nhood, mfma, mfma_waves;
+    const int64_t* lf_start;   // lead-field blocks (bricks) per request
+    const int64_t* ch_start;   // chunks per request
+    const int64_t* xy_start;   // nl * Tp per request
+    const int64_t* lay_start;  // nl per request
+    const int64_t* out_start;  // L * T per request
+    int64_t n_lf, n_ch, n_xy, n_lay, n_out;  // totals
+    const double* layer_k;
+    const double* lead_pos;
+    const double* tgrid;       // FP64 sample times
+    const float2* thl;         // FP32 hi/lo of the same times
+    LayerConst* lc;
+    double2* xy;
+    double* rec;               // MFMA: 16-double G rows; VALU: LP+2 doubles (G, {P,Q}, {a_hi,a_lo})
+    float4* aux;               // MFMA: {P, Q, a_hi, a_lo}
+    double* sums;              // per (request, chunk): [3][LP] sum G, sum G P', sum G Q'
+    int2* bnd;                 // per (request, chunk): {first non-zero tile, first saturated tile}
+    int2* pre_rng;             // per (request, layer, tile): transition chunks [lo, hi)
+    double* pre_ab;            // per (request, layer, tile): [2][LP]; tile n_tiles: [LP] sum G
+    double* partial;
+    double* ecg;
+};
+
+hipError_t launch_ap_tables(const GroupArgs& a, hipStream_t s);
+hipError_t launch_lead_field(const GroupArgs& a, hipStream_t s);
+hipError_t launch_chunk_sums(const GroupArgs& a, hipStream_t s);
+hipError_t launch_layer_prefix(const GroupArgs& a, hipStream_t s);
+hipError_t launch_ecg_fused(const GroupArgs& a, hipStream_t s);
+hipError_t launch_ecg_reduce(const GroupArgs& a, hipStream_t s);
 
 }  // namespace ekgk

@@ -4,17 +4,19 @@
 // job queues topped up by the master (ParallelFramework.h:286-430); an
 // external evaluator is one process per simulation (ExternalEvaluation.h:95-151).
 // Here one worker thread owns n_streams launch slots.  Requests queue FIFO;
-// whenever a slot is free the worker takes the oldest request, adds every
-// queued request with the same heart and shape (leads, layers,
-// neighbourhood, start, length, time step) up to max_batch, admits the group
-// if its device bytes fit the budget and the in-flight cap, and enqueues it
-// on the slot's stream.  Groups on different slots run concurrently, so a
-// request that arrives while others run is admitted as soon as a slot and
-// memory free up, and its kernels overlap with the running ones.  Results
-// are bitwise independent of the grouping: every request's partial sums
-// come from its own block rows over the heart's fixed chunking.
+// whenever a slot is free the worker takes the oldest request and every
+// queued request with the same dipole neighbourhood -- any heart, lead
+// count, duration and time step -- up to max_batch, shrinks the group until
+// its device bytes fit the budget and the in-flight cap, and enqueues it on
+// the slot's stream as one launch group (one launch per kernel).  Groups on
+// different slots run concurrently, so a request that arrives while others
+// run is admitted as soon as a slot and memory free up, and its kernels
+// overlap with the running ones.  Results are bitwise independent of the
+// grouping: every request's sums come from its own work items over its
+// heart's fixed chunking, in a fixed order.
 #include "ekg_server.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -48,18 +50,13 @@ struct Slot {
     Workspace ws;
     bool busy = false;
     std::vector<Request*> reqs;
-    BatchPlan plan;
+    std::vector<int64_t> out_off;
     double* h_out = nullptr;
     size_t h_out_cap = 0;
     size_t bytes = 0;
 };
 
-bool same_shape(const Request& a, const Request& b)
-{
-    return a.heart_id == b.heart_id && a.desc.n_leads == b.desc.n_leads &&
-           a.desc.n_layers == b.desc.n_layers && a.desc.nhood == b.desc.nhood &&
-           a.desc.start == b.desc.start && a.desc.length == b.desc.length && a.desc.dt == b.desc.dt;
-}
+bool same_group(const Request& a, const Request& b) { return a.desc.nhood == b.desc.nhood; }
 
 }  // namespace
 
@@ -98,10 +95,9 @@ void Server::fail_group(std::vector<Request*>& g, int code, const std::string& m
 // Called with mu held once the slot's completion event has fired.
 void Server::retire(Slot& s)
 {
-    const BatchPlan& p = s.plan;
     for (size_t b = 0; b < s.reqs.size(); ++b) {
         Request* r = s.reqs[b];
-        r->ecg.assign(s.h_out + b * p.L * p.T, s.h_out + (b + 1) * p.L * p.T);
+        r->ecg.assign(s.h_out + s.out_off[b], s.h_out + s.out_off[b + 1]);
         r->state = Request::Done;
         r->t_done = Clock::now();
         ++completed;
@@ -155,28 +151,28 @@ void Server::run()
             const int cap_inflight = opts.max_inflight - inflight;
             for (Request* r : pending) {
                 if ((int)group.size() >= opts.max_batch || (int)group.size() >= cap_inflight) break;
-                if (same_shape(*head, *r)) group.push_back(r);
+                if (same_group(*head, *r)) group.push_back(r);
             }
-            Heart* h = hearts[head->heart_id];
-            BatchPlan plan;
+            GroupPlan plan;
             size_t bytes = 0;
             bool ok = !group.empty();
             if (ok) {
                 try {
-                    std::lock_guard<std::mutex> hg(h->mu);
-                    ensure_layout(*h);
-                    std::vector<ekg_sim_desc> ds;
-                    for (Request* r : group) ds.push_back(r->desc);
-                    plan = plan_batch(*h, (int)ds.size(), ds.data());
-                    plan.mfma = use_mfma(ctx, plan.L);
-                    bytes = batch_device_bytes(*h, plan);
+                    std::vector<ReqSpec> specs;
+                    for (Request* r : group) {
+                        Heart* h = hearts[r->heart_id];
+                        std::lock_guard<std::mutex> hg(h->mu);
+                        ensure_layout(*h);
+                        specs.push_back(make_spec(*h, r->desc, true));
+                    }
+                    plan = plan_group(ctx, specs);
+                    bytes = group_device_bytes(plan);
                     // shrink the group until it fits the budget
                     while (group.size() > 1 && bytes_in_use + bytes > opts.hbm_budget_bytes) {
                         group.pop_back();
-                        ds.pop_back();
-                        plan = plan_batch(*h, (int)ds.size(), ds.data());
-                        plan.mfma = use_mfma(ctx, plan.L);
-                        bytes = batch_device_bytes(*h, plan);
+                        specs.pop_back();
+                        plan = plan_group(ctx, specs);
+                        bytes = group_device_bytes(plan);
                     }
                 } catch (const Error& e) {
                     std::vector<Request*> one{head};
@@ -210,32 +206,39 @@ void Server::run()
                 peak_bytes = std::max(peak_bytes, bytes_in_use);
                 free_slot->busy = true;
                 free_slot->reqs = group;
-                free_slot->plan = plan;
                 free_slot->bytes = bytes;
+                free_slot->out_off.assign(1, 0);
+                for (const ReqSpec& q : plan.reqs)
+                    free_slot->out_off.push_back(free_slot->out_off.back() + (int64_t)q.L * q.T);
                 ++groups;
-                std::vector<ekg_sim_desc> ds;
-                for (Request* r : group) ds.push_back(r->desc);
                 lk.unlock();
                 std::string err;
                 int code = EKG_OK;
                 try {
-                    std::lock_guard<std::mutex> hg(h->mu);
+                    // the group's hearts stay locked (in address order) while
+                    // their layouts are read for the launch
+                    std::vector<Heart*> hl;
+                    for (const ReqSpec& q : plan.reqs)
+                        if (std::find(hl.begin(), hl.end(), q.h) == hl.end()) hl.push_back(q.h);
+                    std::sort(hl.begin(), hl.end());
+                    std::vector<std::unique_lock<std::mutex>> locks;
+                    for (Heart* h : hl) locks.emplace_back(h->mu);
                     Slot& s = *free_slot;
-                    const size_t n_out = (size_t)plan.B * plan.L * plan.T;
+                    const size_t n_out = (size_t)s.out_off.back();
                     if (s.h_out_cap < n_out) {
                         if (s.h_out) (void)hipHostFree(s.h_out);
                         s.h_out = nullptr;
                         s.h_out_cap = 0;
-                        hip_check(hipHostMalloc((void**)&s.h_out, n_out * 8, hipHostMallocDefault),
+                        hip_check(hipHostMalloc((void**)&s.h_out, std::max<size_t>(n_out, 1) * 8,
+                                                hipHostMallocDefault),
                                   "hipHostMalloc");
                         s.h_out_cap = n_out;
                     }
-                    upload_batch(plan, ds.data(), s.ws, s.stream);
-                    enqueue_batch(ctx, *h, plan, s.ws.layer_k.as<double>(), s.ws.lead_pos.as<double>(),
-                                  s.ws.ecg.as<double>(), s.ws, s.stream, false);
-                    hip_check(hipMemcpyAsync(s.h_out, s.ws.ecg.p, n_out * 8, hipMemcpyDeviceToHost,
-                                             s.stream),
-                              "download ECGs");
+                    enqueue_group(ctx, plan, s.ws, s.stream, nullptr, nullptr, nullptr, false);
+                    if (n_out)
+                        hip_check(hipMemcpyAsync(s.h_out, s.ws.ecg.p, n_out * 8, hipMemcpyDeviceToHost,
+                                                 s.stream),
+                                  "download ECGs");
                     hip_check(hipEventRecord(s.done, s.stream), "hipEventRecord");
                 } catch (const Error& e) {
                     code = e.code;

@@ -135,9 +135,11 @@ int64_t ekg_num_steps(double length, double dt);
  * ecg: [n_leads][T] FP64. */
 int ekg_simulate(ekg_heart *h, const ekg_sim_desc *d, double *ecg);
 
-/* B ECGs of one heart in one set of launches.  All descs must share n_leads,
- * nhood, start, length and dt; layer_k and lead_pos differ per entry.
- * ecg: [batch][n_leads][T]. */
+/* B ECGs of one heart in one set of launches (one launch per kernel).  The
+ * descriptors must share nhood; lead count, layer count, start, length and
+ * time step may differ per entry.  ECG b is written at offset
+ * sum_{q<b} n_leads_q * T_q, as [n_leads_b][T_b]: for equal shapes that is
+ * [batch][n_leads][T].  Every ECG is bitwise the same as from ekg_simulate. */
 int ekg_simulate_batch(ekg_heart *h, int batch, const ekg_sim_desc *descs, double *ecg);
 
 /* B ECGs whose per-request arrays already live in device memory (HBM):

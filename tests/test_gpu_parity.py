@@ -221,3 +221,27 @@ def test_pipeline_matches_reference_golden(ctx):
     pos = E.move_leads(g["pos_file"], g["disp"])
     got = h.simulate(g["layer_k"], pos, "3D4", 0.0, 500.0, 1.0)
     assert rel_l2(got, g["ecg"]) <= TOL
+
+
+def test_mixed_shapes_one_group_equal_alone(ctx):
+    """Requests with different lead counts, durations, start times and time
+    steps in one launch group equal the single-request calls bitwise and
+    the FP64 oracle within 1e-5."""
+    h = S.make_heart(28, seed=3)
+    g = _heart(ctx, h)
+    g.excite_delay(h.delay)
+    act = O.excite_int(h.layer, h.seed, h.delay).astype(np.float64)
+    rng = np.random.default_rng(5)
+    reqs = []
+    for r, (L, start, length, dt) in enumerate([(2, 0.0, 600.0, 1.0), (7, 50.0, 250.0, 0.5),
+                                                (12, 0.0, 130.0, 0.25), (3, 10.0, 700.0, 1.0),
+                                                (16, 0.0, 64.0, 1.0)]):
+        x = S.random_params(3, L, 900 + r)
+        reqs.append((S.layer_k_from_params(x, 3),
+                     S.move_leads(S.lead_positions(28, L), S.displacements(x, 3, L)), start, length, dt))
+    many = g.simulate_many(reqs)
+    for (k, pos, start, length, dt), got in zip(reqs, many):
+        alone = g.simulate(k, pos, "3D4", start, length, dt)
+        assert np.array_equal(got, alone)
+        ref = O.run(h.layer, act, k, pos, "3D4", start, length, dt)
+        assert rel_l2(got, ref) <= TOL
