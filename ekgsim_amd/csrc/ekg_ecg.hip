@@ -208,30 +208,29 @@ template <> struct Nhood<3> {
 };
 
 // One block of 512 threads per (request, 8^3 brick holding myocardium).
-// For two leads at a time the lead weight m_i of every myocardial cell of the
-// brick and its 1-voxel halo (a compact per-brick list) is computed once into
-// LDS (component arrays, zero outside the myocardium); thread t < n then
-// gathers for the brick's t-th myocardial voxel u
+// For four leads at a time the lead weight m_i of every myocardial cell of
+// the brick and its 1-voxel halo (a compact per-brick list) is computed once
+// in FP64 and kept in LDS as FP32 (zero outside the myocardium); thread t < n
+// then gathers, in FP64, for the brick's t-th myocardial voxel u
 //   G_i(u) = sum_{d in N} M_i(u+d) . d + m_i(u) . s_u,
-//   M_i = m_i on myocardium and 0 elsewhere,  s_u = sum of d with u+d myocardial,
+//   M_i = m_i on myocardium and 0 elsewhere,  s_u = sum of those d,
 // which is sum_{d in N, u+d myo} (m_i(u+d) + m_i(u)) . d -- the coefficient of
 // V_u(t) in the reference's dipole sum regrouped by cell (simulator.cpp:509-537).
-// The voxel's record (row g_off + v of the request) receives
-//   VALU layout (RS = LP + 2 doubles): [0, LP) G zero-padded,
-//     LP: (P_u, Q_u) floats, LP+1: (a_hi, a_lo) floats;
-//   MFMA layout: 16-double G rows and float4 {P_u, Q_u, a_hi, a_lo} in aux;
-// with P_u = k2(1-k3) e^{(k4+k5)(a_u - t_ref)}, Q_u = k2 k3 e^{k5 (a_u - t_ref)},
+// The FP32 rounding of m changes the ECG by about 1e-7 relative or less
+// (G itself stays FP64: its own rounding would cost up to 3e-6 on small
+// hearts).  The voxel's record (row g_off + v of the request) receives a
+// 16-double G row (zero-padded) and aux = {P_u, Q_u, a_hi, a_lo} with
+// P_u = k2(1-k3) e^{(k4+k5)(a_u - t_ref)}, Q_u = k2 k3 e^{k5 (a_u - t_ref)},
 // t_ref the reference time of the voxel's chunk.
-template <int LP, bool kMfma, int NH>
+template <int LP, int NH>
 __global__ __launch_bounds__(512) void lead_field_kernel(
     const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
     const int64_t* __restrict__ lf_start, const double* __restrict__ lead_pos,
     const double* __restrict__ layer_k, double* __restrict__ rec, float4* __restrict__ aux)
 {
     using N = Nhood<NH>;
-    constexpr int RS = kMfma ? 16 : LP + 2;
     __shared__ uint8_t s_myo[LN];
-    __shared__ double s_m[2][3][LN];
+    __shared__ float s_m[4][LN][3];
     const int r = find_req(lf_start, B, blockIdx.x);
     const ReqDev R = reqs[r];
     const HeartDev& H = hearts[R.heart];
@@ -243,7 +242,7 @@ __global__ __launch_bounds__(512) void lead_field_kernel(
     for (int q = t; q < LN; q += blockDim.x) {
         s_myo[q] = 0;
 #pragma unroll
-        for (int w = 0; w < 2; ++w) s_m[w][0][q] = s_m[w][1][q] = s_m[w][2][q] = 0.0;
+        for (int w = 0; w < 4; ++w) s_m[w][q][0] = s_m[w][q][1] = s_m[w][q][2] = 0.0f;
     }
     __syncthreads();
     for (int q = t; q < nc; q += blockDim.x) s_myo[H.halo_cells[c0 + q]] = 1;
@@ -265,44 +264,43 @@ __global__ __launch_bounds__(512) void lead_field_kernel(
     }
     const double* pos = lead_pos + R.pos_off;
     const int n_leads = R.L;
-    // the voxel's whole record is assembled in registers and stored once
-    double row[RS];
+    double row[16];
 #pragma unroll
-    for (int i = 0; i < RS; ++i) row[i] = 0.0;
+    for (int i = 0; i < 16; ++i) row[i] = 0.0;
 #pragma unroll
-    for (int i = 0; i < LP; i += 2) {
-        if (i >= n_leads) break;
-        const bool two = i + 1 < n_leads;
-        for (int q = t; q < nc; q += blockDim.x) {
+    for (int i0 = 0; i0 < LP; i0 += 4) {
+        if (i0 >= n_leads) break;
+        const int n4 = n_leads - i0 < 4 ? n_leads - i0 : 4;
+        // (cell, lead) items over all threads
+        for (int it = t; it < nc * n4; it += blockDim.x) {
+            const int w = it / nc;
+            const int q = it - w * nc;
             const int cell = H.halo_cells[c0 + q];
             const int gz = bi.x - 1 + cell / (LH * LH), gy = bi.y - 1 + (cell / LH) % LH,
                       gx = bi.z - 1 + cell % LH;
-            const double z = (double)(gz + 1), y = (double)(gy + 1), x = (double)(gx + 1);
+            const double* p = pos + 3 * (i0 + w);
             double m0, m1, m2;
-            lead_f(pos[3 * i] - z, pos[3 * i + 1] - y, pos[3 * i + 2] - x, m0, m1, m2);
-            s_m[0][0][cell] = m0;
-            s_m[0][1][cell] = m1;
-            s_m[0][2][cell] = m2;
-            if (two) {
-                lead_f(pos[3 * i + 3] - z, pos[3 * i + 4] - y, pos[3 * i + 5] - x, m0, m1, m2);
-                s_m[1][0][cell] = m0;
-                s_m[1][1][cell] = m1;
-                s_m[1][2][cell] = m2;
-            }
+            lead_f(p[0] - (double)(gz + 1), p[1] - (double)(gy + 1), p[2] - (double)(gx + 1), m0, m1, m2);
+            s_m[w][cell][0] = (float)m0;
+            s_m[w][cell][1] = (float)m1;
+            s_m[w][cell][2] = (float)m2;
         }
         __syncthreads();
         if (t < nv) {
 #pragma unroll
-            for (int w = 0; w < 2; ++w) {
-                double g = s_m[w][0][h] * sd0 + s_m[w][1][h] * sd1 + s_m[w][2][h] * sd2;
+            for (int w = 0; w < 4; ++w) {
+                if (w < n4) {
+                    double g = (double)s_m[w][h][0] * sd0 + (double)s_m[w][h][1] * sd1 +
+                               (double)s_m[w][h][2] * sd2;
 #pragma unroll
-                for (int q = 0; q < N::n; ++q) {
-                    const int hn = h + (N::d[q][0] * LH + N::d[q][1]) * LH + N::d[q][2];
-                    if (N::d[q][0]) g += s_m[w][0][hn] * N::d[q][0];
-                    if (N::d[q][1]) g += s_m[w][1][hn] * N::d[q][1];
-                    if (N::d[q][2]) g += s_m[w][2][hn] * N::d[q][2];
+                    for (int q = 0; q < N::n; ++q) {
+                        const int hn = h + (N::d[q][0] * LH + N::d[q][1]) * LH + N::d[q][2];
+                        if (N::d[q][0]) g += (double)s_m[w][hn][0] * N::d[q][0];
+                        if (N::d[q][1]) g += (double)s_m[w][hn][1] * N::d[q][1];
+                        if (N::d[q][2]) g += (double)s_m[w][hn][2] * N::d[q][2];
+                    }
+                    row[i0 + w] = g;
                 }
-                if (w == 0 || two) row[i + w] = g;
             }
         }
         __syncthreads();
@@ -311,19 +309,12 @@ __global__ __launch_bounds__(512) void lead_field_kernel(
         const int4 ch = H.chunks[H.vox_chunk[v]];
         const double* k = layer_k + R.k_off + 9 * (ch.x - 1);
         const double da = H.vox_a64[v] - (double)__int_as_float(ch.w);
-        const float2 pq{(float)(k[2] * (1.0 - k[3]) * exp((k[4] + k[5]) * da)),
-                        (float)(k[2] * k[3] * exp(k[5] * da))};
         const float2 av = H.vox_a[v];
-        double* gv = rec + (R.g_off + v) * RS;
-        if constexpr (kMfma) {
-            aux[R.g_off + v] = float4{pq.x, pq.y, av.x, av.y};
-        } else {
-            row[LP] = __builtin_bit_cast(double, pq);
-            row[LP + 1] = __builtin_bit_cast(double, av);
-        }
+        aux[R.g_off + v] = float4{(float)(k[2] * (1.0 - k[3]) * exp((k[4] + k[5]) * da)),
+                                  (float)(k[2] * k[3] * exp(k[5] * da)), av.x, av.y};
+        double2* gv = reinterpret_cast<double2*>(rec + (R.g_off + v) * 16);
 #pragma unroll
-        for (int i = 0; i < RS; i += 2)
-            *reinterpret_cast<double2*>(gv + i) = double2{row[i], row[i + 1]};
+        for (int i = 0; i < 16; i += 2) gv[i / 2] = double2{row[i], row[i + 1]};
     }
 }
 
@@ -342,7 +333,7 @@ __global__ __launch_bounds__(512) void lead_field_kernel(
 // are skipped for every voxel (the fused kernel's skip test on the earliest
 // voxel at the tile's last step), tiles from f on are saturated in FP64 for
 // every voxel (test on the latest voxel at the tile's first step).
-template <int LP, bool kMfma>
+template <int LP>
 __global__ __launch_bounds__(256) void chunk_sums_kernel(
     const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
     const int64_t* __restrict__ ch_start, const double* __restrict__ layer_k,
@@ -350,7 +341,6 @@ __global__ __launch_bounds__(256) void chunk_sums_kernel(
     const float2* __restrict__ thl, const double* __restrict__ rec, double* __restrict__ sums,
     int2* __restrict__ bnd)
 {
-    constexpr int RS = kMfma ? 16 : LP + 2;
     __shared__ double s_wave[4][3 * LP];
     const int r = find_req(ch_start, B, blockIdx.x);
     const ReqDev R = reqs[r];
@@ -360,65 +350,41 @@ __global__ __launch_bounds__(256) void chunk_sums_kernel(
     const double* k = layer_k + R.k_off + 9 * (ch.x - 1);
     const double t0 = H.layer_t0[ch.x];
     const double alpha = k[2] * (1.0 - k[3]), beta = k[2] * k[3], k45 = k[4] + k[5], k5 = k[5];
-    const double* recb = rec + R.g_off * RS;
+    const double* recb = rec + R.g_off * 16;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     double* out = sums + (R.c_off + c) * 3 * LP;
-    if constexpr (kMfma) {
-        // 16-double rows: a wave loads 4 whole rows per instruction (lane =
-        // 16 * row slot + lead), one coalesced 512-byte read.  For the wave's
-        // 16 rows, lanes 0-15 compute P' and lanes 16-31 Q' in one exp (lanes
-        // 32-63 repeat them), broadcast by shuffles.
-        const int col = lane & 15, slot = lane >> 4;
-        const bool is_q = (lane >> 4) & 1;
-        const double coef = is_q ? beta : alpha, rate = is_q ? k5 : k45;
-        double sg = 0.0, sp = 0.0, sq = 0.0;
-        for (int v0 = ch.y + 16 * w; v0 < ch.z; v0 += 64) {
-            const int ve = v0 + col;
-            const double e = ve < ch.z ? coef * exp(rate * (H.vox_a64[ve] - t0)) : 0.0;
+    // 16-double rows: a wave loads 4 whole rows per instruction (lane = 16 *
+    // row slot + lead), one coalesced 512-byte read.  For the wave's 16 rows,
+    // lanes 0-15 compute P' and lanes 16-31 Q' in one exp (lanes 32-63 repeat
+    // them), broadcast by shuffles.
+    const int col = lane & 15, slot = lane >> 4;
+    const bool is_q = (lane >> 4) & 1;
+    const double coef = is_q ? beta : alpha, rate = is_q ? k5 : k45;
+    double sg = 0.0, sp = 0.0, sq = 0.0;
+    for (int v0 = ch.y + 16 * w; v0 < ch.z; v0 += 64) {
+        const int ve = v0 + col;
+        const double e = ve < ch.z ? coef * exp(rate * (H.vox_a64[ve] - t0)) : 0.0;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int row = 4 * q + slot;
-                const int v = v0 + row;
-                const double g = v < ch.z ? recb[(int64_t)v * 16 + col] : 0.0;
-                const double P = __shfl(e, row, 64), Q = __shfl(e, 16 + row, 64);
-                sg += g;
-                sp = __builtin_fma(g, P, sp);
-                sq = __builtin_fma(g, Q, sq);
-            }
+        for (int q = 0; q < 4; ++q) {
+            const int row = 4 * q + slot;
+            const int v = v0 + row;
+            const double g = v < ch.z ? recb[(int64_t)v * 16 + col] : 0.0;
+            const double P = __shfl(e, row, 64), Q = __shfl(e, 16 + row, 64);
+            sg += g;
+            sp = __builtin_fma(g, P, sp);
+            sq = __builtin_fma(g, Q, sq);
         }
-        sg += __shfl_down(sg, 16, 64);
-        sp += __shfl_down(sp, 16, 64);
-        sq += __shfl_down(sq, 16, 64);
-        sg += __shfl_down(sg, 32, 64);
-        sp += __shfl_down(sp, 32, 64);
-        sq += __shfl_down(sq, 32, 64);
-        if (lane < LP) {
-            s_wave[w][lane] = sg;
-            s_wave[w][LP + lane] = sp;
-            s_wave[w][2 * LP + lane] = sq;
-        }
-    } else {
-        double part[3 * LP];
-#pragma unroll
-        for (int i = 0; i < 3 * LP; ++i) part[i] = 0.0;
-        for (int v = ch.y + threadIdx.x; v < ch.z; v += blockDim.x) {
-            const double da = H.vox_a64[v] - t0;
-            const double P = alpha * exp(k45 * da), Q = beta * exp(k5 * da);
-#pragma unroll
-            for (int i = 0; i < LP; ++i)
-                if (i < R.L) {
-                    const double g = recb[(int64_t)v * RS + i];
-                    part[i] += g;
-                    part[LP + i] = __builtin_fma(g, P, part[LP + i]);
-                    part[2 * LP + i] = __builtin_fma(g, Q, part[2 * LP + i]);
-                }
-        }
-#pragma unroll
-        for (int i = 0; i < 3 * LP; ++i) {
-            double x = part[i];
-            for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
-            if (lane == 0) s_wave[w][i] = x;
-        }
+    }
+    sg += __shfl_down(sg, 16, 64);
+    sp += __shfl_down(sp, 16, 64);
+    sq += __shfl_down(sq, 16, 64);
+    sg += __shfl_down(sg, 32, 64);
+    sp += __shfl_down(sp, 32, 64);
+    sq += __shfl_down(sq, 32, 64);
+    if (lane < LP) {
+        s_wave[w][lane] = sg;
+        s_wave[w][LP + lane] = sp;
+        s_wave[w][2 * LP + lane] = sq;
     }
     __syncthreads();
     for (int i = threadIdx.x; i < 3 * LP; i += blockDim.x) {
@@ -515,7 +481,7 @@ __global__ __launch_bounds__(64) void layer_prefix_kernel(
 template <int LP>
 __global__ __launch_bounds__(256) void ecg_fused_kernel(
     const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
-    const int64_t* __restrict__ ch_start, const double* __restrict__ rec,
+    const int64_t* __restrict__ ch_start, const double* __restrict__ rec, const float4* __restrict__ aux,
     const LayerConst* __restrict__ lc, const double* __restrict__ layer_k,
     const double2* __restrict__ xy, const float2* __restrict__ thl, const int2* __restrict__ bnd,
     double* __restrict__ partial)
@@ -534,8 +500,8 @@ __global__ __launch_bounds__(256) void ecg_fused_kernel(
     const double tref = (double)__int_as_float(ch.w);
     const double dtr = tref - H.layer_t0[l];
     const double ex = exp((k[4] + k[5]) * dtr), ey = exp(k[5] * dtr);
-    constexpr int RS = LP + 2;
-    const double* Rb = rec + R.g_off * RS;
+    const double* Rb = rec + R.g_off * 16;
+    const float4* Ab = aux + R.g_off;
     const double2* XY = xy + R.xy_off + (int64_t)(l - 1) * R.Tp;
     const float2* th = thl + R.t_off;
 
@@ -564,47 +530,15 @@ __global__ __launch_bounds__(256) void ecg_fused_kernel(
                     return kc.c1 * tau_of(tl.x, tl.y, a.x, a.y) < SKIP_X;
                 });
         }
-        // Both loops keep the next voxel's record in flight while the current
-        // one is consumed (the one-ahead load of the last voxel reads the next
-        // record, which always exists: rows run to V + 1).
-        if (ch.y < s1) {
-            const double* rp = Rb + (int64_t)ch.y * RS;
-            double rn[RS];
+        // voxel data is wave-uniform (scalar loads); the G row feeds FP64 FMAs
+        for (int v = ch.y; v < s2; ++v) {
+            const float4 q = Ab[v];
+            float ap = __builtin_fmaf(q.x, X, q.y * Y);
+            if (v >= s1) ap *= fast_rcp(1.0f + fast_exp2(kc.c1 * tau_of(tt.x, tt.y, q.z, q.w)));
+            const double apd = (double)ap;
+            const double* g = Rb + (int64_t)v * 16;
 #pragma unroll
-            for (int i = 0; i < RS; ++i) rn[i] = rp[i];
-            for (int v = ch.y; v < s1; ++v) {
-                double rr[RS];
-#pragma unroll
-                for (int i = 0; i < RS; ++i) rr[i] = rn[i];
-                rp += RS;
-#pragma unroll
-                for (int i = 0; i < RS; ++i) rn[i] = rp[i];
-                const float2 pq = __builtin_bit_cast(float2, rr[LP]);
-                const double apd = (double)__builtin_fmaf(pq.x, X, pq.y * Y);
-#pragma unroll
-                for (int i = 0; i < LP; ++i) acc[i] = __builtin_fma(rr[i], apd, acc[i]);
-            }
-        }
-        if (s1 < s2) {
-            const double* rp = Rb + (int64_t)s1 * RS;
-            double rn[RS];
-#pragma unroll
-            for (int i = 0; i < RS; ++i) rn[i] = rp[i];
-            for (int v = s1; v < s2; ++v) {
-                double rr[RS];
-#pragma unroll
-                for (int i = 0; i < RS; ++i) rr[i] = rn[i];
-                rp += RS;
-#pragma unroll
-                for (int i = 0; i < RS; ++i) rn[i] = rp[i];
-                const float2 pq = __builtin_bit_cast(float2, rr[LP]);
-                const float2 a = __builtin_bit_cast(float2, rr[LP + 1]);
-                const float tau = tau_of(tt.x, tt.y, a.x, a.y);
-                const float e1 = fast_exp2(kc.c1 * tau);
-                const double apd = (double)(__builtin_fmaf(pq.x, X, pq.y * Y) * fast_rcp(1.0f + e1));
-#pragma unroll
-                for (int i = 0; i < LP; ++i) acc[i] = __builtin_fma(rr[i], apd, acc[i]);
-            }
+            for (int i = 0; i < LP; ++i) acc[i] = __builtin_fma(g[i], apd, acc[i]);
         }
         double* out = partial + R.p_off + ((int64_t)c * LP) * R.Tp + j;
 #pragma unroll
@@ -808,31 +742,22 @@ hipError_t launch_ap_tables(const GroupArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
-template <int LP, bool M, int NH>
+template <int LP, int NH>
 static void lf_launch(const GroupArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL((lead_field_kernel<LP, M, NH>), dim3((unsigned)a.n_lf), dim3(512), 0, s, a.hearts,
+    hipLaunchKernelGGL((lead_field_kernel<LP, NH>), dim3((unsigned)a.n_lf), dim3(512), 0, s, a.hearts,
                        a.reqs, a.B, a.lf_start, a.lead_pos, a.layer_k, a.rec, a.aux);
-}
-
-template <int LP, bool M>
-static void lf_launch_nh(const GroupArgs& a, hipStream_t s)
-{
-    switch (a.nhood) {
-    case 0: lf_launch<LP, M, 0>(a, s); break;
-    case 1: lf_launch<LP, M, 1>(a, s); break;
-    case 2: lf_launch<LP, M, 2>(a, s); break;
-    default: lf_launch<LP, M, 3>(a, s); break;
-    }
 }
 
 template <int LP>
 static void lf_launch_m(const GroupArgs& a, hipStream_t s)
 {
-    if (a.mfma)
-        lf_launch_nh<LP, true>(a, s);
-    else
-        lf_launch_nh<LP, false>(a, s);
+    switch (a.nhood) {
+    case 0: lf_launch<LP, 0>(a, s); break;
+    case 1: lf_launch<LP, 1>(a, s); break;
+    case 2: lf_launch<LP, 2>(a, s); break;
+    default: lf_launch<LP, 3>(a, s); break;
+    }
 }
 
 hipError_t launch_lead_field(const GroupArgs& a, hipStream_t s)
@@ -851,12 +776,8 @@ hipError_t launch_lead_field(const GroupArgs& a, hipStream_t s)
 template <int LP>
 static void sums_launch(const GroupArgs& a, hipStream_t s)
 {
-    if (a.mfma)
-        hipLaunchKernelGGL((chunk_sums_kernel<LP, true>), dim3((unsigned)a.n_ch), dim3(256), 0, s, a.hearts,
-                           a.reqs, a.B, a.ch_start, a.layer_k, a.lc, a.tgrid, a.thl, a.rec, a.sums, a.bnd);
-    else
-        hipLaunchKernelGGL((chunk_sums_kernel<LP, false>), dim3((unsigned)a.n_ch), dim3(256), 0, s, a.hearts,
-                           a.reqs, a.B, a.ch_start, a.layer_k, a.lc, a.tgrid, a.thl, a.rec, a.sums, a.bnd);
+    hipLaunchKernelGGL((chunk_sums_kernel<LP>), dim3((unsigned)a.n_ch), dim3(256), 0, s, a.hearts, a.reqs, a.B,
+                       a.ch_start, a.layer_k, a.lc, a.tgrid, a.thl, a.rec, a.sums, a.bnd);
 }
 
 hipError_t launch_chunk_sums(const GroupArgs& a, hipStream_t s)
@@ -901,7 +822,7 @@ static void fused_launch(const GroupArgs& a, hipStream_t s)
                            a.thl, a.bnd, a.partial);
     } else {
         hipLaunchKernelGGL(ecg_fused_kernel<LP>, dim3((unsigned)a.n_ch), dim3(256), 0, s, a.hearts, a.reqs,
-                           a.B, a.ch_start, a.rec, a.lc, a.layer_k, a.xy, a.thl, a.bnd, a.partial);
+                           a.B, a.ch_start, a.rec, a.aux, a.lc, a.layer_k, a.xy, a.thl, a.bnd, a.partial);
     }
 }
 

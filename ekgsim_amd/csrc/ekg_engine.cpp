@@ -385,7 +385,8 @@ struct ReqSizes {
 ReqSizes req_sizes(const ReqSpec& r, bool mfma)
 {
     const EcgLayout& L = r.h->layout;
-    return ReqSizes{L.V + (mfma ? ekgk::MFMA_PAD : 1), L.n_chunks, L.n_bricks};
+    (void)mfma;
+    return ReqSizes{L.V + ekgk::MFMA_PAD, L.n_chunks, L.n_bricks};
 }
 
 template <class T>
@@ -405,10 +406,10 @@ size_t group_device_bytes(const GroupPlan& g)
         const EcgLayout& L = r.h->layout;
         const int64_t V = L.valid ? L.V : r.h->n_myo;
         const int64_t nch = L.valid ? L.n_chunks : std::max<int64_t>(1, V / 256 + r.h->max_layer + 4);
-        const int64_t rows = V + (g.mfma ? ekgk::MFMA_PAD : 1);
+        const int64_t rows = V + ekgk::MFMA_PAD;
         s += (size_t)r.nl * 9 * 8 + (size_t)r.L * 3 * 8 + (size_t)r.Tp * 16;
         s += (size_t)r.nl * (sizeof(ekgk::LayerConst) + r.Tp * 16);
-        s += g.mfma ? (size_t)rows * (16 * 8 + 16) : (size_t)rows * (LP + 2) * 8;
+        s += (size_t)rows * (16 * 8 + 16);
         s += (size_t)nch * (3 * LP * 8 + 8);
         s += (size_t)r.nl * (r.n_tiles + 1) * (8 + 2 * LP * 8);
         s += (size_t)nch * LP * r.Tp * 8;
@@ -508,8 +509,8 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     // device buffers (grow-only)
     ws.lc.reserve((size_t)st_lay[B] * sizeof(ekgk::LayerConst));
     ws.xy.reserve((size_t)std::max<int64_t>(1, st_xy[B]) * 16);
-    ws.G.reserve((size_t)rows * (g.mfma ? 16 : LP + 2) * 8);
-    if (g.mfma) ws.aux.reserve((size_t)rows * sizeof(float4));
+    ws.G.reserve((size_t)rows * 16 * 8);
+    ws.aux.reserve((size_t)rows * sizeof(float4));
     ws.sums.reserve((size_t)std::max<int64_t>(1, c_off) * 3 * LP * 8);
     ws.bnd.reserve((size_t)std::max<int64_t>(1, c_off) * sizeof(int2));
     ws.pre_rng.reserve((size_t)std::max<int64_t>(1, pre_off) * sizeof(int2));

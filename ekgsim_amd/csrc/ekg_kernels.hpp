@@ -89,8 +89,8 @@ struct GroupArgs {
     const float2* thl;         // FP32 hi/lo of the same times
     LayerConst* lc;
     double2* xy;
-    double* rec;               // MFMA: 16-double G rows; VALU: LP+2 doubles (G, {P,Q}, {a_hi,a_lo})
-    float4* aux;               // MFMA: {P, Q, a_hi, a_lo}
+    double* rec;               // 16-double G row per voxel (leads zero-padded)
+    float4* aux;               // {P, Q, a_hi, a_lo} per voxel
     double* sums;              // per (request, chunk): [3][LP] sum G, sum G P', sum G Q'
     int2* bnd;                 // per (request, chunk): {first non-zero tile, first saturated tile}
     int2* pre_rng;             // per (request, layer, tile): transition chunks [lo, hi)
