@@ -31,7 +31,8 @@ EXPORTS = (
     "ekg_server_create", "ekg_server_destroy", "ekg_server_add_heart", "ekg_server_submit",
     "ekg_server_poll", "ekg_server_wait", "ekg_server_stats", "ekg_simulate_batch_device",
     "ekg_ctx_synchronize", "ekg_ctx_set_kernel", "ekg_ap_value", "ekg_ap_num_genes", "ekg_layer_aps",
-    "ekg_criterion", "ekg_resample",
+    "ekg_criterion", "ekg_resample", "ekg_heart_autotune", "ekg_heart_set_tune", "ekg_heart_get_tune",
+    "ekg_ctx_set_graphs", "ekg_ctx_graph_stats",
 )
 KERNELS = {"auto": 0, "valu": 1, "mfma": 2}
 
@@ -53,6 +54,19 @@ class SimDesc(C.Structure):
         ("length", C.c_double),
         ("dt", C.c_double),
     ]
+
+
+class Tune(C.Structure):
+    _fields_ = [
+        ("chunk_voxels", C.c_int),
+        ("mfma_waves", C.c_int),
+        ("lead_phase", C.c_int),
+        ("kernel", C.c_int),
+        ("ms_per_batch", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class ServerOpts(C.Structure):
@@ -119,6 +133,11 @@ def lib():
     L.ekg_ap_num_genes.argtypes = [C.POINTER(ApModel), C.POINTER(C.c_int)]
     L.ekg_layer_aps.argtypes = [C.POINTER(ApModel), C.c_int, vp, C.c_int, vp, pd]
     L.ekg_criterion.argtypes = [C.c_int, vp, C.c_int64, vp, C.c_int64, C.c_double, pd]
+    L.ekg_heart_autotune.argtypes = [vp, C.POINTER(SimDesc), C.c_int, C.POINTER(Tune)]
+    L.ekg_heart_set_tune.argtypes = [vp, C.POINTER(Tune)]
+    L.ekg_heart_get_tune.argtypes = [vp, C.POINTER(Tune)]
+    L.ekg_ctx_set_graphs.argtypes = [vp, C.c_int]
+    L.ekg_ctx_graph_stats.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.ekg_resample.argtypes = [vp, C.c_int64, C.c_double, C.c_int, vp, C.c_int64, C.POINTER(C.c_int64)]
     _lib = L
     return L
@@ -295,6 +314,16 @@ class Context:
     def heart(self, layer, seed=None):
         return Heart(self, layer, seed)
 
+    def set_graphs(self, enable: bool):
+        """Replay repeated simulation groups from a captured hipGraph."""
+        check(lib().ekg_ctx_set_graphs(self.h, 1 if enable else 0))
+
+    def graph_stats(self):
+        c = C.c_int64()
+        n = C.c_int64()
+        check(lib().ekg_ctx_graph_stats(self.h, C.byref(c), C.byref(n)))
+        return {"captures": c.value, "launches": n.value}
+
 
 class Heart:
     def __init__(self, ctx: Context, layer, seed=None):
@@ -373,6 +402,24 @@ class Heart:
         check(lib().ekg_simulate_batch(self.h, B, arr, _p(out)))
         return out
 
+
+    def autotune(self, layer_k, lead_pos, nhood="3D4", start=0.0, length=600.0, dt=1.0, batch=64):
+        """Time the launch-shape knobs on `batch` copies of one request and
+        keep the fastest for this heart; returns the choice."""
+        d = _Desc(layer_k, lead_pos, nhood, start, length, dt)
+        t = Tune()
+        check(lib().ekg_heart_autotune(self.h, C.byref(d.s), int(batch), C.byref(t)))
+        return t.as_dict()
+
+    def set_tune(self, chunk_voxels=0, mfma_waves=0, lead_phase=0, kernel="auto"):
+        t = Tune(int(chunk_voxels), int(mfma_waves), int(lead_phase),
+                 KERNELS[kernel] if isinstance(kernel, str) else int(kernel), 0.0)
+        check(lib().ekg_heart_set_tune(self.h, C.byref(t)))
+
+    def get_tune(self):
+        t = Tune()
+        check(lib().ekg_heart_get_tune(self.h, C.byref(t)))
+        return t.as_dict()
 
     def simulate_many(self, requests, nhood="3D4"):
         """One launch group of requests that may differ in lead count, layer

@@ -341,6 +341,68 @@ void ekg_move_lead(const double p[3], double du, double dv, double out[3])
     for (int k = 0; k < 3; ++k) out[k] = (p[k] + u[k] * du) + v[k] * dv;
 }
 
+// ---- tuning and graphs ------------------------------------------------------
+
+int ekg_heart_autotune(ekg_heart* h, const ekg_sim_desc* sample, int batch, ekg_tune* out)
+{
+    return guard([&] {
+        if (!h || !sample) ekg::fail(EKG_EINVAL, "NULL argument");
+        const ekg_tune t = ekg::autotune(h->h, *sample, batch);
+        if (out) *out = t;
+    });
+}
+
+int ekg_heart_set_tune(ekg_heart* h, const ekg_tune* t)
+{
+    return guard([&] {
+        if (!h || !t) ekg::fail(EKG_EINVAL, "NULL argument");
+        if (t->chunk_voxels != 0 && (t->chunk_voxels < 256 || t->chunk_voxels > 4096))
+            ekg::fail(EKG_EINVAL, "chunk_voxels must be 0 or 256..4096");
+        if (t->mfma_waves < 0 || t->mfma_waves > 16) ekg::fail(EKG_EINVAL, "mfma_waves must be 0..16");
+        if (t->lead_phase != 0 && t->lead_phase != 2 && t->lead_phase != 4)
+            ekg::fail(EKG_EINVAL, "lead_phase must be 0, 2 or 4");
+        if (t->kernel < EKG_KERNEL_AUTO || t->kernel > EKG_KERNEL_MFMA) ekg::fail(EKG_EINVAL, "unknown kernel");
+        ekg::Ctx& c = *h->h.ctx;
+        std::lock_guard<std::mutex> g1(c.mu);
+        std::lock_guard<std::mutex> g2(h->h.mu);
+        h->h.tune_chunk = t->chunk_voxels;
+        h->h.tune_waves = t->mfma_waves;
+        h->h.tune_phase = t->lead_phase;
+        h->h.tune_kernel = t->kernel;
+    });
+}
+
+int ekg_heart_get_tune(const ekg_heart* h, ekg_tune* t)
+{
+    return guard([&] {
+        if (!h || !t) ekg::fail(EKG_EINVAL, "NULL argument");
+        t->chunk_voxels = h->h.tune_chunk;
+        t->mfma_waves = h->h.tune_waves;
+        t->lead_phase = h->h.tune_phase;
+        t->kernel = h->h.tune_kernel;
+        t->ms_per_batch = 0.0;
+    });
+}
+
+int ekg_ctx_set_graphs(ekg_ctx* ctx, int enable)
+{
+    return guard([&] {
+        if (!ctx) ekg::fail(EKG_EINVAL, "ctx is NULL");
+        std::lock_guard<std::mutex> g(ctx->c.mu);
+        ctx->c.graphs = enable != 0;
+    });
+}
+
+int ekg_ctx_graph_stats(ekg_ctx* ctx, int64_t* captures, int64_t* launches)
+{
+    return guard([&] {
+        if (!ctx) ekg::fail(EKG_EINVAL, "ctx is NULL");
+        std::lock_guard<std::mutex> g(ctx->c.mu);
+        if (captures) *captures = ctx->c.graph_captures;
+        if (launches) *launches = ctx->c.graph_launches;
+    });
+}
+
 // ---- parameter vector -> layer APs, comparison ----------------------------
 
 double ekg_ap_value(const double k[9], double t) { return ekg::ap_value(k, t); }

@@ -208,7 +208,7 @@ template <> struct Nhood<3> {
 };
 
 // One block of 512 threads per (request, 8^3 brick holding myocardium).
-// For four leads at a time the lead weight m_i of every myocardial cell of
+// For WPH (2 or 4, autotuned) leads at a time the lead weight m_i of every myocardial cell of
 // the brick and its 1-voxel halo (a compact per-brick list) is computed once
 // in FP64 and kept in LDS as FP32 (zero outside the myocardium); thread t < n
 // then gathers, in FP64, for the brick's t-th myocardial voxel u
@@ -222,7 +222,7 @@ template <> struct Nhood<3> {
 // 16-double G row (zero-padded) and aux = {P_u, Q_u, a_hi, a_lo} with
 // P_u = k2(1-k3) e^{(k4+k5)(a_u - t_ref)}, Q_u = k2 k3 e^{k5 (a_u - t_ref)},
 // t_ref the reference time of the voxel's chunk.
-template <int LP, int NH>
+template <int LP, int NH, int WPH>
 __global__ __launch_bounds__(512) void lead_field_kernel(
     const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
     const int64_t* __restrict__ lf_start, const double* __restrict__ lead_pos,
@@ -230,7 +230,8 @@ __global__ __launch_bounds__(512) void lead_field_kernel(
 {
     using N = Nhood<NH>;
     __shared__ uint8_t s_myo[LN];
-    __shared__ float s_m[4][LN][3];
+    __shared__ float s_m[WPH][LN][3];
+    __shared__ double s_pos[16][3];
     const int r = find_req(lf_start, B, blockIdx.x);
     const ReqDev R = reqs[r];
     const HeartDev& H = hearts[R.heart];
@@ -239,11 +240,13 @@ __global__ __launch_bounds__(512) void lead_field_kernel(
     const int4 bi = H.bricks[br];
     const int e0 = H.brick_off[br], nv = H.brick_off[br + 1] - e0;
     const int c0 = H.halo_off[br], nc = H.halo_off[br + 1] - c0;
+    const int n_leads = R.L;
     for (int q = t; q < LN; q += blockDim.x) {
         s_myo[q] = 0;
 #pragma unroll
-        for (int w = 0; w < 4; ++w) s_m[w][q][0] = s_m[w][q][1] = s_m[w][q][2] = 0.0f;
+        for (int w = 0; w < WPH; ++w) s_m[w][q][0] = s_m[w][q][1] = s_m[w][q][2] = 0.0f;
     }
+    if (t < 3 * n_leads) s_pos[t / 3][t % 3] = lead_pos[R.pos_off + t];
     __syncthreads();
     for (int q = t; q < nc; q += blockDim.x) s_myo[H.halo_cells[c0 + q]] = 1;
     __syncthreads();
@@ -262,34 +265,43 @@ __global__ __launch_bounds__(512) void lead_field_kernel(
             sd2 += f * N::d[q][2];
         }
     }
-    const double* pos = lead_pos + R.pos_off;
-    const int n_leads = R.L;
     double row[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) row[i] = 0.0;
 #pragma unroll
-    for (int i0 = 0; i0 < LP; i0 += 4) {
+    for (int i0 = 0; i0 < LP; i0 += WPH) {
         if (i0 >= n_leads) break;
-        const int n4 = n_leads - i0 < 4 ? n_leads - i0 : 4;
-        // (cell, lead) items over all threads
-        for (int it = t; it < nc * n4; it += blockDim.x) {
-            const int w = it / nc;
-            const int q = it - w * nc;
-            const int cell = H.halo_cells[c0 + q];
-            const int gz = bi.x - 1 + cell / (LH * LH), gy = bi.y - 1 + (cell / LH) % LH,
-                      gx = bi.z - 1 + cell % LH;
-            const double* p = pos + 3 * (i0 + w);
-            double m0, m1, m2;
-            lead_f(p[0] - (double)(gz + 1), p[1] - (double)(gy + 1), p[2] - (double)(gx + 1), m0, m1, m2);
-            s_m[w][cell][0] = (float)m0;
-            s_m[w][cell][1] = (float)m1;
-            s_m[w][cell][2] = (float)m2;
+        const int nw = n_leads - i0 < WPH ? n_leads - i0 : WPH;
+        // (lead, cell) items over all threads, stepped without division
+        {
+            int w = 0, q = t;
+            while (q >= nc && w < nw) {
+                q -= nc;
+                ++w;
+            }
+            for (; w < nw;) {
+                const int cell = H.halo_cells[c0 + q];
+                const int gz = bi.x - 1 + cell / (LH * LH), gy = bi.y - 1 + (cell / LH) % LH,
+                          gx = bi.z - 1 + cell % LH;
+                const double* p = s_pos[i0 + w];
+                double m0, m1, m2;
+                lead_f(p[0] - (double)(gz + 1), p[1] - (double)(gy + 1), p[2] - (double)(gx + 1), m0, m1,
+                       m2);
+                s_m[w][cell][0] = (float)m0;
+                s_m[w][cell][1] = (float)m1;
+                s_m[w][cell][2] = (float)m2;
+                q += blockDim.x;
+                while (q >= nc && w < nw) {
+                    q -= nc;
+                    ++w;
+                }
+            }
         }
         __syncthreads();
         if (t < nv) {
 #pragma unroll
-            for (int w = 0; w < 4; ++w) {
-                if (w < n4) {
+            for (int w = 0; w < WPH; ++w) {
+                if (w < nw) {
                     double g = (double)s_m[w][h][0] * sd0 + (double)s_m[w][h][1] * sd1 +
                                (double)s_m[w][h][2] * sd2;
 #pragma unroll
@@ -745,8 +757,12 @@ hipError_t launch_ap_tables(const GroupArgs& a, hipStream_t s)
 template <int LP, int NH>
 static void lf_launch(const GroupArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL((lead_field_kernel<LP, NH>), dim3((unsigned)a.n_lf), dim3(512), 0, s, a.hearts,
-                       a.reqs, a.B, a.lf_start, a.lead_pos, a.layer_k, a.rec, a.aux);
+    if (a.lf_phase == 2)
+        hipLaunchKernelGGL((lead_field_kernel<LP, NH, 2>), dim3((unsigned)a.n_lf), dim3(512), 0, s, a.hearts,
+                           a.reqs, a.B, a.lf_start, a.lead_pos, a.layer_k, a.rec, a.aux);
+    else
+        hipLaunchKernelGGL((lead_field_kernel<LP, NH, 4>), dim3((unsigned)a.n_lf), dim3(512), 0, s, a.hearts,
+                           a.reqs, a.B, a.lf_start, a.lead_pos, a.layer_k, a.rec, a.aux);
 }
 
 template <int LP>
@@ -763,6 +779,7 @@ static void lf_launch_m(const GroupArgs& a, hipStream_t s)
 hipError_t launch_lead_field(const GroupArgs& a, hipStream_t s)
 {
     if (a.n_lf == 0) return hipSuccess;
+    if (a.lf_phase != 2 && a.lf_phase != 4) return hipErrorInvalidValue;
     switch (a.LP) {
     case 4: lf_launch_m<4>(a, s); break;
     case 8: lf_launch_m<8>(a, s); break;

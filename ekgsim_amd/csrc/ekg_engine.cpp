@@ -43,6 +43,11 @@ void DevBuf::release()
 
 void Ctx::bind() const { hip_check(hipSetDevice(device), "hipSetDevice"); }
 
+Workspace::~Workspace()
+{
+    if (graph) (void)hipGraphExecDestroy(graph);
+}
+
 hipEvent_t Ctx::get_event()
 {
     if (!free_events.empty()) {
@@ -163,17 +168,19 @@ GroupPlan plan_group(Ctx& ctx, std::vector<ReqSpec> reqs)
     if (reqs.empty()) fail(EKG_EINVAL, "empty group");
     GroupPlan g;
     g.nhood = reqs[0].nhood;
-    int L = 1, kernel = ctx.kernel, waves = 0;
+    int L = 1, kernel = ctx.kernel, waves = 0, phase = 0;
     for (const ReqSpec& r : reqs) {
         if (r.nhood != g.nhood) fail(EKG_EINVAL, "all requests of a launch group must share the neighbourhood");
         L = std::max(L, r.L);
         if (r.h->tune_kernel) kernel = r.h->tune_kernel;
         if (r.h->tune_waves) waves = std::max(waves, r.h->tune_waves);
+        if (r.h->tune_phase && !phase) phase = r.h->tune_phase;
     }
     if (ctx.kernel != EKG_KERNEL_AUTO) kernel = ctx.kernel;  // an explicit context choice wins
     g.LP = L <= 4 ? 4 : L <= 8 ? 8 : L <= 12 ? 12 : 16;
     g.mfma = kernel != EKG_KERNEL_VALU;
     g.waves = waves ? waves : 4;
+    g.lf_phase = phase ? phase : 4;
     g.reqs = std::move(reqs);
     return g;
 }
@@ -568,6 +575,7 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     a.nhood = g.nhood;
     a.mfma = g.mfma ? 1 : 0;
     a.mfma_waves = g.waves;
+    a.lf_phase = g.lf_phase;
     a.lf_start = stp;
     a.ch_start = stp + (B + 1);
     a.xy_start = stp + 2 * (B + 1);
@@ -592,22 +600,57 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     a.pre_ab = ws.pre_ab.as<double>();
     a.partial = ws.partial.as<double>();
     a.ecg = d_ecg;
-    hip_check(ekgk::launch_ap_tables(a, stream), "ap_tables launch");
-    hip_check(ekgk::launch_lead_field(a, stream), "lead_field launch");
-    hip_check(ekgk::launch_chunk_sums(a, stream), "chunk_sums launch");
-    hip_check(ekgk::launch_layer_prefix(a, stream), "layer_prefix launch");
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (time_fused) {
-        e0 = ctx.get_event();
-        e1 = ctx.get_event();
-        hip_check(hipEventRecord(e0, stream), "hipEventRecord");
+    auto launch_all = [&](bool timed) {
+        hip_check(ekgk::launch_ap_tables(a, stream), "ap_tables launch");
+        hip_check(ekgk::launch_lead_field(a, stream), "lead_field launch");
+        hip_check(ekgk::launch_chunk_sums(a, stream), "chunk_sums launch");
+        hip_check(ekgk::launch_layer_prefix(a, stream), "layer_prefix launch");
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (timed) {
+            e0 = ctx.get_event();
+            e1 = ctx.get_event();
+            hip_check(hipEventRecord(e0, stream), "hipEventRecord");
+        }
+        hip_check(ekgk::launch_ecg_fused(a, stream), "ecg_fused launch");
+        if (timed) {
+            hip_check(hipEventRecord(e1, stream), "hipEventRecord");
+            ctx.pending.emplace_back(e0, e1);
+        }
+        hip_check(ekgk::launch_ecg_reduce(a, stream), "ecg_reduce launch");
+    };
+    if (!ctx.graphs || time_fused) {
+        launch_all(time_fused);
+        return;
     }
-    hip_check(ekgk::launch_ecg_fused(a, stream), "ecg_fused launch");
-    if (time_fused) {
-        hip_check(hipEventRecord(e1, stream), "hipEventRecord");
-        ctx.pending.emplace_back(e0, e1);
+    // Graph mode: the six launches of a group are captured once and replayed
+    // while the kernel arguments (buffers, counts, shapes) stay the same; the
+    // per-request values live in device tables and buffers, so new inputs
+    // in the same places need no recapture.  The graph holds kernels only.
+    std::vector<char> key(reinterpret_cast<const char*>(&a), reinterpret_cast<const char*>(&a) + sizeof a);
+    if (!ws.graph || key != ws.graph_key) {
+        if (ws.graph) {
+            (void)hipGraphExecDestroy(ws.graph);
+            ws.graph = nullptr;
+        }
+        ws.graph_key.clear();
+        hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        hipGraph_t graph = nullptr;
+        try {
+            launch_all(false);
+        } catch (...) {
+            (void)hipStreamEndCapture(stream, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
+        const hipError_t e = hipGraphInstantiate(&ws.graph, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        hip_check(e, "hipGraphInstantiate");
+        ws.graph_key.swap(key);
+        ++ctx.graph_captures;
     }
-    hip_check(ekgk::launch_ecg_reduce(a, stream), "ecg_reduce launch");
+    hip_check(hipGraphLaunch(ws.graph, stream), "hipGraphLaunch");
+    ++ctx.graph_launches;
 }
 
 void simulate(Heart& h, int batch, const ekg_sim_desc* descs, double* ecg)
@@ -650,6 +693,81 @@ void simulate_device(Heart& h, const ekg_batch_dev& d)
     ctx.bind();
     GroupPlan g = plan_group(ctx, std::vector<ReqSpec>(d.batch, one));
     enqueue_group(ctx, g, ctx.ws, ctx.stream, d.layer_k_dev, d.lead_pos_dev, d.ecg_dev, ctx.profiling);
+}
+
+ekg_tune autotune(Heart& h, const ekg_sim_desc& sample, int batch)
+{
+    Ctx& ctx = *h.ctx;
+    std::lock_guard<std::mutex> g1(ctx.mu);
+    std::lock_guard<std::mutex> g2(h.mu);
+    if (batch < 1 || batch > 4096) fail(EKG_EINVAL, "autotune batch must be 1..4096");
+    const ReqSpec one = make_spec(h, sample, true);
+    if (one.T == 0) fail(EKG_EINVAL, "the sample simulation has no time steps");
+    if (!h.excited) fail(EKG_ESTATE, "excitation sequence not computed or loaded");
+    ctx.bind();
+    const int saved_kernel = ctx.kernel;
+    const int def_chunk = (int)std::min<int64_t>(4096, std::max<int64_t>(256, h.n_myo / 400));
+    ekg_tune best{def_chunk, 4, 4, EKG_KERNEL_MFMA, 0.0};
+    Workspace ws;
+    hipStream_t st = ctx.stream;
+    hipEvent_t e0 = ctx.get_event(), e1 = ctx.get_event();
+    // time one configuration: one warm-up group, then the best of two
+    auto trial = [&](const ekg_tune& c) {
+        h.tune_chunk = c.chunk_voxels;
+        h.tune_waves = c.mfma_waves;
+        h.tune_phase = c.lead_phase;
+        h.tune_kernel = c.kernel;
+        ctx.kernel = EKG_KERNEL_AUTO;
+        GroupPlan g = plan_group(ctx, std::vector<ReqSpec>(batch, one));
+        double best_ms = 1e30;
+        for (int it = 0; it < 3; ++it) {
+            hip_check(hipEventRecord(e0, st), "hipEventRecord");
+            enqueue_group(ctx, g, ws, st, nullptr, nullptr, nullptr, false);
+            hip_check(hipEventRecord(e1, st), "hipEventRecord");
+            hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+            float ms = 0.f;
+            hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+            if (it > 0) best_ms = std::min(best_ms, (double)ms);
+        }
+        return best_ms;
+    };
+    try {
+        best.ms_per_batch = trial(best);
+        auto sweep = [&](int ekg_tune::*field, std::vector<int> values) {
+            for (int v : values) {
+                if (v == best.*field) continue;
+                ekg_tune c = best;
+                c.*field = v;
+                const double ms = trial(c);
+                if (ms < best.ms_per_batch) {
+                    best = c;
+                    best.ms_per_batch = ms;
+                }
+            }
+        };
+        sweep(&ekg_tune::kernel, {EKG_KERNEL_MFMA, EKG_KERNEL_VALU});
+        std::vector<int> chunks{256, 512, 1024, 2048, 4096};
+        chunks.erase(std::remove(chunks.begin(), chunks.end(), def_chunk), chunks.end());
+        sweep(&ekg_tune::chunk_voxels, chunks);
+        if (best.kernel == EKG_KERNEL_MFMA) sweep(&ekg_tune::mfma_waves, {2, 4, 8, 16});
+        sweep(&ekg_tune::lead_phase, {2, 4});
+    } catch (...) {
+        ctx.kernel = saved_kernel;
+        h.tune_chunk = h.tune_waves = h.tune_phase = h.tune_kernel = 0;
+        ctx.free_events.push_back(e0);
+        ctx.free_events.push_back(e1);
+        throw;
+    }
+    ctx.kernel = saved_kernel;
+    ctx.free_events.push_back(e0);
+    ctx.free_events.push_back(e1);
+    h.tune_chunk = best.chunk_voxels;
+    h.tune_waves = best.mfma_waves;
+    h.tune_phase = best.lead_phase;
+    h.tune_kernel = best.kernel;
+    ensure_layout(h);
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    return best;
 }
 
 void excite(Heart& h, bool is_int, const double* transfer, int rows, int cols, const uint8_t* delay)

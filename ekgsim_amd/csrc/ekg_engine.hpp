@@ -44,6 +44,10 @@ struct Workspace {
     std::vector<float> h_thl;
     std::vector<char> h_tables;    // hearts + reqs + starts as last uploaded
     std::vector<char> h_grids;     // tgrid + thl as last uploaded
+    // captured launch sequence of the last group (graph mode)
+    hipGraphExec_t graph = nullptr;
+    std::vector<char> graph_key;
+    ~Workspace();
 };
 struct Ctx {
     int device = 0;
@@ -51,6 +55,8 @@ struct Ctx {
     Workspace ws;
     bool profiling = false;
     int kernel = EKG_KERNEL_AUTO;  // contraction path of the fused kernel
+    bool graphs = false;           // replay repeated launch groups from a captured hipGraph
+    int64_t graph_launches = 0, graph_captures = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;  // fused launches not yet timed
     std::vector<hipEvent_t> free_events;
     int64_t fused_launches = 0;
@@ -91,6 +97,7 @@ struct Heart {
     // tuning chosen by ekg_heart_autotune (0 = defaults)
     int tune_chunk = 0;   // target voxels per chunk
     int tune_waves = 0;   // waves per block of the matrix-core fused kernel
+    int tune_phase = 0;   // leads per lead-field phase (2 or 4)
     int tune_kernel = 0;  // EKG_KERNEL_* override
     std::mutex mu;
 };
@@ -109,7 +116,7 @@ struct ReqSpec {
 // A launch group: requests sharing the neighbourhood and the kernel path.
 struct GroupPlan {
     std::vector<ReqSpec> reqs;
-    int LP = 4, nhood = 0, waves = 4;
+    int LP = 4, nhood = 0, waves = 4, lf_phase = 4;
     bool mfma = true;
 };
 
@@ -129,6 +136,10 @@ size_t group_device_bytes(const GroupPlan& g);
 void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, const double* d_k,
                    const double* d_pos, double* d_ecg, bool time_fused);
 void simulate(Heart& h, int batch, const ekg_sim_desc* descs, double* ecg);
+// Times the tunable knobs on `batch` copies of `sample` and keeps the fastest
+// on the heart (coordinate descent: contraction path, chunk size, fused
+// waves per block, lead-field phase width).
+ekg_tune autotune(Heart& h, const ekg_sim_desc& sample, int batch);
 void simulate_device(Heart& h, const ekg_batch_dev& d);
 
 void excite(Heart& h, bool is_int, const double* transfer, int rows, int cols,

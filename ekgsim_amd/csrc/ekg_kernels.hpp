@@ -77,6 +77,7 @@ struct GroupArgs {
     const HeartDev* hearts;
     const ReqDev* reqs;
     int B, LP, nhood, mfma, mfma_waves;
+    int lf_phase;              // leads per lead-field phase (2 or 4)
     const int64_t* lf_start;   // lead-field blocks (bricks) per request
     const int64_t* ch_start;   // chunks per request
     const int64_t* xy_start;   // nl * Tp per request

@@ -164,6 +164,36 @@ typedef struct ekg_batch_dev {
 int ekg_simulate_batch_device(ekg_heart *h, const ekg_batch_dev *d);
 int ekg_ctx_synchronize(ekg_ctx *ctx);
 
+/* ---- launch tuning and graph replay (no reference counterpart) ---------- */
+/* Tunable launch shape of a heart's simulations.  Every setting gives ECGs
+ * within the same tolerance; they differ in speed only. */
+typedef struct ekg_tune {
+    int chunk_voxels;     /* target myocardial voxels per chunk, 256..4096 */
+    int mfma_waves;       /* waves per block of the matrix-core fused kernel, 1..16 */
+    int lead_phase;       /* leads per lead-field phase, 2 or 4 */
+    int kernel;           /* EKG_KERNEL_MFMA or EKG_KERNEL_VALU */
+    double ms_per_batch;  /* measured device time of the chosen shape (autotune only) */
+} ekg_tune;
+
+/* Autotuner: times the launch-shape knobs (coordinate descent: contraction
+ * path, chunk size, fused-kernel waves, lead-field phase width) on `batch`
+ * copies of `sample` on this heart's GPU and keeps the fastest for the
+ * heart's later simulations; `out` (optional) receives the choice.  An
+ * explicit ekg_ctx_set_kernel choice still overrides the tuned path. */
+int ekg_heart_autotune(ekg_heart *h, const ekg_sim_desc *sample, int batch, ekg_tune *out);
+/* Apply a stored tuning (e.g. cached from an earlier autotune on the same
+ * heart and shape); chunk_voxels 0 restores the default layout. */
+int ekg_heart_set_tune(ekg_heart *h, const ekg_tune *t);
+int ekg_heart_get_tune(const ekg_heart *h, ekg_tune *t);
+
+/* Graph mode: the six kernel launches of a simulation group are captured
+ * into a hipGraph on first use and replayed while the group keeps its
+ * shape and buffers (a repeated batch with new coefficients or lead
+ * positions in the same device arrays replays without recapture).  Off by
+ * default; groups timed with ekg_ctx_set_profiling run eagerly. */
+int ekg_ctx_set_graphs(ekg_ctx *ctx, int enable);
+int ekg_ctx_graph_stats(ekg_ctx *ctx, int64_t *captures, int64_t *launches);
+
 /* ---- lead geometry helpers (host, FP64) --------------------------------- */
 /* U/V displacement basis of a measuring point (EkgSim::loadMeasuringPoints,
  * sim_lib.h:167-193) and the displaced position p + du*U + dv*V

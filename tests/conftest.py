@@ -4,6 +4,14 @@ import sys
 import numpy as np
 import pytest
 
+# torch ships its own HIP runtime under the same soname as ROCm's; loaded
+# first, it is the one the library binds to as well, and torch's device
+# enumeration works in tests that hand torch-allocated buffers to the library.
+try:
+    import torch  # noqa: F401
+except Exception:  # pragma: no cover - torch is part of the image
+    torch = None
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
