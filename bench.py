@@ -135,6 +135,8 @@ def main():
                     help="lead-contraction path of the fused kernel")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the batched steps (for rocprofv3 runs)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--no-graph", action="store_true", help="launch the timed steps eagerly")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -187,21 +189,36 @@ def main():
         heart.simulate_batch_device(d_k.data_ptr(), d_p.data_ptr(), d_e.data_ptr(), B, 3, L,
                                     "3D4", 0.0, args.length, 1.0)
 
+    tuning = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tuning = heart.autotune(ks[0], pos[0], "3D4", 0.0, args.length, 1.0, batch=min(B, 64))
+        tuning["seconds"] = round(time.perf_counter() - t0, 2)
+        log(f"autotune: {tuning}")
     for _ in range(args.warmup):
         step()
     ctx.synchronize()
+    # fused-kernel time: HIP events on the context stream over eager steps
     ctx.set_profiling(True)
     ctx.reset_profiling()
-    torch.cuda.synchronize()
+    for _ in range(min(args.steps, 5)):
+        step()
     ctx.synchronize()
+    nl, kms = ctx.kernel_time("ecg_fused")
+    ctx.set_profiling(False)
+    # timed steps (replayed from a captured hipGraph unless --no-graph)
+    ctx.set_graphs(not args.no_graph)
+    step()
+    ctx.synchronize()
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     ctx.synchronize()
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    nl, kms = ctx.kernel_time("ecg_fused")
-    ctx.set_profiling(False)
+    gstats = ctx.graph_stats()
+    ctx.set_graphs(False)
     ms_step = el * 1e3 / args.steps
     value = B * args.steps / el
     avg_kernel_s = kms / 1e3 / max(nl, 1)
@@ -299,6 +316,7 @@ def main():
                          f"heart ({V} myocardial voxels), {L} leads, {T} steps of 1 ms, 3D4 dipoles"),
             "grid": n, "myocardial_voxels": V, "leads": L, "steps_per_ecg": T, "batch": B,
             "neighbourhood": "3D4", "excitation": "int32 per-voxel delay 1-3, 5 endocardial seeds",
+            "launch": {"autotuned": tuning, "graph_replay": not args.no_graph, "graph": gstats},
         },
         "plain_ecgs_per_s": round(plain, 2),
         "plain_p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
