@@ -136,6 +136,8 @@ def main():
     ap.add_argument("--profile-only", action="store_true",
                     help="only the batched steps (for rocprofv3 runs)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--tune", default=None,
+                    help="fixed launch shape chunk,waves,phase,kernel (e.g. 2048,2,2,2) instead of autotuning")
     ap.add_argument("--no-graph", action="store_true", help="launch the timed steps eagerly")
     args = ap.parse_args()
 
@@ -190,7 +192,11 @@ def main():
                                     "3D4", 0.0, args.length, 1.0)
 
     tuning = None
-    if not args.no_autotune:
+    if args.tune:
+        c, w, ph, kern = (int(v) for v in args.tune.split(","))
+        heart.set_tune(c, w, ph, kern)
+        tuning = heart.get_tune()
+    elif not args.no_autotune:
         t0 = time.perf_counter()
         tuning = heart.autotune(ks[0], pos[0], "3D4", 0.0, args.length, 1.0, batch=min(B, 64))
         tuning["seconds"] = round(time.perf_counter() - t0, 2)

@@ -158,20 +158,18 @@ __global__ void ap_tables_kernel(const HeartDev* __restrict__ hearts, const ReqD
 // Lead field
 // ---------------------------------------------------------------------------
 
-// f(e) = e * (1/(l*sqrt(l))), l = |e|^2: the lead weight m of simulator.cpp:531-533
-// for e = p - (u+1).  1/(l*sqrt(l)) = rsq(l)^3 with two Newton steps on the
-// hardware FP64 reciprocal square root (quadratic convergence: a few ulp).
-__device__ __forceinline__ void lead_f(double e0, double e1, double e2, double& m0, double& m1,
-                                       double& m2)
+// f(e) = e / |e|^3: the lead weight m of simulator.cpp:531-533 for
+// e = p - (u+1).  Computed in FP32 (the weights are kept in FP32): e from the
+// electrode's FP32 hi/lo split, so it is within an ulp of the FP64 difference,
+// then the hardware reciprocal square root with one Newton step.
+__device__ __forceinline__ void lead_f32(float e0, float e1, float e2, float& m0, float& m1, float& m2)
 {
-    double l = e0 * e0;
-    l = __builtin_fma(e1, e1, l);
-    l = __builtin_fma(e2, e2, l);
-    double r = __builtin_amdgcn_rsq(l);
-    const double h = 0.5 * l;
-    r = r * __builtin_fma(-h * r, r, 1.5);
-    r = r * __builtin_fma(-h * r, r, 1.5);
-    const double s = r * r * r;
+    float l = e0 * e0;
+    l = __builtin_fmaf(e1, e1, l);
+    l = __builtin_fmaf(e2, e2, l);
+    float r = __builtin_amdgcn_rsqf(l);
+    r = r * __builtin_fmaf(-0.5f * l * r, r, 1.5f);
+    const float s = r * r * r;
     m0 = e0 * s;
     m1 = e1 * s;
     m2 = e2 * s;
@@ -208,20 +206,22 @@ template <> struct Nhood<3> {
 };
 
 // One block of 512 threads per (request, 8^3 brick holding myocardium).
-// For WPH (2 or 4, autotuned) leads at a time the lead weight m_i of every myocardial cell of
-// the brick and its 1-voxel halo (a compact per-brick list) is computed once
-// in FP64 and kept in LDS as FP32 (zero outside the myocardium); thread t < n
-// then gathers, in FP64, for the brick's t-th myocardial voxel u
+// For WPH (2 or 4, autotuned) leads at a time the lead weight m_i of every
+// myocardial cell of the brick and its 1-voxel halo (a compact per-brick list,
+// one thread per cell looping over the leads) is computed in FP32 and kept in
+// LDS (zero outside the myocardium); thread t < n then gathers, in FP64, for
+// the brick's t-th myocardial voxel u
 //   G_i(u) = sum_{d in N} M_i(u+d) . d + m_i(u) . s_u,
 //   M_i = m_i on myocardium and 0 elsewhere,  s_u = sum of those d,
 // which is sum_{d in N, u+d myo} (m_i(u+d) + m_i(u)) . d -- the coefficient of
 // V_u(t) in the reference's dipole sum regrouped by cell (simulator.cpp:509-537).
-// The FP32 rounding of m changes the ECG by about 1e-7 relative or less
-// (G itself stays FP64: its own rounding would cost up to 3e-6 on small
-// hearts).  The voxel's record (row g_off + v of the request) receives a
-// 16-double G row (zero-padded) and aux = {P_u, Q_u, a_hi, a_lo} with
-// P_u = k2(1-k3) e^{(k4+k5)(a_u - t_ref)}, Q_u = k2 k3 e^{k5 (a_u - t_ref)},
-// t_ref the reference time of the voxel's chunk.
+// The FP32 weights change the ECG by a few 1e-7 relative (an error in m_i(w)
+// enters only through V differences across the dipoles at w); G itself is
+// summed and kept in FP64, since its own rounding multiplies V_u directly and
+// would cost up to 3e-6 on small hearts.  The voxel's record (row g_off + v of
+// the request) receives a 16-double G row (zero-padded) and
+// aux = {P_u, Q_u, a_hi, a_lo} with P_u = k2(1-k3) e^{(k4+k5)(a_u - t_ref)},
+// Q_u = k2 k3 e^{k5 (a_u - t_ref)}, t_ref the reference time of the voxel's chunk.
 template <int LP, int NH, int WPH>
 __global__ __launch_bounds__(512) void lead_field_kernel(
     const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
@@ -229,9 +229,17 @@ __global__ __launch_bounds__(512) void lead_field_kernel(
     const double* __restrict__ layer_k, double* __restrict__ rec, float4* __restrict__ aux)
 {
     using N = Nhood<NH>;
+    // Lead weights use the halo box with padded row and plane strides (RS, PS
+    // cells): with 16-byte cells, a 10-cell row stride puts lanes of adjacent
+    // rows on the same LDS banks.
+    constexpr int RS = 12, PS = RS * LH, MLN = PS * LH;
+    constexpr int HM = PS + RS + 1;       // largest backward neighbour offset
+    constexpr int ROWS = WPH * MLN / 8;   // G rows of 128 bytes that fit in the lead-weight area
     __shared__ uint8_t s_myo[LN];
-    __shared__ float s_m[WPH][LN][3];
-    __shared__ double s_pos[16][3];
+    __shared__ uint16_t s_cell[LN];
+    __shared__ float4 s_m[WPH][MLN];      // {m_x, m_y, m_z, 0}: one 16-byte read per neighbour
+    __shared__ float s_p[16][2][3];       // electrode {hi, lo} per component
+    __shared__ int s_v[LB * LB * LB];     // sorted voxel index of each brick voxel
     const int r = find_req(lf_start, B, blockIdx.x);
     const ReqDev R = reqs[r];
     const HeartDev& H = hearts[R.heart];
@@ -241,29 +249,44 @@ __global__ __launch_bounds__(512) void lead_field_kernel(
     const int e0 = H.brick_off[br], nv = H.brick_off[br + 1] - e0;
     const int c0 = H.halo_off[br], nc = H.halo_off[br + 1] - c0;
     const int n_leads = R.L;
-    for (int q = t; q < LN; q += blockDim.x) {
-        s_myo[q] = 0;
+    for (int q = t; q < LN; q += blockDim.x) s_myo[q] = 0;
+    for (int q = t; q < MLN; q += blockDim.x) {
 #pragma unroll
-        for (int w = 0; w < WPH; ++w) s_m[w][q][0] = s_m[w][q][1] = s_m[w][q][2] = 0.0f;
+        for (int w = 0; w < WPH; ++w) s_m[w][q] = float4{0.0f, 0.0f, 0.0f, 0.0f};
     }
-    if (t < 3 * n_leads) s_pos[t / 3][t % 3] = lead_pos[R.pos_off + t];
+    if (t < 3 * n_leads) {
+        const double p = lead_pos[R.pos_off + t];
+        const float hi = (float)p;
+        s_p[t / 3][0][t % 3] = hi;
+        s_p[t / 3][1][t % 3] = (float)(p - (double)hi);
+    }
     __syncthreads();
-    for (int q = t; q < nc; q += blockDim.x) s_myo[H.halo_cells[c0 + q]] = 1;
+    for (int q = t; q < nc; q += blockDim.x) {
+        const int c = H.halo_cells[c0 + q];
+        s_myo[c] = 1;
+        s_cell[q] = (uint16_t)c;
+    }
     __syncthreads();
-    int h = 0, v = -1;
+    int hs = HM, v = -1;
     double sd0 = 0.0, sd1 = 0.0, sd2 = 0.0;
     if (t < nv) {
         const int2 bv = H.brick_vox[e0 + t];
         v = bv.x;
+        s_v[t] = v;
         const int lz = bv.y >> 6, ly = (bv.y >> 3) & 7, lx = bv.y & 7;
-        h = ((lz + 1) * LH + (ly + 1)) * LH + (lx + 1);
+        const int h = ((lz + 1) * LH + (ly + 1)) * LH + (lx + 1);
+        hs = (lz + 1) * PS + (ly + 1) * RS + (lx + 1);
+        int i0 = 0, i1 = 0, i2 = 0;
 #pragma unroll
         for (int q = 0; q < N::n; ++q) {
-            const double f = s_myo[h + (N::d[q][0] * LH + N::d[q][1]) * LH + N::d[q][2]];
-            sd0 += f * N::d[q][0];
-            sd1 += f * N::d[q][1];
-            sd2 += f * N::d[q][2];
+            const int f = s_myo[h + (N::d[q][0] * LH + N::d[q][1]) * LH + N::d[q][2]];
+            i0 += f * N::d[q][0];
+            i1 += f * N::d[q][1];
+            i2 += f * N::d[q][2];
         }
+        sd0 = (double)i0;
+        sd1 = (double)i1;
+        sd2 = (double)i2;
     }
     double row[16];
 #pragma unroll
@@ -272,28 +295,19 @@ __global__ __launch_bounds__(512) void lead_field_kernel(
     for (int i0 = 0; i0 < LP; i0 += WPH) {
         if (i0 >= n_leads) break;
         const int nw = n_leads - i0 < WPH ? n_leads - i0 : WPH;
-        // (lead, cell) items over all threads, stepped without division
-        {
-            int w = 0, q = t;
-            while (q >= nc && w < nw) {
-                q -= nc;
-                ++w;
-            }
-            for (; w < nw;) {
-                const int cell = H.halo_cells[c0 + q];
-                const int gz = bi.x - 1 + cell / (LH * LH), gy = bi.y - 1 + (cell / LH) % LH,
-                          gx = bi.z - 1 + cell % LH;
-                const double* p = s_pos[i0 + w];
-                double m0, m1, m2;
-                lead_f(p[0] - (double)(gz + 1), p[1] - (double)(gy + 1), p[2] - (double)(gx + 1), m0, m1,
-                       m2);
-                s_m[w][cell][0] = (float)m0;
-                s_m[w][cell][1] = (float)m1;
-                s_m[w][cell][2] = (float)m2;
-                q += blockDim.x;
-                while (q >= nc && w < nw) {
-                    q -= nc;
-                    ++w;
+        for (int q = t; q < nc; q += blockDim.x) {
+            const int cell = s_cell[q];
+            const int cz = cell / (LH * LH), cy = (cell / LH) % LH, cx = cell % LH;
+            // grid coordinate + 1 of the cell (the reference's voxel centre)
+            const float fz = (float)(bi.x + cz), fy = (float)(bi.y + cy), fx = (float)(bi.z + cx);
+#pragma unroll
+            for (int w = 0; w < WPH; ++w) {
+                if (w < nw) {
+                    const float(*p)[3] = s_p[i0 + w];
+                    float m0, m1, m2;
+                    lead_f32((p[0][0] - fz) + p[1][0], (p[0][1] - fy) + p[1][1], (p[0][2] - fx) + p[1][2],
+                             m0, m1, m2);
+                    s_m[w][cz * PS + cy * RS + cx] = float4{m0, m1, m2, 0.0f};
                 }
             }
         }
@@ -302,14 +316,16 @@ __global__ __launch_bounds__(512) void lead_field_kernel(
 #pragma unroll
             for (int w = 0; w < WPH; ++w) {
                 if (w < nw) {
-                    double g = (double)s_m[w][h][0] * sd0 + (double)s_m[w][h][1] * sd1 +
-                               (double)s_m[w][h][2] * sd2;
+                    // non-negative constant offsets from the lowest neighbour
+                    const float4* mb = &s_m[w][hs - HM];
+                    const float4 mu = mb[HM];
+                    double g = (double)mu.x * sd0 + (double)mu.y * sd1 + (double)mu.z * sd2;
 #pragma unroll
                     for (int q = 0; q < N::n; ++q) {
-                        const int hn = h + (N::d[q][0] * LH + N::d[q][1]) * LH + N::d[q][2];
-                        if (N::d[q][0]) g += (double)s_m[w][hn][0] * N::d[q][0];
-                        if (N::d[q][1]) g += (double)s_m[w][hn][1] * N::d[q][1];
-                        if (N::d[q][2]) g += (double)s_m[w][hn][2] * N::d[q][2];
+                        const float4 mn = mb[HM + N::d[q][0] * PS + N::d[q][1] * RS + N::d[q][2]];
+                        if (N::d[q][0]) g += (double)mn.x * N::d[q][0];
+                        if (N::d[q][1]) g += (double)mn.y * N::d[q][1];
+                        if (N::d[q][2]) g += (double)mn.z * N::d[q][2];
                     }
                     row[i0 + w] = g;
                 }
@@ -324,9 +340,25 @@ __global__ __launch_bounds__(512) void lead_field_kernel(
         const float2 av = H.vox_a[v];
         aux[R.g_off + v] = float4{(float)(k[2] * (1.0 - k[3]) * exp((k[4] + k[5]) * da)),
                                   (float)(k[2] * k[3] * exp(k[5] * da)), av.x, av.y};
-        double2* gv = reinterpret_cast<double2*>(rec + (R.g_off + v) * 16);
+    }
+    // The G rows leave through LDS (the lead-weight area is free after the
+    // last gather): each 8-lane group then stores one whole 128-byte row per
+    // instruction, instead of every lane touching its own line with each
+    // 16-byte store (the rows of a brick are scattered in sorted order).
+    double2* s_rows = reinterpret_cast<double2*>(&s_m[0][0]);
+    for (int base = 0; base < nv; base += ROWS) {
+        const int rr = t - base;
+        if (t < nv && rr >= 0 && rr < ROWS) {
 #pragma unroll
-        for (int i = 0; i < 16; i += 2) gv[i / 2] = double2{row[i], row[i + 1]};
+            for (int i = 0; i < 8; ++i) s_rows[rr * 8 + (i ^ (rr & 7))] = double2{row[2 * i], row[2 * i + 1]};
+        }
+        __syncthreads();
+        const int nr = nv - base < ROWS ? nv - base : ROWS;
+        for (int q = t; q < nr * 8; q += blockDim.x) {
+            const int r8 = q >> 3, i = q & 7;
+            reinterpret_cast<double2*>(rec + (R.g_off + s_v[base + r8]) * 16)[i] = s_rows[r8 * 8 + (i ^ (r8 & 7))];
+        }
+        __syncthreads();
     }
 }
 

@@ -180,7 +180,7 @@ GroupPlan plan_group(Ctx& ctx, std::vector<ReqSpec> reqs)
     g.LP = L <= 4 ? 4 : L <= 8 ? 8 : L <= 12 ? 12 : 16;
     g.mfma = kernel != EKG_KERNEL_VALU;
     g.waves = waves ? waves : 4;
-    g.lf_phase = phase ? phase : 4;
+    g.lf_phase = phase ? phase : 2;
     g.reqs = std::move(reqs);
     return g;
 }
@@ -707,7 +707,7 @@ ekg_tune autotune(Heart& h, const ekg_sim_desc& sample, int batch)
     ctx.bind();
     const int saved_kernel = ctx.kernel;
     const int def_chunk = (int)std::min<int64_t>(4096, std::max<int64_t>(256, h.n_myo / 400));
-    ekg_tune best{def_chunk, 4, 4, EKG_KERNEL_MFMA, 0.0};
+    ekg_tune best{def_chunk, 4, 2, EKG_KERNEL_MFMA, 0.0};
     Workspace ws;
     hipStream_t st = ctx.stream;
     hipEvent_t e0 = ctx.get_event(), e1 = ctx.get_event();

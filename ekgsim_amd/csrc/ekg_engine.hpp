@@ -116,7 +116,7 @@ struct ReqSpec {
 // A launch group: requests sharing the neighbourhood and the kernel path.
 struct GroupPlan {
     std::vector<ReqSpec> reqs;
-    int LP = 4, nhood = 0, waves = 4, lf_phase = 4;
+    int LP = 4, nhood = 0, waves = 4, lf_phase = 2;
     bool mfma = true;
 };
 
