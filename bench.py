@@ -50,7 +50,34 @@ def window_voxel_steps(act, k1, start, dt, T):
     return int(np.maximum(jhi - jlo, 0).sum())
 
 
-# HBM bytes per launch of the fused kernel from rocprofv3 --pmc FETCH_SIZE and
+def lead_field_cells(layer):
+    """Lead-weight evaluations per lead of one request in the chunk kernel:
+    each myocardial voxel evaluates its own cell and its myocardial 3D4
+    (corner) neighbours."""
+    myo = layer > 0
+    P = np.pad(myo, 1)
+    n0, n1, n2 = myo.shape
+    cnt = np.ones(myo.shape, np.int64)
+    for d0 in (-1, 1):
+        for d1 in (-1, 1):
+            for d2 in (-1, 1):
+                cnt += P[1 + d0:1 + d0 + n0, 1 + d1:1 + d1 + n1, 1 + d2:1 + d2 + n2]
+    return int(cnt[myo].sum())
+
+
+# Algorithmic work of the chunk kernel, counted per unit:
+#   per lead-weight evaluation (voxel, lead, cell): 22 FP32 flops (electrode
+#     offset 6, |e|^2 5, rsq 1, Newton step 5, r^3 2, m 3) and 3 FP64 adds;
+#   per window voxel-step (the transition tiles): 2L FP64 MFMA flops and 14
+#     FP32 ops (10 AP assembly + 4 transcendental).
+# FP32 counts half against the FP64 peak (the FP32 vector rate is twice it).
+FP32_PER_CELL, FP64_PER_CELL = 22, 3
+FP32_PER_WINDOW_STEP = 14
+# Naive per-voxel-step model: every voxel at every step, 38 VALU lane-cycles
+# at L = 12, against 256 CU x 64 lanes x 2.4 GHz.
+NAIVE_LANE_CYCLES, LANE_PEAK = 38, 256 * 64 * 2.4e9
+
+# HBM bytes per launch of the dominant kernel from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE, each scaled by the counter's reading for a copy kernel of known
 # size (profiles/ecg_fused_mfma_128cubed_batch256_hbm_traffic.md); None until
 # measured.
@@ -231,10 +258,14 @@ def main():
     act_grid = heart.activation()
     act_myo = act_grid[h.layer > 0]
     win = sum(window_voxel_steps(act_myo, float(k[0][1]), 0.0, 1.0, T) for k in ks)
-    flops = 2.0 * L * win
+    cells = B * L * lead_field_cells(h.layer) if args.kernel != "valu" else 0
+    f64 = 2.0 * L * win + FP64_PER_CELL * cells
+    f32 = FP32_PER_WINDOW_STEP * win + FP32_PER_CELL * cells
+    flops = f64 + 0.5 * f32
     achieved = flops / avg_kernel_s
-    log(f"batched: {ms_step:.2f} ms/step, {value:.1f} ECGs/s; fused kernel {avg_kernel_s * 1e3:.2f} ms; "
-        f"{win / B / V:.1f} window steps per voxel; {achieved / 1e12:.2f} TFLOP/s FP64 contraction")
+    naive_rate = value * V * T
+    log(f"batched: {ms_step:.2f} ms/step, {value:.1f} ECGs/s; dominant kernel {avg_kernel_s * 1e3:.2f} ms; "
+        f"{win / B / V:.1f} window steps per voxel; {achieved / 1e12:.2f} TFLOP/s FP64-equivalent")
     if args.profile_only:
         print(json.dumps({"ms_per_step": ms_step, "fused_ms": avg_kernel_s * 1e3}))
         return
@@ -335,15 +366,22 @@ def main():
         "excitation_ms": round(ca_ms, 2),
         "roofline": {
             "bound": "mfma",
-            "kernel": "ecg_fused_mfma_kernel" if args.kernel != "valu" else "ecg_fused_kernel",
+            "kernel": "ecg_chunk_kernel" if args.kernel != "valu" else "ecg_fused_kernel",
             "achieved": round(achieved / 1e12, 4),
             "peak": MFMA_FP64_PEAK / 1e12,
             "unit": "TFLOP/s",
             "frac": round(achieved / MFMA_FP64_PEAK, 5),
             "traffic": TRAFFIC_BYTES,
             "algorithmic_flops_per_launch": flops,
+            "fp64_flops_per_launch": f64,
+            "fp32_flops_per_launch": f32,
+            "lead_weight_evaluations_per_launch": cells,
             "window_voxel_steps_per_launch": win,
-            "flops_per_window_voxel_step": 2 * L,
+            "per_unit": {"lead_weight_evaluation": {"fp32": FP32_PER_CELL, "fp64": FP64_PER_CELL},
+                         "window_voxel_step": {"fp32": FP32_PER_WINDOW_STEP, "fp64": 2 * L},
+                         "fp32_weight": 0.5},
+            "naive_voxel_steps_per_s": naive_rate,
+            "naive_model_frac": round(naive_rate * NAIVE_LANE_CYCLES / LANE_PEAK, 3),
             "kernel_ms": round(avg_kernel_s * 1e3, 4),
             "launches": nl,
         },

@@ -356,9 +356,10 @@ int ekg_heart_set_tune(ekg_heart* h, const ekg_tune* t)
 {
     return guard([&] {
         if (!h || !t) ekg::fail(EKG_EINVAL, "NULL argument");
-        if (t->chunk_voxels != 0 && (t->chunk_voxels < 256 || t->chunk_voxels > 4096))
-            ekg::fail(EKG_EINVAL, "chunk_voxels must be 0 or 256..4096");
-        if (t->mfma_waves < 0 || t->mfma_waves > 16) ekg::fail(EKG_EINVAL, "mfma_waves must be 0..16");
+        if (t->chunk_voxels != 0 && (t->chunk_voxels < 64 || t->chunk_voxels > ekgk::CHUNK_MAX))
+            ekg::fail(EKG_EINVAL, "chunk_voxels must be 0 or 64..512");
+        if (t->mfma_waves != 0 && (t->mfma_waves < 2 || t->mfma_waves > 16))
+            ekg::fail(EKG_EINVAL, "mfma_waves must be 0 or 2..16");
         if (t->lead_phase != 0 && t->lead_phase != 2 && t->lead_phase != 4)
             ekg::fail(EKG_EINVAL, "lead_phase must be 0, 2 or 4");
         if (t->kernel < EKG_KERNEL_AUTO || t->kernel > EKG_KERNEL_MFMA) ekg::fail(EKG_EINVAL, "unknown kernel");

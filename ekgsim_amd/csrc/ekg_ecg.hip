@@ -478,19 +478,38 @@ __global__ __launch_bounds__(64) void layer_prefix_kernel(
     const int cb = l <= H.max_layer ? H.layer_chunk[l] : 0;
     const int ce = l <= H.max_layer ? H.layer_chunk[l + 1] : 0;
     const int64_t slot = R.pre_off + (int64_t)(l - 1) * (R.n_tiles + 1);
+    const int2* bd = bnd + R.c_off;
+    const double* sc = sums + R.c_off * 3 * LP;
     double A = 0.0, Bq = 0.0;
     int lo = cb, hi = cb;
     for (int tile = 0; tile < R.n_tiles; ++tile) {
-        while (lo < ce && bnd[R.c_off + lo].y <= tile) {
-            if (i < LP) {
-                const double* s = sums + (R.c_off + lo) * 3 * LP;
-                A += s[LP + i];
-                Bq += s[2 * LP + i];
+        // chunks saturated by this tile: a prefix (wave-parallel search)
+        const int lo1 = first_false(lo, ce, [&](int c) { return bd[c].y <= tile; });
+        if (i < LP) {
+            // in chunk order; the loads do not depend on the running sums
+            int c = lo;
+            for (; c + 4 <= lo1; c += 4) {
+                const double p0 = sc[(c + 0) * 3 * LP + LP + i], q0 = sc[(c + 0) * 3 * LP + 2 * LP + i];
+                const double p1 = sc[(c + 1) * 3 * LP + LP + i], q1 = sc[(c + 1) * 3 * LP + 2 * LP + i];
+                const double p2 = sc[(c + 2) * 3 * LP + LP + i], q2 = sc[(c + 2) * 3 * LP + 2 * LP + i];
+                const double p3 = sc[(c + 3) * 3 * LP + LP + i], q3 = sc[(c + 3) * 3 * LP + 2 * LP + i];
+                A += p0;
+                Bq += q0;
+                A += p1;
+                Bq += q1;
+                A += p2;
+                Bq += q2;
+                A += p3;
+                Bq += q3;
             }
-            ++lo;
+            for (; c < lo1; ++c) {
+                A += sc[c * 3 * LP + LP + i];
+                Bq += sc[c * 3 * LP + 2 * LP + i];
+            }
         }
+        lo = lo1;
         if (hi < lo) hi = lo;
-        while (hi < ce && bnd[R.c_off + hi].x <= tile) ++hi;
+        hi = first_false(hi, ce, [&](int c) { return bd[c].x <= tile; });
         if (i == 0) pre_rng[slot + tile] = int2{lo, hi};
         if (i < LP) {
             pre_ab[(slot + tile) * 2 * LP + i] = A;
@@ -499,7 +518,16 @@ __global__ __launch_bounds__(64) void layer_prefix_kernel(
     }
     if (i < LP) {
         double g = 0.0;
-        for (int c = cb; c < ce; ++c) g += sums[(R.c_off + c) * 3 * LP + i];
+        int c = cb;
+        for (; c + 4 <= ce; c += 4) {
+            const double g0 = sc[(c + 0) * 3 * LP + i], g1 = sc[(c + 1) * 3 * LP + i];
+            const double g2 = sc[(c + 2) * 3 * LP + i], g3 = sc[(c + 3) * 3 * LP + i];
+            g += g0;
+            g += g1;
+            g += g2;
+            g += g3;
+        }
+        for (; c < ce; ++c) g += sc[c * 3 * LP + i];
         pre_ab[(slot + R.n_tiles) * 2 * LP + i] = g;
         pre_ab[(slot + R.n_tiles) * 2 * LP + LP + i] = 0.0;
     }
@@ -593,132 +621,252 @@ __global__ __launch_bounds__(256) void ecg_fused_kernel(
 
 typedef double mfma_f64x4 __attribute__((ext_vector_type(4)));
 
-// Matrix-core variant: the lead contraction on v_mfma_f64_16x16x4_f64,
-// D[lead i][step j] += sum_k G[voxel k][lead i] * V[voxel k][step j].  gfx950
-// lane layout (checked by a one-hot probe): A(i,k) in lane 16k+i, B(k,j) in
-// lane 16k+j, D(i,j) in lane 16(i%4)+j, register i/4.  Block = `nw` waves of
-// one (request, chunk); a wave takes tiles z + w, z + w + nw, ... < f.  A
-// tile is four 16-step sub-tiles m, each classified on its own: saturated in
-// FP64 for every voxel (exact closed form X_l(t) sum G P' + Y_l(t) sum G Q',
-// no voxel loop), or evaluated over the voxels before its skip point.  Per
-// group of 4 voxels: one coalesced 512-byte load of the four G rows (A), one
-// float4 {P, Q, a} per lane (voxel k = lane/16), an AP per (voxel, step) and
-// one MFMA per evaluated sub-tile.  Groups that reach past the chunk end are
-// masked; the spare rows per request keep their loads in bounds.
-template <int LP>
-__global__ __launch_bounds__(1024) void ecg_fused_mfma_kernel(
+// ---------------------------------------------------------------------------
+// Chunk kernel (matrix-core path): lead field, chunk sums and transition tiles
+// of one (request, chunk) in one block
+// ---------------------------------------------------------------------------
+
+// Bit of neighbour offset d in a voxel's 3x3x3 myocardium mask.
+__device__ constexpr int cube_bit(int d0, int d1, int d2) { return (d0 + 1) * 9 + (d1 + 1) * 3 + (d2 + 1); }
+
+constexpr int CHUNK_SEG = 8;  // fixed summation segments of the chunk sums
+
+// Block = `nw` waves of one (request, chunk) of at most NMAX voxels; the
+// chunk's G rows never leave the CU.
+//  1. Thread 0 finds the tile bounds {z, f} (as chunk_sums_kernel).  Each
+//     voxel u of the chunk (one per thread, strided) gets its G row from its
+//     own cell and its myocardial neighbours directly: per lead the FP32 lead
+//     weight of each of those cells and the FP64 sum
+//       G_i(u) = m_i(u) . s_u + sum_{d in N, u+d myo} m_i(u+d) . d
+//     in the lead-field kernel's order (its non-myocardial neighbours add
+//     zeros), into LDS with the voxel's {P, Q, a} and FP64 {P', Q'}.
+//  2. Chunk sums sum G, sum G P', sum G Q' in a fixed order (CHUNK_SEG
+//     strided segments, then the segments in order), to LDS and to `sums`.
+//  3. Transition tiles, 16-step sub-tiles spread over the waves: a sub-tile
+//     saturated in FP64 for every voxel takes the exact closed form
+//     X_l(t) sum G P' + Y_l(t) sum G Q'; otherwise its voxels split as in
+//     ecg_fused_kernel into [0, s1) saturated, [s1, s2) general and skipped
+//     from s2 on, and the lead contraction runs on v_mfma_f64_16x16x4_f64,
+//     D[lead i][step j] += sum_k G[voxel k][lead i] * V[voxel k][step j],
+//     one MFMA per group of 4 voxels below s2.  gfx950 lane layout (checked
+//     by a one-hot probe): A(i,k) in lane 16k+i, B(k,j) in lane 16k+j,
+//     D(i,j) in lane 16(i%4)+j, register i/4.  Lanes past the chunk end
+//     contribute zeros.
+// The host keeps every chunk of a group at most NMAX voxels.
+template <int LP, int NH, int NMAX>
+__global__ __launch_bounds__(1024) void ecg_chunk_kernel(
     const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
-    const int64_t* __restrict__ ch_start, const double* __restrict__ G16,
-    const float4* __restrict__ aux, const double* __restrict__ sums,
-    const LayerConst* __restrict__ lc, const double* __restrict__ layer_k,
+    const int64_t* __restrict__ ch_start, const double* __restrict__ lead_pos,
+    const double* __restrict__ layer_k, const LayerConst* __restrict__ lc,
     const double2* __restrict__ xy, const double* __restrict__ tgrid, const float2* __restrict__ thl,
-    const int2* __restrict__ bnd, double* __restrict__ partial)
+    double* __restrict__ sums, int2* __restrict__ bnd, double* __restrict__ partial)
 {
-    const int lane = threadIdx.x & 63;
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    using N = Nhood<NH>;
+    __shared__ double s_G[NMAX * LP];       // G rows of the chunk's voxels
+    __shared__ float4 s_aux[NMAX];          // {P, Q, a_hi, a_lo} relative to the chunk's t_ref
+    __shared__ double2 s_pq[NMAX];          // {P', Q'} relative to the layer's t0
+    __shared__ double s_part[CHUNK_SEG][3 * LP];
+    __shared__ double s_sum[3 * LP];
+    __shared__ float s_p[16][2][3];         // electrode {hi, lo} per component
+    __shared__ int s_tb[2];
+    const int t = threadIdx.x, lane = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
     const int nw = blockDim.x >> 6;
     const int r = find_req(ch_start, B, blockIdx.x);
     const ReqDev R = reqs[r];
     const HeartDev& H = hearts[R.heart];
     const int c = (int)(blockIdx.x - ch_start[r]);
-    const int2 tb = bnd[R.c_off + c];
-    if (tb.x + w >= tb.y) return;
     const int4 ch = H.chunks[c];
-    const int l = ch.x;
-    const LayerConst kc = lc[R.lay_off + l - 1];
+    const int l = ch.x, v0 = ch.y, n = ch.z - ch.y;
     const double* kk = layer_k + R.k_off + 9 * (l - 1);
+    const LayerConst kc = lc[R.lay_off + l - 1];
+    const double t0 = H.layer_t0[l];
     const double tref = (double)__int_as_float(ch.w);
-    const double dtr = tref - H.layer_t0[l];
-    const double ex = exp((kk[4] + kk[5]) * dtr), ey = exp(kk[5] * dtr);
+    const double alpha = kk[2] * (1.0 - kk[3]), beta = kk[2] * kk[3], k45 = kk[4] + kk[5], k5 = kk[5];
     const double c1d = -kk[1] * LOG2E_D;
+    const double* tg = tgrid + R.t_off;
+    const float2* th = thl + R.t_off;
+    if (t < 3 * R.L) {
+        const double p = lead_pos[R.pos_off + t];
+        const float hi = (float)p;
+        s_p[t / 3][0][t % 3] = hi;
+        s_p[t / 3][1][t % 3] = (float)(p - (double)hi);
+    }
+    if (t == 0) {
+        const double a_max = H.vox_a64[ch.z - 1];
+        const float2 a_min = H.vox_a[ch.y];
+        const int f = c1d < 0.0 ? first_true(R.n_tiles, [&](int tile) {
+            return c1d * (tg[tile * 64] - a_max) <= SAT64_X;
+        }) : R.n_tiles;
+        const int z = (kc.c1 < 0.0f && kc.skip != 0.0f) ? first_true(R.n_tiles, [&](int tile) {
+            const float2 tl = th[tile * 64 + 63];
+            return kc.c1 * tau_of(tl.x, tl.y, a_min.x, a_min.y) < SKIP_X;
+        }) : 0;
+        s_tb[0] = z < f ? z : f;
+        s_tb[1] = f;
+        bnd[R.c_off + c] = int2{z < f ? z : f, f};
+    }
+    __syncthreads();
+
+    // 1. G rows and AP factors
+    for (int q = t; q < n; q += blockDim.x) {
+        const int v = v0 + q;
+        const uint2 geo = H.vox_geo[v];
+        const int gz = (int)(geo.x & 1023u), gy = (int)((geo.x >> 10) & 1023u), gx = (int)(geo.x >> 20);
+        const uint32_t mk = geo.y;
+        int i0 = 0, i1 = 0, i2 = 0;
+#pragma unroll
+        for (int qn = 0; qn < N::n; ++qn) {
+            const int f = (int)((mk >> cube_bit(N::d[qn][0], N::d[qn][1], N::d[qn][2])) & 1u);
+            i0 += f * N::d[qn][0];
+            i1 += f * N::d[qn][1];
+            i2 += f * N::d[qn][2];
+        }
+        const double sd0 = (double)i0, sd1 = (double)i1, sd2 = (double)i2;
+        // grid coordinate + 1 of the voxel (the reference's voxel centre)
+        const float fz = (float)(gz + 1), fy = (float)(gy + 1), fx = (float)(gx + 1);
+        double* grow = s_G + q * LP;
+        // two leads per pass: two independent FP64 accumulation chains.  A
+        // cell's lead weight is a function of the cell alone, (p_hi - c) + p_lo
+        // then lead_f32, bitwise the same wherever it is evaluated: its FP32
+        // rounding then cancels between the two ends of each dipole, and only
+        // V differences across the dipole see it.  (A weight derived per
+        // evaluating voxel, e.g. from e(u) - d, does not cancel: ~1e-5.)
+#pragma unroll 1
+        for (int i = 0; i < LP; i += 2) {
+            double g0 = 0.0, g1 = 0.0;
+            if (i < R.L) {
+                const float(*pa)[3] = s_p[i];
+                const float(*pb)[3] = s_p[i + 1 < R.L ? i + 1 : i];
+                const float ea0 = (pa[0][0] - fz) + pa[1][0], ea1 = (pa[0][1] - fy) + pa[1][1],
+                            ea2 = (pa[0][2] - fx) + pa[1][2];
+                const float eb0 = (pb[0][0] - fz) + pb[1][0], eb1 = (pb[0][1] - fy) + pb[1][1],
+                            eb2 = (pb[0][2] - fx) + pb[1][2];
+                float a0, a1, a2, b0, b1, b2;
+                lead_f32(ea0, ea1, ea2, a0, a1, a2);
+                lead_f32(eb0, eb1, eb2, b0, b1, b2);
+                g0 = (double)a0 * sd0 + (double)a1 * sd1 + (double)a2 * sd2;
+                g1 = (double)b0 * sd0 + (double)b1 * sd1 + (double)b2 * sd2;
+#pragma unroll
+                for (int qn = 0; qn < N::n; ++qn) {
+                    if ((mk >> cube_bit(N::d[qn][0], N::d[qn][1], N::d[qn][2])) & 1u) {
+                        const float cz = (float)(gz + 1 + N::d[qn][0]), cy = (float)(gy + 1 + N::d[qn][1]),
+                                    cx = (float)(gx + 1 + N::d[qn][2]);
+                        lead_f32((pa[0][0] - cz) + pa[1][0], (pa[0][1] - cy) + pa[1][1], (pa[0][2] - cx) + pa[1][2],
+                                 a0, a1, a2);
+                        lead_f32((pb[0][0] - cz) + pb[1][0], (pb[0][1] - cy) + pb[1][1], (pb[0][2] - cx) + pb[1][2],
+                                 b0, b1, b2);
+                        if (N::d[qn][0]) {
+                            g0 += (double)a0 * N::d[qn][0];
+                            g1 += (double)b0 * N::d[qn][0];
+                        }
+                        if (N::d[qn][1]) {
+                            g0 += (double)a1 * N::d[qn][1];
+                            g1 += (double)b1 * N::d[qn][1];
+                        }
+                        if (N::d[qn][2]) {
+                            g0 += (double)a2 * N::d[qn][2];
+                            g1 += (double)b2 * N::d[qn][2];
+                        }
+                    }
+                }
+                if (i + 1 >= R.L) g1 = 0.0;
+            }
+            grow[i] = g0;
+            grow[i + 1] = g1;
+        }
+        const double a = H.vox_a64[v];
+        const float2 av = H.vox_a[v];
+        s_aux[q] = float4{(float)(alpha * exp(k45 * (a - tref))), (float)(beta * exp(k5 * (a - tref))), av.x, av.y};
+        s_pq[q] = double2{alpha * exp(k45 * (a - t0)), beta * exp(k5 * (a - t0))};
+    }
+    __syncthreads();
+
+    // 2. chunk sums: lead i = t % 16, segment t / 16 takes voxels seg, seg + CHUNK_SEG, ...
+    {
+        const int i = t & 15, sg = t >> 4;
+        if (sg < CHUNK_SEG && i < LP) {
+            double sG = 0.0, sP = 0.0, sQ = 0.0;
+            for (int q = sg; q < n; q += CHUNK_SEG) {
+                const double g = s_G[q * LP + i];
+                const double2 pq = s_pq[q];
+                sG += g;
+                sP = __builtin_fma(g, pq.x, sP);
+                sQ = __builtin_fma(g, pq.y, sQ);
+            }
+            s_part[sg][i] = sG;
+            s_part[sg][LP + i] = sP;
+            s_part[sg][2 * LP + i] = sQ;
+        }
+    }
+    __syncthreads();
+    if (t < 3 * LP) {
+        double sum = 0.0;
+#pragma unroll
+        for (int sg = 0; sg < CHUNK_SEG; ++sg) sum += s_part[sg][t];
+        s_sum[t] = sum;
+        sums[(R.c_off + c) * 3 * LP + t] = sum;
+    }
+    __syncthreads();
+
+    // 3. transition tiles [z, f)
+    const int tz = s_tb[0], tf = s_tb[1];
+    if (4 * (tf - tz) <= w) return;  // no sub-tile for this wave
+    const double ex = exp(k45 * (tref - t0)), ey = exp(k5 * (tref - t0));
     const double a_last = H.vox_a64[ch.z - 1];
     const int jl = lane & 15;   // step column
     const int kv = lane >> 4;   // voxel slot of the B operand / lead group of D
-    const double* Gb = G16 + R.g_off * 16;
-    const float4* Ab = aux + R.g_off;
-    const double* sm = sums + (R.c_off + c) * 3 * LP;
+    const int ia = lane & 15;   // lead of the A operand
     const double2* XY = xy + R.xy_off + (int64_t)(l - 1) * R.Tp;
-    const double* tg = tgrid + R.t_off;
-    const float2* th = thl + R.t_off;
     double* out = partial + R.p_off + (int64_t)c * LP * R.Tp;
 
-    for (int tile = tb.x + w; tile < tb.y; tile += nw) {
-        float X[4], Y[4], TH[4], TL[4];
-        bool sat[4];
-        int s1[4], s2[4];
-        int vend = ch.y;
+    // 16-step sub-tiles over the waves: each wave owns whole accumulators
+    const int nsub = (tf - tz) * 4;
+    for (int sb = w; sb < nsub; sb += nw) {
+        const int j0 = (tz + (sb >> 2)) * 64 + 16 * (sb & 3);
+        const int j = j0 + jl;
+        const double2 xyj = XY[j];
+        if (c1d * (tg[j0] - a_last) <= SAT64_X) {
+            // every voxel saturated in FP64: the exact closed form
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            const int j0 = tile * 64 + 16 * m;
-            sat[m] = c1d * (tg[j0] - a_last) <= SAT64_X;
-            const double2 xyj = XY[j0 + jl];
-            X[m] = (float)fmin(xyj.x * ex, XMAX);
-            Y[m] = (float)fmin(xyj.y * ey, XMAX);
-            const float2 t2 = th[j0 + jl];
-            TH[m] = t2.x;
-            TL[m] = t2.y;
-            s1[m] = ch.y;
-            s2[m] = ch.z;
-            if (!sat[m] && kc.c1 < 0.0f) {
-                const float2 tf = th[j0], tl = th[j0 + 15];
-                s1[m] = first_false(ch.y, ch.z, [&](int v) {
-                    const float2 a = H.vox_a[v];
-                    return kc.c1 * tau_of(tf.x, tf.y, a.x, a.y) <= SAT_X;
-                });
-                if (kc.skip != 0.0f)
-                    s2[m] = first_false(s1[m], ch.z, [&](int v) {
-                        const float2 a = H.vox_a[v];
-                        return kc.c1 * tau_of(tl.x, tl.y, a.x, a.y) < SKIP_X;
-                    });
+            for (int qd = 0; qd < 4; ++qd) {
+                const int i = 4 * qd + kv;
+                if (i < R.L && i < LP)
+                    out[(int64_t)i * R.Tp + j] = __builtin_fma(xyj.x, s_sum[LP + i], xyj.y * s_sum[2 * LP + i]);
             }
-            if (!sat[m] && s2[m] > vend) vend = s2[m];
+            continue;
         }
-        mfma_f64x4 acc[4];
-#pragma unroll
-        for (int m = 0; m < 4; ++m) acc[m] = mfma_f64x4{0.0, 0.0, 0.0, 0.0};
-
-        // Groups of 4 voxels; the next group's A rows and {P, Q, a} are loaded
-        // while the current group's MFMAs run (vector loads complete in order).
-        // Every group starts below ch.z <= V, so the look-ahead group starts at
-        // most at V + 3 and its rows stay below V + 7 < V + MFMA_PAD.
-        int vg = ch.y;
-        if (vg < vend) {
-            double a_n = Gb[(int64_t)vg * 16 + lane];
-            float4 q_n = Ab[vg + kv];
-            for (; vg < vend; vg += 4) {
-                const bool ok = vg + kv < ch.z;
-                const double a_op = ok ? a_n : 0.0;
-                const float4 q = q_n;
-                a_n = Gb[(int64_t)(vg + 4) * 16 + lane];
-                q_n = Ab[vg + 4 + kv];
-#pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                    if (sat[m] || vg >= s2[m]) continue;
-                    float ap = __builtin_fmaf(q.x, X[m], q.y * Y[m]);
-                    if (vg + 4 > s1[m]) {
-                        const float tau = tau_of(TH[m], TL[m], q.z, q.w);
-                        ap *= fast_rcp(1.0f + fast_exp2(kc.c1 * tau));
-                    }
-                    const double bv = ok ? (double)ap : 0.0;
-                    acc[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_op, bv, acc[m], 0, 0, 0);
-                }
-            }
+        const float X = (float)fmin(xyj.x * ex, XMAX);
+        const float Y = (float)fmin(xyj.y * ey, XMAX);
+        const float2 tt = th[j];
+        int s1 = 0, s2 = n;
+        if (kc.c1 < 0.0f) {
+            const float2 tf2 = th[j0], tl2 = th[j0 + 15];
+            s1 = first_false(0, n, [&](int q) {
+                const float4 a4 = s_aux[q];
+                return kc.c1 * tau_of(tf2.x, tf2.y, a4.z, a4.w) <= SAT_X;
+            });
+            if (kc.skip != 0.0f)
+                s2 = first_false(s1, n, [&](int q) {
+                    const float4 a4 = s_aux[q];
+                    return kc.c1 * tau_of(tl2.x, tl2.y, a4.z, a4.w) < SKIP_X;
+                });
+        }
+        mfma_f64x4 acc = mfma_f64x4{0.0, 0.0, 0.0, 0.0};
+        for (int vg = 0; vg < s2; vg += 4) {
+            const bool ok = vg + kv < n;
+            const double a_op = (ok && ia < LP) ? s_G[(vg + kv) * LP + ia] : 0.0;
+            const float4 q = ok ? s_aux[vg + kv] : float4{0.0f, 0.0f, 0.0f, 0.0f};
+            float ap = __builtin_fmaf(q.x, X, q.y * Y);
+            if (vg + 4 > s1) ap *= fast_rcp(1.0f + fast_exp2(kc.c1 * tau_of(tt.x, tt.y, q.z, q.w)));
+            const double bv = ok ? (double)ap : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a_op, bv, acc, 0, 0, 0);
         }
         // D(i, j) sits in lane 16(i%4)+j, register i/4: lead i = 4q + lane/16
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            const int j = tile * 64 + 16 * m + jl;
-            double2 xyj = double2{0.0, 0.0};
-            if (sat[m]) xyj = XY[j];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int i = 4 * q + kv;
-                if (i < R.L && i < LP) {
-                    double v = acc[m][q];
-                    if (sat[m]) v = __builtin_fma(xyj.x, sm[LP + i], xyj.y * sm[2 * LP + i]);
-                    out[(int64_t)i * R.Tp + j] = v;
-                }
-            }
+        for (int qd = 0; qd < 4; ++qd) {
+            const int i = 4 * qd + kv;
+            if (i < R.L && i < LP) out[(int64_t)i * R.Tp + j] = acc[qd];
         }
     }
 }
@@ -864,26 +1012,62 @@ hipError_t launch_layer_prefix(const GroupArgs& a, hipStream_t s)
 template <int LP>
 static void fused_launch(const GroupArgs& a, hipStream_t s)
 {
-    if (a.mfma) {
-        const int nw = a.mfma_waves;
-        hipLaunchKernelGGL(ecg_fused_mfma_kernel<LP>, dim3((unsigned)a.n_ch), dim3(64 * nw), 0, s, a.hearts,
-                           a.reqs, a.B, a.ch_start, a.rec, a.aux, a.sums, a.lc, a.layer_k, a.xy, a.tgrid,
-                           a.thl, a.bnd, a.partial);
-    } else {
-        hipLaunchKernelGGL(ecg_fused_kernel<LP>, dim3((unsigned)a.n_ch), dim3(256), 0, s, a.hearts, a.reqs,
-                           a.B, a.ch_start, a.rec, a.aux, a.lc, a.layer_k, a.xy, a.thl, a.bnd, a.partial);
-    }
+    hipLaunchKernelGGL(ecg_fused_kernel<LP>, dim3((unsigned)a.n_ch), dim3(256), 0, s, a.hearts, a.reqs, a.B,
+                       a.ch_start, a.rec, a.aux, a.lc, a.layer_k, a.xy, a.thl, a.bnd, a.partial);
 }
 
 hipError_t launch_ecg_fused(const GroupArgs& a, hipStream_t s)
 {
     if (a.n_ch == 0) return hipSuccess;
-    if (a.mfma && (a.mfma_waves < 1 || a.mfma_waves > 16)) return hipErrorInvalidValue;
     switch (a.LP) {
     case 4: fused_launch<4>(a, s); break;
     case 8: fused_launch<8>(a, s); break;
     case 12: fused_launch<12>(a, s); break;
     case 16: fused_launch<16>(a, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int LP, int NH, int NMAX>
+static void chunk_launch3(const GroupArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL((ecg_chunk_kernel<LP, NH, NMAX>), dim3((unsigned)a.n_ch), dim3(64 * a.mfma_waves), 0, s,
+                       a.hearts, a.reqs, a.B, a.ch_start, a.lead_pos, a.layer_k, a.lc, a.xy, a.tgrid, a.thl,
+                       a.sums, a.bnd, a.partial);
+}
+
+template <int LP, int NH>
+static void chunk_launch2(const GroupArgs& a, hipStream_t s)
+{
+    if (a.chunk_max <= 256)
+        chunk_launch3<LP, NH, 256>(a, s);
+    else
+        chunk_launch3<LP, NH, 512>(a, s);
+}
+
+template <int LP>
+static void chunk_launch1(const GroupArgs& a, hipStream_t s)
+{
+    switch (a.nhood) {
+    case 0: chunk_launch2<LP, 0>(a, s); break;
+    case 1: chunk_launch2<LP, 1>(a, s); break;
+    case 2: chunk_launch2<LP, 2>(a, s); break;
+    default: chunk_launch2<LP, 3>(a, s); break;
+    }
+}
+
+hipError_t launch_ecg_chunk(const GroupArgs& a, hipStream_t s)
+{
+    if (a.n_ch == 0) return hipSuccess;
+    // the chunk sums use 128 threads; every chunk fits the kernel's LDS rows
+    if (a.mfma_waves < 2 || a.mfma_waves > 16 || a.chunk_max < 1 || a.chunk_max > CHUNK_MAX)
+        return hipErrorInvalidValue;
+    switch (a.LP) {
+    case 4: chunk_launch1<4>(a, s); break;
+    case 8: chunk_launch1<8>(a, s); break;
+    case 12: chunk_launch1<12>(a, s); break;
+    case 16: chunk_launch1<16>(a, s); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

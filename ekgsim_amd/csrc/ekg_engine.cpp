@@ -199,7 +199,7 @@ void ensure_layout(Heart& h)
     // size replaces it.  Chunks are layer-pure and span at most
     // CHUNK_SPAN_MS of activation time.
     const int64_t target = h.tune_chunk ? h.tune_chunk
-                                        : std::min<int64_t>(4096, std::max<int64_t>(256, h.n_myo / 400));
+                                        : std::min<int64_t>(ekgk::CHUNK_MAX, std::max<int64_t>(256, h.n_myo / 400));
     if (L.valid && L.chunk_target == target) return;
     L.valid = false;
     std::vector<int64_t> idx;
@@ -258,11 +258,27 @@ void ensure_layout(Heart& h)
         for (int v2 = L.h_chunks[c].y; v2 < L.h_chunks[c].z; ++v2) vchunk[v2] = c;
     std::vector<double> a64(V ? V : 1);
     std::vector<float2> af(V ? V : 1);
+    std::vector<uint2> geo(V ? V : 1);
+    if (h.n0 > 1024 || h.n1 > 1024 || h.n2 > 1024) fail(EKG_EINVAL, "grid dimensions above 1024");
     for (int64_t i = 0; i < V; ++i) {
         const double a = h.h_act[idx[i]];
         a64[i] = a;
         const float hi = (float)a;
         af[i] = float2{hi, (float)(a - (double)hi)};
+        // grid coordinates and the 3x3x3 myocardium mask of the voxel
+        const int64_t x = idx[i];
+        const int i2 = (int)(x % h.n2);
+        const int i1 = (int)((x / h.n2) % h.n1), i0 = (int)(x / ((int64_t)h.n2 * h.n1));
+        uint32_t mask = 0;
+        for (int d0 = -1; d0 <= 1; ++d0)
+            for (int d1 = -1; d1 <= 1; ++d1)
+                for (int d2 = -1; d2 <= 1; ++d2) {
+                    const int z = i0 + d0, y = i1 + d1, xx = i2 + d2;
+                    if (z < 0 || z >= h.n0 || y < 0 || y >= h.n1 || xx < 0 || xx >= h.n2) continue;
+                    if (h.h_layer[((int64_t)z * h.n1 + y) * h.n2 + xx])
+                        mask |= 1u << ((d0 + 1) * 9 + (d1 + 1) * 3 + (d2 + 1));
+                }
+        geo[i] = make_uint2((uint32_t)i0 | ((uint32_t)i1 << 10) | ((uint32_t)i2 << 20), mask);
     }
     // 8^3 bricks of the grid that hold myocardium, with their voxels
     // {sorted index, local cell} grouped by brick (counting sort)
@@ -325,6 +341,7 @@ void ensure_layout(Heart& h)
     L.brick_vox.reserve(bvox.size() * sizeof(int2));
     L.vox_a64.reserve(a64.size() * sizeof(double));
     L.vox_a.reserve(af.size() * sizeof(float2));
+    L.vox_geo.reserve(geo.size() * sizeof(uint2));
     L.chunks.reserve(std::max<size_t>(1, L.h_chunks.size()) * sizeof(int4));
     hip_check(hipMemcpy(L.vox_chunk.p, vchunk.data(), vchunk.size() * sizeof(int), hipMemcpyHostToDevice),
               "upload voxel chunks");
@@ -346,6 +363,8 @@ void ensure_layout(Heart& h)
               "upload activation times");
     hip_check(hipMemcpy(L.vox_a.p, af.data(), af.size() * sizeof(float2), hipMemcpyHostToDevice),
               "upload activation times");
+    hip_check(hipMemcpy(L.vox_geo.p, geo.data(), geo.size() * sizeof(uint2), hipMemcpyHostToDevice),
+              "upload voxel geometry");
     if (!L.h_chunks.empty())
         hip_check(hipMemcpy(L.chunks.p, L.h_chunks.data(), L.h_chunks.size() * sizeof(int4),
                             hipMemcpyHostToDevice),
@@ -368,6 +387,7 @@ void ensure_layout(Heart& h)
     d.vox_chunk = L.vox_chunk.as<int>();
     d.vox_a64 = L.vox_a64.as<double>();
     d.vox_a = L.vox_a.as<float2>();
+    d.vox_geo = L.vox_geo.as<uint2>();
     d.chunks = L.chunks.as<int4>();
     d.layer_chunk = L.layer_chunk.as<int>();
     d.layer_t0 = L.layer_t0.as<double>();
@@ -392,8 +412,8 @@ struct ReqSizes {
 ReqSizes req_sizes(const ReqSpec& r, bool mfma)
 {
     const EcgLayout& L = r.h->layout;
-    (void)mfma;
-    return ReqSizes{L.V + ekgk::MFMA_PAD, L.n_chunks, L.n_bricks};
+    // the chunk kernel (matrix-core path) keeps G rows on chip: no records
+    return ReqSizes{mfma ? 0 : L.V + ekgk::MFMA_PAD, L.n_chunks, L.n_bricks};
 }
 
 template <class T>
@@ -413,7 +433,7 @@ size_t group_device_bytes(const GroupPlan& g)
         const EcgLayout& L = r.h->layout;
         const int64_t V = L.valid ? L.V : r.h->n_myo;
         const int64_t nch = L.valid ? L.n_chunks : std::max<int64_t>(1, V / 256 + r.h->max_layer + 4);
-        const int64_t rows = V + ekgk::MFMA_PAD;
+        const int64_t rows = g.mfma ? 0 : V + ekgk::MFMA_PAD;
         s += (size_t)r.nl * 9 * 8 + (size_t)r.L * 3 * 8 + (size_t)r.Tp * 16;
         s += (size_t)r.nl * (sizeof(ekgk::LayerConst) + r.Tp * 16);
         s += (size_t)rows * (16 * 8 + 16);
@@ -516,8 +536,8 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     // device buffers (grow-only)
     ws.lc.reserve((size_t)st_lay[B] * sizeof(ekgk::LayerConst));
     ws.xy.reserve((size_t)std::max<int64_t>(1, st_xy[B]) * 16);
-    ws.G.reserve((size_t)rows * 16 * 8);
-    ws.aux.reserve((size_t)rows * sizeof(float4));
+    ws.G.reserve((size_t)std::max<int64_t>(1, rows) * 16 * 8);
+    ws.aux.reserve((size_t)std::max<int64_t>(1, rows) * sizeof(float4));
     ws.sums.reserve((size_t)std::max<int64_t>(1, c_off) * 3 * LP * 8);
     ws.bnd.reserve((size_t)std::max<int64_t>(1, c_off) * sizeof(int2));
     ws.pre_rng.reserve((size_t)std::max<int64_t>(1, pre_off) * sizeof(int2));
@@ -576,6 +596,8 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     a.mfma = g.mfma ? 1 : 0;
     a.mfma_waves = g.waves;
     a.lf_phase = g.lf_phase;
+    a.chunk_max = 0;
+    for (Heart* hp : hs) a.chunk_max = std::max(a.chunk_max, hp->layout.chunk_target);
     a.lf_start = stp;
     a.ch_start = stp + (B + 1);
     a.xy_start = stp + 2 * (B + 1);
@@ -601,20 +623,33 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     a.partial = ws.partial.as<double>();
     a.ecg = d_ecg;
     auto launch_all = [&](bool timed) {
-        hip_check(ekgk::launch_ap_tables(a, stream), "ap_tables launch");
-        hip_check(ekgk::launch_lead_field(a, stream), "lead_field launch");
-        hip_check(ekgk::launch_chunk_sums(a, stream), "chunk_sums launch");
-        hip_check(ekgk::launch_layer_prefix(a, stream), "layer_prefix launch");
+        // The timed kernel is the dominant one of each path: the chunk kernel
+        // (matrix cores) or the fused transition-tile kernel (vector ALUs).
         hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (timed) {
+        auto tic = [&]() {
+            if (!timed) return;
             e0 = ctx.get_event();
             e1 = ctx.get_event();
             hip_check(hipEventRecord(e0, stream), "hipEventRecord");
-        }
-        hip_check(ekgk::launch_ecg_fused(a, stream), "ecg_fused launch");
-        if (timed) {
+        };
+        auto toc = [&]() {
+            if (!timed) return;
             hip_check(hipEventRecord(e1, stream), "hipEventRecord");
             ctx.pending.emplace_back(e0, e1);
+        };
+        hip_check(ekgk::launch_ap_tables(a, stream), "ap_tables launch");
+        if (g.mfma) {
+            tic();
+            hip_check(ekgk::launch_ecg_chunk(a, stream), "ecg_chunk launch");
+            toc();
+            hip_check(ekgk::launch_layer_prefix(a, stream), "layer_prefix launch");
+        } else {
+            hip_check(ekgk::launch_lead_field(a, stream), "lead_field launch");
+            hip_check(ekgk::launch_chunk_sums(a, stream), "chunk_sums launch");
+            hip_check(ekgk::launch_layer_prefix(a, stream), "layer_prefix launch");
+            tic();
+            hip_check(ekgk::launch_ecg_fused(a, stream), "ecg_fused launch");
+            toc();
         }
         hip_check(ekgk::launch_ecg_reduce(a, stream), "ecg_reduce launch");
     };
@@ -706,7 +741,7 @@ ekg_tune autotune(Heart& h, const ekg_sim_desc& sample, int batch)
     if (!h.excited) fail(EKG_ESTATE, "excitation sequence not computed or loaded");
     ctx.bind();
     const int saved_kernel = ctx.kernel;
-    const int def_chunk = (int)std::min<int64_t>(4096, std::max<int64_t>(256, h.n_myo / 400));
+    const int def_chunk = (int)std::min<int64_t>(ekgk::CHUNK_MAX, std::max<int64_t>(256, h.n_myo / 400));
     ekg_tune best{def_chunk, 4, 2, EKG_KERNEL_MFMA, 0.0};
     Workspace ws;
     hipStream_t st = ctx.stream;
@@ -746,7 +781,7 @@ ekg_tune autotune(Heart& h, const ekg_sim_desc& sample, int batch)
             }
         };
         sweep(&ekg_tune::kernel, {EKG_KERNEL_MFMA, EKG_KERNEL_VALU});
-        std::vector<int> chunks{256, 512, 1024, 2048, 4096};
+        std::vector<int> chunks{128, 256, 512};
         chunks.erase(std::remove(chunks.begin(), chunks.end(), def_chunk), chunks.end());
         sweep(&ekg_tune::chunk_voxels, chunks);
         if (best.kernel == EKG_KERNEL_MFMA) sweep(&ekg_tune::mfma_waves, {2, 4, 8, 16});

@@ -80,7 +80,7 @@ struct EcgLayout {
     std::vector<int4> h_chunks;
     std::vector<int> h_layer_chunk;
     std::vector<double> h_layer_t0;
-    DevBuf vox_chunk, vox_a64, vox_a, chunks, bricks, brick_off, brick_vox, halo_off, halo_cells,
+    DevBuf vox_chunk, vox_a64, vox_a, vox_geo, chunks, bricks, brick_off, brick_vox, halo_off, halo_cells,
         layer_chunk, layer_t0;
     ekgk::HeartDev dev{};          // device pointers and sizes for the kernels
 };

@@ -12,6 +12,10 @@ namespace ekgk {
 // plus a one-group look-ahead past the last chunk).
 constexpr int MFMA_PAD = 8;
 
+// Largest chunk (myocardial voxels) of a layout: the chunk kernel keeps a
+// whole chunk's G rows in LDS.
+constexpr int CHUNK_MAX = 512;
+
 // ---- excitation sequence ---------------------------------------------------
 
 template <bool kInt> struct CaT { using type = double; };
@@ -48,6 +52,7 @@ struct HeartDev {
     const int* vox_chunk;        // [V] chunk of each sorted voxel
     const double* vox_a64;       // [V] activation time
     const float2* vox_a;         // [V] activation time (hi, lo)
+    const uint2* vox_geo;        // [V] {z | y << 10 | x << 20, 3x3x3 myocardium mask (bit (dz+1)*9+(dy+1)*3+dx+1)}
     const int4* chunks;          // [n_chunks] {layer, v_begin, v_end, t_ref as float bits}
     const int* layer_chunk;      // [max_layer + 2]: chunks of layer l are [layer_chunk[l], layer_chunk[l+1])
     const double* layer_t0;      // [max_layer + 1]: reference time of layer l
@@ -78,6 +83,7 @@ struct GroupArgs {
     const ReqDev* reqs;
     int B, LP, nhood, mfma, mfma_waves;
     int lf_phase;              // leads per lead-field phase (2 or 4)
+    int chunk_max;             // largest chunk target of the group's hearts (chunk kernel LDS rows)
     const int64_t* lf_start;   // lead-field blocks (bricks) per request
     const int64_t* ch_start;   // chunks per request
     const int64_t* xy_start;   // nl * Tp per request
@@ -105,6 +111,8 @@ hipError_t launch_lead_field(const GroupArgs& a, hipStream_t s);
 hipError_t launch_chunk_sums(const GroupArgs& a, hipStream_t s);
 hipError_t launch_layer_prefix(const GroupArgs& a, hipStream_t s);
 hipError_t launch_ecg_fused(const GroupArgs& a, hipStream_t s);
+// Matrix-core path: lead field, chunk sums and transition tiles in one kernel.
+hipError_t launch_ecg_chunk(const GroupArgs& a, hipStream_t s);
 hipError_t launch_ecg_reduce(const GroupArgs& a, hipStream_t s);
 
 }  // namespace ekgk

@@ -168,8 +168,8 @@ int ekg_ctx_synchronize(ekg_ctx *ctx);
 /* Tunable launch shape of a heart's simulations.  Every setting gives ECGs
  * within the same tolerance; they differ in speed only. */
 typedef struct ekg_tune {
-    int chunk_voxels;     /* target myocardial voxels per chunk, 256..4096 */
-    int mfma_waves;       /* waves per block of the matrix-core fused kernel, 1..16 */
+    int chunk_voxels;     /* target myocardial voxels per chunk, 64..512 */
+    int mfma_waves;       /* waves per block of the matrix-core chunk kernel, 2..16 */
     int lead_phase;       /* leads per lead-field phase, 2 or 4 */
     int kernel;           /* EKG_KERNEL_MFMA or EKG_KERNEL_VALU */
     double ms_per_batch;  /* measured device time of the chosen shape (autotune only) */
