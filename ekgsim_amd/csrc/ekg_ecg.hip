@@ -175,6 +175,19 @@ __device__ __forceinline__ void lead_f32(float e0, float e1, float e2, float& m0
     m2 = e2 * s;
 }
 
+// s = 1/|e|^3 in FP32 with the hardware reciprocal square root alone.  With
+// e in FP32, the lead weight's dipole term s (d . e) is an exact FP64 product
+// (24 + at most 29 significant bits), so a cell's weight enters every G row
+// that uses it with the same value.
+__device__ __forceinline__ float lead_s32(float e0, float e1, float e2)
+{
+    float l = e0 * e0;
+    l = __builtin_fmaf(e1, e1, l);
+    l = __builtin_fmaf(e2, e2, l);
+    const float r = __builtin_amdgcn_rsqf(l);
+    return r * r * r;
+}
+
 constexpr int LB = 8;                        // lead-field brick edge
 constexpr int LH = LB + 2;                   // with halo
 constexpr int LN = LH * LH * LH;             // halo cells
@@ -635,11 +648,11 @@ constexpr int CHUNK_SEG = 8;  // fixed summation segments of the chunk sums
 // chunk's G rows never leave the CU.
 //  1. Thread 0 finds the tile bounds {z, f} (as chunk_sums_kernel).  Each
 //     voxel u of the chunk (one per thread, strided) gets its G row from its
-//     own cell and its myocardial neighbours directly: per lead the FP32 lead
-//     weight of each of those cells and the FP64 sum
-//       G_i(u) = m_i(u) . s_u + sum_{d in N, u+d myo} m_i(u+d) . d
-//     in the lead-field kernel's order (its non-myocardial neighbours add
-//     zeros), into LDS with the voxel's {P, Q, a} and FP64 {P', Q'}.
+//     own cell and its myocardial neighbours directly: per lead the lead
+//     weight m = e s of each of those cells (FP32 e and s, lead_s32) and the
+//     FP64 sum of the exact products
+//       G_i(u) = s(u) (e(u) . s_u) + sum_{d in N, u+d myo} s(u+d) (e(u+d) . d)
+//     into LDS, with the voxel's {P, Q, a} and FP64 {P', Q'}.
 //  2. Chunk sums sum G, sum G P', sum G Q' in a fixed order (CHUNK_SEG
 //     strided segments, then the segments in order), to LDS and to `sums`.
 //  3. Transition tiles, 16-step sub-tiles spread over the waves: a sub-tile
@@ -651,7 +664,9 @@ constexpr int CHUNK_SEG = 8;  // fixed summation segments of the chunk sums
 //     one MFMA per group of 4 voxels below s2.  gfx950 lane layout (checked
 //     by a one-hot probe): A(i,k) in lane 16k+i, B(k,j) in lane 16k+j,
 //     D(i,j) in lane 16(i%4)+j, register i/4.  Lanes past the chunk end
-//     contribute zeros.
+//     contribute zeros.  Four accumulator chains per sub-tile (group g into
+//     chain g % 4) keep the matrix cores busy; their sum is taken in a fixed
+//     order.
 // The host keeps every chunk of a group at most NMAX voxels.
 template <int LP, int NH, int NMAX>
 __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
@@ -692,23 +707,26 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
         s_p[t / 3][0][t % 3] = hi;
         s_p[t / 3][1][t % 3] = (float)(p - (double)hi);
     }
-    if (t == 0) {
+    if (w == 0) {
+        // tile bounds, wave-parallel (the same predicates as chunk_sums_kernel)
         const double a_max = H.vox_a64[ch.z - 1];
         const float2 a_min = H.vox_a[ch.y];
-        const int f = c1d < 0.0 ? first_true(R.n_tiles, [&](int tile) {
-            return c1d * (tg[tile * 64] - a_max) <= SAT64_X;
+        const int f = c1d < 0.0 ? first_false(0, R.n_tiles, [&](int tile) {
+            return !(c1d * (tg[tile * 64] - a_max) <= SAT64_X);
         }) : R.n_tiles;
-        const int z = (kc.c1 < 0.0f && kc.skip != 0.0f) ? first_true(R.n_tiles, [&](int tile) {
+        const int z = (kc.c1 < 0.0f && kc.skip != 0.0f) ? first_false(0, R.n_tiles, [&](int tile) {
             const float2 tl = th[tile * 64 + 63];
-            return kc.c1 * tau_of(tl.x, tl.y, a_min.x, a_min.y) < SKIP_X;
+            return !(kc.c1 * tau_of(tl.x, tl.y, a_min.x, a_min.y) < SKIP_X);
         }) : 0;
-        s_tb[0] = z < f ? z : f;
-        s_tb[1] = f;
-        bnd[R.c_off + c] = int2{z < f ? z : f, f};
+        if (lane == 0) {
+            s_tb[0] = z < f ? z : f;
+            s_tb[1] = f;
+            bnd[R.c_off + c] = int2{z < f ? z : f, f};
+        }
     }
     __syncthreads();
 
-    // 1. G rows and AP factors
+    // 1. G rows and AP factors, one voxel per thread (strided)
     for (int q = t; q < n; q += blockDim.x) {
         const int v = v0 + q;
         const uint2 geo = H.vox_geo[v];
@@ -725,13 +743,12 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
         const double sd0 = (double)i0, sd1 = (double)i1, sd2 = (double)i2;
         // grid coordinate + 1 of the voxel (the reference's voxel centre)
         const float fz = (float)(gz + 1), fy = (float)(gy + 1), fx = (float)(gx + 1);
-        double* grow = s_G + q * LP;
-        // two leads per pass: two independent FP64 accumulation chains.  A
-        // cell's lead weight is a function of the cell alone, (p_hi - c) + p_lo
-        // then lead_f32, bitwise the same wherever it is evaluated: its FP32
-        // rounding then cancels between the two ends of each dipole, and only
-        // V differences across the dipole see it.  (A weight derived per
-        // evaluating voxel, e.g. from e(u) - d, does not cancel: ~1e-5.)
+        // Two leads per pass (two independent FP64 accumulation chains).  A
+        // cell's weight m = e s is a function of the cell alone: e = (p_hi - c)
+        // + p_lo and s = lead_s32(e) in FP32, then exact FP64 products, so its
+        // rounding cancels between the two ends of each dipole and only V
+        // differences across the dipole see it (a weight derived per
+        // evaluating voxel, e.g. from e(u) - d, does not cancel: ~1e-5).
 #pragma unroll 1
         for (int i = 0; i < LP; i += 2) {
             double g0 = 0.0, g1 = 0.0;
@@ -742,38 +759,40 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
                             ea2 = (pa[0][2] - fx) + pa[1][2];
                 const float eb0 = (pb[0][0] - fz) + pb[1][0], eb1 = (pb[0][1] - fy) + pb[1][1],
                             eb2 = (pb[0][2] - fx) + pb[1][2];
-                float a0, a1, a2, b0, b1, b2;
-                lead_f32(ea0, ea1, ea2, a0, a1, a2);
-                lead_f32(eb0, eb1, eb2, b0, b1, b2);
-                g0 = (double)a0 * sd0 + (double)a1 * sd1 + (double)a2 * sd2;
-                g1 = (double)b0 * sd0 + (double)b1 * sd1 + (double)b2 * sd2;
-#pragma unroll
+                // own cell: m(u) . s_u
+                g0 = (double)lead_s32(ea0, ea1, ea2) * ((double)ea0 * sd0 + (double)ea1 * sd1 + (double)ea2 * sd2);
+                g1 = (double)lead_s32(eb0, eb1, eb2) * ((double)eb0 * sd0 + (double)eb1 * sd1 + (double)eb2 * sd2);
+    #pragma unroll
                 for (int qn = 0; qn < N::n; ++qn) {
                     if ((mk >> cube_bit(N::d[qn][0], N::d[qn][1], N::d[qn][2])) & 1u) {
                         const float cz = (float)(gz + 1 + N::d[qn][0]), cy = (float)(gy + 1 + N::d[qn][1]),
                                     cx = (float)(gx + 1 + N::d[qn][2]);
-                        lead_f32((pa[0][0] - cz) + pa[1][0], (pa[0][1] - cy) + pa[1][1], (pa[0][2] - cx) + pa[1][2],
-                                 a0, a1, a2);
-                        lead_f32((pb[0][0] - cz) + pb[1][0], (pb[0][1] - cy) + pb[1][1], (pb[0][2] - cx) + pb[1][2],
-                                 b0, b1, b2);
+                        const float na0 = (pa[0][0] - cz) + pa[1][0], na1 = (pa[0][1] - cy) + pa[1][1],
+                                    na2 = (pa[0][2] - cx) + pa[1][2];
+                        const float nb0 = (pb[0][0] - cz) + pb[1][0], nb1 = (pb[0][1] - cy) + pb[1][1],
+                                    nb2 = (pb[0][2] - cx) + pb[1][2];
+                        // d . e exactly in FP64 (d has entries -1, 0, 1)
+                        double da = 0.0, db = 0.0;
                         if (N::d[qn][0]) {
-                            g0 += (double)a0 * N::d[qn][0];
-                            g1 += (double)b0 * N::d[qn][0];
+                            da += (double)na0 * N::d[qn][0];
+                            db += (double)nb0 * N::d[qn][0];
                         }
                         if (N::d[qn][1]) {
-                            g0 += (double)a1 * N::d[qn][1];
-                            g1 += (double)b1 * N::d[qn][1];
+                            da += (double)na1 * N::d[qn][1];
+                            db += (double)nb1 * N::d[qn][1];
                         }
                         if (N::d[qn][2]) {
-                            g0 += (double)a2 * N::d[qn][2];
-                            g1 += (double)b2 * N::d[qn][2];
+                            da += (double)na2 * N::d[qn][2];
+                            db += (double)nb2 * N::d[qn][2];
                         }
+                        g0 = __builtin_fma((double)lead_s32(na0, na1, na2), da, g0);
+                        g1 = __builtin_fma((double)lead_s32(nb0, nb1, nb2), db, g1);
                     }
                 }
                 if (i + 1 >= R.L) g1 = 0.0;
             }
-            grow[i] = g0;
-            grow[i + 1] = g1;
+            s_G[q * LP + i] = g0;
+            s_G[q * LP + i + 1] = g1;
         }
         const double a = H.vox_a64[v];
         const float2 av = H.vox_a[v];
@@ -852,21 +871,45 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
                     return kc.c1 * tau_of(tl2.x, tl2.y, a4.z, a4.w) < SKIP_X;
                 });
         }
-        mfma_f64x4 acc = mfma_f64x4{0.0, 0.0, 0.0, 0.0};
-        for (int vg = 0; vg < s2; vg += 4) {
-            const bool ok = vg + kv < n;
-            const double a_op = (ok && ia < LP) ? s_G[(vg + kv) * LP + ia] : 0.0;
-            const float4 q = ok ? s_aux[vg + kv] : float4{0.0f, 0.0f, 0.0f, 0.0f};
-            float ap = __builtin_fmaf(q.x, X, q.y * Y);
-            if (vg + 4 > s1) ap *= fast_rcp(1.0f + fast_exp2(kc.c1 * tau_of(tt.x, tt.y, q.z, q.w)));
-            const double bv = ok ? (double)ap : 0.0;
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a_op, bv, acc, 0, 0, 0);
+        // s1, s2 are wave-uniform (ballot searches): scalar loop control
+        const int s1u = __builtin_amdgcn_readfirstlane(s1), s2u = __builtin_amdgcn_readfirstlane(s2);
+        // four independent accumulator chains (group g goes to chain g % 4),
+        // summed in a fixed order at the end; each block of 4 groups loads its
+        // operands first (clamped rows, zeroed past the chunk end or the lead
+        // count), then evaluates and contracts
+        mfma_f64x4 acc[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[u] = mfma_f64x4{0.0, 0.0, 0.0, 0.0};
+        const int ic = ia < LP ? ia : LP - 1;
+        for (int vb = 0; vb < s2u; vb += 16) {
+            double gv[4];
+            float4 qv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int vr = vb + 4 * u + kv;
+                const int vc = vr < n ? vr : n - 1;
+                gv[u] = s_G[vc * LP + ic];
+                qv[u] = s_aux[vc];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int vg = vb + 4 * u;
+                if (vg < s2u) {
+                    const bool ok = vg + kv < n;
+                    float ap = __builtin_fmaf(qv[u].x, X, qv[u].y * Y);
+                    if (vg + 4 > s1u)
+                        ap *= fast_rcp(1.0f + fast_exp2(kc.c1 * tau_of(tt.x, tt.y, qv[u].z, qv[u].w)));
+                    const double a_op = (ok && ia < LP) ? gv[u] : 0.0;
+                    const double bv = ok ? (double)ap : 0.0;
+                    acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_op, bv, acc[u], 0, 0, 0);
+                }
+            }
         }
         // D(i, j) sits in lane 16(i%4)+j, register i/4: lead i = 4q + lane/16
 #pragma unroll
         for (int qd = 0; qd < 4; ++qd) {
             const int i = 4 * qd + kv;
-            if (i < R.L && i < LP) out[(int64_t)i * R.Tp + j] = acc[qd];
+            if (i < R.L && i < LP) out[(int64_t)i * R.Tp + j] = (acc[0][qd] + acc[1][qd]) + (acc[2][qd] + acc[3][qd]);
         }
     }
 }
