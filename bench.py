@@ -33,18 +33,18 @@ METRIC = ("simulated ECGs/sec at 128^3, 600 ms, 12 leads (batched vs plain) at "
           "≤1e-5 rel-L2; p99")
 # MI355X dense FP64 matrix-core peak (the lead contraction's precision), FLOP/s
 MFMA_FP64_PEAK = 78.6e12
-# Sigmoid window of the fused kernel in the exp2 domain x = c1 (t - a),
-# c1 = -k1 log2(e): outside (SAT64_X, SKIP_X) a voxel's AP is an exact closed
-# form (sigmoid 1) or below 2^-64 of its plateau, so only window voxel-steps
-# need an individually evaluated AP.
-SAT64_X, SKIP_X = -54.0, 64.0
+# Sigmoid window of the transition tiles in the exp2 domain x = c1 (t - a),
+# c1 = -k1 log2(e): outside (SATC_X, SKIP_X) a voxel's AP takes the FP64
+# closed form (sigmoid within 2^-24.5 of 1, as in the FP32 AP) or is below
+# 2^-36 of its plateau, so only window voxel-steps need an evaluated AP.
+SATC_X, SKIP_X = -24.5, 48.0
 
 
 def window_voxel_steps(act, k1, start, dt, T):
-    """Voxel-steps (u, j) with SAT64_X < c1 (t_j - a_u) < SKIP_X, t_j = start + j dt."""
+    """Voxel-steps (u, j) with SATC_X < c1 (t_j - a_u) < SKIP_X, t_j = start + j dt."""
     c1 = -k1 * math.log2(math.e)
-    lo = act + SKIP_X / c1          # t > a - 64/|c1|
-    hi = act + SAT64_X / c1         # t < a + 54/|c1|
+    lo = act + SKIP_X / c1          # t > a - 48/|c1|
+    hi = act + SATC_X / c1          # t < a + 24.5/|c1|
     jlo = np.clip(np.floor((lo - start) / dt) + 1, 0, T)
     jhi = np.clip(np.ceil((hi - start) / dt), 0, T)
     return int(np.maximum(jhi - jlo, 0).sum())

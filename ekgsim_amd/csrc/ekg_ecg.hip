@@ -47,16 +47,18 @@ __device__ __forceinline__ int find_req(const int64_t* __restrict__ start, int B
 }
 
 // Sigmoid bounds in the exp2 domain.  With x = c1*tau, c1 = -k1 log2(e):
-//   x >= SKIP_X for every lane  -> S < 2^-64: with k4 + k5 <= k1/4 (the
+//   x >= SKIP_X for every lane  -> S < 2^-48: with k4 + k5 <= k1/4 (the
 //                                  LayerConst skip flag) the cell's AP is below
-//                                  k2 e^-33 and the voxel is skipped;
+//                                  k2 2^-36 < k2 e^-24 and the voxel is skipped;
 //   x <= SAT_X for every lane   -> exp2(x) < 2^-24, 1 + exp2(x) rounds to 1,
 //                                  S = rcp(1) = 1 exactly in FP32: no sigmoid;
-//   x <= SAT64_X                -> 1 + 2^x rounds to 1 in FP64 as well: the
-//                                  reference's sigmoid is exactly 1.
-constexpr float SKIP_X = 64.0f;
+//   x <= SATC_X (FP64 test on   -> every voxel's S is within 2^-24.5 of 1, as
+//   the latest voxel of a          in the FP32 AP: the chunk or sub-tile takes
+//   chunk at a tile's first        the exact FP64 closed form X_l sum G P' +
+//   step)                          Y_l sum G Q' instead of FP32 evaluation.
+constexpr float SKIP_X = 48.0f;
 constexpr float SAT_X = -24.5f;
-constexpr double SAT64_X = -54.0;
+constexpr double SATC_X = -24.5;
 constexpr double LOG2E_D = 1.4426950408889634074;
 // Largest FP32 lane factor kept; with S = 0 exactly in such lanes the
 // product stays 0 instead of 0 * inf.
@@ -457,7 +459,7 @@ __global__ __launch_bounds__(256) void chunk_sums_kernel(
         const double* tg = tgrid + R.t_off;
         const float2* th = thl + R.t_off;
         const int f = c1d < 0.0 ? first_true(R.n_tiles, [&](int tile) {
-            return c1d * (tg[tile * 64] - a_max) <= SAT64_X;
+            return c1d * (tg[tile * 64] - a_max) <= SATC_X;
         }) : R.n_tiles;
         const int z = (kc.c1 < 0.0f && kc.skip != 0.0f) ? first_true(R.n_tiles, [&](int tile) {
             const float2 tl = th[tile * 64 + 63];
@@ -646,13 +648,13 @@ constexpr int CHUNK_SEG = 8;  // fixed summation segments of the chunk sums
 
 // Block = `nw` waves of one (request, chunk) of at most NMAX voxels; the
 // chunk's G rows never leave the CU.
-//  1. Thread 0 finds the tile bounds {z, f} (as chunk_sums_kernel).  Each
-//     voxel u of the chunk (one per thread, strided) gets its G row from its
+//  1. Each voxel u of the chunk (one per thread, strided) gets its G row from its
 //     own cell and its myocardial neighbours directly: per lead the lead
 //     weight m = e s of each of those cells (FP32 e and s, lead_s32) and the
 //     FP64 sum of the exact products
 //       G_i(u) = s(u) (e(u) . s_u) + sum_{d in N, u+d myo} s(u+d) (e(u+d) . d)
-//     into LDS, with the voxel's {P, Q, a} and FP64 {P', Q'}.
+//     into LDS, with the voxel's {P, Q, a} and FP64 {P', Q'}.  Wave 0 then
+//     finds the tile bounds {z, f} (as chunk_sums_kernel).
 //  2. Chunk sums sum G, sum G P', sum G Q' in a fixed order (CHUNK_SEG
 //     strided segments, then the segments in order), to LDS and to `sums`.
 //  3. Transition tiles, 16-step sub-tiles spread over the waves: a sub-tile
@@ -706,23 +708,6 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
         const float hi = (float)p;
         s_p[t / 3][0][t % 3] = hi;
         s_p[t / 3][1][t % 3] = (float)(p - (double)hi);
-    }
-    if (w == 0) {
-        // tile bounds, wave-parallel (the same predicates as chunk_sums_kernel)
-        const double a_max = H.vox_a64[ch.z - 1];
-        const float2 a_min = H.vox_a[ch.y];
-        const int f = c1d < 0.0 ? first_false(0, R.n_tiles, [&](int tile) {
-            return !(c1d * (tg[tile * 64] - a_max) <= SAT64_X);
-        }) : R.n_tiles;
-        const int z = (kc.c1 < 0.0f && kc.skip != 0.0f) ? first_false(0, R.n_tiles, [&](int tile) {
-            const float2 tl = th[tile * 64 + 63];
-            return !(kc.c1 * tau_of(tl.x, tl.y, a_min.x, a_min.y) < SKIP_X);
-        }) : 0;
-        if (lane == 0) {
-            s_tb[0] = z < f ? z : f;
-            s_tb[1] = f;
-            bnd[R.c_off + c] = int2{z < f ? z : f, f};
-        }
     }
     __syncthreads();
 
@@ -799,6 +784,24 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
         s_aux[q] = float4{(float)(alpha * exp(k45 * (a - tref))), (float)(beta * exp(k5 * (a - tref))), av.x, av.y};
         s_pq[q] = double2{alpha * exp(k45 * (a - t0)), beta * exp(k5 * (a - t0))};
     }
+    if (w == 0) {
+        // tile bounds, wave-parallel (the same predicates as chunk_sums_kernel);
+        // needed from the transition tiles on, so after this wave's G rows
+        const double a_max = H.vox_a64[ch.z - 1];
+        const float2 a_min = H.vox_a[ch.y];
+        const int f = c1d < 0.0 ? first_false(0, R.n_tiles, [&](int tile) {
+            return !(c1d * (tg[tile * 64] - a_max) <= SATC_X);
+        }) : R.n_tiles;
+        const int z = (kc.c1 < 0.0f && kc.skip != 0.0f) ? first_false(0, R.n_tiles, [&](int tile) {
+            const float2 tl = th[tile * 64 + 63];
+            return !(kc.c1 * tau_of(tl.x, tl.y, a_min.x, a_min.y) < SKIP_X);
+        }) : 0;
+        if (lane == 0) {
+            s_tb[0] = z < f ? z : f;
+            s_tb[1] = f;
+            bnd[R.c_off + c] = int2{z < f ? z : f, f};
+        }
+    }
     __syncthreads();
 
     // 2. chunk sums: lead i = t % 16, segment t / 16 takes voxels seg, seg + CHUNK_SEG, ...
@@ -845,7 +848,7 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
         const int j0 = (tz + (sb >> 2)) * 64 + 16 * (sb & 3);
         const int j = j0 + jl;
         const double2 xyj = XY[j];
-        if (c1d * (tg[j0] - a_last) <= SAT64_X) {
+        if (c1d * (tg[j0] - a_last) <= SATC_X) {
             // every voxel saturated in FP64: the exact closed form
 #pragma unroll
             for (int qd = 0; qd < 4; ++qd) {
@@ -951,7 +954,17 @@ __global__ void ecg_reduce_kernel(const HeartDev* __restrict__ hearts, const Req
                 s = __builtin_fma(lc[R.lay_off + l - 1].k0, pre_ab[(slot + R.n_tiles) * 2 * LP + i], s);
                 const int2 rg = pre_rng[slot + tile];
                 const double* p = partial + R.p_off + (int64_t)i * R.Tp + j;
-                for (int c = rg.x; c < rg.y; ++c) s += p[(int64_t)c * LP * R.Tp];
+                const int64_t st = (int64_t)LP * R.Tp;
+                // eight independent loads in flight, added in chunk order
+                int c = rg.x;
+                for (; c + 8 <= rg.y; c += 8) {
+                    double x[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) x[u] = p[(int64_t)(c + u) * st];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) s += x[u];
+                }
+                for (; c < rg.y; ++c) s += p[(int64_t)c * st];
             }
         }
         ecg[R.ecg_off + loc] = s;
