@@ -66,22 +66,22 @@ def lead_field_cells(layer):
 
 
 # Algorithmic work of the chunk kernel, counted per unit:
-#   per lead-weight evaluation (voxel, lead, cell): 22 FP32 flops (electrode
-#     offset 6, |e|^2 5, rsq 1, Newton step 5, r^3 2, m 3) and 3 FP64 adds;
+#   per lead-weight evaluation (voxel, lead, cell): 14 FP32 flops (electrode
+#     offset 6, |e|^2 5, rsq 1, r^3 2) and 4 FP64 flops (d . e, one FMA);
 #   per window voxel-step (the transition tiles): 2L FP64 MFMA flops and 14
 #     FP32 ops (10 AP assembly + 4 transcendental).
 # FP32 counts half against the FP64 peak (the FP32 vector rate is twice it).
-FP32_PER_CELL, FP64_PER_CELL = 22, 3
+FP32_PER_CELL, FP64_PER_CELL = 14, 4
 FP32_PER_WINDOW_STEP = 14
 # Naive per-voxel-step model: every voxel at every step, 38 VALU lane-cycles
 # at L = 12, against 256 CU x 64 lanes x 2.4 GHz.
 NAIVE_LANE_CYCLES, LANE_PEAK = 38, 256 * 64 * 2.4e9
 
-# HBM bytes per launch of the dominant kernel from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE, each scaled by the counter's reading for a copy kernel of known
-# size (profiles/ecg_fused_mfma_128cubed_batch256_hbm_traffic.md); None until
-# measured.
-TRAFFIC_BYTES = None
+# HBM bytes per launch of the dominant kernel on this workload (chunk target
+# 256, 4 waves) from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled
+# by its reading for a 256 MiB calibration copy
+# (profiles/ecg_chunk_kernel_128cubed_batch256_hbm_traffic.md).
+TRAFFIC_BYTES = 4861287967
 
 
 def log(*a):
@@ -166,6 +166,8 @@ def main():
     ap.add_argument("--tune", default=None,
                     help="fixed launch shape chunk,waves,phase,kernel (e.g. 2048,2,2,2) instead of autotuning")
     ap.add_argument("--no-graph", action="store_true", help="launch the timed steps eagerly")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="one device-to-device copy of MIB MiB before the steps (scale for HBM counters)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -228,6 +230,12 @@ def main():
         tuning = heart.autotune(ks[0], pos[0], "3D4", 0.0, args.length, 1.0, batch=min(B, 64))
         tuning["seconds"] = round(time.perf_counter() - t0, 2)
         log(f"autotune: {tuning}")
+    if args.calibrate_copy > 0:
+        src = torch.zeros(args.calibrate_copy << 17, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for _ in range(args.warmup):
         step()
     ctx.synchronize()
@@ -267,7 +275,8 @@ def main():
     log(f"batched: {ms_step:.2f} ms/step, {value:.1f} ECGs/s; dominant kernel {avg_kernel_s * 1e3:.2f} ms; "
         f"{win / B / V:.1f} window steps per voxel; {achieved / 1e12:.2f} TFLOP/s FP64-equivalent")
     if args.profile_only:
-        print(json.dumps({"ms_per_step": ms_step, "fused_ms": avg_kernel_s * 1e3}))
+        print(json.dumps({"ms_per_step": ms_step, "dominant_kernel_ms": avg_kernel_s * 1e3,
+                          "calibration_copy_bytes": args.calibrate_copy << 20}))
         return
 
     ecg_gpu = d_e.cpu().numpy()
