@@ -645,6 +645,7 @@ typedef double mfma_f64x4 __attribute__((ext_vector_type(4)));
 __device__ constexpr int cube_bit(int d0, int d1, int d2) { return (d0 + 1) * 9 + (d1 + 1) * 3 + (d2 + 1); }
 
 constexpr int CHUNK_SEG = 8;  // fixed summation segments of the chunk sums
+constexpr int CH_LEADS = 2;   // leads per pass of the chunk kernel's lead field (divides LP)
 
 // Block = `nw` waves of one (request, chunk) of at most NMAX voxels; the
 // chunk's G rows never leave the CU.
@@ -728,56 +729,57 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
         const double sd0 = (double)i0, sd1 = (double)i1, sd2 = (double)i2;
         // grid coordinate + 1 of the voxel (the reference's voxel centre)
         const float fz = (float)(gz + 1), fy = (float)(gy + 1), fx = (float)(gx + 1);
-        // Two leads per pass (two independent FP64 accumulation chains).  A
+        // CH_LEADS leads per pass (independent FP64 accumulation chains).  A
         // cell's weight m = e s is a function of the cell alone: e = (p_hi - c)
         // + p_lo and s = lead_s32(e) in FP32, then exact FP64 products, so its
         // rounding cancels between the two ends of each dipole and only V
         // differences across the dipole see it (a weight derived per
         // evaluating voxel, e.g. from e(u) - d, does not cancel: ~1e-5).
 #pragma unroll 1
-        for (int i = 0; i < LP; i += 2) {
-            double g0 = 0.0, g1 = 0.0;
-            if (i < R.L) {
-                const float(*pa)[3] = s_p[i];
-                const float(*pb)[3] = s_p[i + 1 < R.L ? i + 1 : i];
-                const float ea0 = (pa[0][0] - fz) + pa[1][0], ea1 = (pa[0][1] - fy) + pa[1][1],
-                            ea2 = (pa[0][2] - fx) + pa[1][2];
-                const float eb0 = (pb[0][0] - fz) + pb[1][0], eb1 = (pb[0][1] - fy) + pb[1][1],
-                            eb2 = (pb[0][2] - fx) + pb[1][2];
-                // own cell: m(u) . s_u
-                g0 = (double)lead_s32(ea0, ea1, ea2) * ((double)ea0 * sd0 + (double)ea1 * sd1 + (double)ea2 * sd2);
-                g1 = (double)lead_s32(eb0, eb1, eb2) * ((double)eb0 * sd0 + (double)eb1 * sd1 + (double)eb2 * sd2);
-    #pragma unroll
+        for (int i0 = 0; i0 < LP; i0 += CH_LEADS) {
+            double g[CH_LEADS];
+#pragma unroll
+            for (int w2 = 0; w2 < CH_LEADS; ++w2) g[w2] = 0.0;
+            if (i0 < R.L) {
+                float ph[CH_LEADS][3], pl[CH_LEADS][3];
+#pragma unroll
+                for (int w2 = 0; w2 < CH_LEADS; ++w2) {
+                    // leads past the request's count repeat its first lead of the pass
+                    const int ii = i0 + w2 < R.L ? i0 + w2 : i0;
+#pragma unroll
+                    for (int c3 = 0; c3 < 3; ++c3) {
+                        ph[w2][c3] = s_p[ii][0][c3];
+                        pl[w2][c3] = s_p[ii][1][c3];
+                    }
+                }
+#pragma unroll
+                for (int w2 = 0; w2 < CH_LEADS; ++w2) {
+                    // own cell: m(u) . s_u
+                    const float e0 = (ph[w2][0] - fz) + pl[w2][0], e1 = (ph[w2][1] - fy) + pl[w2][1],
+                                e2 = (ph[w2][2] - fx) + pl[w2][2];
+                    g[w2] = (double)lead_s32(e0, e1, e2) * ((double)e0 * sd0 + (double)e1 * sd1 + (double)e2 * sd2);
+                }
+#pragma unroll
                 for (int qn = 0; qn < N::n; ++qn) {
                     if ((mk >> cube_bit(N::d[qn][0], N::d[qn][1], N::d[qn][2])) & 1u) {
                         const float cz = (float)(gz + 1 + N::d[qn][0]), cy = (float)(gy + 1 + N::d[qn][1]),
                                     cx = (float)(gx + 1 + N::d[qn][2]);
-                        const float na0 = (pa[0][0] - cz) + pa[1][0], na1 = (pa[0][1] - cy) + pa[1][1],
-                                    na2 = (pa[0][2] - cx) + pa[1][2];
-                        const float nb0 = (pb[0][0] - cz) + pb[1][0], nb1 = (pb[0][1] - cy) + pb[1][1],
-                                    nb2 = (pb[0][2] - cx) + pb[1][2];
-                        // d . e exactly in FP64 (d has entries -1, 0, 1)
-                        double da = 0.0, db = 0.0;
-                        if (N::d[qn][0]) {
-                            da += (double)na0 * N::d[qn][0];
-                            db += (double)nb0 * N::d[qn][0];
+#pragma unroll
+                        for (int w2 = 0; w2 < CH_LEADS; ++w2) {
+                            const float n0 = (ph[w2][0] - cz) + pl[w2][0], n1 = (ph[w2][1] - cy) + pl[w2][1],
+                                        n2 = (ph[w2][2] - cx) + pl[w2][2];
+                            // d . e exactly in FP64 (d has entries -1, 0, 1)
+                            double de = 0.0;
+                            if (N::d[qn][0]) de += (double)n0 * N::d[qn][0];
+                            if (N::d[qn][1]) de += (double)n1 * N::d[qn][1];
+                            if (N::d[qn][2]) de += (double)n2 * N::d[qn][2];
+                            g[w2] = __builtin_fma((double)lead_s32(n0, n1, n2), de, g[w2]);
                         }
-                        if (N::d[qn][1]) {
-                            da += (double)na1 * N::d[qn][1];
-                            db += (double)nb1 * N::d[qn][1];
-                        }
-                        if (N::d[qn][2]) {
-                            da += (double)na2 * N::d[qn][2];
-                            db += (double)nb2 * N::d[qn][2];
-                        }
-                        g0 = __builtin_fma((double)lead_s32(na0, na1, na2), da, g0);
-                        g1 = __builtin_fma((double)lead_s32(nb0, nb1, nb2), db, g1);
                     }
                 }
-                if (i + 1 >= R.L) g1 = 0.0;
             }
-            s_G[q * LP + i] = g0;
-            s_G[q * LP + i + 1] = g1;
+#pragma unroll
+            for (int w2 = 0; w2 < CH_LEADS; ++w2) s_G[q * LP + i0 + w2] = i0 + w2 < R.L ? g[w2] : 0.0;
         }
         const double a = H.vox_a64[v];
         const float2 av = H.vox_a[v];
