@@ -741,39 +741,53 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
 #pragma unroll
             for (int w2 = 0; w2 < CH_LEADS; ++w2) g[w2] = 0.0;
             if (i0 < R.L) {
-                float ph[CH_LEADS][3], pl[CH_LEADS][3];
+                // Per lead, the electrode offset of a cell is (p_hi - c) + p_lo per
+                // axis, and the voxel and its neighbours span only three values of
+                // c per axis: E[axis][d + 1] in FP32 and FP64, and the first term
+                // of |e|^2 per x-offset.  Each cell's weight is then formed from
+                // exactly the operands (and in the order) its own evaluation uses.
+                float E[CH_LEADS][3][3], SQ0[CH_LEADS][3];
+                double E64[CH_LEADS][3][3];
+                const float fc[3] = {fz, fy, fx};
 #pragma unroll
                 for (int w2 = 0; w2 < CH_LEADS; ++w2) {
                     // leads past the request's count repeat its first lead of the pass
                     const int ii = i0 + w2 < R.L ? i0 + w2 : i0;
 #pragma unroll
                     for (int c3 = 0; c3 < 3; ++c3) {
-                        ph[w2][c3] = s_p[ii][0][c3];
-                        pl[w2][c3] = s_p[ii][1][c3];
+                        const float ph = s_p[ii][0][c3], pl = s_p[ii][1][c3];
+#pragma unroll
+                        for (int k3 = 0; k3 < 3; ++k3) {
+                            E[w2][c3][k3] = (ph - (fc[c3] + (float)(k3 - 1))) + pl;
+                            E64[w2][c3][k3] = (double)E[w2][c3][k3];
+                        }
                     }
+#pragma unroll
+                    for (int k3 = 0; k3 < 3; ++k3) SQ0[w2][k3] = E[w2][0][k3] * E[w2][0][k3];
                 }
 #pragma unroll
                 for (int w2 = 0; w2 < CH_LEADS; ++w2) {
                     // own cell: m(u) . s_u
-                    const float e0 = (ph[w2][0] - fz) + pl[w2][0], e1 = (ph[w2][1] - fy) + pl[w2][1],
-                                e2 = (ph[w2][2] - fx) + pl[w2][2];
-                    g[w2] = (double)lead_s32(e0, e1, e2) * ((double)e0 * sd0 + (double)e1 * sd1 + (double)e2 * sd2);
+                    float l = __builtin_fmaf(E[w2][1][1], E[w2][1][1], SQ0[w2][1]);
+                    l = __builtin_fmaf(E[w2][2][1], E[w2][2][1], l);
+                    const float r = __builtin_amdgcn_rsqf(l);
+                    g[w2] = (double)(r * r * r) * (E64[w2][0][1] * sd0 + E64[w2][1][1] * sd1 + E64[w2][2][1] * sd2);
                 }
 #pragma unroll
                 for (int qn = 0; qn < N::n; ++qn) {
                     if ((mk >> cube_bit(N::d[qn][0], N::d[qn][1], N::d[qn][2])) & 1u) {
-                        const float cz = (float)(gz + 1 + N::d[qn][0]), cy = (float)(gy + 1 + N::d[qn][1]),
-                                    cx = (float)(gx + 1 + N::d[qn][2]);
+                        const int k0 = N::d[qn][0] + 1, k1 = N::d[qn][1] + 1, k2 = N::d[qn][2] + 1;
 #pragma unroll
                         for (int w2 = 0; w2 < CH_LEADS; ++w2) {
-                            const float n0 = (ph[w2][0] - cz) + pl[w2][0], n1 = (ph[w2][1] - cy) + pl[w2][1],
-                                        n2 = (ph[w2][2] - cx) + pl[w2][2];
+                            float l = __builtin_fmaf(E[w2][1][k1], E[w2][1][k1], SQ0[w2][k0]);
+                            l = __builtin_fmaf(E[w2][2][k2], E[w2][2][k2], l);
+                            const float r = __builtin_amdgcn_rsqf(l);
                             // d . e exactly in FP64 (d has entries -1, 0, 1)
                             double de = 0.0;
-                            if (N::d[qn][0]) de += (double)n0 * N::d[qn][0];
-                            if (N::d[qn][1]) de += (double)n1 * N::d[qn][1];
-                            if (N::d[qn][2]) de += (double)n2 * N::d[qn][2];
-                            g[w2] = __builtin_fma((double)lead_s32(n0, n1, n2), de, g[w2]);
+                            if (N::d[qn][0]) de += E64[w2][0][k0] * N::d[qn][0];
+                            if (N::d[qn][1]) de += E64[w2][1][k1] * N::d[qn][1];
+                            if (N::d[qn][2]) de += E64[w2][2][k2] * N::d[qn][2];
+                            g[w2] = __builtin_fma((double)(r * r * r), de, g[w2]);
                         }
                     }
                 }
