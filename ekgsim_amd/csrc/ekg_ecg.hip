@@ -710,6 +710,16 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
         s_p[t / 3][0][t % 3] = hi;
         s_p[t / 3][1][t % 3] = (float)(p - (double)hi);
     }
+    // operands of the tile-bound tests, loaded now and used after the G rows
+    // (one probe per lane when the request has at most 64 tiles)
+    const double a_max = H.vox_a64[ch.z - 1];
+    const float2 a_min = H.vox_a[ch.y];
+    double tg_probe = 0.0;
+    float2 th_probe = float2{0.0f, 0.0f};
+    if (w == 0 && lane < R.n_tiles) {
+        tg_probe = tg[lane * 64];
+        th_probe = th[lane * 64 + 63];
+    }
     __syncthreads();
 
     // 1. G rows and AP factors, one voxel per thread (strided)
@@ -801,17 +811,28 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
         s_pq[q] = double2{alpha * exp(k45 * (a - t0)), beta * exp(k5 * (a - t0))};
     }
     if (w == 0) {
-        // tile bounds, wave-parallel (the same predicates as chunk_sums_kernel);
-        // needed from the transition tiles on, so after this wave's G rows
-        const double a_max = H.vox_a64[ch.z - 1];
-        const float2 a_min = H.vox_a[ch.y];
-        const int f = c1d < 0.0 ? first_false(0, R.n_tiles, [&](int tile) {
-            return !(c1d * (tg[tile * 64] - a_max) <= SATC_X);
-        }) : R.n_tiles;
-        const int z = (kc.c1 < 0.0f && kc.skip != 0.0f) ? first_false(0, R.n_tiles, [&](int tile) {
-            const float2 tl = th[tile * 64 + 63];
-            return !(kc.c1 * tau_of(tl.x, tl.y, a_min.x, a_min.y) < SKIP_X);
-        }) : 0;
+        // tile bounds (the same predicates as chunk_sums_kernel), needed from
+        // the transition tiles on: one ballot over the prefetched probes, or
+        // the wave-parallel search for requests with more than 64 tiles
+        const bool sat_ok = c1d < 0.0, skip_ok = kc.c1 < 0.0f && kc.skip != 0.0f;
+        int f = R.n_tiles, z = 0;
+        if (R.n_tiles <= 64) {
+            const bool fp = lane < R.n_tiles && sat_ok && c1d * (tg_probe - a_max) <= SATC_X;
+            const bool zp = lane < R.n_tiles && skip_ok &&
+                            kc.c1 * tau_of(th_probe.x, th_probe.y, a_min.x, a_min.y) < SKIP_X;
+            // both predicates are monotone in the tile: the count of false
+            // probes is the first true tile
+            if (sat_ok) f = R.n_tiles - __popcll(__ballot(fp));
+            if (skip_ok) z = R.n_tiles - __popcll(__ballot(zp));
+        } else {
+            if (sat_ok)
+                f = first_false(0, R.n_tiles, [&](int tile) { return !(c1d * (tg[tile * 64] - a_max) <= SATC_X); });
+            if (skip_ok)
+                z = first_false(0, R.n_tiles, [&](int tile) {
+                    const float2 tl = th[tile * 64 + 63];
+                    return !(kc.c1 * tau_of(tl.x, tl.y, a_min.x, a_min.y) < SKIP_X);
+                });
+        }
         if (lane == 0) {
             s_tb[0] = z < f ? z : f;
             s_tb[1] = f;
@@ -825,7 +846,24 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
         const int i = t & 15, sg = t >> 4;
         if (sg < CHUNK_SEG && i < LP) {
             double sG = 0.0, sP = 0.0, sQ = 0.0;
-            for (int q = sg; q < n; q += CHUNK_SEG) {
+            int q = sg;
+            // four rows' loads in flight, summed in row order
+            for (; q + 3 * CHUNK_SEG < n; q += 4 * CHUNK_SEG) {
+                double gq[4];
+                double2 pq[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    gq[u] = s_G[(q + u * CHUNK_SEG) * LP + i];
+                    pq[u] = s_pq[q + u * CHUNK_SEG];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    sG += gq[u];
+                    sP = __builtin_fma(gq[u], pq[u].x, sP);
+                    sQ = __builtin_fma(gq[u], pq[u].y, sQ);
+                }
+            }
+            for (; q < n; q += CHUNK_SEG) {
                 const double g = s_G[q * LP + i];
                 const double2 pq = s_pq[q];
                 sG += g;
