@@ -227,7 +227,7 @@ def main():
         tuning = heart.get_tune()
     elif not args.no_autotune:
         t0 = time.perf_counter()
-        tuning = heart.autotune(ks[0], pos[0], "3D4", 0.0, args.length, 1.0, batch=min(B, 64))
+        tuning = heart.autotune(ks[0], pos[0], "3D4", 0.0, args.length, 1.0, batch=B)
         tuning["seconds"] = round(time.perf_counter() - t0, 2)
         log(f"autotune: {tuning}")
     if args.calibrate_copy > 0:

@@ -473,78 +473,91 @@ __global__ __launch_bounds__(256) void chunk_sums_kernel(
 // Layer prefix tables
 // ---------------------------------------------------------------------------
 
-// One wave per (request, layer); lane i < LP is lead i.  Within a layer the
-// chunks are ordered by activation, so both tile bounds are non-decreasing
-// along the chunk list: for tile tau the saturated chunks are a prefix
-// [begin, lo) and the chunks in transition a range [lo, hi).  Writes per tile
-// {lo, hi} and the prefix sums A = sum_{c < lo} sum G P', B = sum_{c < lo}
-// sum G Q'; slot n_tiles holds the layer's sum G (for k0).  Fixed order.
+// One block per (request, layer).  Within a layer the chunks are ordered by
+// activation, so both tile bounds are non-decreasing along the chunk list:
+// for tile tau the saturated chunks are a prefix [begin, lo) and the chunks in
+// transition a range [lo, hi).  Writes per tile {lo, hi} and the prefix sums
+// A = sum_{c < lo} sum G P', B = sum_{c < lo} sum G Q'; slot n_tiles holds the
+// layer's sum G (for k0).  The sums have a fixed order that depends only on
+// the layout: the layer's chunks form consecutive segments of `seg` chunks
+// (at most PRE_SEGS segments), each summed in chunk order; the segment sums
+// are accumulated in order; a prefix at lo is the accumulated sum of the
+// segments before lo's segment plus that segment's chunks before lo, in order.
+constexpr int PRE_SEGS = 128;
+
 template <int LP>
-__global__ __launch_bounds__(64) void layer_prefix_kernel(
+__global__ __launch_bounds__(256) void layer_prefix_kernel(
     const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
     const int64_t* __restrict__ lay_start, const double* __restrict__ sums,
     const int2* __restrict__ bnd, int2* __restrict__ pre_rng, double* __restrict__ pre_ab)
 {
+    constexpr int NC3 = 3 * LP;
+    __shared__ double s_seg[PRE_SEGS + 1][NC3];   // [k][col]: sum of the segments before k
     const int r = find_req(lay_start, B, blockIdx.x);
     const ReqDev R = reqs[r];
     const HeartDev& H = hearts[R.heart];
     const int l = (int)(blockIdx.x - lay_start[r]) + 1;
-    const int i = threadIdx.x;
+    const int t = threadIdx.x;
     const int cb = l <= H.max_layer ? H.layer_chunk[l] : 0;
     const int ce = l <= H.max_layer ? H.layer_chunk[l + 1] : 0;
+    const int nc = ce - cb;
+    const int seg = nc <= PRE_SEGS ? 1 : (nc + PRE_SEGS - 1) / PRE_SEGS;
+    const int nseg = (nc + seg - 1) / seg;
     const int64_t slot = R.pre_off + (int64_t)(l - 1) * (R.n_tiles + 1);
     const int2* bd = bnd + R.c_off;
-    const double* sc = sums + R.c_off * 3 * LP;
-    double A = 0.0, Bq = 0.0;
-    int lo = cb, hi = cb;
-    for (int tile = 0; tile < R.n_tiles; ++tile) {
-        // chunks saturated by this tile: a prefix (wave-parallel search)
-        const int lo1 = first_false(lo, ce, [&](int c) { return bd[c].y <= tile; });
-        if (i < LP) {
-            // in chunk order; the loads do not depend on the running sums
-            int c = lo;
-            for (; c + 4 <= lo1; c += 4) {
-                const double p0 = sc[(c + 0) * 3 * LP + LP + i], q0 = sc[(c + 0) * 3 * LP + 2 * LP + i];
-                const double p1 = sc[(c + 1) * 3 * LP + LP + i], q1 = sc[(c + 1) * 3 * LP + 2 * LP + i];
-                const double p2 = sc[(c + 2) * 3 * LP + LP + i], q2 = sc[(c + 2) * 3 * LP + 2 * LP + i];
-                const double p3 = sc[(c + 3) * 3 * LP + LP + i], q3 = sc[(c + 3) * 3 * LP + 2 * LP + i];
-                A += p0;
-                Bq += q0;
-                A += p1;
-                Bq += q1;
-                A += p2;
-                Bq += q2;
-                A += p3;
-                Bq += q3;
-            }
-            for (; c < lo1; ++c) {
-                A += sc[c * 3 * LP + LP + i];
-                Bq += sc[c * 3 * LP + 2 * LP + i];
-            }
-        }
-        lo = lo1;
-        if (hi < lo) hi = lo;
-        hi = first_false(hi, ce, [&](int c) { return bd[c].x <= tile; });
-        if (i == 0) pre_rng[slot + tile] = int2{lo, hi};
-        if (i < LP) {
-            pre_ab[(slot + tile) * 2 * LP + i] = A;
-            pre_ab[(slot + tile) * 2 * LP + LP + i] = Bq;
+    const double* sc = sums + R.c_off * NC3;
+    // segment sums (columns: sum G, sum G P', sum G Q' per lead)
+    for (int it = t; it < nseg * NC3; it += blockDim.x) {
+        const int k = it / NC3, col = it - k * NC3;
+        const int c0 = cb + k * seg, c1 = c0 + seg < ce ? c0 + seg : ce;
+        double acc = 0.0;
+        for (int c = c0; c < c1; ++c) acc += sc[(int64_t)c * NC3 + col];
+        s_seg[k + 1][col] = acc;
+    }
+    __syncthreads();
+    if (t < NC3) {
+        double run = 0.0;
+        s_seg[0][t] = 0.0;
+        for (int k = 1; k <= nseg; ++k) {
+            run += s_seg[k][t];
+            s_seg[k][t] = run;
         }
     }
-    if (i < LP) {
-        double g = 0.0;
-        int c = cb;
-        for (; c + 4 <= ce; c += 4) {
-            const double g0 = sc[(c + 0) * 3 * LP + i], g1 = sc[(c + 1) * 3 * LP + i];
-            const double g2 = sc[(c + 2) * 3 * LP + i], g3 = sc[(c + 3) * 3 * LP + i];
-            g += g0;
-            g += g1;
-            g += g2;
-            g += g3;
+    __syncthreads();
+    // per (tile, lead): transition range by binary searches, prefix at lo
+    for (int it = t; it < R.n_tiles * LP; it += blockDim.x) {
+        const int tile = it / LP, i = it - tile * LP;
+        int a0 = cb, a1 = ce;
+        while (a0 < a1) {
+            const int mid = (a0 + a1) >> 1;
+            if (bd[mid].y <= tile)
+                a0 = mid + 1;
+            else
+                a1 = mid;
         }
-        for (; c < ce; ++c) g += sc[c * 3 * LP + i];
-        pre_ab[(slot + R.n_tiles) * 2 * LP + i] = g;
-        pre_ab[(slot + R.n_tiles) * 2 * LP + LP + i] = 0.0;
+        const int lo = a0;
+        a1 = ce;
+        while (a0 < a1) {
+            const int mid = (a0 + a1) >> 1;
+            if (bd[mid].x <= tile)
+                a0 = mid + 1;
+            else
+                a1 = mid;
+        }
+        const int hi = a0;
+        const int k = (lo - cb) / seg;
+        double A = s_seg[k][LP + i], Bq = s_seg[k][2 * LP + i];
+        for (int c = cb + k * seg; c < lo; ++c) {
+            A += sc[(int64_t)c * NC3 + LP + i];
+            Bq += sc[(int64_t)c * NC3 + 2 * LP + i];
+        }
+        if (i == 0) pre_rng[slot + tile] = int2{lo, hi};
+        pre_ab[(slot + tile) * 2 * LP + i] = A;
+        pre_ab[(slot + tile) * 2 * LP + LP + i] = Bq;
+    }
+    if (t < LP) {
+        pre_ab[(slot + R.n_tiles) * 2 * LP + t] = s_seg[nseg][t];
+        pre_ab[(slot + R.n_tiles) * 2 * LP + LP + t] = 0.0;
     }
 }
 
@@ -1106,7 +1119,7 @@ hipError_t launch_layer_prefix(const GroupArgs& a, hipStream_t s)
     switch (a.LP) {
 #define EKG_PREFIX(LPV)                                                                                  \
     case LPV:                                                                                            \
-        hipLaunchKernelGGL(layer_prefix_kernel<LPV>, dim3((unsigned)a.n_lay), dim3(64), 0, s, a.hearts, \
+        hipLaunchKernelGGL(layer_prefix_kernel<LPV>, dim3((unsigned)a.n_lay), dim3(256), 0, s, a.hearts, \
                            a.reqs, a.B, a.lay_start, a.sums, a.bnd, a.pre_rng, a.pre_ab);               \
         break;
         EKG_PREFIX(4)
@@ -1150,7 +1163,11 @@ static void chunk_launch3(const GroupArgs& a, hipStream_t s)
 template <int LP, int NH>
 static void chunk_launch2(const GroupArgs& a, hipStream_t s)
 {
-    if (a.chunk_max <= 256)
+    // LDS rows for the group's largest chunk: the smaller the rows, the more
+    // blocks share a CU
+    if (a.chunk_max <= 128)
+        chunk_launch3<LP, NH, 128>(a, s);
+    else if (a.chunk_max <= 256)
         chunk_launch3<LP, NH, 256>(a, s);
     else
         chunk_launch3<LP, NH, 512>(a, s);

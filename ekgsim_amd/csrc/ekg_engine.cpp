@@ -11,6 +11,13 @@
 
 namespace ekg {
 
+// Default launch shape of the matrix-core path (measured on MI355X at 128^3 x
+// 256 requests: 128-voxel chunks in 2-wave blocks keep eight blocks per CU;
+// larger blocks leave waves idle in the transition tiles while holding LDS).
+constexpr int DEFAULT_CHUNK = 128;
+constexpr int DEFAULT_WAVES = 2;
+
+
 [[noreturn]] void fail(int code, const std::string& msg) { throw Error{code, msg}; }
 
 void hip_check(hipError_t e, const char* what)
@@ -179,7 +186,7 @@ GroupPlan plan_group(Ctx& ctx, std::vector<ReqSpec> reqs)
     if (ctx.kernel != EKG_KERNEL_AUTO) kernel = ctx.kernel;  // an explicit context choice wins
     g.LP = L <= 4 ? 4 : L <= 8 ? 8 : L <= 12 ? 12 : 16;
     g.mfma = kernel != EKG_KERNEL_VALU;
-    g.waves = waves ? waves : 4;
+    g.waves = waves ? waves : DEFAULT_WAVES;
     g.lf_phase = phase ? phase : 2;
     g.reqs = std::move(reqs);
     return g;
@@ -194,12 +201,9 @@ void ensure_layout(Heart& h)
 {
     if (!h.excited) fail(EKG_ESTATE, "excitation sequence not computed or loaded");
     EcgLayout& L = h.layout;
-    // Chunk size: enough waves for one simulation to fill the GPU
-    // (~400 chunks), bounded so that per-wave setup stays small; an autotuned
-    // size replaces it.  Chunks are layer-pure and span at most
-    // CHUNK_SPAN_MS of activation time.
-    const int64_t target = h.tune_chunk ? h.tune_chunk
-                                        : std::min<int64_t>(ekgk::CHUNK_MAX, std::max<int64_t>(256, h.n_myo / 400));
+    // Chunk size: DEFAULT_CHUNK voxels unless the heart is tuned.  Chunks
+    // are layer-pure and span at most CHUNK_SPAN_MS of activation time.
+    const int64_t target = h.tune_chunk ? h.tune_chunk : DEFAULT_CHUNK;
     if (L.valid && L.chunk_target == target) return;
     L.valid = false;
     std::vector<int64_t> idx;
@@ -741,8 +745,8 @@ ekg_tune autotune(Heart& h, const ekg_sim_desc& sample, int batch)
     if (!h.excited) fail(EKG_ESTATE, "excitation sequence not computed or loaded");
     ctx.bind();
     const int saved_kernel = ctx.kernel;
-    const int def_chunk = (int)std::min<int64_t>(ekgk::CHUNK_MAX, std::max<int64_t>(256, h.n_myo / 400));
-    ekg_tune best{def_chunk, 4, 2, EKG_KERNEL_MFMA, 0.0};
+    const int def_chunk = DEFAULT_CHUNK;
+    ekg_tune best{def_chunk, DEFAULT_WAVES, 2, EKG_KERNEL_MFMA, 0.0};
     Workspace ws;
     hipStream_t st = ctx.stream;
     hipEvent_t e0 = ctx.get_event(), e1 = ctx.get_event();
@@ -781,11 +785,28 @@ ekg_tune autotune(Heart& h, const ekg_sim_desc& sample, int batch)
             }
         };
         sweep(&ekg_tune::kernel, {EKG_KERNEL_MFMA, EKG_KERNEL_VALU});
-        std::vector<int> chunks{128, 256, 512};
-        chunks.erase(std::remove(chunks.begin(), chunks.end(), def_chunk), chunks.end());
-        sweep(&ekg_tune::chunk_voxels, chunks);
-        if (best.kernel == EKG_KERNEL_MFMA) sweep(&ekg_tune::mfma_waves, {2, 4, 8, 16});
-        sweep(&ekg_tune::lead_phase, {2, 4});
+        if (best.kernel == EKG_KERNEL_MFMA) {
+            // chunk size and block size interact (LDS rows per block against
+            // blocks per CU): every pair
+            const ekg_tune base = best;
+            for (int chunk : {64, 128, 256, 512})
+                for (int waves : {2, 4, 8}) {
+                    if (chunk == base.chunk_voxels && waves == base.mfma_waves) continue;
+                    ekg_tune c = base;
+                    c.chunk_voxels = chunk;
+                    c.mfma_waves = waves;
+                    const double ms = trial(c);
+                    if (ms < best.ms_per_batch) {
+                        best = c;
+                        best.ms_per_batch = ms;
+                    }
+                }
+        } else {
+            std::vector<int> chunks{64, 128, 256, 512};
+            chunks.erase(std::remove(chunks.begin(), chunks.end(), def_chunk), chunks.end());
+            sweep(&ekg_tune::chunk_voxels, chunks);
+            sweep(&ekg_tune::lead_phase, {2, 4});
+        }
     } catch (...) {
         ctx.kernel = saved_kernel;
         h.tune_chunk = h.tune_waves = h.tune_phase = h.tune_kernel = 0;

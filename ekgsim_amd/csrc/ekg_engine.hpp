@@ -116,7 +116,7 @@ struct ReqSpec {
 // A launch group: requests sharing the neighbourhood and the kernel path.
 struct GroupPlan {
     std::vector<ReqSpec> reqs;
-    int LP = 4, nhood = 0, waves = 4, lf_phase = 2;
+    int LP = 4, nhood = 0, waves = 2, lf_phase = 2;
     bool mfma = true;
 };
 
@@ -137,8 +137,9 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
                    const double* d_pos, double* d_ecg, bool time_fused);
 void simulate(Heart& h, int batch, const ekg_sim_desc* descs, double* ecg);
 // Times the tunable knobs on `batch` copies of `sample` and keeps the fastest
-// on the heart (coordinate descent: contraction path, chunk size, fused
-// waves per block, lead-field phase width).
+// on the heart: the contraction path, then for the matrix-core path every
+// (chunk size, waves per block) pair, for the vector-ALU path the chunk size
+// and the lead-field phase width.
 ekg_tune autotune(Heart& h, const ekg_sim_desc& sample, int batch);
 void simulate_device(Heart& h, const ekg_batch_dev& d);
 

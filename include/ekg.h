@@ -175,11 +175,12 @@ typedef struct ekg_tune {
     double ms_per_batch;  /* measured device time of the chosen shape (autotune only) */
 } ekg_tune;
 
-/* Autotuner: times the launch-shape knobs (coordinate descent: contraction
- * path, chunk size, fused-kernel waves, lead-field phase width) on `batch`
- * copies of `sample` on this heart's GPU and keeps the fastest for the
- * heart's later simulations; `out` (optional) receives the choice.  An
- * explicit ekg_ctx_set_kernel choice still overrides the tuned path. */
+/* Autotuner: times the launch-shape knobs on `batch` copies of `sample` on
+ * this heart's GPU -- the contraction path, then every (chunk size, waves per
+ * block) pair of the matrix-core path, or the chunk size and lead-field phase
+ * width of the vector-ALU path -- and keeps the fastest for the heart's later
+ * simulations; `out` (optional) receives the choice.  An explicit
+ * ekg_ctx_set_kernel choice still overrides the tuned path. */
 int ekg_heart_autotune(ekg_heart *h, const ekg_sim_desc *sample, int batch, ekg_tune *out);
 /* Apply a stored tuning (e.g. cached from an earlier autotune on the same
  * heart and shape); chunk_voxels 0 restores the default layout. */
