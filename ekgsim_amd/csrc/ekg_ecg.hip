@@ -477,7 +477,7 @@ __global__ __launch_bounds__(256) void chunk_sums_kernel(
 // activation, so both tile bounds are non-decreasing along the chunk list:
 // for tile tau the saturated chunks are a prefix [begin, lo) and the chunks in
 // transition a range [lo, hi).  Writes per tile {lo, hi} and the prefix sums
-// A = sum_{c < lo} sum G P', B = sum_{c < lo} sum G Q'; slot n_tiles holds the
+// A = sum_{c < lo} sum G P', B = sum_{c < lo} sum G Q'; slot n_bt holds the
 // layer's sum G (for k0).  The sums have a fixed order that depends only on
 // the layout: the layer's chunks form consecutive segments of `seg` chunks
 // (at most PRE_SEGS segments), each summed in chunk order; the segment sums
@@ -503,7 +503,8 @@ __global__ __launch_bounds__(256) void layer_prefix_kernel(
     const int nc = ce - cb;
     const int seg = nc <= PRE_SEGS ? 1 : (nc + PRE_SEGS - 1) / PRE_SEGS;
     const int nseg = (nc + seg - 1) / seg;
-    const int64_t slot = R.pre_off + (int64_t)(l - 1) * (R.n_tiles + 1);
+    const int nbt = R.Tp >> R.bt_shift;   // bound tiles
+    const int64_t slot = R.pre_off + (int64_t)(l - 1) * (nbt + 1);
     const int2* bd = bnd + R.c_off;
     const double* sc = sums + R.c_off * NC3;
     // segment sums (columns: sum G, sum G P', sum G Q' per lead)
@@ -525,7 +526,7 @@ __global__ __launch_bounds__(256) void layer_prefix_kernel(
     }
     __syncthreads();
     // per (tile, lead): transition range by binary searches, prefix at lo
-    for (int it = t; it < R.n_tiles * LP; it += blockDim.x) {
+    for (int it = t; it < nbt * LP; it += blockDim.x) {
         const int tile = it / LP, i = it - tile * LP;
         int a0 = cb, a1 = ce;
         while (a0 < a1) {
@@ -556,8 +557,8 @@ __global__ __launch_bounds__(256) void layer_prefix_kernel(
         pre_ab[(slot + tile) * 2 * LP + LP + i] = Bq;
     }
     if (t < LP) {
-        pre_ab[(slot + R.n_tiles) * 2 * LP + t] = s_seg[nseg][t];
-        pre_ab[(slot + R.n_tiles) * 2 * LP + LP + t] = 0.0;
+        pre_ab[(slot + nbt) * 2 * LP + t] = s_seg[nseg][t];
+        pre_ab[(slot + nbt) * 2 * LP + LP + t] = 0.0;
     }
 }
 
@@ -668,14 +669,15 @@ constexpr int CH_LEADS = 2;   // leads per pass of the chunk kernel's lead field
 //     FP64 sum of the exact products
 //       G_i(u) = s(u) (e(u) . s_u) + sum_{d in N, u+d myo} s(u+d) (e(u+d) . d)
 //     into LDS, with the voxel's {P, Q, a} and FP64 {P', Q'}.  Wave 0 then
-//     finds the tile bounds {z, f} (as chunk_sums_kernel).
+//     finds the bounds {z, f} in 16-step bound tiles (the chunk_sums_kernel
+//     predicates at that granularity): before z every voxel is skipped, from
+//     f on every voxel is saturated and the reduction takes the closed form
+//     X_l(t) sum G P' + Y_l(t) sum G Q'.
 //  2. Chunk sums sum G, sum G P', sum G Q' in a fixed order (CHUNK_SEG
-//     strided segments, then the segments in order), to LDS and to `sums`.
-//  3. Transition tiles, 16-step sub-tiles spread over the waves: a sub-tile
-//     saturated in FP64 for every voxel takes the exact closed form
-//     X_l(t) sum G P' + Y_l(t) sum G Q'; otherwise its voxels split as in
-//     ecg_fused_kernel into [0, s1) saturated, [s1, s2) general and skipped
-//     from s2 on, and the lead contraction runs on v_mfma_f64_16x16x4_f64,
+//     strided segments, then the segments in order), to `sums`.
+//  3. Transition sub-tiles [z, f), spread over the waves: the chunk's voxels
+//     split as in ecg_fused_kernel into [0, s1) saturated, [s1, s2) general
+//     and skipped from s2 on, and the lead contraction runs on v_mfma_f64_16x16x4_f64,
 //     D[lead i][step j] += sum_k G[voxel k][lead i] * V[voxel k][step j],
 //     one MFMA per group of 4 voxels below s2.  gfx950 lane layout (checked
 //     by a one-hot probe): A(i,k) in lane 16k+i, B(k,j) in lane 16k+j,
@@ -697,7 +699,6 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
     __shared__ float4 s_aux[NMAX];          // {P, Q, a_hi, a_lo} relative to the chunk's t_ref
     __shared__ double2 s_pq[NMAX];          // {P', Q'} relative to the layer's t0
     __shared__ double s_part[CHUNK_SEG][3 * LP];
-    __shared__ double s_sum[3 * LP];
     __shared__ float s_p[16][2][3];         // electrode {hi, lo} per component
     __shared__ int s_tb[2];
     const int t = threadIdx.x, lane = t & 63;
@@ -723,15 +724,17 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
         s_p[t / 3][0][t % 3] = hi;
         s_p[t / 3][1][t % 3] = (float)(p - (double)hi);
     }
-    // operands of the tile-bound tests, loaded now and used after the G rows
-    // (one probe per lane when the request has at most 64 tiles)
+    // operands of the bound-tile tests (16-step sub-tiles), loaded now and
+    // used after the G rows: one probe per lane when there are at most 64
+    constexpr int BT = 1 << BT_SHIFT_MFMA;
+    const int nbt = R.Tp >> BT_SHIFT_MFMA;
     const double a_max = H.vox_a64[ch.z - 1];
     const float2 a_min = H.vox_a[ch.y];
     double tg_probe = 0.0;
     float2 th_probe = float2{0.0f, 0.0f};
-    if (w == 0 && lane < R.n_tiles) {
-        tg_probe = tg[lane * 64];
-        th_probe = th[lane * 64 + 63];
+    if (w == 0 && lane < nbt) {
+        tg_probe = tg[lane * BT];
+        th_probe = th[lane * BT + BT - 1];
     }
     __syncthreads();
 
@@ -828,21 +831,21 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
         // the transition tiles on: one ballot over the prefetched probes, or
         // the wave-parallel search for requests with more than 64 tiles
         const bool sat_ok = c1d < 0.0, skip_ok = kc.c1 < 0.0f && kc.skip != 0.0f;
-        int f = R.n_tiles, z = 0;
-        if (R.n_tiles <= 64) {
-            const bool fp = lane < R.n_tiles && sat_ok && c1d * (tg_probe - a_max) <= SATC_X;
-            const bool zp = lane < R.n_tiles && skip_ok &&
+        int f = nbt, z = 0;
+        if (nbt <= 64) {
+            const bool fp = lane < nbt && sat_ok && c1d * (tg_probe - a_max) <= SATC_X;
+            const bool zp = lane < nbt && skip_ok &&
                             kc.c1 * tau_of(th_probe.x, th_probe.y, a_min.x, a_min.y) < SKIP_X;
             // both predicates are monotone in the tile: the count of false
             // probes is the first true tile
-            if (sat_ok) f = R.n_tiles - __popcll(__ballot(fp));
-            if (skip_ok) z = R.n_tiles - __popcll(__ballot(zp));
+            if (sat_ok) f = nbt - __popcll(__ballot(fp));
+            if (skip_ok) z = nbt - __popcll(__ballot(zp));
         } else {
             if (sat_ok)
-                f = first_false(0, R.n_tiles, [&](int tile) { return !(c1d * (tg[tile * 64] - a_max) <= SATC_X); });
+                f = first_false(0, nbt, [&](int tile) { return !(c1d * (tg[tile * BT] - a_max) <= SATC_X); });
             if (skip_ok)
-                z = first_false(0, R.n_tiles, [&](int tile) {
-                    const float2 tl = th[tile * 64 + 63];
+                z = first_false(0, nbt, [&](int tile) {
+                    const float2 tl = th[tile * BT + BT - 1];
                     return !(kc.c1 * tau_of(tl.x, tl.y, a_min.x, a_min.y) < SKIP_X);
                 });
         }
@@ -893,38 +896,27 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
         double sum = 0.0;
 #pragma unroll
         for (int sg = 0; sg < CHUNK_SEG; ++sg) sum += s_part[sg][t];
-        s_sum[t] = sum;
         sums[(R.c_off + c) * 3 * LP + t] = sum;
     }
     __syncthreads();
 
     // 3. transition tiles [z, f)
     const int tz = s_tb[0], tf = s_tb[1];
-    if (4 * (tf - tz) <= w) return;  // no sub-tile for this wave
+    if (tf - tz <= w) return;  // no sub-tile for this wave
     const double ex = exp(k45 * (tref - t0)), ey = exp(k5 * (tref - t0));
-    const double a_last = H.vox_a64[ch.z - 1];
     const int jl = lane & 15;   // step column
     const int kv = lane >> 4;   // voxel slot of the B operand / lead group of D
     const int ia = lane & 15;   // lead of the A operand
     const double2* XY = xy + R.xy_off + (int64_t)(l - 1) * R.Tp;
     double* out = partial + R.p_off + (int64_t)c * LP * R.Tp;
 
-    // 16-step sub-tiles over the waves: each wave owns whole accumulators
-    const int nsub = (tf - tz) * 4;
-    for (int sb = w; sb < nsub; sb += nw) {
-        const int j0 = (tz + (sb >> 2)) * 64 + 16 * (sb & 3);
+    // 16-step sub-tiles [z, f) over the waves, each wave owning whole
+    // accumulators.  Every one of them has work: before f the chunk's latest
+    // voxel is not saturated, from z on its earliest is not skipped.
+    for (int sb = tz + w; sb < tf; sb += nw) {
+        const int j0 = sb * BT;
         const int j = j0 + jl;
         const double2 xyj = XY[j];
-        if (c1d * (tg[j0] - a_last) <= SATC_X) {
-            // every voxel saturated in FP64: the exact closed form
-#pragma unroll
-            for (int qd = 0; qd < 4; ++qd) {
-                const int i = 4 * qd + kv;
-                if (i < R.L && i < LP)
-                    out[(int64_t)i * R.Tp + j] = __builtin_fma(xyj.x, s_sum[LP + i], xyj.y * s_sum[2 * LP + i]);
-            }
-            continue;
-        }
         const float X = (float)fmin(xyj.x * ex, XMAX);
         const float Y = (float)fmin(xyj.y * ey, XMAX);
         const float2 tt = th[j];
@@ -1010,15 +1002,16 @@ __global__ void ecg_reduce_kernel(const HeartDev* __restrict__ hearts, const Req
         const int j = (int)(loc % R.T);
         double s = 0.0;
         if (j < R.Tn) {
-            const int tile = j >> 6;
+            const int tile = j >> R.bt_shift;
+            const int nbt = R.Tp >> R.bt_shift;
             const int nl = R.nl < H.max_layer ? R.nl : H.max_layer;
             for (int l = 1; l <= nl; ++l) {
-                const int64_t slot = R.pre_off + (int64_t)(l - 1) * (R.n_tiles + 1);
+                const int64_t slot = R.pre_off + (int64_t)(l - 1) * (nbt + 1);
                 const double2 xyj = xy[R.xy_off + (int64_t)(l - 1) * R.Tp + j];
                 const double* ab = pre_ab + (slot + tile) * 2 * LP;
                 s = __builtin_fma(xyj.x, ab[i], s);
                 s = __builtin_fma(xyj.y, ab[LP + i], s);
-                s = __builtin_fma(lc[R.lay_off + l - 1].k0, pre_ab[(slot + R.n_tiles) * 2 * LP + i], s);
+                s = __builtin_fma(lc[R.lay_off + l - 1].k0, pre_ab[(slot + nbt) * 2 * LP + i], s);
                 const int2 rg = pre_rng[slot + tile];
                 const double* p = partial + R.p_off + (int64_t)i * R.Tp + j;
                 const int64_t st = (int64_t)LP * R.Tp;

@@ -442,7 +442,8 @@ size_t group_device_bytes(const GroupPlan& g)
         s += (size_t)r.nl * (sizeof(ekgk::LayerConst) + r.Tp * 16);
         s += (size_t)rows * (16 * 8 + 16);
         s += (size_t)nch * (3 * LP * 8 + 8);
-        s += (size_t)r.nl * (r.n_tiles + 1) * (8 + 2 * LP * 8);
+        s += (size_t)r.nl * ((r.Tp >> (g.mfma ? ekgk::BT_SHIFT_MFMA : ekgk::BT_SHIFT_VALU)) + 1) *
+             (8 + 2 * LP * 8);
         s += (size_t)nch * LP * r.Tp * 8;
         s += (size_t)r.L * r.T * 8;
         s += sizeof(ekgk::ReqDev) + sizeof(ekgk::HeartDev) + 5 * 8;
@@ -508,7 +509,7 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
         R.Tn = (int)q.Tn;
         R.Tp = (int)q.Tp;
         R.n_tiles = q.n_tiles;
-        R.pad = 0;
+        R.bt_shift = g.mfma ? ekgk::BT_SHIFT_MFMA : ekgk::BT_SHIFT_VALU;
         R.k_off = dev_in ? (int64_t)r * q.nl * 9 : k_off;
         R.pos_off = dev_in ? (int64_t)r * q.L * 3 : pos_off;
         R.ecg_off = st_out[r];
@@ -528,7 +529,7 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
         rows += z.rows;
         c_off += z.chunks;
         p_off += z.chunks * LP * q.Tp;
-        pre_off += (int64_t)q.nl * (q.n_tiles + 1);
+        pre_off += (int64_t)q.nl * ((q.Tp >> R.bt_shift) + 1);
         st_lf[r + 1] = st_lf[r] + z.bricks;
         st_ch[r + 1] = st_ch[r] + z.chunks;
         st_xy[r + 1] = st_xy[r] + (int64_t)q.nl * q.Tp;

@@ -16,6 +16,13 @@ constexpr int MFMA_PAD = 8;
 // whole chunk's G rows in LDS.
 constexpr int CHUNK_MAX = 512;
 
+// Bound tiles: the chunk bounds {z, f}, the prefix tables and the reduction
+// work in tiles of 2^BT_SHIFT steps -- 16 on the matrix-core path (its
+// transition work goes by 16-step sub-tiles), 64 on the vector-ALU path (one
+// step per lane of a wave).
+constexpr int BT_SHIFT_MFMA = 4;
+constexpr int BT_SHIFT_VALU = 6;
+
 // ---- excitation sequence ---------------------------------------------------
 
 template <bool kInt> struct CaT { using type = double; };
@@ -62,7 +69,8 @@ struct HeartDev {
 
 // One request of a launch group.  Offsets index the group's packed arrays.
 struct ReqDev {
-    int heart, L, nl, T, Tn, Tp, n_tiles, pad;
+    int heart, L, nl, T, Tn, Tp, n_tiles;
+    int bt_shift;     // log2 of the bound tiles of the chunk bounds and prefix tables (16 or 64 steps)
     int64_t k_off;    // layer_k[k_off + 9*(l-1) + q]
     int64_t pos_off;  // lead_pos[pos_off + 3*i + c]
     int64_t ecg_off;  // ecg[ecg_off + i*T + j]
@@ -72,7 +80,7 @@ struct ReqDev {
     int64_t g_off;    // record row of sorted voxel v: g_off + v
     int64_t c_off;    // per-chunk arrays: sums[(c_off + c)*3*LP ...], bnd[c_off + c]
     int64_t p_off;    // partial[p_off + (c*LP + i)*Tp + j]
-    int64_t pre_off;  // prefix tables: [(pre_off + (l-1)*(n_tiles+1) + tile)]
+    int64_t pre_off;  // prefix tables: [(pre_off + (l-1)*(n_bt+1) + bound tile)], n_bt = Tp >> bt_shift
 };
 
 // A launch group: requests of any hearts, lead counts and time grids that
@@ -99,9 +107,9 @@ struct GroupArgs {
     double* rec;               // 16-double G row per voxel (leads zero-padded)
     float4* aux;               // {P, Q, a_hi, a_lo} per voxel
     double* sums;              // per (request, chunk): [3][LP] sum G, sum G P', sum G Q'
-    int2* bnd;                 // per (request, chunk): {first non-zero tile, first saturated tile}
-    int2* pre_rng;             // per (request, layer, tile): transition chunks [lo, hi)
-    double* pre_ab;            // per (request, layer, tile): [2][LP]; tile n_tiles: [LP] sum G
+    int2* bnd;                 // per (request, chunk): {first non-zero, first saturated} bound tile
+    int2* pre_rng;             // per (request, layer, bound tile): transition chunks [lo, hi)
+    double* pre_ab;            // per (request, layer, bound tile): [2][LP]; tile n_bt: [LP] sum G
     double* partial;
     double* ecg;
 };
