@@ -78,10 +78,10 @@ FP32_PER_WINDOW_STEP = 14
 NAIVE_LANE_CYCLES, LANE_PEAK = 38, 256 * 64 * 2.4e9
 
 # HBM bytes per launch of the dominant kernel on this workload (chunk target
-# 256, 4 waves) from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled
+# 128, 2 waves) from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled
 # by its reading for a 256 MiB calibration copy
 # (profiles/ecg_chunk_kernel_128cubed_batch256_hbm_traffic.md).
-TRAFFIC_BYTES = 4861287967
+TRAFFIC_BYTES = 5219316228
 
 
 def log(*a):
