@@ -32,9 +32,21 @@
 namespace ekgk {
 
 // Request owning work item x: the largest r with start[r] <= x (requests
-// without work items share their start with the next one).
+// without work items share their start with the next one).  A batch whose
+// requests all hold the same number of items is answered by one guess and one
+// check (two dependent loads instead of a log2(B)-deep chain); any other
+// batch falls through to the binary search.
 __device__ __forceinline__ int find_req(const int64_t* __restrict__ start, int B, int64_t x)
 {
+    if (B <= 1) return 0;
+    const int64_t s0 = start[0], w = start[1] - s0, d = x - s0;
+    if (w > 0 && d >= 0) {
+        const int64_t g = (d < 0x7fffffff && w < 0x7fffffff) ? (int64_t)((uint32_t)d / (uint32_t)w) : d / w;
+        if (g < B) {
+            const int r = (int)g;
+            if (start[r] <= x && (r + 1 == B || start[r + 1] > x)) return r;
+        }
+    }
     int lo = 0, hi = B;
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
@@ -681,8 +693,9 @@ constexpr int CH_LEADS = 2;   // leads per pass of the chunk kernel's lead field
 //     D[lead i][step j] += sum_k G[voxel k][lead i] * V[voxel k][step j],
 //     one MFMA per group of 4 voxels below s2.  gfx950 lane layout (checked
 //     by a one-hot probe): A(i,k) in lane 16k+i, B(k,j) in lane 16k+j,
-//     D(i,j) in lane 16(i%4)+j, register i/4.  Lanes past the chunk end
-//     contribute zeros.  Four accumulator chains per sub-tile (group g into
+//     D(i,j) in lane 16(i%4)+j, register i/4.  LDS rows past the chunk end
+//     (to the next multiple of 16) hold zeros and contribute nothing.  Four
+//     accumulator chains per sub-tile (group g into
 //     chain g % 4) keep the matrix cores busy; their sum is taken in a fixed
 //     order.
 // The host keeps every chunk of a group at most NMAX voxels.
@@ -736,6 +749,8 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
         tg_probe = tg[lane * BT];
         th_probe = th[lane * BT + BT - 1];
     }
+    // factors from the layer's t0 to the chunk's t_ref: P = P' e^{(k4+k5)(t0 - t_ref)}, Q = Q' e^{k5 (t0 - t_ref)}
+    const double eP = exp(k45 * (t0 - tref)), eQ = exp(k5 * (t0 - tref));
     __syncthreads();
 
     // 1. G rows and AP factors, one voxel per thread (strided)
@@ -823,8 +838,16 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
         }
         const double a = H.vox_a64[v];
         const float2 av = H.vox_a[v];
-        s_aux[q] = float4{(float)(alpha * exp(k45 * (a - tref))), (float)(beta * exp(k5 * (a - tref))), av.x, av.y};
-        s_pq[q] = double2{alpha * exp(k45 * (a - t0)), beta * exp(k5 * (a - t0))};
+        const double pp = alpha * exp(k45 * (a - t0)), qq = beta * exp(k5 * (a - t0));
+        s_aux[q] = float4{(float)(pp * eP), (float)(qq * eQ), av.x, av.y};
+        s_pq[q] = double2{pp, qq};
+    }
+    // Rows n .. (n rounded up to 16): zero G and P = Q = 0, so that the
+    // contraction loads whole groups of 16 rows unclamped and they add 0.
+    for (int q = n + t; q < ((n + 15) & ~15); q += blockDim.x) {
+#pragma unroll
+        for (int i = 0; i < LP; ++i) s_G[q * LP + i] = 0.0;
+        s_aux[q] = float4{0.0f, 0.0f, 0.0f, 0.0f};
     }
     if (w == 0) {
         // tile bounds (the same predicates as chunk_sums_kernel), needed from
@@ -937,33 +960,32 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
         const int s1u = __builtin_amdgcn_readfirstlane(s1), s2u = __builtin_amdgcn_readfirstlane(s2);
         // four independent accumulator chains (group g goes to chain g % 4),
         // summed in a fixed order at the end; each block of 4 groups loads its
-        // operands first (clamped rows, zeroed past the chunk end or the lead
-        // count), then evaluates and contracts
+        // operands first, then evaluates and contracts.  Rows past the chunk
+        // end up to the next multiple of 16 are zero (G = P = Q = 0), and
+        // lanes of A rows i >= LP repeat lead LP - 1: rows of D past the lead
+        // count are never stored, so neither needs a select.
         mfma_f64x4 acc[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) acc[u] = mfma_f64x4{0.0, 0.0, 0.0, 0.0};
         const int ic = ia < LP ? ia : LP - 1;
+        const double* gcol = s_G + kv * LP + ic;
+        const float4* acol = s_aux + kv;
         for (int vb = 0; vb < s2u; vb += 16) {
             double gv[4];
             float4 qv[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const int vr = vb + 4 * u + kv;
-                const int vc = vr < n ? vr : n - 1;
-                gv[u] = s_G[vc * LP + ic];
-                qv[u] = s_aux[vc];
+                gv[u] = gcol[(vb + 4 * u) * LP];
+                qv[u] = acol[vb + 4 * u];
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int vg = vb + 4 * u;
                 if (vg < s2u) {
-                    const bool ok = vg + kv < n;
                     float ap = __builtin_fmaf(qv[u].x, X, qv[u].y * Y);
                     if (vg + 4 > s1u)
                         ap *= fast_rcp(1.0f + fast_exp2(kc.c1 * tau_of(tt.x, tt.y, qv[u].z, qv[u].w)));
-                    const double a_op = (ok && ia < LP) ? gv[u] : 0.0;
-                    const double bv = ok ? (double)ap : 0.0;
-                    acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_op, bv, acc[u], 0, 0, 0);
+                    acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(gv[u], (double)ap, acc[u], 0, 0, 0);
                 }
             }
         }
