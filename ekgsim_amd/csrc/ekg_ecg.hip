@@ -702,7 +702,7 @@ constexpr int CH_LEADS = 2;   // leads per pass of the chunk kernel's lead field
 template <int LP, int NH, int NMAX>
 __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
     const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
-    const int64_t* __restrict__ ch_start, const double* __restrict__ lead_pos,
+    const int64_t* __restrict__ ch_start, int ch_uniform, const double* __restrict__ lead_pos,
     const double* __restrict__ layer_k, const LayerConst* __restrict__ lc,
     const double2* __restrict__ xy, const double* __restrict__ tgrid, const float2* __restrict__ thl,
     double* __restrict__ sums, int2* __restrict__ bnd, double* __restrict__ partial)
@@ -717,10 +717,20 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
     const int t = threadIdx.x, lane = t & 63;
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
     const int nw = blockDim.x >> 6;
-    const int r = find_req(ch_start, B, blockIdx.x);
+    // Chunk-major when every request has the same number of chunks (one
+    // heart): the requests of a chunk run back to back and read its voxel
+    // records from L2 (blocks go round-robin over the XCDs, so each XCD
+    // fetches a chunk's records once).
+    int r, c;
+    if (ch_uniform > 0) {
+        r = (int)(blockIdx.x % (unsigned)B);
+        c = (int)(blockIdx.x / (unsigned)B);
+    } else {
+        r = find_req(ch_start, B, blockIdx.x);
+        c = (int)(blockIdx.x - ch_start[r]);
+    }
     const ReqDev R = reqs[r];
     const HeartDev& H = hearts[R.heart];
-    const int c = (int)(blockIdx.x - ch_start[r]);
     const int4 ch = H.chunks[c];
     const int l = ch.x, v0 = ch.y, n = ch.z - ch.y;
     const double* kk = layer_k + R.k_off + 9 * (l - 1);
@@ -1171,8 +1181,8 @@ template <int LP, int NH, int NMAX>
 static void chunk_launch3(const GroupArgs& a, hipStream_t s)
 {
     hipLaunchKernelGGL((ecg_chunk_kernel<LP, NH, NMAX>), dim3((unsigned)a.n_ch), dim3(64 * a.mfma_waves), 0, s,
-                       a.hearts, a.reqs, a.B, a.ch_start, a.lead_pos, a.layer_k, a.lc, a.xy, a.tgrid, a.thl,
-                       a.sums, a.bnd, a.partial);
+                       a.hearts, a.reqs, a.B, a.ch_start, a.ch_uniform, a.lead_pos, a.layer_k, a.lc, a.xy,
+                       a.tgrid, a.thl, a.sums, a.bnd, a.partial);
 }
 
 template <int LP, int NH>

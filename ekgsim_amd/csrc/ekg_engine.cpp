@@ -610,6 +610,10 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     a.out_start = stp + 4 * (B + 1);
     a.n_lf = st_lf[B];
     a.n_ch = st_ch[B];
+    // every request with the same chunk count: the chunk kernel runs chunk-major
+    a.ch_uniform = B > 0 ? (int)(st_ch[1] - st_ch[0]) : 0;
+    for (int r = 1; r < B && a.ch_uniform; ++r)
+        if (st_ch[r + 1] - st_ch[r] != a.ch_uniform) a.ch_uniform = 0;
     a.n_xy = st_xy[B];
     a.n_lay = st_lay[B];
     a.n_out = st_out[B];

@@ -98,6 +98,7 @@ struct GroupArgs {
     const int64_t* lay_start;  // nl per request
     const int64_t* out_start;  // L * T per request
     int64_t n_lf, n_ch, n_xy, n_lay, n_out;  // totals
+    int ch_uniform;            // chunks per request when all requests have the same count, else 0
     const double* layer_k;
     const double* lead_pos;
     const double* tgrid;       // FP64 sample times
