@@ -62,6 +62,7 @@ class Tune(C.Structure):
         ("mfma_waves", C.c_int),
         ("lead_phase", C.c_int),
         ("kernel", C.c_int),
+        ("class_min", C.c_int),
         ("ms_per_batch", C.c_double),
     ]
 
@@ -411,9 +412,11 @@ class Heart:
         check(lib().ekg_heart_autotune(self.h, C.byref(d.s), int(batch), C.byref(t)))
         return t.as_dict()
 
-    def set_tune(self, chunk_voxels=0, mfma_waves=0, lead_phase=0, kernel="auto"):
+    def set_tune(self, chunk_voxels=0, mfma_waves=0, lead_phase=0, kernel="auto", class_min=0):
+        """class_min: activation classes of at least this many voxels take the
+        class path (0 = default, -1 = none)."""
         t = Tune(int(chunk_voxels), int(mfma_waves), int(lead_phase),
-                 KERNELS[kernel] if isinstance(kernel, str) else int(kernel), 0.0)
+                 KERNELS[kernel] if isinstance(kernel, str) else int(kernel), int(class_min), 0.0)
         check(lib().ekg_heart_set_tune(self.h, C.byref(t)))
 
     def get_tune(self):

@@ -363,6 +363,7 @@ int ekg_heart_set_tune(ekg_heart* h, const ekg_tune* t)
         if (t->lead_phase != 0 && t->lead_phase != 2 && t->lead_phase != 4)
             ekg::fail(EKG_EINVAL, "lead_phase must be 0, 2 or 4");
         if (t->kernel < EKG_KERNEL_AUTO || t->kernel > EKG_KERNEL_MFMA) ekg::fail(EKG_EINVAL, "unknown kernel");
+        if (t->class_min < -1) ekg::fail(EKG_EINVAL, "class_min must be -1, 0 or a voxel count");
         ekg::Ctx& c = *h->h.ctx;
         std::lock_guard<std::mutex> g1(c.mu);
         std::lock_guard<std::mutex> g2(h->h.mu);
@@ -370,6 +371,7 @@ int ekg_heart_set_tune(ekg_heart* h, const ekg_tune* t)
         h->h.tune_waves = t->mfma_waves;
         h->h.tune_phase = t->lead_phase;
         h->h.tune_kernel = t->kernel;
+        h->h.tune_class = t->class_min;
     });
 }
 
@@ -381,6 +383,7 @@ int ekg_heart_get_tune(const ekg_heart* h, ekg_tune* t)
         t->mfma_waves = h->h.tune_waves;
         t->lead_phase = h->h.tune_phase;
         t->kernel = h->h.tune_kernel;
+        t->class_min = h->h.tune_class;
         t->ms_per_batch = 0.0;
     });
 }

@@ -731,7 +731,9 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
     }
     const ReqDev R = reqs[r];
     const HeartDev& H = hearts[R.heart];
-    const int4 ch = H.chunks[c];
+    // chunk indices from n_chunks on are class chunks: G rows and sums only
+    const bool cls = c >= H.n_chunks;
+    const int4 ch = cls ? H.cchunks[c - H.n_chunks] : H.chunks[c];
     const int l = ch.x, v0 = ch.y, n = ch.z - ch.y;
     const double* kk = layer_k + R.k_off + 9 * (l - 1);
     const LayerConst kc = lc[R.lay_off + l - 1];
@@ -859,7 +861,7 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
         for (int i = 0; i < LP; ++i) s_G[q * LP + i] = 0.0;
         s_aux[q] = float4{0.0f, 0.0f, 0.0f, 0.0f};
     }
-    if (w == 0) {
+    if (w == 0 && !cls) {
         // tile bounds (the same predicates as chunk_sums_kernel), needed from
         // the transition tiles on: one ballot over the prefetched probes, or
         // the wave-parallel search for requests with more than 64 tiles
@@ -931,6 +933,7 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
         for (int sg = 0; sg < CHUNK_SEG; ++sg) sum += s_part[sg][t];
         sums[(R.c_off + c) * 3 * LP + t] = sum;
     }
+    if (cls) return;  // a class's terms come from its sums (ecg_class_kernel)
     __syncthreads();
 
     // 3. transition tiles [z, f)
@@ -1061,6 +1064,79 @@ __global__ void ecg_reduce_kernel(const HeartDev* __restrict__ hearts, const Req
         }
         ecg[R.ecg_off + loc] = s;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Activation classes
+// ---------------------------------------------------------------------------
+
+// csums[(cls_off + k)*3*LP + v]: the sums of class k's chunks (sum G, sum G P',
+// sum G Q' per lead), added in chunk order.
+template <int LP>
+__global__ void class_sums_kernel(const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
+                                  const int64_t* __restrict__ cs_start, int64_t n_cs,
+                                  const double* __restrict__ sums, double* __restrict__ csums)
+{
+    const int64_t n = n_cs * 3 * LP;
+    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t item = idx / (3 * LP);
+        const int v = (int)(idx - item * (3 * LP));
+        const int r = find_req(cs_start, B, item);
+        const ReqDev R = reqs[r];
+        const HeartDev& H = hearts[R.heart];
+        const int4 cd = H.classes[item - cs_start[r]];
+        double s = 0.0;
+        for (int q = cd.y; q < cd.z; ++q) s += sums[(R.c_off + H.n_chunks + q) * 3 * LP + v];
+        csums[item * 3 * LP + v] = s;
+    }
+}
+
+// ecg[r][i][t] += sum over the heart's activation classes, in class order, of
+//   S(t - a) (X_l(t) sum G P' + Y_l(t) sum G Q') + k0 sum G,
+// with S(tau) = 1 / (1 + exp(-k1 tau)) in FP64, the factor of the AP
+// (Wohlfart.h:195-203) at the cell time tau = t - a (simulator.cpp:154-170).
+// Every voxel of a class has the same layer and activation time, hence the
+// same AP, so the class's dipole terms factor through its sums.  Samples from
+// Tn on receive nothing, as in the reduction.  One wave per (request, 64
+// steps), one step per lane, every lead in registers.
+template <int LP>
+__global__ __launch_bounds__(64) void ecg_class_kernel(
+    const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
+    const int64_t* __restrict__ cl_start, const double* __restrict__ layer_k,
+    const double2* __restrict__ xy, const double* __restrict__ tgrid, const double* __restrict__ csums,
+    double* __restrict__ ecg)
+{
+    const int r = find_req(cl_start, B, blockIdx.x);
+    const ReqDev R = reqs[r];
+    const HeartDev& H = hearts[R.heart];
+    const int j = (int)(blockIdx.x - cl_start[r]) * 64 + (int)threadIdx.x;
+    if (j >= R.Tn) return;
+    const double t = tgrid[R.t_off + j];
+    double acc[LP];
+#pragma unroll
+    for (int i = 0; i < LP; ++i) acc[i] = 0.0;
+    int lp = 0;
+    double X = 0.0, Y = 0.0, k0 = 0.0, k1 = 0.0;
+    for (int k = 0; k < H.n_classes; ++k) {
+        const int l = H.classes[k].x;
+        if (l != lp) {  // classes are ordered by layer
+            const double2 xyj = xy[R.xy_off + (int64_t)(l - 1) * R.Tp + j];
+            X = xyj.x;
+            Y = xyj.y;
+            k0 = layer_k[R.k_off + 9 * (l - 1)];
+            k1 = layer_k[R.k_off + 9 * (l - 1) + 1];
+            lp = l;
+        }
+        const double S = 1.0 / (1.0 + exp(-k1 * (t - H.class_a[k])));
+        const double* cs = csums + (R.cls_off + k) * 3 * LP;
+#pragma unroll
+        for (int i = 0; i < LP; ++i)
+            acc[i] += __builtin_fma(k0, cs[i], S * __builtin_fma(X, cs[LP + i], Y * cs[2 * LP + i]));
+    }
+#pragma unroll
+    for (int i = 0; i < LP; ++i)
+        if (i < R.L) ecg[R.ecg_off + (int64_t)i * R.T + j] += acc[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -1220,6 +1296,45 @@ hipError_t launch_ecg_chunk(const GroupArgs& a, hipStream_t s)
     case 8: chunk_launch1<8>(a, s); break;
     case 12: chunk_launch1<12>(a, s); break;
     case 16: chunk_launch1<16>(a, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_class_sums(const GroupArgs& a, hipStream_t s)
+{
+    if (a.n_cs == 0) return hipSuccess;
+    switch (a.LP) {
+#define EKG_CSUMS(LPV)                                                                                  \
+    case LPV:                                                                                           \
+        hipLaunchKernelGGL(class_sums_kernel<LPV>, dim3(grid_for(a.n_cs * 3 * LPV, 256)), dim3(256), 0, s, \
+                           a.hearts, a.reqs, a.B, a.cs_start, a.n_cs, a.sums, a.csums);                  \
+        break;
+        EKG_CSUMS(4)
+        EKG_CSUMS(8)
+        EKG_CSUMS(12)
+        EKG_CSUMS(16)
+#undef EKG_CSUMS
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_ecg_class(const GroupArgs& a, hipStream_t s)
+{
+    if (a.n_cl == 0) return hipSuccess;
+    if (a.n_cl > 0x7fffffffLL) return hipErrorInvalidValue;
+    switch (a.LP) {
+#define EKG_CLASS(LPV)                                                                                  \
+    case LPV:                                                                                           \
+        hipLaunchKernelGGL(ecg_class_kernel<LPV>, dim3((unsigned)a.n_cl), dim3(64), 0, s, a.hearts, a.reqs, \
+                           a.B, a.cl_start, a.layer_k, a.xy, a.tgrid, a.csums, a.ecg);                   \
+        break;
+        EKG_CLASS(4)
+        EKG_CLASS(8)
+        EKG_CLASS(12)
+        EKG_CLASS(16)
+#undef EKG_CLASS
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -170,6 +170,13 @@ ReqSpec make_spec(Heart& h, const ekg_sim_desc& d, bool host_values)
     return r;
 }
 
+bool heart_mfma(const Ctx& ctx, const Heart& h)
+{
+    int kernel = h.tune_kernel ? h.tune_kernel : ctx.kernel;
+    if (ctx.kernel != EKG_KERNEL_AUTO) kernel = ctx.kernel;  // as in plan_group
+    return kernel != EKG_KERNEL_VALU;
+}
+
 GroupPlan plan_group(Ctx& ctx, std::vector<ReqSpec> reqs)
 {
     if (reqs.empty()) fail(EKG_EINVAL, "empty group");
@@ -196,15 +203,21 @@ GroupPlan plan_group(Ctx& ctx, std::vector<ReqSpec> reqs)
 // scales exponentials by e^{(k4+k5)(a - t_ref)} per voxel; a bounded span
 // keeps that factor far from FP32 overflow for any realistic k4, k5.
 constexpr double CHUNK_SPAN_MS = 16.0;
+// Smallest activation class by default: a run of at least this many voxels of
+// one layer with one activation time takes the class path.
+constexpr int DEFAULT_CLASS_MIN = 64;
 
-void ensure_layout(Heart& h)
+void ensure_layout(Heart& h, bool mfma)
 {
     if (!h.excited) fail(EKG_ESTATE, "excitation sequence not computed or loaded");
     EcgLayout& L = h.layout;
     // Chunk size: DEFAULT_CHUNK voxels unless the heart is tuned.  Chunks
     // are layer-pure and span at most CHUNK_SPAN_MS of activation time.
     const int64_t target = h.tune_chunk ? h.tune_chunk : DEFAULT_CHUNK;
-    if (L.valid && L.chunk_target == target) return;
+    // Activation classes (matrix-core path only): runs of one layer and one
+    // activation time with at least cmin voxels.
+    const int cmin = (!mfma || h.tune_class < 0) ? 0 : (h.tune_class > 0 ? h.tune_class : DEFAULT_CLASS_MIN);
+    if (L.valid && L.chunk_target == target && L.class_min == cmin) return;
     L.valid = false;
     std::vector<int64_t> idx;
     idx.reserve(h.n_myo);
@@ -218,41 +231,76 @@ void ensure_layout(Heart& h)
     L.V = V;
     L.max_layer = h.max_layer;
     L.h_chunks.clear();
+    L.h_cchunks.clear();
+    L.h_classes.clear();
+    L.h_class_a.clear();
     L.chunk_target = (int)target;
+    L.class_min = cmin;
+    // reference time of layer l: its earliest activation time, rounded to FP32
+    L.h_layer_t0.assign(h.max_layer + 1, 0.0);
+    std::vector<int64_t> order;  // the layout's voxel order
+    order.reserve(V);
+    auto tbits_of = [](double a) {
+        const float t = (float)a;
+        int b;
+        std::memcpy(&b, &t, 4);
+        return b;
+    };
     int64_t v = 0;
     while (v < V) {
         const int lay = h.h_layer[idx[v]];
         int64_t e = v;
         while (e < V && h.h_layer[idx[e]] == lay) ++e;
-        const int64_t n = e - v;
+        L.h_layer_t0[lay] = (double)(float)h.h_act[idx[v]];
+        // voxels outside classes first, then the classes' runs
+        std::vector<std::pair<int64_t, int64_t>> runs;
+        const int64_t m0 = (int64_t)order.size();
+        for (int64_t s0 = v; s0 < e;) {
+            int64_t s1 = s0 + 1;
+            while (s1 < e && h.h_act[idx[s1]] == h.h_act[idx[s0]]) ++s1;
+            if (cmin > 0 && s1 - s0 >= cmin)
+                runs.push_back({s0, s1});
+            else
+                order.insert(order.end(), idx.begin() + s0, idx.begin() + s1);
+            s0 = s1;
+        }
+        const int64_t m1 = (int64_t)order.size();
+        // chunks: k near-equal pieces of at most `target`, each further cut
+        // where the activation span would exceed CHUNK_SPAN_MS
+        const int64_t n = m1 - m0;
         const int64_t k = (n + target - 1) / target;
         for (int64_t c = 0; c < k; ++c) {
-            int64_t b0 = v + n * c / k;
-            const int64_t b1 = v + n * (c + 1) / k;
+            int64_t b0 = m0 + n * c / k;
+            const int64_t b1 = m0 + n * (c + 1) / k;
             while (b0 < b1) {
-                const float tref = (float)h.h_act[idx[b0]];
+                const float tref = (float)h.h_act[order[b0]];
                 int64_t b2 = b0 + 1;
-                while (b2 < b1 && h.h_act[idx[b2]] - (double)tref <= CHUNK_SPAN_MS) ++b2;
-                int tbits;
-                std::memcpy(&tbits, &tref, 4);
-                L.h_chunks.push_back(int4{lay, (int)b0, (int)b2, tbits});
+                while (b2 < b1 && h.h_act[order[b2]] - (double)tref <= CHUNK_SPAN_MS) ++b2;
+                L.h_chunks.push_back(int4{lay, (int)b0, (int)b2, tbits_of(h.h_act[order[b0]])});
                 b0 = b2;
             }
         }
+        // class chunks: near-equal pieces of at most `target` per class
+        for (const auto& rr : runs) {
+            const int64_t c0 = (int64_t)order.size(), nc = rr.second - rr.first;
+            order.insert(order.end(), idx.begin() + rr.first, idx.begin() + rr.second);
+            const int64_t kc = (nc + target - 1) / target;
+            const int first = (int)L.h_cchunks.size();
+            const double a = h.h_act[idx[rr.first]];
+            for (int64_t c = 0; c < kc; ++c)
+                L.h_cchunks.push_back(int4{lay, (int)(c0 + nc * c / kc), (int)(c0 + nc * (c + 1) / kc), tbits_of(a)});
+            L.h_classes.push_back(int4{lay, first, (int)L.h_cchunks.size(), 0});
+            L.h_class_a.push_back(a);
+        }
         v = e;
     }
+    idx.swap(order);
     L.n_chunks = (int)L.h_chunks.size();
-    // chunks of layer l: [layer_chunk[l], layer_chunk[l+1]); reference time of
-    // layer l: t_ref of its first chunk
+    L.n_cchunks = (int)L.h_cchunks.size();
+    L.n_classes = (int)L.h_classes.size();
+    // chunks of layer l: [layer_chunk[l], layer_chunk[l+1])
     L.h_layer_chunk.assign(h.max_layer + 2, L.n_chunks);
-    L.h_layer_t0.assign(h.max_layer + 1, 0.0);
-    for (int c = L.n_chunks - 1; c >= 0; --c) {
-        const int lay = L.h_chunks[c].x;
-        L.h_layer_chunk[lay] = c;
-        float t;
-        std::memcpy(&t, &L.h_chunks[c].w, 4);
-        L.h_layer_t0[lay] = (double)t;
-    }
+    for (int c = L.n_chunks - 1; c >= 0; --c) L.h_layer_chunk[L.h_chunks[c].x] = c;
     L.h_layer_chunk[0] = 0;
     for (int l = h.max_layer; l >= 1; --l)
         if (L.h_layer_chunk[l] > L.h_layer_chunk[l + 1]) L.h_layer_chunk[l] = L.h_layer_chunk[l + 1];
@@ -373,6 +421,20 @@ void ensure_layout(Heart& h)
         hip_check(hipMemcpy(L.chunks.p, L.h_chunks.data(), L.h_chunks.size() * sizeof(int4),
                             hipMemcpyHostToDevice),
                   "upload chunks");
+    L.cchunks.reserve(std::max<size_t>(1, L.h_cchunks.size()) * sizeof(int4));
+    L.classes.reserve(std::max<size_t>(1, L.h_classes.size()) * sizeof(int4));
+    L.class_a.reserve(std::max<size_t>(1, L.h_class_a.size()) * sizeof(double));
+    if (!L.h_cchunks.empty()) {
+        hip_check(hipMemcpy(L.cchunks.p, L.h_cchunks.data(), L.h_cchunks.size() * sizeof(int4),
+                            hipMemcpyHostToDevice),
+                  "upload class chunks");
+        hip_check(hipMemcpy(L.classes.p, L.h_classes.data(), L.h_classes.size() * sizeof(int4),
+                            hipMemcpyHostToDevice),
+                  "upload classes");
+        hip_check(hipMemcpy(L.class_a.p, L.h_class_a.data(), L.h_class_a.size() * sizeof(double),
+                            hipMemcpyHostToDevice),
+                  "upload class times");
+    }
     L.layer_chunk.reserve(L.h_layer_chunk.size() * sizeof(int));
     L.layer_t0.reserve(L.h_layer_t0.size() * sizeof(double));
     hip_check(hipMemcpy(L.layer_chunk.p, L.h_layer_chunk.data(), L.h_layer_chunk.size() * sizeof(int),
@@ -395,6 +457,9 @@ void ensure_layout(Heart& h)
     d.chunks = L.chunks.as<int4>();
     d.layer_chunk = L.layer_chunk.as<int>();
     d.layer_t0 = L.layer_t0.as<double>();
+    d.cchunks = L.cchunks.as<int4>();
+    d.classes = L.classes.as<int4>();
+    d.class_a = L.class_a.as<double>();
     d.n0 = h.n0;
     d.n1 = h.n1;
     d.n2 = h.n2;
@@ -402,7 +467,8 @@ void ensure_layout(Heart& h)
     d.n_chunks = L.n_chunks;
     d.n_bricks = L.n_bricks;
     d.max_layer = h.max_layer;
-    d.pad = 0;
+    d.n_cchunks = L.n_cchunks;
+    d.n_classes = L.n_classes;
     L.valid = true;
 }
 
@@ -410,14 +476,16 @@ namespace {
 
 // Per-request sizes of a group's packed arrays.
 struct ReqSizes {
-    int64_t rows, chunks, bricks;
+    int64_t rows, chunks, mixed, bricks, classes;
 };
 
 ReqSizes req_sizes(const ReqSpec& r, bool mfma)
 {
     const EcgLayout& L = r.h->layout;
-    // the chunk kernel (matrix-core path) keeps G rows on chip: no records
-    return ReqSizes{mfma ? 0 : L.V + ekgk::MFMA_PAD, L.n_chunks, L.n_bricks};
+    // the chunk kernel (matrix-core path) keeps G rows on chip: no records;
+    // per-chunk sums for every chunk, partial rows for chunks outside classes
+    return ReqSizes{mfma ? 0 : L.V + ekgk::MFMA_PAD, (int64_t)L.n_chunks + L.n_cchunks, L.n_chunks, L.n_bricks,
+                    L.n_classes};
 }
 
 template <class T>
@@ -436,17 +504,22 @@ size_t group_device_bytes(const GroupPlan& g)
     for (const ReqSpec& r : g.reqs) {
         const EcgLayout& L = r.h->layout;
         const int64_t V = L.valid ? L.V : r.h->n_myo;
-        const int64_t nch = L.valid ? L.n_chunks : std::max<int64_t>(1, V / 256 + r.h->max_layer + 4);
+        // a layout for the other path is sized as one without classes
+        const bool fits = L.valid && (L.class_min == 0 || g.mfma);
+        const int64_t est = std::max<int64_t>(1, V / 64 + r.h->max_layer + 4);
+        const int64_t nmix = fits ? L.n_chunks : est;
+        const int64_t nall = fits ? (int64_t)L.n_chunks + L.n_cchunks : est;
+        const int64_t ncls = fits ? L.n_classes : 0;
         const int64_t rows = g.mfma ? 0 : V + ekgk::MFMA_PAD;
         s += (size_t)r.nl * 9 * 8 + (size_t)r.L * 3 * 8 + (size_t)r.Tp * 16;
         s += (size_t)r.nl * (sizeof(ekgk::LayerConst) + r.Tp * 16);
         s += (size_t)rows * (16 * 8 + 16);
-        s += (size_t)nch * (3 * LP * 8 + 8);
+        s += (size_t)nall * (3 * LP * 8 + 8) + (size_t)ncls * 3 * LP * 8;
         s += (size_t)r.nl * ((r.Tp >> (g.mfma ? ekgk::BT_SHIFT_MFMA : ekgk::BT_SHIFT_VALU)) + 1) *
              (8 + 2 * LP * 8);
-        s += (size_t)nch * LP * r.Tp * 8;
+        s += (size_t)nmix * LP * r.Tp * 8;
         s += (size_t)r.L * r.T * 8;
-        s += sizeof(ekgk::ReqDev) + sizeof(ekgk::HeartDev) + 5 * 8;
+        s += sizeof(ekgk::ReqDev) + sizeof(ekgk::HeartDev) + 7 * 8;
     }
     return s;
 }
@@ -467,7 +540,8 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     ws.h_tgrid.clear();
     ws.h_thl.clear();
     std::vector<std::pair<std::pair<double, double>, std::pair<int64_t, int64_t>>> grids;  // (start, dt) -> (Tp, off)
-    std::vector<int64_t> st_lf(B + 1, 0), st_ch(B + 1, 0), st_xy(B + 1, 0), st_lay(B + 1, 0), st_out(B + 1, 0);
+    std::vector<int64_t> st_lf(B + 1, 0), st_ch(B + 1, 0), st_xy(B + 1, 0), st_lay(B + 1, 0), st_out(B + 1, 0),
+        st_cs(B + 1, 0), st_cl(B + 1, 0);
     int64_t k_off = 0, pos_off = 0, rows = 0, c_off = 0, p_off = 0, pre_off = 0;
     if (!dev_in) {
         ws.h_k.clear();
@@ -476,7 +550,7 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     for (int r = 0; r < B; ++r) {
         ReqSpec& q = g.reqs[r];
         Heart& h = *q.h;
-        ensure_layout(h);
+        ensure_layout(h, g.mfma);
         int hi = -1;
         for (size_t i = 0; i < hs.size(); ++i)
             if (hs[i] == &h) hi = (int)i;
@@ -520,6 +594,7 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
         R.c_off = c_off;
         R.p_off = p_off;
         R.pre_off = pre_off;
+        R.cls_off = st_cs[r];
         if (!dev_in) {
             ws.h_k.insert(ws.h_k.end(), q.k, q.k + 9 * q.nl);
             ws.h_pos.insert(ws.h_pos.end(), q.pos, q.pos + 3 * q.L);
@@ -528,15 +603,17 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
         }
         rows += z.rows;
         c_off += z.chunks;
-        p_off += z.chunks * LP * q.Tp;
+        p_off += z.mixed * LP * q.Tp;
         pre_off += (int64_t)q.nl * ((q.Tp >> R.bt_shift) + 1);
         st_lf[r + 1] = st_lf[r] + z.bricks;
         st_ch[r + 1] = st_ch[r] + z.chunks;
         st_xy[r + 1] = st_xy[r] + (int64_t)q.nl * q.Tp;
         st_lay[r + 1] = st_lay[r] + q.nl;
         st_out[r + 1] = st_out[r] + (int64_t)q.L * q.T;
+        st_cs[r + 1] = st_cs[r] + z.classes;
+        st_cl[r + 1] = st_cl[r] + (z.classes > 0 ? q.n_tiles : 0);
     }
-    if (st_lf[B] > 0x7fffffffLL || st_ch[B] > 0x7fffffffLL || st_lay[B] > 0x7fffffffLL)
+    if (st_lf[B] > 0x7fffffffLL || st_ch[B] > 0x7fffffffLL || st_lay[B] > 0x7fffffffLL || st_cl[B] > 0x7fffffffLL)
         fail(EKG_EINVAL, "launch group too large");
     // device buffers (grow-only)
     ws.lc.reserve((size_t)st_lay[B] * sizeof(ekgk::LayerConst));
@@ -548,6 +625,7 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     ws.pre_rng.reserve((size_t)std::max<int64_t>(1, pre_off) * sizeof(int2));
     ws.pre_ab.reserve((size_t)std::max<int64_t>(1, pre_off) * 2 * LP * 8);
     ws.partial.reserve((size_t)std::max<int64_t>(1, p_off) * 8);
+    ws.csums.reserve((size_t)std::max<int64_t>(1, st_cs[B]) * 3 * LP * 8);
     if (!d_ecg) {
         ws.ecg.reserve((size_t)std::max<int64_t>(1, st_out[B]) * 8);
         d_ecg = ws.ecg.as<double>();
@@ -559,7 +637,7 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     const size_t o_req = tables.size();
     append(tables, rd.data(), rd.size());
     const size_t o_st = tables.size();
-    for (auto* st : {&st_lf, &st_ch, &st_xy, &st_lay, &st_out}) append(tables, st->data(), st->size());
+    for (auto* st : {&st_lf, &st_ch, &st_xy, &st_lay, &st_out, &st_cs, &st_cl}) append(tables, st->data(), st->size());
     void* before = ws.hearts.p;
     ws.hearts.reserve(tables.size());
     if (ws.hearts.p != before) ws.h_tables.clear();
@@ -608,6 +686,10 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     a.xy_start = stp + 2 * (B + 1);
     a.lay_start = stp + 3 * (B + 1);
     a.out_start = stp + 4 * (B + 1);
+    a.cs_start = stp + 5 * (B + 1);
+    a.cl_start = stp + 6 * (B + 1);
+    a.n_cs = st_cs[B];
+    a.n_cl = st_cl[B];
     a.n_lf = st_lf[B];
     a.n_ch = st_ch[B];
     // every request with the same chunk count: the chunk kernel runs chunk-major
@@ -630,6 +712,7 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     a.pre_rng = ws.pre_rng.as<int2>();
     a.pre_ab = ws.pre_ab.as<double>();
     a.partial = ws.partial.as<double>();
+    a.csums = ws.csums.as<double>();
     a.ecg = d_ecg;
     auto launch_all = [&](bool timed) {
         // The timed kernel is the dominant one of each path: the chunk kernel
@@ -651,6 +734,7 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
             tic();
             hip_check(ekgk::launch_ecg_chunk(a, stream), "ecg_chunk launch");
             toc();
+            hip_check(ekgk::launch_class_sums(a, stream), "class_sums launch");
             hip_check(ekgk::launch_layer_prefix(a, stream), "layer_prefix launch");
         } else {
             hip_check(ekgk::launch_lead_field(a, stream), "lead_field launch");
@@ -661,12 +745,13 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
             toc();
         }
         hip_check(ekgk::launch_ecg_reduce(a, stream), "ecg_reduce launch");
+        hip_check(ekgk::launch_ecg_class(a, stream), "ecg_class launch");
     };
     if (!ctx.graphs || time_fused) {
         launch_all(time_fused);
         return;
     }
-    // Graph mode: the six launches of a group are captured once and replayed
+    // Graph mode: the launches of a group are captured once and replayed
     // while the kernel arguments (buffers, counts, shapes) stay the same; the
     // per-request values live in device tables and buffers, so new inputs
     // in the same places need no recapture.  The graph holds kernels only.
@@ -751,7 +836,7 @@ ekg_tune autotune(Heart& h, const ekg_sim_desc& sample, int batch)
     ctx.bind();
     const int saved_kernel = ctx.kernel;
     const int def_chunk = DEFAULT_CHUNK;
-    ekg_tune best{def_chunk, DEFAULT_WAVES, 2, EKG_KERNEL_MFMA, 0.0};
+    ekg_tune best{def_chunk, DEFAULT_WAVES, 2, EKG_KERNEL_MFMA, h.tune_class, 0.0};
     Workspace ws;
     hipStream_t st = ctx.stream;
     hipEvent_t e0 = ctx.get_event(), e1 = ctx.get_event();
@@ -761,6 +846,7 @@ ekg_tune autotune(Heart& h, const ekg_sim_desc& sample, int batch)
         h.tune_waves = c.mfma_waves;
         h.tune_phase = c.lead_phase;
         h.tune_kernel = c.kernel;
+        h.tune_class = c.class_min;
         ctx.kernel = EKG_KERNEL_AUTO;
         GroupPlan g = plan_group(ctx, std::vector<ReqSpec>(batch, one));
         double best_ms = 1e30;
@@ -826,7 +912,8 @@ ekg_tune autotune(Heart& h, const ekg_sim_desc& sample, int batch)
     h.tune_waves = best.mfma_waves;
     h.tune_phase = best.lead_phase;
     h.tune_kernel = best.kernel;
-    ensure_layout(h);
+    h.tune_class = best.class_min;
+    ensure_layout(h, best.kernel != EKG_KERNEL_VALU);
     hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
     return best;
 }

@@ -39,7 +39,7 @@ struct DevBuf {
 // Device workspace of one launch group on one stream (grow-only buffers).
 struct Workspace {
     DevBuf hearts, reqs, starts, layer_k, lead_pos, tgrid, thl, lc, xy, G, aux, sums, bnd, pre_rng,
-        pre_ab, partial, ecg;
+        pre_ab, partial, csums, ecg;
     std::vector<double> h_k, h_pos, h_tgrid;
     std::vector<float> h_thl;
     std::vector<char> h_tables;    // hearts + reqs + starts as last uploaded
@@ -69,7 +69,11 @@ struct Ctx {
 };
 
 // Per-heart layout for the ECG kernels: myocardial voxels sorted by
-// (layer, activation time, grid index), cut into layer-pure chunks.
+// (layer, activation time, grid index), cut into layer-pure chunks.  With
+// activation classes on, each layer lists first its voxels outside classes
+// (in that order, cut into chunks) and then its classes, each a run of
+// voxels with one activation time of at least class_min voxels, cut into
+// class chunks.
 struct EcgLayout {
     bool valid = false;
     int64_t V = 0;
@@ -77,11 +81,14 @@ struct EcgLayout {
     int n_bricks = 0;
     int max_layer = 0;
     int chunk_target = 0;          // voxels per chunk the layout was cut with
-    std::vector<int4> h_chunks;
+    int class_min = -1;            // smallest class it was built with (0: no classes)
+    int n_cchunks = 0, n_classes = 0;
+    std::vector<int4> h_chunks, h_cchunks, h_classes;
+    std::vector<double> h_class_a;
     std::vector<int> h_layer_chunk;
     std::vector<double> h_layer_t0;
     DevBuf vox_chunk, vox_a64, vox_a, vox_geo, chunks, bricks, brick_off, brick_vox, halo_off, halo_cells,
-        layer_chunk, layer_t0;
+        layer_chunk, layer_t0, cchunks, classes, class_a;
     ekgk::HeartDev dev{};          // device pointers and sizes for the kernels
 };
 struct Heart {
@@ -99,6 +106,7 @@ struct Heart {
     int tune_waves = 0;   // waves per block of the matrix-core fused kernel
     int tune_phase = 0;   // leads per lead-field phase (2 or 4)
     int tune_kernel = 0;  // EKG_KERNEL_* override
+    int tune_class = 0;   // smallest activation class (voxels) of the matrix-core path; 0 default, -1 none
     std::mutex mu;
 };
 
@@ -126,7 +134,11 @@ int64_t num_steps(double length, double dt);
 ReqSpec make_spec(Heart& h, const ekg_sim_desc& d, bool host_values);
 // Builds a group from specs (all with the same neighbourhood).
 GroupPlan plan_group(Ctx& ctx, std::vector<ReqSpec> reqs);
-void ensure_layout(Heart& h);
+// Builds (or keeps) the heart's layout for the matrix-core path (mfma, with
+// activation classes as tuned) or the vector-ALU path (no classes).
+void ensure_layout(Heart& h, bool mfma);
+// Whether the heart's simulations take the matrix-core path (as plan_group decides).
+bool heart_mfma(const Ctx& ctx, const Heart& h);
 size_t group_device_bytes(const GroupPlan& g);
 // Enqueue a whole group on `stream`.  With d_k/d_pos null the host values of
 // the specs are packed and uploaded; otherwise request r reads

@@ -63,8 +63,16 @@ struct HeartDev {
     const int4* chunks;          // [n_chunks] {layer, v_begin, v_end, t_ref as float bits}
     const int* layer_chunk;      // [max_layer + 2]: chunks of layer l are [layer_chunk[l], layer_chunk[l+1])
     const double* layer_t0;      // [max_layer + 1]: reference time of layer l
+    // Activation classes: runs of one layer and one activation time, whose
+    // voxels all have the same AP.  Their voxels are cut into class chunks
+    // (chunk index n_chunks + k of the per-chunk arrays); the chunks above
+    // hold the other voxels.
+    const int4* cchunks;         // [n_cchunks] {layer, v_begin, v_end, t_ref as float bits}
+    const int4* classes;         // [n_classes] {layer, first class chunk, end class chunk, 0}
+    const double* class_a;       // [n_classes] activation time of the class
     int n0, n1, n2;
-    int V, n_chunks, n_bricks, max_layer, pad;
+    int V, n_chunks, n_bricks, max_layer;
+    int n_cchunks, n_classes;
 };
 
 // One request of a launch group.  Offsets index the group's packed arrays.
@@ -81,6 +89,7 @@ struct ReqDev {
     int64_t c_off;    // per-chunk arrays: sums[(c_off + c)*3*LP ...], bnd[c_off + c]
     int64_t p_off;    // partial[p_off + (c*LP + i)*Tp + j]
     int64_t pre_off;  // prefix tables: [(pre_off + (l-1)*(n_bt+1) + bound tile)], n_bt = Tp >> bt_shift
+    int64_t cls_off;  // class sums: csums[(cls_off + class)*3*LP ...]
 };
 
 // A launch group: requests of any hearts, lead counts and time grids that
@@ -97,7 +106,9 @@ struct GroupArgs {
     const int64_t* xy_start;   // nl * Tp per request
     const int64_t* lay_start;  // nl per request
     const int64_t* out_start;  // L * T per request
-    int64_t n_lf, n_ch, n_xy, n_lay, n_out;  // totals
+    const int64_t* cs_start;   // activation classes per request
+    const int64_t* cl_start;   // 64-step tiles of the class kernel per request (0 without classes)
+    int64_t n_lf, n_ch, n_xy, n_lay, n_out, n_cs, n_cl;  // totals
     int ch_uniform;            // chunks per request when all requests have the same count, else 0
     const double* layer_k;
     const double* lead_pos;
@@ -112,6 +123,7 @@ struct GroupArgs {
     int2* pre_rng;             // per (request, layer, bound tile): transition chunks [lo, hi)
     double* pre_ab;            // per (request, layer, bound tile): [2][LP]; tile n_bt: [LP] sum G
     double* partial;
+    double* csums;             // per (request, class): [3][LP] sum G, sum G P', sum G Q'
     double* ecg;
 };
 
@@ -123,5 +135,9 @@ hipError_t launch_ecg_fused(const GroupArgs& a, hipStream_t s);
 // Matrix-core path: lead field, chunk sums and transition tiles in one kernel.
 hipError_t launch_ecg_chunk(const GroupArgs& a, hipStream_t s);
 hipError_t launch_ecg_reduce(const GroupArgs& a, hipStream_t s);
+// Activation classes: sums of the class chunks per class, then the classes'
+// terms added to the reduced ECG.
+hipError_t launch_class_sums(const GroupArgs& a, hipStream_t s);
+hipError_t launch_ecg_class(const GroupArgs& a, hipStream_t s);
 
 }  // namespace ekgk

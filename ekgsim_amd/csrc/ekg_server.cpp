@@ -162,7 +162,7 @@ void Server::run()
                     for (Request* r : group) {
                         Heart* h = hearts[r->heart_id];
                         std::lock_guard<std::mutex> hg(h->mu);
-                        ensure_layout(*h);
+                        ensure_layout(*h, heart_mfma(ctx, *h));
                         specs.push_back(make_spec(*h, r->desc, true));
                     }
                     plan = plan_group(ctx, specs);
@@ -312,7 +312,7 @@ int server_add_heart(Server& s, Heart& h)
     std::lock_guard<std::mutex> g(s.mu);
     {
         std::lock_guard<std::mutex> hg(h.mu);
-        ensure_layout(h);
+        ensure_layout(h, heart_mfma(s.ctx, h));
     }
     s.hearts.push_back(&h);
     return (int)s.hearts.size() - 1;

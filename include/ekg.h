@@ -172,6 +172,10 @@ typedef struct ekg_tune {
     int mfma_waves;       /* waves per block of the matrix-core chunk kernel, 2..16 */
     int lead_phase;       /* leads per lead-field phase, 2 or 4 */
     int kernel;           /* EKG_KERNEL_MFMA or EKG_KERNEL_VALU */
+    int class_min;        /* activation classes on the matrix-core path: a run of at least
+                             this many voxels of one layer with one activation time is
+                             summed as one class (all its voxels share the AP); 0 = the
+                             default (64), -1 = no classes (per-voxel APs throughout) */
     double ms_per_batch;  /* measured device time of the chosen shape (autotune only) */
 } ekg_tune;
 
@@ -187,7 +191,7 @@ int ekg_heart_autotune(ekg_heart *h, const ekg_sim_desc *sample, int batch, ekg_
 int ekg_heart_set_tune(ekg_heart *h, const ekg_tune *t);
 int ekg_heart_get_tune(const ekg_heart *h, ekg_tune *t);
 
-/* Graph mode: the six kernel launches of a simulation group are captured
+/* Graph mode: the kernel launches of a simulation group are captured
  * into a hipGraph on first use and replayed while the group keeps its
  * shape and buffers (a repeated batch with new coefficients or lead
  * positions in the same device arrays replays without recapture).  Off by

@@ -1,0 +1,132 @@
+"""Activation classes on the matrix-core path.
+
+Voxels of one layer with one activation time have the same AP, so their
+dipole terms factor through the sums of their lead weights: a run of at
+least `class_min` such voxels is summed as a class.  The ECG must stay within
+rel-L2 1e-5 of the FP64 oracle whatever the threshold:
+
+* integer-delay hearts (every activation time an integer, so runs of
+  thousands of voxels), with classes at the default threshold, with every
+  voxel in a class (class_min 1: layers with no chunk outside the classes),
+  with a low threshold that leaves small runs outside, and with classes off;
+  on both neighbourhoods, 16 leads and six layers;
+* a heart from the FP64 transfer-matrix excitation, whose times are mostly
+  distinct, gives the same result with classes on or off;
+* a request gives the same bits alone, in a uniform batch and in a server
+  group mixing a heart with classes and one without.
+"""
+import numpy as np
+import pytest
+
+from ekgsim_amd import synth as S
+from oracle import oracle as O
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def _int_heart(ctx, n, seed, n_layers=3):
+    from ekgsim_amd import Heart
+    h = S.make_heart(n, seed=seed, n_layers=n_layers)
+    g = Heart(ctx, h.layer, h.seed)
+    g.excite_delay(h.delay)
+    act = O.excite_int(h.layer, h.seed, h.delay).astype(np.float64)
+    return h, g, act
+
+
+def _req(n, L, s, n_layers=3):
+    x = S.random_params(n_layers, L, s)
+    return (S.layer_k_from_params(x, n_layers),
+            S.move_leads(S.lead_positions(n, L), S.displacements(x, n_layers, L)))
+
+
+def _class_runs(h, act):
+    lay = h.layer.ravel()
+    m = lay > 0
+    _, cnt = np.unique(np.stack([lay[m].astype(np.float64), act.ravel()[m]], 1), axis=0, return_counts=True)
+    return cnt
+
+
+@pytest.mark.parametrize("nhood", ["3D4", "3D8"])
+def test_integer_heart_every_threshold(ctx, nhood):
+    h, g, act = _int_heart(ctx, 40, 9)
+    runs = _class_runs(h, act)
+    assert runs.max() >= 64 and runs.min() < 16   # large classes and small runs both occur
+    k, pos = _req(40, 6, 31)
+    ref = O.run(h.layer, act, k, pos, nhood, 0.0, 400.0, 1.0)
+    out = {}
+    for cm in (0, 1, 16, -1):
+        g.set_tune(class_min=cm)
+        out[cm] = g.simulate(k, pos, nhood, 0.0, 400.0, 1.0)
+        err = rel_l2(out[cm], ref)
+        print(f"{nhood} class_min {cm}: rel-L2 {err:.3e}")
+        assert err <= TOL, cm
+    assert g.get_tune()["class_min"] == -1
+    g.set_tune()
+    assert g.get_tune()["class_min"] == 0
+    g.close()
+
+
+def test_six_layers_16_leads_short_step(ctx):
+    h, g, act = _int_heart(ctx, 24, 5, n_layers=6)
+    k, pos = _req(24, 16, 321, n_layers=6)
+    ref = O.run(h.layer, act, k, pos, "3D8", 0.0, 300.0, 0.5)
+    for cm in (0, 1):
+        g.set_tune(class_min=cm)
+        err = rel_l2(g.simulate(k, pos, "3D8", 0.0, 300.0, 0.5), ref)
+        print(f"six layers 3D8 16 leads dt 0.5, class_min {cm}: rel-L2 {err:.3e}")
+        assert err <= TOL
+    g.set_tune()
+    g.close()
+
+
+def test_transfer_heart_same_with_or_without_classes(ctx):
+    from ekgsim_amd import Heart
+    h = S.make_heart(36, seed=4)
+    T = S.layered_transfer(3, 12, 1.0, 1.5)
+    act = O.excite_fp64(h.layer, h.seed, T)
+    g = Heart(ctx, h.layer, h.seed)
+    g.excite_transfer(T)
+    k, pos = _req(36, 4, 77)
+    ref = O.run(h.layer, act, k, pos, "3D4", 0.0, 300.0, 1.0)
+    on = g.simulate(k, pos, "3D4", 0.0, 300.0, 1.0)
+    g.set_tune(class_min=-1)
+    off = g.simulate(k, pos, "3D4", 0.0, 300.0, 1.0)
+    g.set_tune()
+    assert rel_l2(on, ref) <= TOL and rel_l2(off, ref) <= TOL
+    assert rel_l2(on, off) <= 1e-9
+    g.close()
+
+
+def test_same_bits_alone_in_batch_and_in_mixed_groups(ctx):
+    from ekgsim_amd import Heart, Server
+    h1, g1, act1 = _int_heart(ctx, 32, 2)
+    h2 = S.make_heart(28, seed=6)
+    T = S.layered_transfer(3, 12, 1.0, 1.5)
+    g2 = Heart(ctx, h2.layer, h2.seed)
+    g2.excite_transfer(T)
+    reqs1 = [_req(32, 3, 500 + i) for i in range(5)]
+    reqs2 = [_req(28, 3, 600 + i) for i in range(5)]
+    alone1 = [g1.simulate(k, p, "3D4", 0.0, 250.0, 1.0) for k, p in reqs1]
+    alone2 = [g2.simulate(k, p, "3D4", 0.0, 250.0, 1.0) for k, p in reqs2]
+    # a uniform batch of one heart runs chunk-major with classes
+    batch = g1.simulate_batch(np.stack([k for k, _ in reqs1]), np.stack([p for _, p in reqs1]),
+                              "3D4", 0.0, 250.0, 1.0)
+    for i in range(5):
+        assert np.array_equal(batch[i], alone1[i])
+    srv = Server(ctx, max_batch=10, n_streams=1)
+    i1, i2 = srv.add_heart(g1), srv.add_heart(g2)
+    tickets = []
+    for a, b in zip(reqs1, reqs2):
+        tickets.append(srv.submit(i1, a[0], a[1], "3D4", 0.0, 250.0, 1.0))
+        tickets.append(srv.submit(i2, b[0], b[1], "3D4", 0.0, 250.0, 1.0))
+    got = [srv.wait(t)[0] for t in tickets]
+    srv.close()
+    for i in range(5):
+        assert np.array_equal(got[2 * i], alone1[i])
+        assert np.array_equal(got[2 * i + 1], alone2[i])
+    ref = O.run(h1.layer, act1, reqs1[0][0], reqs1[0][1], "3D4", 0.0, 250.0, 1.0)
+    assert rel_l2(alone1[0], ref) <= TOL
+    g1.close()
+    g2.close()
