@@ -3,12 +3,15 @@
 
 One step = one batch of 256 parameter vectors (4 free AP coefficients per
 endo/mid/epi layer + 2 electrode displacements per lead) simulated on the
-cached 128^3 ellipsoid heart: lead field, AP tables, fused AP+lead kernel and
-the FP64 reduction, with the per-request inputs already in HBM.  Beside it:
-the plain drop-in (one ekg_simulate call per simulation, host buffers in and
-out), a short continuous-batching burst (p50/p99 latency), accuracy against
-the FP64 oracle and the compiled reference, and the reference's CPU path
-timed on the host cores.
+cached 128^3 ellipsoid heart: AP tables, the chunk kernel (lead field, chunk
+sums, per-voxel APs contracted on the matrix cores), the activation-class
+terms and the FP64 reduction, with the per-request inputs already in HBM.
+Beside it: the same batch with activation classes off (every voxel's AP
+evaluated, the path of hearts whose activation times are distinct), the plain
+drop-in (one ekg_simulate call per simulation, host buffers in and out), a
+short continuous-batching burst (p50/p99 latency), accuracy against the FP64
+oracle and the compiled reference, and the reference's CPU path timed on the
+host cores.
 
 Prints one JSON line on stdout; details go to stderr.
 """
@@ -38,6 +41,21 @@ MFMA_FP64_PEAK = 78.6e12
 # closed form (sigmoid within 2^-24.5 of 1, as in the FP32 AP) or is below
 # 2^-36 of its plateau, so only window voxel-steps need an evaluated AP.
 SATC_X, SKIP_X = -24.5, 48.0
+# Activation classes of the matrix-core path: runs of at least this many
+# voxels of one layer with one activation time share the AP and are summed
+# per class (the library's default threshold).
+DEFAULT_CLASS_MIN = 64
+
+
+def class_members(lay, act, cmin):
+    """Mask of the voxels in activation classes (cmin <= 0: none) and the
+    number of classes."""
+    if cmin <= 0:
+        return np.zeros(lay.shape, bool), 0
+    key = np.stack([lay.astype(np.float64), act], 1)
+    _, inv, cnt = np.unique(key, axis=0, return_inverse=True, return_counts=True)
+    big = cnt >= cmin
+    return big[inv.ravel()], int(big.sum())
 
 
 def window_voxel_steps(act, k1, start, dt, T):
@@ -164,7 +182,10 @@ def main():
                     help="only the batched steps (for rocprofv3 runs)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--tune", default=None,
-                    help="fixed launch shape chunk,waves,phase,kernel (e.g. 2048,2,2,2) instead of autotuning")
+                    help="fixed launch shape chunk,waves,phase,kernel[,class_min] (e.g. 256,4,2,2) "
+                         "instead of autotuning")
+    ap.add_argument("--no-general", action="store_true",
+                    help="skip the timing with activation classes off")
     ap.add_argument("--no-graph", action="store_true", help="launch the timed steps eagerly")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="one device-to-device copy of MIB MiB before the steps (scale for HBM counters)")
@@ -220,10 +241,25 @@ def main():
         heart.simulate_batch_device(d_k.data_ptr(), d_p.data_ptr(), d_e.data_ptr(), B, 3, L,
                                     "3D4", 0.0, args.length, 1.0)
 
+    def timed_steps(graphs):
+        ctx.set_graphs(graphs)
+        step()
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        gs = ctx.graph_stats()
+        ctx.set_graphs(False)
+        return el, gs
+
     tuning = None
     if args.tune:
-        c, w, ph, kern = (int(v) for v in args.tune.split(","))
-        heart.set_tune(c, w, ph, kern)
+        tv = [int(v) for v in args.tune.split(",")]
+        heart.set_tune(*tv)
         tuning = heart.get_tune()
     elif not args.no_autotune:
         t0 = time.perf_counter()
@@ -248,24 +284,17 @@ def main():
     nl, kms = ctx.kernel_time("ecg_fused")
     ctx.set_profiling(False)
     # timed steps (replayed from a captured hipGraph unless --no-graph)
-    ctx.set_graphs(not args.no_graph)
-    step()
-    ctx.synchronize()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    ctx.synchronize()
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    gstats = ctx.graph_stats()
-    ctx.set_graphs(False)
+    el, gstats = timed_steps(not args.no_graph)
     ms_step = el * 1e3 / args.steps
     value = B * args.steps / el
     avg_kernel_s = kms / 1e3 / max(nl, 1)
     act_grid = heart.activation()
     act_myo = act_grid[h.layer > 0]
-    win = sum(window_voxel_steps(act_myo, float(k[0][1]), 0.0, 1.0, T) for k in ks)
+    # voxels in activation classes have no per-voxel AP in the chunk kernel
+    tn = heart.get_tune()
+    cmin = 0 if args.kernel == "valu" or tn["class_min"] < 0 else (tn["class_min"] or DEFAULT_CLASS_MIN)
+    in_cls, n_classes = class_members(h.layer[h.layer > 0], act_myo, cmin)
+    win = sum(window_voxel_steps(act_myo[~in_cls], float(k[0][1]), 0.0, 1.0, T) for k in ks)
     cells = B * L * lead_field_cells(h.layer) if args.kernel != "valu" else 0
     f64 = 2.0 * L * win + FP64_PER_CELL * cells
     f32 = FP32_PER_WINDOW_STEP * win + FP32_PER_CELL * cells
@@ -294,6 +323,27 @@ def main():
             den += float(np.sum(we ** 2))
         ref_win_err = math.sqrt(num / den)
         log(f"rel-L2 vs FP64 oracle {errs}; vs {cpu['kind']} on windows {ref_win_err:.2e}")
+
+    # ---- the same batch with activation classes off: every voxel's AP ----
+    general = None
+    if not args.no_general and args.kernel != "valu" and n_classes > 0:
+        saved = heart.get_tune()
+        heart.set_tune(class_min=-1)
+        gtune = None
+        if args.tune:
+            heart.set_tune(saved["chunk_voxels"], saved["mfma_waves"], saved["lead_phase"], saved["kernel"], -1)
+        elif not args.no_autotune:
+            gtune = heart.autotune(ks[0], pos[0], "3D4", 0.0, args.length, 1.0, batch=B)
+        for _ in range(args.warmup):
+            step()
+        gel, _ = timed_steps(not args.no_graph)
+        gecg = d_e.cpu().numpy()
+        gerr = {r: float(np.linalg.norm(gecg[r] - ref) / np.linalg.norm(ref)) for r, ref in ref_full.items()}
+        general = {"ecgs_per_s": round(B * args.steps / gel, 2), "ms_per_step": round(gel * 1e3 / args.steps, 3),
+                   "autotuned": gtune, "rel_l2_max_vs_fp64": max(gerr.values()) if gerr else None}
+        log(f"activation classes off: {general}")
+        heart.set_tune(saved["chunk_voxels"], saved["mfma_waves"], saved["lead_phase"], saved["kernel"],
+                       saved["class_min"])
 
     # ---- plain drop-in: one ekg_simulate per simulation, host in/out ----
     lat = []
@@ -355,7 +405,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 AP / f64 lead sums",
+        "dtype": "f32 lead weights and per-voxel APs, f64 class APs and sums",
         "data": "synthetic (seeded ellipsoid heart, random parameter vectors)",
         "config": {
             "workload": (f"batched: {B} parameter vectors per step on one cached {n}^3 ellipsoid "
@@ -363,7 +413,10 @@ def main():
             "grid": n, "myocardial_voxels": V, "leads": L, "steps_per_ecg": T, "batch": B,
             "neighbourhood": "3D4", "excitation": "int32 per-voxel delay 1-3, 5 endocardial seeds",
             "launch": {"autotuned": tuning, "graph_replay": not args.no_graph, "graph": gstats},
+            "activation_classes": {"class_min": cmin, "classes": n_classes,
+                                   "voxel_fraction": round(float(in_cls.mean()), 5)},
         },
+        "classes_off": general,
         "plain_ecgs_per_s": round(plain, 2),
         "plain_p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
         "plain_p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),

@@ -673,6 +673,104 @@ __device__ constexpr int cube_bit(int d0, int d1, int d2) { return (d0 + 1) * 9 
 constexpr int CHUNK_SEG = 8;  // fixed summation segments of the chunk sums
 constexpr int CH_LEADS = 2;   // leads per pass of the chunk kernel's lead field (divides LP)
 
+// G row of one myocardial voxel u from its own cell and its myocardial
+// neighbours in the dipole neighbourhood N, CH_LEADS leads per pass:
+//   G_i(u) = s(u) (e(u) . s_u) + sum_{d in N, u+d myo} s(u+d) (e(u+d) . d),
+// s_u the sum of those d.  A cell's weight m = e s is a function of the cell
+// alone: e = (p_hi - c) + p_lo and s = |e|^-3 (rsq cubed) in FP32, then exact
+// FP64 products, so its rounding cancels between the two ends of each dipole
+// and only V differences across the dipole see it (a weight derived per
+// evaluating voxel, e.g. from e(u) - d, does not cancel: ~1e-5).
+template <int NH>
+struct VoxelField {
+    float fc[3];          // grid coordinate + 1 per axis (the reference's voxel centre)
+    double sd0, sd1, sd2;  // s_u
+    uint32_t mk;           // 3x3x3 myocardium mask
+
+    __device__ __forceinline__ explicit VoxelField(uint2 geo)
+    {
+        using N = Nhood<NH>;
+        const int gz = (int)(geo.x & 1023u), gy = (int)((geo.x >> 10) & 1023u), gx = (int)(geo.x >> 20);
+        mk = geo.y;
+        int i0 = 0, i1 = 0, i2 = 0;
+#pragma unroll
+        for (int qn = 0; qn < N::n; ++qn) {
+            const int f = (int)((mk >> cube_bit(N::d[qn][0], N::d[qn][1], N::d[qn][2])) & 1u);
+            i0 += f * N::d[qn][0];
+            i1 += f * N::d[qn][1];
+            i2 += f * N::d[qn][2];
+        }
+        sd0 = (double)i0;
+        sd1 = (double)i1;
+        sd2 = (double)i2;
+        fc[0] = (float)(gz + 1);
+        fc[1] = (float)(gy + 1);
+        fc[2] = (float)(gx + 1);
+    }
+
+    // g[w] = G_{i0+w}(u); leads at or past L give 0.  s_p[i][0|1][c]: the
+    // electrode's FP32 hi/lo per component.
+    __device__ __forceinline__ void leads(int i0, int L, const float (*s_p)[2][3], double g[CH_LEADS]) const
+    {
+        using N = Nhood<NH>;
+#pragma unroll
+        for (int w2 = 0; w2 < CH_LEADS; ++w2) g[w2] = 0.0;
+        if (i0 >= L) return;
+        // Per lead, the electrode offset of a cell is (p_hi - c) + p_lo per
+        // axis, and the voxel and its neighbours span only three values of c
+        // per axis: E[axis][d + 1] in FP32 and FP64, and the first term of
+        // |e|^2 per x-offset.  Each cell's weight is then formed from exactly
+        // the operands (and in the order) its own evaluation uses.
+        float E[CH_LEADS][3][3], SQ0[CH_LEADS][3];
+        double E64[CH_LEADS][3][3];
+#pragma unroll
+        for (int w2 = 0; w2 < CH_LEADS; ++w2) {
+            // leads past the request's count repeat its first lead of the pass
+            const int ii = i0 + w2 < L ? i0 + w2 : i0;
+#pragma unroll
+            for (int c3 = 0; c3 < 3; ++c3) {
+                const float ph = s_p[ii][0][c3], pl = s_p[ii][1][c3];
+#pragma unroll
+                for (int k3 = 0; k3 < 3; ++k3) {
+                    E[w2][c3][k3] = (ph - (fc[c3] + (float)(k3 - 1))) + pl;
+                    E64[w2][c3][k3] = (double)E[w2][c3][k3];
+                }
+            }
+#pragma unroll
+            for (int k3 = 0; k3 < 3; ++k3) SQ0[w2][k3] = E[w2][0][k3] * E[w2][0][k3];
+        }
+#pragma unroll
+        for (int w2 = 0; w2 < CH_LEADS; ++w2) {
+            // own cell: m(u) . s_u
+            float l = __builtin_fmaf(E[w2][1][1], E[w2][1][1], SQ0[w2][1]);
+            l = __builtin_fmaf(E[w2][2][1], E[w2][2][1], l);
+            const float r = __builtin_amdgcn_rsqf(l);
+            g[w2] = (double)(r * r * r) * (E64[w2][0][1] * sd0 + E64[w2][1][1] * sd1 + E64[w2][2][1] * sd2);
+        }
+#pragma unroll
+        for (int qn = 0; qn < N::n; ++qn) {
+            if ((mk >> cube_bit(N::d[qn][0], N::d[qn][1], N::d[qn][2])) & 1u) {
+                const int k0 = N::d[qn][0] + 1, k1 = N::d[qn][1] + 1, k2 = N::d[qn][2] + 1;
+#pragma unroll
+                for (int w2 = 0; w2 < CH_LEADS; ++w2) {
+                    float l = __builtin_fmaf(E[w2][1][k1], E[w2][1][k1], SQ0[w2][k0]);
+                    l = __builtin_fmaf(E[w2][2][k2], E[w2][2][k2], l);
+                    const float r = __builtin_amdgcn_rsqf(l);
+                    // d . e exactly in FP64 (d has entries -1, 0, 1)
+                    double de = 0.0;
+                    if (N::d[qn][0]) de += E64[w2][0][k0] * N::d[qn][0];
+                    if (N::d[qn][1]) de += E64[w2][1][k1] * N::d[qn][1];
+                    if (N::d[qn][2]) de += E64[w2][2][k2] * N::d[qn][2];
+                    g[w2] = __builtin_fma((double)(r * r * r), de, g[w2]);
+                }
+            }
+        }
+#pragma unroll
+        for (int w2 = 0; w2 < CH_LEADS; ++w2)
+            if (i0 + w2 >= L) g[w2] = 0.0;
+    }
+};
+
 // Block = `nw` waves of one (request, chunk) of at most NMAX voxels; the
 // chunk's G rows never leave the CU.
 //  1. Each voxel u of the chunk (one per thread, strided) gets its G row from its
@@ -731,9 +829,7 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
     }
     const ReqDev R = reqs[r];
     const HeartDev& H = hearts[R.heart];
-    // chunk indices from n_chunks on are class chunks: G rows and sums only
-    const bool cls = c >= H.n_chunks;
-    const int4 ch = cls ? H.cchunks[c - H.n_chunks] : H.chunks[c];
+    const int4 ch = H.chunks[c];
     const int l = ch.x, v0 = ch.y, n = ch.z - ch.y;
     const double* kk = layer_k + R.k_off + 9 * (l - 1);
     const LayerConst kc = lc[R.lay_off + l - 1];
@@ -768,85 +864,13 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
     // 1. G rows and AP factors, one voxel per thread (strided)
     for (int q = t; q < n; q += blockDim.x) {
         const int v = v0 + q;
-        const uint2 geo = H.vox_geo[v];
-        const int gz = (int)(geo.x & 1023u), gy = (int)((geo.x >> 10) & 1023u), gx = (int)(geo.x >> 20);
-        const uint32_t mk = geo.y;
-        int i0 = 0, i1 = 0, i2 = 0;
-#pragma unroll
-        for (int qn = 0; qn < N::n; ++qn) {
-            const int f = (int)((mk >> cube_bit(N::d[qn][0], N::d[qn][1], N::d[qn][2])) & 1u);
-            i0 += f * N::d[qn][0];
-            i1 += f * N::d[qn][1];
-            i2 += f * N::d[qn][2];
-        }
-        const double sd0 = (double)i0, sd1 = (double)i1, sd2 = (double)i2;
-        // grid coordinate + 1 of the voxel (the reference's voxel centre)
-        const float fz = (float)(gz + 1), fy = (float)(gy + 1), fx = (float)(gx + 1);
-        // CH_LEADS leads per pass (independent FP64 accumulation chains).  A
-        // cell's weight m = e s is a function of the cell alone: e = (p_hi - c)
-        // + p_lo and s = lead_s32(e) in FP32, then exact FP64 products, so its
-        // rounding cancels between the two ends of each dipole and only V
-        // differences across the dipole see it (a weight derived per
-        // evaluating voxel, e.g. from e(u) - d, does not cancel: ~1e-5).
+        const VoxelField<NH> vf(H.vox_geo[v]);
 #pragma unroll 1
         for (int i0 = 0; i0 < LP; i0 += CH_LEADS) {
             double g[CH_LEADS];
+            vf.leads(i0, R.L, s_p, g);
 #pragma unroll
-            for (int w2 = 0; w2 < CH_LEADS; ++w2) g[w2] = 0.0;
-            if (i0 < R.L) {
-                // Per lead, the electrode offset of a cell is (p_hi - c) + p_lo per
-                // axis, and the voxel and its neighbours span only three values of
-                // c per axis: E[axis][d + 1] in FP32 and FP64, and the first term
-                // of |e|^2 per x-offset.  Each cell's weight is then formed from
-                // exactly the operands (and in the order) its own evaluation uses.
-                float E[CH_LEADS][3][3], SQ0[CH_LEADS][3];
-                double E64[CH_LEADS][3][3];
-                const float fc[3] = {fz, fy, fx};
-#pragma unroll
-                for (int w2 = 0; w2 < CH_LEADS; ++w2) {
-                    // leads past the request's count repeat its first lead of the pass
-                    const int ii = i0 + w2 < R.L ? i0 + w2 : i0;
-#pragma unroll
-                    for (int c3 = 0; c3 < 3; ++c3) {
-                        const float ph = s_p[ii][0][c3], pl = s_p[ii][1][c3];
-#pragma unroll
-                        for (int k3 = 0; k3 < 3; ++k3) {
-                            E[w2][c3][k3] = (ph - (fc[c3] + (float)(k3 - 1))) + pl;
-                            E64[w2][c3][k3] = (double)E[w2][c3][k3];
-                        }
-                    }
-#pragma unroll
-                    for (int k3 = 0; k3 < 3; ++k3) SQ0[w2][k3] = E[w2][0][k3] * E[w2][0][k3];
-                }
-#pragma unroll
-                for (int w2 = 0; w2 < CH_LEADS; ++w2) {
-                    // own cell: m(u) . s_u
-                    float l = __builtin_fmaf(E[w2][1][1], E[w2][1][1], SQ0[w2][1]);
-                    l = __builtin_fmaf(E[w2][2][1], E[w2][2][1], l);
-                    const float r = __builtin_amdgcn_rsqf(l);
-                    g[w2] = (double)(r * r * r) * (E64[w2][0][1] * sd0 + E64[w2][1][1] * sd1 + E64[w2][2][1] * sd2);
-                }
-#pragma unroll
-                for (int qn = 0; qn < N::n; ++qn) {
-                    if ((mk >> cube_bit(N::d[qn][0], N::d[qn][1], N::d[qn][2])) & 1u) {
-                        const int k0 = N::d[qn][0] + 1, k1 = N::d[qn][1] + 1, k2 = N::d[qn][2] + 1;
-#pragma unroll
-                        for (int w2 = 0; w2 < CH_LEADS; ++w2) {
-                            float l = __builtin_fmaf(E[w2][1][k1], E[w2][1][k1], SQ0[w2][k0]);
-                            l = __builtin_fmaf(E[w2][2][k2], E[w2][2][k2], l);
-                            const float r = __builtin_amdgcn_rsqf(l);
-                            // d . e exactly in FP64 (d has entries -1, 0, 1)
-                            double de = 0.0;
-                            if (N::d[qn][0]) de += E64[w2][0][k0] * N::d[qn][0];
-                            if (N::d[qn][1]) de += E64[w2][1][k1] * N::d[qn][1];
-                            if (N::d[qn][2]) de += E64[w2][2][k2] * N::d[qn][2];
-                            g[w2] = __builtin_fma((double)(r * r * r), de, g[w2]);
-                        }
-                    }
-                }
-            }
-#pragma unroll
-            for (int w2 = 0; w2 < CH_LEADS; ++w2) s_G[q * LP + i0 + w2] = i0 + w2 < R.L ? g[w2] : 0.0;
+            for (int w2 = 0; w2 < CH_LEADS; ++w2) s_G[q * LP + i0 + w2] = g[w2];
         }
         const double a = H.vox_a64[v];
         const float2 av = H.vox_a[v];
@@ -861,7 +885,7 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
         for (int i = 0; i < LP; ++i) s_G[q * LP + i] = 0.0;
         s_aux[q] = float4{0.0f, 0.0f, 0.0f, 0.0f};
     }
-    if (w == 0 && !cls) {
+    if (w == 0) {
         // tile bounds (the same predicates as chunk_sums_kernel), needed from
         // the transition tiles on: one ballot over the prefetched probes, or
         // the wave-parallel search for requests with more than 64 tiles
@@ -933,7 +957,6 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
         for (int sg = 0; sg < CHUNK_SEG; ++sg) sum += s_part[sg][t];
         sums[(R.c_off + c) * 3 * LP + t] = sum;
     }
-    if (cls) return;  // a class's terms come from its sums (ecg_class_kernel)
     __syncthreads();
 
     // 3. transition tiles [z, f)
@@ -1069,6 +1092,82 @@ __global__ void ecg_reduce_kernel(const HeartDev* __restrict__ hearts, const Req
 // ---------------------------------------------------------------------------
 // Activation classes
 // ---------------------------------------------------------------------------
+
+constexpr int CC_THREADS = 256;
+
+// Class chunks: one block per (request, class chunk).  Every voxel of a class
+// has the same AP, so only the sum of its G rows is needed: each thread adds
+// the rows of voxels t, t + CC_THREADS, ... in registers (FP64, in that
+// order; the lead field as in the chunk kernel), the lanes of a wave are added
+// by an xor butterfly (the same operations on every run) and the waves in
+// order, and sums[(c_off + n_chunks + k)*3*LP] receives {sum G, P' sum G,
+// Q' sum G} with the class's P' = k2(1-k3) e^{(k4+k5)(a - t0)} and
+// Q' = k2 k3 e^{k5 (a - t0)}: the layout the class sums and the reduction
+// read for chunks.
+template <int LP, int NH>
+__global__ __launch_bounds__(CC_THREADS) void ecg_class_chunk_kernel(
+    const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
+    const int64_t* __restrict__ cc_start, int cc_uniform, const double* __restrict__ lead_pos,
+    const double* __restrict__ layer_k, double* __restrict__ sums)
+{
+    __shared__ float s_p[16][2][3];
+    __shared__ double s_w[CC_THREADS / 64][LP];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    // chunk-major when every request has the same class chunks (one heart)
+    int r, k;
+    if (cc_uniform > 0) {
+        r = (int)(blockIdx.x % (unsigned)B);
+        k = (int)(blockIdx.x / (unsigned)B);
+    } else {
+        r = find_req(cc_start, B, blockIdx.x);
+        k = (int)(blockIdx.x - cc_start[r]);
+    }
+    const ReqDev R = reqs[r];
+    const HeartDev& H = hearts[R.heart];
+    const int4 ch = H.cchunks[k];
+    if (t < 3 * R.L) {
+        const double p = lead_pos[R.pos_off + t];
+        const float hi = (float)p;
+        s_p[t / 3][0][t % 3] = hi;
+        s_p[t / 3][1][t % 3] = (float)(p - (double)hi);
+    }
+    __syncthreads();
+    double acc[LP];
+#pragma unroll
+    for (int i = 0; i < LP; ++i) acc[i] = 0.0;
+    for (int q = ch.y + t; q < ch.z; q += CC_THREADS) {
+        const VoxelField<NH> vf(H.vox_geo[q]);
+#pragma unroll
+        for (int i0 = 0; i0 < LP; i0 += CH_LEADS) {
+            double g[CH_LEADS];
+            vf.leads(i0, R.L, s_p, g);
+#pragma unroll
+            for (int w2 = 0; w2 < CH_LEADS; ++w2) acc[i0 + w2] += g[w2];
+        }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+        for (int i = 0; i < LP; ++i) acc[i] += __shfl_xor(acc[i], off);
+    if (lane == 0)
+#pragma unroll
+        for (int i = 0; i < LP; ++i) s_w[w][i] = acc[i];
+    __syncthreads();
+    if (t < LP) {
+        double sg = 0.0;
+#pragma unroll
+        for (int ww = 0; ww < CC_THREADS / 64; ++ww) sg += s_w[ww][t];
+        const int l = ch.x;
+        const double* kk = layer_k + R.k_off + 9 * (l - 1);
+        const double da = H.vox_a64[ch.y] - H.layer_t0[l];
+        const double pp = kk[2] * (1.0 - kk[3]) * exp((kk[4] + kk[5]) * da);
+        const double qq = kk[2] * kk[3] * exp(kk[5] * da);
+        double* o = sums + (R.c_off + H.n_chunks + k) * 3 * LP;
+        o[t] = sg;
+        o[LP + t] = pp * sg;
+        o[2 * LP + t] = qq * sg;
+    }
+}
 
 // csums[(cls_off + k)*3*LP + v]: the sums of class k's chunks (sum G, sum G P',
 // sum G Q' per lead), added in chunk order.
@@ -1296,6 +1395,35 @@ hipError_t launch_ecg_chunk(const GroupArgs& a, hipStream_t s)
     case 8: chunk_launch1<8>(a, s); break;
     case 12: chunk_launch1<12>(a, s); break;
     case 16: chunk_launch1<16>(a, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int LP>
+static void class_chunk_launch(const GroupArgs& a, hipStream_t s)
+{
+#define EKG_CCHUNK(NHV)                                                                                   \
+    hipLaunchKernelGGL((ecg_class_chunk_kernel<LP, NHV>), dim3((unsigned)a.n_cc), dim3(CC_THREADS), 0, s, \
+                       a.hearts, a.reqs, a.B, a.cc_start, a.cc_uniform, a.lead_pos, a.layer_k, a.sums)
+    switch (a.nhood) {
+    case 0: EKG_CCHUNK(0); break;
+    case 1: EKG_CCHUNK(1); break;
+    case 2: EKG_CCHUNK(2); break;
+    default: EKG_CCHUNK(3); break;
+    }
+#undef EKG_CCHUNK
+}
+
+hipError_t launch_class_chunks(const GroupArgs& a, hipStream_t s)
+{
+    if (a.n_cc == 0) return hipSuccess;
+    if (a.n_cc > 0x7fffffffLL) return hipErrorInvalidValue;
+    switch (a.LP) {
+    case 4: class_chunk_launch<4>(a, s); break;
+    case 8: class_chunk_launch<8>(a, s); break;
+    case 12: class_chunk_launch<12>(a, s); break;
+    case 16: class_chunk_launch<16>(a, s); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -206,6 +206,9 @@ constexpr double CHUNK_SPAN_MS = 16.0;
 // Smallest activation class by default: a run of at least this many voxels of
 // one layer with one activation time takes the class path.
 constexpr int DEFAULT_CLASS_MIN = 64;
+// Largest class chunk (voxels): the class-chunk kernel sums its lead weights
+// in registers, four voxels per thread at this size.
+constexpr int64_t CLASS_CHUNK = 1024;
 
 void ensure_layout(Heart& h, bool mfma)
 {
@@ -280,11 +283,11 @@ void ensure_layout(Heart& h, bool mfma)
                 b0 = b2;
             }
         }
-        // class chunks: near-equal pieces of at most `target` per class
+        // class chunks: near-equal pieces of at most CLASS_CHUNK per class
         for (const auto& rr : runs) {
             const int64_t c0 = (int64_t)order.size(), nc = rr.second - rr.first;
             order.insert(order.end(), idx.begin() + rr.first, idx.begin() + rr.second);
-            const int64_t kc = (nc + target - 1) / target;
+            const int64_t kc = (nc + CLASS_CHUNK - 1) / CLASS_CHUNK;
             const int first = (int)L.h_cchunks.size();
             const double a = h.h_act[idx[rr.first]];
             for (int64_t c = 0; c < kc; ++c)
@@ -476,7 +479,7 @@ namespace {
 
 // Per-request sizes of a group's packed arrays.
 struct ReqSizes {
-    int64_t rows, chunks, mixed, bricks, classes;
+    int64_t rows, chunks, mixed, cchunks, bricks, classes;
 };
 
 ReqSizes req_sizes(const ReqSpec& r, bool mfma)
@@ -484,8 +487,8 @@ ReqSizes req_sizes(const ReqSpec& r, bool mfma)
     const EcgLayout& L = r.h->layout;
     // the chunk kernel (matrix-core path) keeps G rows on chip: no records;
     // per-chunk sums for every chunk, partial rows for chunks outside classes
-    return ReqSizes{mfma ? 0 : L.V + ekgk::MFMA_PAD, (int64_t)L.n_chunks + L.n_cchunks, L.n_chunks, L.n_bricks,
-                    L.n_classes};
+    return ReqSizes{mfma ? 0 : L.V + ekgk::MFMA_PAD, (int64_t)L.n_chunks + L.n_cchunks, L.n_chunks, L.n_cchunks,
+                    L.n_bricks, L.n_classes};
 }
 
 template <class T>
@@ -519,7 +522,7 @@ size_t group_device_bytes(const GroupPlan& g)
              (8 + 2 * LP * 8);
         s += (size_t)nmix * LP * r.Tp * 8;
         s += (size_t)r.L * r.T * 8;
-        s += sizeof(ekgk::ReqDev) + sizeof(ekgk::HeartDev) + 7 * 8;
+        s += sizeof(ekgk::ReqDev) + sizeof(ekgk::HeartDev) + 8 * 8;
     }
     return s;
 }
@@ -541,7 +544,7 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     ws.h_thl.clear();
     std::vector<std::pair<std::pair<double, double>, std::pair<int64_t, int64_t>>> grids;  // (start, dt) -> (Tp, off)
     std::vector<int64_t> st_lf(B + 1, 0), st_ch(B + 1, 0), st_xy(B + 1, 0), st_lay(B + 1, 0), st_out(B + 1, 0),
-        st_cs(B + 1, 0), st_cl(B + 1, 0);
+        st_cc(B + 1, 0), st_cs(B + 1, 0), st_cl(B + 1, 0);
     int64_t k_off = 0, pos_off = 0, rows = 0, c_off = 0, p_off = 0, pre_off = 0;
     if (!dev_in) {
         ws.h_k.clear();
@@ -606,14 +609,16 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
         p_off += z.mixed * LP * q.Tp;
         pre_off += (int64_t)q.nl * ((q.Tp >> R.bt_shift) + 1);
         st_lf[r + 1] = st_lf[r] + z.bricks;
-        st_ch[r + 1] = st_ch[r] + z.chunks;
+        st_ch[r + 1] = st_ch[r] + z.mixed;
+        st_cc[r + 1] = st_cc[r] + z.cchunks;
         st_xy[r + 1] = st_xy[r] + (int64_t)q.nl * q.Tp;
         st_lay[r + 1] = st_lay[r] + q.nl;
         st_out[r + 1] = st_out[r] + (int64_t)q.L * q.T;
         st_cs[r + 1] = st_cs[r] + z.classes;
         st_cl[r + 1] = st_cl[r] + (z.classes > 0 ? q.n_tiles : 0);
     }
-    if (st_lf[B] > 0x7fffffffLL || st_ch[B] > 0x7fffffffLL || st_lay[B] > 0x7fffffffLL || st_cl[B] > 0x7fffffffLL)
+    if (st_lf[B] > 0x7fffffffLL || st_ch[B] > 0x7fffffffLL || st_lay[B] > 0x7fffffffLL || st_cc[B] > 0x7fffffffLL ||
+        st_cl[B] > 0x7fffffffLL)
         fail(EKG_EINVAL, "launch group too large");
     // device buffers (grow-only)
     ws.lc.reserve((size_t)st_lay[B] * sizeof(ekgk::LayerConst));
@@ -637,7 +642,8 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     const size_t o_req = tables.size();
     append(tables, rd.data(), rd.size());
     const size_t o_st = tables.size();
-    for (auto* st : {&st_lf, &st_ch, &st_xy, &st_lay, &st_out, &st_cs, &st_cl}) append(tables, st->data(), st->size());
+    for (auto* st : {&st_lf, &st_ch, &st_xy, &st_lay, &st_out, &st_cc, &st_cs, &st_cl})
+        append(tables, st->data(), st->size());
     void* before = ws.hearts.p;
     ws.hearts.reserve(tables.size());
     if (ws.hearts.p != before) ws.h_tables.clear();
@@ -686,8 +692,10 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     a.xy_start = stp + 2 * (B + 1);
     a.lay_start = stp + 3 * (B + 1);
     a.out_start = stp + 4 * (B + 1);
-    a.cs_start = stp + 5 * (B + 1);
-    a.cl_start = stp + 6 * (B + 1);
+    a.cc_start = stp + 5 * (B + 1);
+    a.cs_start = stp + 6 * (B + 1);
+    a.cl_start = stp + 7 * (B + 1);
+    a.n_cc = st_cc[B];
     a.n_cs = st_cs[B];
     a.n_cl = st_cl[B];
     a.n_lf = st_lf[B];
@@ -696,6 +704,9 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     a.ch_uniform = B > 0 ? (int)(st_ch[1] - st_ch[0]) : 0;
     for (int r = 1; r < B && a.ch_uniform; ++r)
         if (st_ch[r + 1] - st_ch[r] != a.ch_uniform) a.ch_uniform = 0;
+    a.cc_uniform = B > 0 ? (int)(st_cc[1] - st_cc[0]) : 0;
+    for (int r = 1; r < B && a.cc_uniform; ++r)
+        if (st_cc[r + 1] - st_cc[r] != a.cc_uniform) a.cc_uniform = 0;
     a.n_xy = st_xy[B];
     a.n_lay = st_lay[B];
     a.n_out = st_out[B];
@@ -715,8 +726,9 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     a.csums = ws.csums.as<double>();
     a.ecg = d_ecg;
     auto launch_all = [&](bool timed) {
-        // The timed kernel is the dominant one of each path: the chunk kernel
-        // (matrix cores) or the fused transition-tile kernel (vector ALUs).
+        // The timed kernel is the dominant one of each path: the class-chunk
+        // kernel, or without class chunks the chunk kernel (matrix-core
+        // path), or the fused transition-tile kernel (vector ALUs).
         hipEvent_t e0 = nullptr, e1 = nullptr;
         auto tic = [&]() {
             if (!timed) return;
@@ -731,9 +743,13 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
         };
         hip_check(ekgk::launch_ap_tables(a, stream), "ap_tables launch");
         if (g.mfma) {
-            tic();
+            // the class-chunk kernel is timed when the group has class chunks
+            if (a.n_cc == 0) tic();
             hip_check(ekgk::launch_ecg_chunk(a, stream), "ecg_chunk launch");
-            toc();
+            if (a.n_cc == 0) toc();
+            if (a.n_cc > 0) tic();
+            hip_check(ekgk::launch_class_chunks(a, stream), "class_chunk launch");
+            if (a.n_cc > 0) toc();
             hip_check(ekgk::launch_class_sums(a, stream), "class_sums launch");
             hip_check(ekgk::launch_layer_prefix(a, stream), "layer_prefix launch");
         } else {

@@ -65,8 +65,8 @@ struct HeartDev {
     const double* layer_t0;      // [max_layer + 1]: reference time of layer l
     // Activation classes: runs of one layer and one activation time, whose
     // voxels all have the same AP.  Their voxels are cut into class chunks
-    // (chunk index n_chunks + k of the per-chunk arrays); the chunks above
-    // hold the other voxels.
+    // (index n_chunks + k of the per-chunk sums); the chunks above hold the
+    // other voxels.
     const int4* cchunks;         // [n_cchunks] {layer, v_begin, v_end, t_ref as float bits}
     const int4* classes;         // [n_classes] {layer, first class chunk, end class chunk, 0}
     const double* class_a;       // [n_classes] activation time of the class
@@ -106,10 +106,12 @@ struct GroupArgs {
     const int64_t* xy_start;   // nl * Tp per request
     const int64_t* lay_start;  // nl per request
     const int64_t* out_start;  // L * T per request
+    const int64_t* cc_start;   // class chunks per request
     const int64_t* cs_start;   // activation classes per request
     const int64_t* cl_start;   // 64-step tiles of the class kernel per request (0 without classes)
-    int64_t n_lf, n_ch, n_xy, n_lay, n_out, n_cs, n_cl;  // totals
+    int64_t n_lf, n_ch, n_xy, n_lay, n_out, n_cc, n_cs, n_cl;  // totals
     int ch_uniform;            // chunks per request when all requests have the same count, else 0
+    int cc_uniform;            // the same for class chunks
     const double* layer_k;
     const double* lead_pos;
     const double* tgrid;       // FP64 sample times
@@ -135,8 +137,9 @@ hipError_t launch_ecg_fused(const GroupArgs& a, hipStream_t s);
 // Matrix-core path: lead field, chunk sums and transition tiles in one kernel.
 hipError_t launch_ecg_chunk(const GroupArgs& a, hipStream_t s);
 hipError_t launch_ecg_reduce(const GroupArgs& a, hipStream_t s);
-// Activation classes: sums of the class chunks per class, then the classes'
-// terms added to the reduced ECG.
+// Activation classes: the lead-weight sums of the class chunks, their sums
+// per class, then the classes' terms added to the reduced ECG.
+hipError_t launch_class_chunks(const GroupArgs& a, hipStream_t s);
 hipError_t launch_class_sums(const GroupArgs& a, hipStream_t s);
 hipError_t launch_ecg_class(const GroupArgs& a, hipStream_t s);
 
