@@ -1096,21 +1096,26 @@ __global__ void ecg_reduce_kernel(const HeartDev* __restrict__ hearts, const Req
 constexpr int CC_THREADS = 256;
 
 // Class chunks: one block per (request, class chunk).  Every voxel of a class
-// has the same AP, so only the sum of its G rows is needed: each thread adds
-// the rows of voxels t, t + CC_THREADS, ... in registers (FP64, in that
-// order; the lead field as in the chunk kernel), the lanes of a wave are added
-// by an xor butterfly (the same operations on every run) and the waves in
-// order, and sums[(c_off + n_chunks + k)*3*LP] receives {sum G, P' sum G,
-// Q' sum G} with the class's P' = k2(1-k3) e^{(k4+k5)(a - t0)} and
-// Q' = k2 k3 e^{k5 (a - t0)}: the layout the class sums and the reduction
-// read for chunks.
-template <int LP, int NH>
+// has the same AP, so only the sum of the chunk's G rows is needed, and by
+// regrouping the dipoles per cell it is
+//   sum_u G(u) = sum_w s(w) (e(w) . D(w))
+// over the chunk's cells w with a non-zero integer vector D(w) (host-built
+// per neighbourhood; cells inside the chunk cancel).  s(w) and the FP32 e(w)
+// are the chunk kernel's bit for bit (same operands and order), and
+// e(w) . D(w) is exact in FP64 (FP32 components times small integers), so a
+// cell's contributions here and in chunk-kernel G rows cancel across every
+// dipole as they do between G rows.  Each thread adds its cells t, t + CC_THREADS, ...
+// (two at a time) in registers in that order, the lanes of a wave are added
+// by an xor butterfly and the waves in order, and sums[(c_off + n_chunks +
+// k)*3*LP] receives {sum G, P' sum G, Q' sum G} with the class's
+// P' = k2(1-k3) e^{(k4+k5)(a - t0)} and Q' = k2 k3 e^{k5 (a - t0)}.
+template <int LP>
 __global__ __launch_bounds__(CC_THREADS) void ecg_class_chunk_kernel(
     const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
     const int64_t* __restrict__ cc_start, int cc_uniform, const double* __restrict__ lead_pos,
     const double* __restrict__ layer_k, double* __restrict__ sums)
 {
-    __shared__ float s_p[16][2][3];
+    __shared__ float4 s_ph[LP], s_pl[LP];  // electrode FP32 hi / lo per component
     __shared__ double s_w[CC_THREADS / 64][LP];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     // chunk-major when every request has the same class chunks (one heart)
@@ -1125,24 +1130,63 @@ __global__ __launch_bounds__(CC_THREADS) void ecg_class_chunk_kernel(
     const ReqDev R = reqs[r];
     const HeartDev& H = hearts[R.heart];
     const int4 ch = H.cchunks[k];
-    if (t < 3 * R.L) {
-        const double p = lead_pos[R.pos_off + t];
-        const float hi = (float)p;
-        s_p[t / 3][0][t % 3] = hi;
-        s_p[t / 3][1][t % 3] = (float)(p - (double)hi);
+    if (t < LP) {
+        // leads past the request's count repeat lead 0 (their sums are zeroed)
+        const int i = t < R.L ? t : 0;
+        float hi[3], lo[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double p = lead_pos[R.pos_off + 3 * i + c];
+            hi[c] = (float)p;
+            lo[c] = (float)(p - (double)hi[c]);
+        }
+        s_ph[t] = float4{hi[0], hi[1], hi[2], 0.0f};
+        s_pl[t] = float4{lo[0], lo[1], lo[2], 0.0f};
     }
     __syncthreads();
     double acc[LP];
 #pragma unroll
     for (int i = 0; i < LP; ++i) acc[i] = 0.0;
-    for (int q = ch.y + t; q < ch.z; q += CC_THREADS) {
-        const VoxelField<NH> vf(H.vox_geo[q]);
+    const int cb = H.ccell_off[k], ce = H.ccell_off[k + 1];
+    for (int q = cb + t; q < ce; q += 2 * CC_THREADS) {
+        // reload the electrodes from LDS in every iteration rather than keep
+        // all of them in registers (which would halve the waves per SIMD)
+        asm volatile("" ::: "memory");
+        const bool two = q + CC_THREADS < ce;
+        const uint2 c0 = H.ccells[q];
+        const uint2 c1 = two ? H.ccells[q + CC_THREADS] : c0;
+        // packed coordinates z | y << 10 | x << 20: component c is axis c
+        float f0[3], f1[3];
+        double d0[3], d1[3];
 #pragma unroll
-        for (int i0 = 0; i0 < LP; i0 += CH_LEADS) {
-            double g[CH_LEADS];
-            vf.leads(i0, R.L, s_p, g);
+        for (int c = 0; c < 3; ++c) {
+            f0[c] = (float)((int)((c0.x >> (10 * c)) & 1023u) + 1);
+            f1[c] = (float)((int)((c1.x >> (10 * c)) & 1023u) + 1);
+            d0[c] = (double)(int)(int8_t)((c0.y >> (8 * c)) & 255u);
+            d1[c] = two ? (double)(int)(int8_t)((c1.y >> (8 * c)) & 255u) : 0.0;
+        }
 #pragma unroll
-            for (int w2 = 0; w2 < CH_LEADS; ++w2) acc[i0 + w2] += g[w2];
+        for (int i = 0; i < LP; ++i) {
+            const float4 ph = s_ph[i], pl = s_pl[i];
+            {
+                const float e0 = (ph.x - f0[0]) + pl.x, e1 = (ph.y - f0[1]) + pl.y, e2 = (ph.z - f0[2]) + pl.z;
+                float l = e0 * e0;
+                l = __builtin_fmaf(e1, e1, l);
+                l = __builtin_fmaf(e2, e2, l);
+                const float rr = __builtin_amdgcn_rsqf(l);
+                // D . e exactly: FP32 components times small integers
+                const double eD = __builtin_fma(d0[2], (double)e2, __builtin_fma(d0[1], (double)e1, d0[0] * (double)e0));
+                acc[i] = __builtin_fma((double)(rr * rr * rr), eD, acc[i]);
+            }
+            {
+                const float e0 = (ph.x - f1[0]) + pl.x, e1 = (ph.y - f1[1]) + pl.y, e2 = (ph.z - f1[2]) + pl.z;
+                float l = e0 * e0;
+                l = __builtin_fmaf(e1, e1, l);
+                l = __builtin_fmaf(e2, e2, l);
+                const float rr = __builtin_amdgcn_rsqf(l);
+                const double eD = __builtin_fma(d1[2], (double)e2, __builtin_fma(d1[1], (double)e1, d1[0] * (double)e0));
+                acc[i] = __builtin_fma((double)(rr * rr * rr), eD, acc[i]);
+            }
         }
     }
 #pragma unroll
@@ -1157,6 +1201,7 @@ __global__ __launch_bounds__(CC_THREADS) void ecg_class_chunk_kernel(
         double sg = 0.0;
 #pragma unroll
         for (int ww = 0; ww < CC_THREADS / 64; ++ww) sg += s_w[ww][t];
+        if (t >= R.L) sg = 0.0;
         const int l = ch.x;
         const double* kk = layer_k + R.k_off + 9 * (l - 1);
         const double da = H.vox_a64[ch.y] - H.layer_t0[l];
@@ -1191,33 +1236,43 @@ __global__ void class_sums_kernel(const HeartDev* __restrict__ hearts, const Req
     }
 }
 
-// ecg[r][i][t] += sum over the heart's activation classes, in class order, of
+// ecg[r][i][t] += sum over the heart's activation classes of
 //   S(t - a) (X_l(t) sum G P' + Y_l(t) sum G Q') + k0 sum G,
 // with S(tau) = 1 / (1 + exp(-k1 tau)) in FP64, the factor of the AP
 // (Wohlfart.h:195-203) at the cell time tau = t - a (simulator.cpp:154-170).
 // Every voxel of a class has the same layer and activation time, hence the
 // same AP, so the class's dipole terms factor through its sums.  Samples from
-// Tn on receive nothing, as in the reduction.  One wave per (request, 64
-// steps), one step per lane, every lead in registers.
+// Tn on receive nothing, as in the reduction.  One block of CL_WAVES waves
+// per (request, 64 steps), one step per lane: wave w adds the classes
+// [w n / CL_WAVES, (w+1) n / CL_WAVES) in order, then the waves' sums are
+// added in wave order.
+constexpr int CL_WAVES = 4;
+
 template <int LP>
-__global__ __launch_bounds__(64) void ecg_class_kernel(
+__global__ __launch_bounds__(64 * CL_WAVES) void ecg_class_kernel(
     const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
     const int64_t* __restrict__ cl_start, const double* __restrict__ layer_k,
     const double2* __restrict__ xy, const double* __restrict__ tgrid, const double* __restrict__ csums,
     double* __restrict__ ecg)
 {
+    __shared__ double s_acc[CL_WAVES - 1][LP][64];
     const int r = find_req(cl_start, B, blockIdx.x);
     const ReqDev R = reqs[r];
     const HeartDev& H = hearts[R.heart];
-    const int j = (int)(blockIdx.x - cl_start[r]) * 64 + (int)threadIdx.x;
-    if (j >= R.Tn) return;
-    const double t = tgrid[R.t_off + j];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int j0 = (int)(blockIdx.x - cl_start[r]) * 64;
+    if (j0 >= R.Tn) return;  // the whole block
+    const int j = j0 + lane;
+    const bool live = j < R.Tn;
+    const double t = tgrid[R.t_off + j];  // j < Tp: in the grid
     double acc[LP];
 #pragma unroll
     for (int i = 0; i < LP; ++i) acc[i] = 0.0;
+    const int k0c = (int)((int64_t)H.n_classes * w / CL_WAVES), k1c = (int)((int64_t)H.n_classes * (w + 1) / CL_WAVES);
     int lp = 0;
     double X = 0.0, Y = 0.0, k0 = 0.0, k1 = 0.0;
-    for (int k = 0; k < H.n_classes; ++k) {
+    for (int k = k0c; k < k1c; ++k) {
         const int l = H.classes[k].x;
         if (l != lp) {  // classes are ordered by layer
             const double2 xyj = xy[R.xy_off + (int64_t)(l - 1) * R.Tp + j];
@@ -1233,9 +1288,19 @@ __global__ __launch_bounds__(64) void ecg_class_kernel(
         for (int i = 0; i < LP; ++i)
             acc[i] += __builtin_fma(k0, cs[i], S * __builtin_fma(X, cs[LP + i], Y * cs[2 * LP + i]));
     }
+    if (w > 0)
 #pragma unroll
-    for (int i = 0; i < LP; ++i)
-        if (i < R.L) ecg[R.ecg_off + (int64_t)i * R.T + j] += acc[i];
+        for (int i = 0; i < LP; ++i) s_acc[w - 1][i][lane] = acc[i];
+    __syncthreads();
+    if (w == 0 && live) {
+#pragma unroll
+        for (int i = 0; i < LP; ++i) {
+            double v = acc[i];
+#pragma unroll
+            for (int ww = 0; ww < CL_WAVES - 1; ++ww) v += s_acc[ww][i][lane];
+            if (i < R.L) ecg[R.ecg_off + (int64_t)i * R.T + j] += v;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1403,16 +1468,8 @@ hipError_t launch_ecg_chunk(const GroupArgs& a, hipStream_t s)
 template <int LP>
 static void class_chunk_launch(const GroupArgs& a, hipStream_t s)
 {
-#define EKG_CCHUNK(NHV)                                                                                   \
-    hipLaunchKernelGGL((ecg_class_chunk_kernel<LP, NHV>), dim3((unsigned)a.n_cc), dim3(CC_THREADS), 0, s, \
-                       a.hearts, a.reqs, a.B, a.cc_start, a.cc_uniform, a.lead_pos, a.layer_k, a.sums)
-    switch (a.nhood) {
-    case 0: EKG_CCHUNK(0); break;
-    case 1: EKG_CCHUNK(1); break;
-    case 2: EKG_CCHUNK(2); break;
-    default: EKG_CCHUNK(3); break;
-    }
-#undef EKG_CCHUNK
+    hipLaunchKernelGGL((ecg_class_chunk_kernel<LP>), dim3((unsigned)a.n_cc), dim3(CC_THREADS), 0, s, a.hearts,
+                       a.reqs, a.B, a.cc_start, a.cc_uniform, a.lead_pos, a.layer_k, a.sums);
 }
 
 hipError_t launch_class_chunks(const GroupArgs& a, hipStream_t s)
@@ -1455,7 +1512,8 @@ hipError_t launch_ecg_class(const GroupArgs& a, hipStream_t s)
     switch (a.LP) {
 #define EKG_CLASS(LPV)                                                                                  \
     case LPV:                                                                                           \
-        hipLaunchKernelGGL(ecg_class_kernel<LPV>, dim3((unsigned)a.n_cl), dim3(64), 0, s, a.hearts, a.reqs, \
+        hipLaunchKernelGGL(ecg_class_kernel<LPV>, dim3((unsigned)a.n_cl), dim3(64 * CL_WAVES), 0, s, a.hearts,  \
+                           a.reqs,                                                                        \
                            a.B, a.cl_start, a.layer_k, a.xy, a.tgrid, a.csums, a.ecg);                   \
         break;
         EKG_CLASS(4)

@@ -207,8 +207,8 @@ constexpr double CHUNK_SPAN_MS = 16.0;
 // one layer with one activation time takes the class path.
 constexpr int DEFAULT_CLASS_MIN = 64;
 // Largest class chunk (voxels): the class-chunk kernel sums its lead weights
-// in registers, four voxels per thread at this size.
-constexpr int64_t CLASS_CHUNK = 1024;
+// cell by cell in registers (see ensure_cells).
+constexpr int64_t CLASS_CHUNK = 4096;
 
 void ensure_layout(Heart& h, bool mfma)
 {
@@ -239,6 +239,7 @@ void ensure_layout(Heart& h, bool mfma)
     L.h_class_a.clear();
     L.chunk_target = (int)target;
     L.class_min = cmin;
+    for (bool& b : L.cells_built) b = false;
     // reference time of layer l: its earliest activation time, rounded to FP32
     L.h_layer_t0.assign(h.max_layer + 1, 0.0);
     std::vector<int64_t> order;  // the layout's voxel order
@@ -420,6 +421,7 @@ void ensure_layout(Heart& h, bool mfma)
               "upload activation times");
     hip_check(hipMemcpy(L.vox_geo.p, geo.data(), geo.size() * sizeof(uint2), hipMemcpyHostToDevice),
               "upload voxel geometry");
+    L.h_geo.swap(geo);
     if (!L.h_chunks.empty())
         hip_check(hipMemcpy(L.chunks.p, L.h_chunks.data(), L.h_chunks.size() * sizeof(int4),
                             hipMemcpyHostToDevice),
@@ -473,6 +475,78 @@ void ensure_layout(Heart& h, bool mfma)
     d.n_cchunks = L.n_cchunks;
     d.n_classes = L.n_classes;
     L.valid = true;
+}
+
+// The sum of the G rows of a set of voxels S, regrouped by cell: a voxel u
+// contributes m(u) . s_u (s_u = sum of the d in N with u + d myocardial) and
+// m(u + d) . d for each such d, so
+//   sum_{u in S} G(u) = sum_w m(w) . D(w),
+//   D(w) = [w in S] s_w + sum_{d in N, w - d in S, w myocardial} d,
+// an integer vector per cell; cells inside S cancel (D = 0) and are dropped.
+// Each remaining cell's weight is evaluated once per class chunk.
+void ensure_cells(Heart& h, int nhood)
+{
+    EcgLayout& L = h.layout;
+    if (nhood < 0 || nhood > 3) fail(EKG_EINVAL, "unknown neighbourhood");
+    if (L.cells_built[nhood]) return;
+    int8_t dif[26][3];
+    const int nd = nhood_offsets(nhood, dif);
+    struct Ent {
+        uint32_t cell;
+        int d0, d1, d2;
+    };
+    std::vector<Ent> ent;
+    std::vector<uint2> cells;
+    std::vector<int> off(L.n_cchunks + 1, 0);
+    auto bit = [](int d0, int d1, int d2) { return (d0 + 1) * 9 + (d1 + 1) * 3 + (d2 + 1); };
+    for (int k = 0; k < L.n_cchunks; ++k) {
+        const int4 cc = L.h_cchunks[k];
+        ent.clear();
+        for (int v = cc.y; v < cc.z; ++v) {
+            const uint2 g = L.h_geo[v];
+            const int z = (int)(g.x & 1023u), y = (int)((g.x >> 10) & 1023u), x = (int)(g.x >> 20);
+            int s0 = 0, s1 = 0, s2 = 0;
+            for (int q = 0; q < nd; ++q) {
+                if (!((g.y >> bit(dif[q][0], dif[q][1], dif[q][2])) & 1u)) continue;
+                const int zz = z + dif[q][0], yy = y + dif[q][1], xx = x + dif[q][2];
+                ent.push_back(Ent{(uint32_t)zz | ((uint32_t)yy << 10) | ((uint32_t)xx << 20), dif[q][0], dif[q][1],
+                                  dif[q][2]});
+                s0 += dif[q][0];
+                s1 += dif[q][1];
+                s2 += dif[q][2];
+            }
+            ent.push_back(Ent{g.x, s0, s1, s2});
+        }
+        std::sort(ent.begin(), ent.end(), [](const Ent& a, const Ent& b) { return a.cell < b.cell; });
+        for (size_t i = 0; i < ent.size();) {
+            size_t j = i;
+            int D0 = 0, D1 = 0, D2 = 0;
+            for (; j < ent.size() && ent[j].cell == ent[i].cell; ++j) {
+                D0 += ent[j].d0;
+                D1 += ent[j].d1;
+                D2 += ent[j].d2;
+            }
+            if (D0 || D1 || D2) {
+                if (std::abs(D0) > 127 || std::abs(D1) > 127 || std::abs(D2) > 127)
+                    fail(EKG_EINVAL, "dipole vector out of range");
+                cells.push_back(make_uint2(ent[i].cell, (uint32_t)(uint8_t)(int8_t)D0 |
+                                                            ((uint32_t)(uint8_t)(int8_t)D1 << 8) |
+                                                            ((uint32_t)(uint8_t)(int8_t)D2 << 16)));
+            }
+            i = j;
+        }
+        if (cells.size() > 0x7fffffffu) fail(EKG_EINVAL, "too many class cells");
+        off[k + 1] = (int)cells.size();
+    }
+    if (cells.empty()) cells.push_back(make_uint2(0, 0));
+    h.ctx->bind();
+    L.ccells[nhood].reserve(cells.size() * sizeof(uint2));
+    L.ccell_off[nhood].reserve(off.size() * sizeof(int));
+    hip_check(hipMemcpy(L.ccells[nhood].p, cells.data(), cells.size() * sizeof(uint2), hipMemcpyHostToDevice),
+              "upload class cells");
+    hip_check(hipMemcpy(L.ccell_off[nhood].p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice),
+              "upload class cell offsets");
+    L.cells_built[nhood] = true;
 }
 
 namespace {
@@ -554,13 +628,17 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
         ReqSpec& q = g.reqs[r];
         Heart& h = *q.h;
         ensure_layout(h, g.mfma);
+        if (g.mfma) ensure_cells(h, g.nhood);
         int hi = -1;
         for (size_t i = 0; i < hs.size(); ++i)
             if (hs[i] == &h) hi = (int)i;
         if (hi < 0) {
             hi = (int)hs.size();
             hs.push_back(&h);
-            hdev.push_back(h.layout.dev);
+            ekgk::HeartDev d = h.layout.dev;
+            d.ccells = g.mfma ? h.layout.ccells[g.nhood].as<uint2>() : nullptr;
+            d.ccell_off = g.mfma ? h.layout.ccell_off[g.nhood].as<int>() : nullptr;
+            hdev.push_back(d);
         }
         int64_t t_off = -1;
         for (auto& gr : grids)

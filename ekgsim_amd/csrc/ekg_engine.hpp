@@ -85,6 +85,12 @@ struct EcgLayout {
     int n_cchunks = 0, n_classes = 0;
     std::vector<int4> h_chunks, h_cchunks, h_classes;
     std::vector<double> h_class_a;
+    std::vector<uint2> h_geo;      // per voxel of the layout: packed grid coordinates, 3x3x3 myocardium mask
+    // Per dipole neighbourhood (EKG_NHOOD_*), built on first use: the cells of
+    // each class chunk with a non-zero integer dipole vector D (see
+    // ensure_cells), cells [ccell_off[k], ccell_off[k+1]) for class chunk k.
+    bool cells_built[4] = {false, false, false, false};
+    DevBuf ccells[4], ccell_off[4];
     std::vector<int> h_layer_chunk;
     std::vector<double> h_layer_t0;
     DevBuf vox_chunk, vox_a64, vox_a, vox_geo, chunks, bricks, brick_off, brick_vox, halo_off, halo_cells,
@@ -137,6 +143,8 @@ GroupPlan plan_group(Ctx& ctx, std::vector<ReqSpec> reqs);
 // Builds (or keeps) the heart's layout for the matrix-core path (mfma, with
 // activation classes as tuned) or the vector-ALU path (no classes).
 void ensure_layout(Heart& h, bool mfma);
+// Cell lists of the heart's class chunks for dipole neighbourhood `nhood`.
+void ensure_cells(Heart& h, int nhood);
 // Whether the heart's simulations take the matrix-core path (as plan_group decides).
 bool heart_mfma(const Ctx& ctx, const Heart& h);
 size_t group_device_bytes(const GroupPlan& g);
