@@ -41,21 +41,6 @@ MFMA_FP64_PEAK = 78.6e12
 # closed form (sigmoid within 2^-24.5 of 1, as in the FP32 AP) or is below
 # 2^-36 of its plateau, so only window voxel-steps need an evaluated AP.
 SATC_X, SKIP_X = -24.5, 48.0
-# Activation classes of the matrix-core path: runs of at least this many
-# voxels of one layer with one activation time share the AP and are summed
-# per class (the library's default threshold).
-DEFAULT_CLASS_MIN = 64
-
-
-def class_members(lay, act, cmin):
-    """Mask of the voxels in activation classes (cmin <= 0: none) and the
-    number of classes."""
-    if cmin <= 0:
-        return np.zeros(lay.shape, bool), 0
-    key = np.stack([lay.astype(np.float64), act], 1)
-    _, inv, cnt = np.unique(key, axis=0, return_inverse=True, return_counts=True)
-    big = cnt >= cmin
-    return big[inv.ravel()], int(big.sum())
 
 
 def window_voxel_steps(act, k1, start, dt, T):
@@ -83,12 +68,16 @@ def lead_field_cells(layer):
     return int(cnt[myo].sum())
 
 
-# Algorithmic work of the chunk kernel, counted per unit:
+# Algorithmic work, counted per unit.  Class-chunk kernel (the dominant one
+# when the heart has activation classes), per (class cell, lead): 14 FP32
+# flops (offset e 6, |e|^2 5, rsq 1, |e|^-3 2) and 7 FP64 flops (D . e 5, the
+# accumulating FMA 2).  Chunk kernel (dominant without classes):
 #   per lead-weight evaluation (voxel, lead, cell): 14 FP32 flops (electrode
 #     offset 6, |e|^2 5, rsq 1, r^3 2) and 4 FP64 flops (d . e, one FMA);
 #   per window voxel-step (the transition tiles): 2L FP64 MFMA flops and 14
 #     FP32 ops (10 AP assembly + 4 transcendental).
 # FP32 counts half against the FP64 peak (the FP32 vector rate is twice it).
+FP32_PER_CLASS_CELL, FP64_PER_CLASS_CELL = 14, 7
 FP32_PER_CELL, FP64_PER_CELL = 14, 4
 FP32_PER_WINDOW_STEP = 14
 # Naive per-voxel-step model: every voxel at every step, 38 VALU lane-cycles
@@ -290,19 +279,26 @@ def main():
     avg_kernel_s = kms / 1e3 / max(nl, 1)
     act_grid = heart.activation()
     act_myo = act_grid[h.layer > 0]
-    # voxels in activation classes have no per-voxel AP in the chunk kernel
-    tn = heart.get_tune()
-    cmin = 0 if args.kernel == "valu" or tn["class_min"] < 0 else (tn["class_min"] or DEFAULT_CLASS_MIN)
-    in_cls, n_classes = class_members(h.layer[h.layer > 0], act_myo, cmin)
-    win = sum(window_voxel_steps(act_myo[~in_cls], float(k[0][1]), 0.0, 1.0, T) for k in ks)
-    cells = B * L * lead_field_cells(h.layer) if args.kernel != "valu" else 0
-    f64 = 2.0 * L * win + FP64_PER_CELL * cells
-    f32 = FP32_PER_WINDOW_STEP * win + FP32_PER_CELL * cells
+    info = heart.layout_info("3D4") if args.kernel != "valu" else None
+    n_classes = info["classes"] if info and heart.get_tune()["class_min"] >= 0 else 0
+    win = cells = ccells = 0
+    if n_classes > 0:
+        # the class-chunk kernel: every class cell once per (request, lead)
+        dom = "ecg_class_chunk_kernel"
+        ccells = B * L * info["class_cells"]
+        f64 = FP64_PER_CLASS_CELL * ccells
+        f32 = FP32_PER_CLASS_CELL * ccells
+    else:
+        dom = "ecg_chunk_kernel" if args.kernel != "valu" else "ecg_fused_kernel"
+        win = sum(window_voxel_steps(act_myo, float(k[0][1]), 0.0, 1.0, T) for k in ks)
+        cells = B * L * lead_field_cells(h.layer) if args.kernel != "valu" else 0
+        f64 = 2.0 * L * win + FP64_PER_CELL * cells
+        f32 = FP32_PER_WINDOW_STEP * win + FP32_PER_CELL * cells
     flops = f64 + 0.5 * f32
     achieved = flops / avg_kernel_s
     naive_rate = value * V * T
-    log(f"batched: {ms_step:.2f} ms/step, {value:.1f} ECGs/s; dominant kernel {avg_kernel_s * 1e3:.2f} ms; "
-        f"{win / B / V:.1f} window steps per voxel; {achieved / 1e12:.2f} TFLOP/s FP64-equivalent")
+    log(f"batched: {ms_step:.2f} ms/step, {value:.1f} ECGs/s; {dom} {avg_kernel_s * 1e3:.2f} ms; "
+        f"{achieved / 1e12:.2f} TFLOP/s FP64-equivalent; layout {info}")
     if args.profile_only:
         print(json.dumps({"ms_per_step": ms_step, "dominant_kernel_ms": avg_kernel_s * 1e3,
                           "calibration_copy_bytes": args.calibrate_copy << 20}))
@@ -413,8 +409,7 @@ def main():
             "grid": n, "myocardial_voxels": V, "leads": L, "steps_per_ecg": T, "batch": B,
             "neighbourhood": "3D4", "excitation": "int32 per-voxel delay 1-3, 5 endocardial seeds",
             "launch": {"autotuned": tuning, "graph_replay": not args.no_graph, "graph": gstats},
-            "activation_classes": {"class_min": cmin, "classes": n_classes,
-                                   "voxel_fraction": round(float(in_cls.mean()), 5)},
+            "layout": info,
         },
         "classes_off": general,
         "plain_ecgs_per_s": round(plain, 2),
@@ -428,7 +423,7 @@ def main():
         "excitation_ms": round(ca_ms, 2),
         "roofline": {
             "bound": "mfma",
-            "kernel": "ecg_chunk_kernel" if args.kernel != "valu" else "ecg_fused_kernel",
+            "kernel": dom,
             "achieved": round(achieved / 1e12, 4),
             "peak": MFMA_FP64_PEAK / 1e12,
             "unit": "TFLOP/s",
@@ -437,9 +432,11 @@ def main():
             "algorithmic_flops_per_launch": flops,
             "fp64_flops_per_launch": f64,
             "fp32_flops_per_launch": f32,
+            "class_cell_leads_per_launch": ccells,
             "lead_weight_evaluations_per_launch": cells,
             "window_voxel_steps_per_launch": win,
-            "per_unit": {"lead_weight_evaluation": {"fp32": FP32_PER_CELL, "fp64": FP64_PER_CELL},
+            "per_unit": {"class_cell_lead": {"fp32": FP32_PER_CLASS_CELL, "fp64": FP64_PER_CLASS_CELL},
+                         "lead_weight_evaluation": {"fp32": FP32_PER_CELL, "fp64": FP64_PER_CELL},
                          "window_voxel_step": {"fp32": FP32_PER_WINDOW_STEP, "fp64": 2 * L},
                          "fp32_weight": 0.5},
             "naive_voxel_steps_per_s": naive_rate,

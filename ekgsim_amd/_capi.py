@@ -32,7 +32,7 @@ EXPORTS = (
     "ekg_server_poll", "ekg_server_wait", "ekg_server_stats", "ekg_simulate_batch_device",
     "ekg_ctx_synchronize", "ekg_ctx_set_kernel", "ekg_ap_value", "ekg_ap_num_genes", "ekg_layer_aps",
     "ekg_criterion", "ekg_resample", "ekg_heart_autotune", "ekg_heart_set_tune", "ekg_heart_get_tune",
-    "ekg_ctx_set_graphs", "ekg_ctx_graph_stats",
+    "ekg_ctx_set_graphs", "ekg_ctx_graph_stats", "ekg_heart_layout_info",
 )
 KERNELS = {"auto": 0, "valu": 1, "mfma": 2}
 
@@ -68,6 +68,18 @@ class Tune(C.Structure):
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class LayoutInfo(C.Structure):
+    _fields_ = [
+        ("voxels", C.c_int64),
+        ("class_voxels", C.c_int64),
+        ("class_cells", C.c_int64),
+        ("chunks", C.c_int),
+        ("class_chunks", C.c_int),
+        ("classes", C.c_int),
+        ("pad", C.c_int),
+    ]
 
 
 class ServerOpts(C.Structure):
@@ -137,6 +149,7 @@ def lib():
     L.ekg_heart_autotune.argtypes = [vp, C.POINTER(SimDesc), C.c_int, C.POINTER(Tune)]
     L.ekg_heart_set_tune.argtypes = [vp, C.POINTER(Tune)]
     L.ekg_heart_get_tune.argtypes = [vp, C.POINTER(Tune)]
+    L.ekg_heart_layout_info.argtypes = [vp, C.c_int, C.POINTER(LayoutInfo)]
     L.ekg_ctx_set_graphs.argtypes = [vp, C.c_int]
     L.ekg_ctx_graph_stats.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.ekg_resample.argtypes = [vp, C.c_int64, C.c_double, C.c_int, vp, C.c_int64, C.POINTER(C.c_int64)]
@@ -423,6 +436,13 @@ class Heart:
         t = Tune()
         check(lib().ekg_heart_get_tune(self.h, C.byref(t)))
         return t.as_dict()
+
+    def layout_info(self, nhood="3D4"):
+        """Chunks, activation classes and class cells of the matrix-core
+        layout as currently tuned."""
+        o = LayoutInfo()
+        check(lib().ekg_heart_layout_info(self.h, NHOOD[nhood], C.byref(o)))
+        return {f: getattr(o, f) for f, _ in o._fields_ if f != "pad"}
 
     def simulate_many(self, requests, nhood="3D4"):
         """One launch group of requests that may differ in lead count, layer

@@ -388,6 +388,28 @@ int ekg_heart_get_tune(const ekg_heart* h, ekg_tune* t)
     });
 }
 
+int ekg_heart_layout_info(ekg_heart* h, int nhood, ekg_layout_info* out)
+{
+    return guard([&] {
+        if (!h || !out) ekg::fail(EKG_EINVAL, "NULL argument");
+        if (nhood < EKG_NHOOD_2D4 || nhood > EKG_NHOOD_3D8) ekg::fail(EKG_EINVAL, "unknown neighbourhood");
+        ekg::Ctx& c = *h->h.ctx;
+        std::lock_guard<std::mutex> g1(c.mu);
+        std::lock_guard<std::mutex> g2(h->h.mu);
+        ekg::ensure_layout(h->h, true);
+        ekg::ensure_cells(h->h, nhood);
+        const ekg::EcgLayout& L = h->h.layout;
+        ekg_layout_info o{};
+        o.voxels = L.V;
+        for (const int4& cc : L.h_cchunks) o.class_voxels += cc.z - cc.y;
+        o.class_cells = L.n_ccells[nhood];
+        o.chunks = L.n_chunks;
+        o.class_chunks = L.n_cchunks;
+        o.classes = L.n_classes;
+        *out = o;
+    });
+}
+
 int ekg_ctx_set_graphs(ekg_ctx* ctx, int enable)
 {
     return guard([&] {

@@ -491,53 +491,59 @@ void ensure_cells(Heart& h, int nhood)
     if (L.cells_built[nhood]) return;
     int8_t dif[26][3];
     const int nd = nhood_offsets(nhood, dif);
-    struct Ent {
-        uint32_t cell;
-        int d0, d1, d2;
-    };
-    std::vector<Ent> ent;
+    // cell -> slot of the current chunk (-1: none); reset after each chunk
+    std::vector<int> slot((size_t)h.nvox, -1);
+    std::vector<int64_t> touched;
+    std::vector<int> acc;  // D per slot, three ints
     std::vector<uint2> cells;
     std::vector<int> off(L.n_cchunks + 1, 0);
     auto bit = [](int d0, int d1, int d2) { return (d0 + 1) * 9 + (d1 + 1) * 3 + (d2 + 1); };
+    auto add = [&](int z, int y, int x, int d0, int d1, int d2) {
+        const int64_t gi = ((int64_t)z * h.n1 + y) * h.n2 + x;
+        int sl = slot[gi];
+        if (sl < 0) {
+            sl = slot[gi] = (int)touched.size();
+            touched.push_back(gi);
+            acc.insert(acc.end(), {0, 0, 0});
+        }
+        acc[3 * sl] += d0;
+        acc[3 * sl + 1] += d1;
+        acc[3 * sl + 2] += d2;
+    };
     for (int k = 0; k < L.n_cchunks; ++k) {
         const int4 cc = L.h_cchunks[k];
-        ent.clear();
+        touched.clear();
+        acc.clear();
         for (int v = cc.y; v < cc.z; ++v) {
             const uint2 g = L.h_geo[v];
             const int z = (int)(g.x & 1023u), y = (int)((g.x >> 10) & 1023u), x = (int)(g.x >> 20);
             int s0 = 0, s1 = 0, s2 = 0;
             for (int q = 0; q < nd; ++q) {
                 if (!((g.y >> bit(dif[q][0], dif[q][1], dif[q][2])) & 1u)) continue;
-                const int zz = z + dif[q][0], yy = y + dif[q][1], xx = x + dif[q][2];
-                ent.push_back(Ent{(uint32_t)zz | ((uint32_t)yy << 10) | ((uint32_t)xx << 20), dif[q][0], dif[q][1],
-                                  dif[q][2]});
+                add(z + dif[q][0], y + dif[q][1], x + dif[q][2], dif[q][0], dif[q][1], dif[q][2]);
                 s0 += dif[q][0];
                 s1 += dif[q][1];
                 s2 += dif[q][2];
             }
-            ent.push_back(Ent{g.x, s0, s1, s2});
+            add(z, y, x, s0, s1, s2);
         }
-        std::sort(ent.begin(), ent.end(), [](const Ent& a, const Ent& b) { return a.cell < b.cell; });
-        for (size_t i = 0; i < ent.size();) {
-            size_t j = i;
-            int D0 = 0, D1 = 0, D2 = 0;
-            for (; j < ent.size() && ent[j].cell == ent[i].cell; ++j) {
-                D0 += ent[j].d0;
-                D1 += ent[j].d1;
-                D2 += ent[j].d2;
-            }
-            if (D0 || D1 || D2) {
-                if (std::abs(D0) > 127 || std::abs(D1) > 127 || std::abs(D2) > 127)
-                    fail(EKG_EINVAL, "dipole vector out of range");
-                cells.push_back(make_uint2(ent[i].cell, (uint32_t)(uint8_t)(int8_t)D0 |
-                                                            ((uint32_t)(uint8_t)(int8_t)D1 << 8) |
-                                                            ((uint32_t)(uint8_t)(int8_t)D2 << 16)));
-            }
-            i = j;
+        // cells in first-touch order (the voxel order of the chunk)
+        for (size_t sl = 0; sl < touched.size(); ++sl) {
+            const int64_t gi = touched[sl];
+            slot[gi] = -1;
+            const int D0 = acc[3 * sl], D1 = acc[3 * sl + 1], D2 = acc[3 * sl + 2];
+            if (!(D0 || D1 || D2)) continue;
+            if (std::abs(D0) > 127 || std::abs(D1) > 127 || std::abs(D2) > 127)
+                fail(EKG_EINVAL, "dipole vector out of range");
+            const int x = (int)(gi % h.n2), y = (int)((gi / h.n2) % h.n1), z = (int)(gi / ((int64_t)h.n2 * h.n1));
+            cells.push_back(make_uint2((uint32_t)z | ((uint32_t)y << 10) | ((uint32_t)x << 20),
+                                       (uint32_t)(uint8_t)(int8_t)D0 | ((uint32_t)(uint8_t)(int8_t)D1 << 8) |
+                                           ((uint32_t)(uint8_t)(int8_t)D2 << 16)));
         }
         if (cells.size() > 0x7fffffffu) fail(EKG_EINVAL, "too many class cells");
         off[k + 1] = (int)cells.size();
     }
+    L.n_ccells[nhood] = (int64_t)cells.size();
     if (cells.empty()) cells.push_back(make_uint2(0, 0));
     h.ctx->bind();
     L.ccells[nhood].reserve(cells.size() * sizeof(uint2));

@@ -90,6 +90,7 @@ struct EcgLayout {
     // each class chunk with a non-zero integer dipole vector D (see
     // ensure_cells), cells [ccell_off[k], ccell_off[k+1]) for class chunk k.
     bool cells_built[4] = {false, false, false, false};
+    int64_t n_ccells[4] = {0, 0, 0, 0};
     DevBuf ccells[4], ccell_off[4];
     std::vector<int> h_layer_chunk;
     std::vector<double> h_layer_t0;

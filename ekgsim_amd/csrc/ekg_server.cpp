@@ -312,7 +312,11 @@ int server_add_heart(Server& s, Heart& h)
     std::lock_guard<std::mutex> g(s.mu);
     {
         std::lock_guard<std::mutex> hg(h.mu);
-        ensure_layout(h, heart_mfma(s.ctx, h));
+        const bool mfma = heart_mfma(s.ctx, h);
+        ensure_layout(h, mfma);
+        // class cells of every neighbourhood now, not in a request's latency
+        if (mfma)
+            for (int nh = EKG_NHOOD_2D4; nh <= EKG_NHOOD_3D8; ++nh) ensure_cells(h, nh);
     }
     s.hearts.push_back(&h);
     return (int)s.hearts.size() - 1;

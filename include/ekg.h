@@ -191,6 +191,21 @@ int ekg_heart_autotune(ekg_heart *h, const ekg_sim_desc *sample, int batch, ekg_
 int ekg_heart_set_tune(ekg_heart *h, const ekg_tune *t);
 int ekg_heart_get_tune(const ekg_heart *h, ekg_tune *t);
 
+/* Layout of the heart's matrix-core path as currently tuned (built on
+ * demand): chunks, activation classes and, for dipole neighbourhood `nhood`,
+ * the class cells whose lead weights each simulation evaluates once.  No
+ * counterpart in the reference (diagnostics of this library's layout). */
+typedef struct ekg_layout_info {
+    int64_t voxels;        /* myocardial voxels */
+    int64_t class_voxels;  /* of which in activation classes */
+    int64_t class_cells;   /* cells of the class chunks with a non-zero dipole vector */
+    int chunks;            /* chunks of the voxels outside classes */
+    int class_chunks;
+    int classes;
+    int pad;
+} ekg_layout_info;
+int ekg_heart_layout_info(ekg_heart *h, int nhood, ekg_layout_info *out);
+
 /* Graph mode: the kernel launches of a simulation group are captured
  * into a hipGraph on first use and replayed while the group keeps its
  * shape and buffers (a repeated batch with new coefficients or lead

@@ -68,6 +68,23 @@ def test_integer_heart_every_threshold(ctx, nhood):
     g.close()
 
 
+def test_layout_info(ctx):
+    h, g, act = _int_heart(ctx, 40, 9)
+    runs = _class_runs(h, act)
+    info = g.layout_info("3D4")
+    print(info)
+    assert info["voxels"] == int((h.layer > 0).sum())
+    assert info["class_voxels"] == int(runs[runs >= 64].sum())
+    assert info["classes"] == int((runs >= 64).sum())
+    assert 0 < info["class_cells"] <= 9 * info["class_voxels"]
+    assert info["class_chunks"] >= info["classes"] and info["chunks"] > 0
+    g.set_tune(class_min=-1)
+    off = g.layout_info("3D8")
+    assert off["classes"] == 0 and off["class_voxels"] == 0 and off["class_cells"] == 0
+    g.set_tune()
+    g.close()
+
+
 def test_six_layers_16_leads_short_step(ctx):
     h, g, act = _int_heart(ctx, 24, 5, n_layers=6)
     k, pos = _req(24, 16, 321, n_layers=6)
