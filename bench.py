@@ -85,10 +85,11 @@ FP32_PER_WINDOW_STEP = 14
 NAIVE_LANE_CYCLES, LANE_PEAK = 38, 256 * 64 * 2.4e9
 
 # HBM bytes per launch of the dominant kernel on this workload (chunk target
-# 128, 2 waves) from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled
-# by its reading for a 256 MiB calibration copy
-# (profiles/ecg_chunk_kernel_128cubed_batch256_hbm_traffic.md).
-TRAFFIC_BYTES = 5219316228
+# 256, 4 waves), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled
+# by its reading for a 256 MiB calibration copy: with activation classes
+# (profiles/ecg_class_chunk_kernel_128cubed_batch256_hbm_traffic.md) and
+# without (profiles/ecg_chunk_kernel_128cubed_batch256_hbm_traffic.md).
+TRAFFIC_BYTES = {"ecg_class_chunk_kernel": 121094073, "ecg_chunk_kernel": 1396124157}
 
 
 def log(*a):
@@ -428,7 +429,7 @@ def main():
             "peak": MFMA_FP64_PEAK / 1e12,
             "unit": "TFLOP/s",
             "frac": round(achieved / MFMA_FP64_PEAK, 5),
-            "traffic": TRAFFIC_BYTES,
+            "traffic": TRAFFIC_BYTES.get(dom),
             "algorithmic_flops_per_launch": flops,
             "fp64_flops_per_launch": f64,
             "fp32_flops_per_launch": f32,
