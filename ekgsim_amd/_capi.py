@@ -32,7 +32,8 @@ EXPORTS = (
     "ekg_server_poll", "ekg_server_wait", "ekg_server_stats", "ekg_simulate_batch_device",
     "ekg_ctx_synchronize", "ekg_ctx_set_kernel", "ekg_ap_value", "ekg_ap_num_genes", "ekg_layer_aps",
     "ekg_criterion", "ekg_resample", "ekg_heart_autotune", "ekg_heart_set_tune", "ekg_heart_get_tune",
-    "ekg_ctx_set_graphs", "ekg_ctx_graph_stats", "ekg_heart_layout_info",
+    "ekg_ctx_set_graphs", "ekg_ctx_graph_stats", "ekg_heart_layout_info", "ekg_criteria_device",
+    "ekg_simulate_criteria",
 )
 KERNELS = {"auto": 0, "valu": 1, "mfma": 2}
 
@@ -150,6 +151,8 @@ def lib():
     L.ekg_heart_set_tune.argtypes = [vp, C.POINTER(Tune)]
     L.ekg_heart_get_tune.argtypes = [vp, C.POINTER(Tune)]
     L.ekg_heart_layout_info.argtypes = [vp, C.c_int, C.POINTER(LayoutInfo)]
+    L.ekg_criteria_device.argtypes = [vp, C.POINTER(CritDesc), vp, C.c_int, C.c_int, C.c_int64, vp]
+    L.ekg_simulate_criteria.argtypes = [vp, C.c_int, C.POINTER(SimDesc), C.POINTER(CritDesc), vp]
     L.ekg_ctx_set_graphs.argtypes = [vp, C.c_int]
     L.ekg_ctx_graph_stats.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.ekg_resample.argtypes = [vp, C.c_int64, C.c_double, C.c_int, vp, C.c_int64, C.POINTER(C.c_int64)]
@@ -264,6 +267,30 @@ def resample(src, factor, start):
     return out
 
 
+class CritDesc(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int),
+        ("peaks", C.c_int),
+        ("raw", C.c_int),
+        ("n_targets", C.c_int),
+        ("target_len", C.c_int64),
+        ("targets", C.c_void_p),
+        ("target_offsets", C.c_void_p),
+    ]
+
+
+class _Crit:
+    """Keeps the target arrays behind one CritDesc alive."""
+
+    def __init__(self, mode, targets, offsets, peaks, raw):
+        self.t = np.ascontiguousarray(np.atleast_2d(targets), np.float64)
+        self.o = np.ascontiguousarray(offsets, np.float64).reshape(-1)
+        if self.o.size != self.t.shape[0]:
+            raise ValueError("one offset per target")
+        self.s = CritDesc(COMPARE.get(mode, mode), int(bool(peaks)), int(bool(raw)), self.t.shape[0],
+                          self.t.shape[1], self.t.ctypes.data, self.o.ctypes.data)
+
+
 class BatchDev(C.Structure):
     _fields_ = [
         ("batch", C.c_int),
@@ -327,6 +354,14 @@ class Context:
 
     def heart(self, layer, seed=None):
         return Heart(self, layer, seed)
+
+    def criteria_device(self, mode, targets, offsets, ecg_ptr, batch, n_leads, n_steps, out_ptr,
+                        peaks=False, raw=False):
+        """Criteria of device ECGs [batch][n_leads][n_steps] into a device
+        buffer [batch][n_crit (x2 with peaks)], on the context's stream."""
+        c = _Crit(mode, targets, offsets, peaks, raw)
+        check(lib().ekg_criteria_device(self.h, C.byref(c.s), C.c_void_p(ecg_ptr), int(batch), int(n_leads),
+                                        int(n_steps), C.c_void_p(out_ptr)))
 
     def set_graphs(self, enable: bool):
         """Replay repeated simulation groups from a captured hipGraph."""
@@ -416,6 +451,21 @@ class Heart:
         check(lib().ekg_simulate_batch(self.h, B, arr, _p(out)))
         return out
 
+
+    def simulate_criteria(self, layer_k, lead_pos, mode, targets, offsets, nhood="3D4", start=0.0,
+                          length=600.0, dt=1.0, peaks=False, raw=False):
+        """layer_k: (B, n_layers, 9); lead_pos: (B, L, 3) -> criteria (B, n_crit (x2 with peaks));
+        the ECGs stay on the device."""
+        layer_k = np.asarray(layer_k, np.float64)
+        lead_pos = np.asarray(lead_pos, np.float64)
+        B = layer_k.shape[0]
+        ds = [_Desc(layer_k[b], lead_pos[b], nhood, start, length, dt) for b in range(B)]
+        arr = (SimDesc * B)(*[d.s for d in ds])
+        c = _Crit(mode, targets, offsets, peaks, raw)
+        n_crit = min(lead_pos.shape[1], c.t.shape[0])
+        out = np.zeros((B, n_crit * (2 if peaks else 1)), np.float64)
+        check(lib().ekg_simulate_criteria(self.h, B, arr, C.byref(c.s), _p(out)))
+        return out
 
     def autotune(self, layer_k, lead_pos, nhood="3D4", start=0.0, length=600.0, dt=1.0, batch=64):
         """Time the launch-shape knobs on `batch` copies of one request and

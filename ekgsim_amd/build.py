@@ -29,7 +29,7 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-p
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HOST = ["-ffp-contract=off", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include")]
 
-KERNEL_SRC = ["ekg_ca.hip", "ekg_ecg.hip"]
+KERNEL_SRC = ["ekg_ca.hip", "ekg_ecg.hip", "ekg_crit.hip"]
 HOST_SRC = ["ekg_engine.cpp", "ekg_capi.cpp", "ekg_server.cpp", "ekg_model.cpp"]
 CLI_SRC = ["ini.cpp", "sim_lib.cpp", "ekgsim_app.cpp", "ekgsim_main.cpp"]
 

@@ -388,6 +388,24 @@ int ekg_heart_get_tune(const ekg_heart* h, ekg_tune* t)
     });
 }
 
+int ekg_criteria_device(ekg_ctx* ctx, const ekg_criteria_desc* c, const double* d_ecg, int batch, int n_leads,
+                        int64_t n_steps, double* d_out)
+{
+    return guard([&] {
+        if (!ctx || !c) ekg::fail(EKG_EINVAL, "NULL argument");
+        ekg::criteria_device(ctx->c, *c, d_ecg, batch, n_leads, n_steps, d_out);
+    });
+}
+
+int ekg_simulate_criteria(ekg_heart* h, int batch, const ekg_sim_desc* descs, const ekg_criteria_desc* c,
+                          double* criteria)
+{
+    return guard([&] {
+        if (!h || !c) ekg::fail(EKG_EINVAL, "NULL argument");
+        ekg::simulate_criteria(h->h, batch, descs, *c, criteria);
+    });
+}
+
 int ekg_heart_layout_info(ekg_heart* h, int nhood, ekg_layout_info* out)
 {
     return guard([&] {

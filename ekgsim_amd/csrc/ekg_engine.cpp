@@ -1018,6 +1018,78 @@ ekg_tune autotune(Heart& h, const ekg_sim_desc& sample, int batch)
     return best;
 }
 
+namespace {
+
+int crit_count(const ekg_criteria_desc& c, int n_leads)
+{
+    if (c.mode < EKG_CMP_RMS || c.mode > EKG_CMP_VECTOR_CORRELATION) fail(EKG_EINVAL, "unknown comparison mode");
+    if (c.n_targets < 1 || c.target_len < 1 || !c.targets || !c.target_offsets)
+        fail(EKG_EINVAL, "criteria need at least one non-empty target and its offset");
+    return std::min(n_leads, c.n_targets);
+}
+
+// Targets and offsets into the context's buffers on its stream.
+void upload_targets(Ctx& ctx, const ekg_criteria_desc& c)
+{
+    const size_t nt = (size_t)c.n_targets * (size_t)c.target_len;
+    ctx.crit_tg.reserve(nt * 8);
+    ctx.crit_off.reserve((size_t)c.n_targets * 8);
+    hip_check(hipMemcpyAsync(ctx.crit_tg.p, c.targets, nt * 8, hipMemcpyHostToDevice, ctx.stream), "upload targets");
+    hip_check(hipMemcpyAsync(ctx.crit_off.p, c.target_offsets, (size_t)c.n_targets * 8, hipMemcpyHostToDevice,
+                             ctx.stream),
+              "upload target offsets");
+}
+
+}  // namespace
+
+void criteria_device(Ctx& ctx, const ekg_criteria_desc& c, const double* d_ecg, int batch, int n_leads,
+                     int64_t n_steps, double* d_out)
+{
+    std::lock_guard<std::mutex> g(ctx.mu);
+    if (batch < 0 || n_leads < 1 || n_steps < 1) fail(EKG_EINVAL, "bad batch shape");
+    if (!d_ecg || !d_out) fail(EKG_EINVAL, "NULL device buffer");
+    const int n_crit = crit_count(c, n_leads);
+    if (batch == 0) return;
+    ctx.bind();
+    upload_targets(ctx, c);
+    hip_check(ekgk::launch_criteria(c.mode, c.peaks ? 1 : 0, c.raw ? 1 : 0, d_ecg, batch, n_leads, n_steps,
+                                    ctx.crit_tg.as<double>(), ctx.crit_off.as<double>(), n_crit, c.target_len, d_out,
+                                    ctx.stream),
+              "criteria launch");
+}
+
+void simulate_criteria(Heart& h, int batch, const ekg_sim_desc* descs, const ekg_criteria_desc& c,
+                       double* criteria)
+{
+    Ctx& ctx = *h.ctx;
+    std::lock_guard<std::mutex> g1(ctx.mu);
+    std::lock_guard<std::mutex> g2(h.mu);
+    if (!criteria) fail(EKG_EINVAL, "criteria output buffer is required");
+    if (batch < 1 || !descs) fail(EKG_EINVAL, "batch must hold at least one descriptor");
+    std::vector<ReqSpec> specs;
+    for (int b = 0; b < batch; ++b) specs.push_back(make_spec(h, descs[b], true));
+    const int L = specs[0].L;
+    const int64_t T = specs[0].T;
+    for (const ReqSpec& r : specs)
+        if (r.L != L || r.T != T) fail(EKG_EINVAL, "every request of a criteria batch needs the same leads and steps");
+    if (T < 1) fail(EKG_EINVAL, "no time steps");
+    const int n_crit = crit_count(c, L);
+    const size_t per = (size_t)n_crit * (c.peaks ? 2 : 1);
+    ctx.bind();
+    GroupPlan g = plan_group(ctx, std::move(specs));
+    enqueue_group(ctx, g, ctx.ws, ctx.stream, nullptr, nullptr, nullptr, ctx.profiling);
+    upload_targets(ctx, c);
+    ctx.crit_out.reserve((size_t)batch * per * 8);
+    hip_check(ekgk::launch_criteria(c.mode, c.peaks ? 1 : 0, c.raw ? 1 : 0, ctx.ws.ecg.as<double>(), batch, L, T,
+                                    ctx.crit_tg.as<double>(), ctx.crit_off.as<double>(), n_crit, c.target_len,
+                                    ctx.crit_out.as<double>(), ctx.stream),
+              "criteria launch");
+    hip_check(hipMemcpyAsync(criteria, ctx.crit_out.p, (size_t)batch * per * 8, hipMemcpyDeviceToHost, ctx.stream),
+              "download criteria");
+    hip_check(hipStreamSynchronize(ctx.stream), "hipStreamSynchronize");
+    if (ctx.profiling) ctx.collect_timings();
+}
+
 void excite(Heart& h, bool is_int, const double* transfer, int rows, int cols, const uint8_t* delay)
 {
     Ctx& ctx = *h.ctx;

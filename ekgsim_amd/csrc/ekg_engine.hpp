@@ -62,6 +62,7 @@ struct Ctx {
     int64_t fused_launches = 0;
     double fused_ms = 0.0;
     DevBuf nb_table;  // [4][26][4] int8 {d0, d1, d2, cube bit} of the four dipole neighbourhoods
+    DevBuf crit_tg, crit_off, crit_out;  // criteria: targets, offsets, output of ekg_simulate_criteria
     std::mutex mu;    // serialises ABI calls on this context
     void bind() const;
     hipEvent_t get_event();
@@ -163,6 +164,11 @@ void simulate(Heart& h, int batch, const ekg_sim_desc* descs, double* ecg);
 // and the lead-field phase width.
 ekg_tune autotune(Heart& h, const ekg_sim_desc& sample, int batch);
 void simulate_device(Heart& h, const ekg_batch_dev& d);
+// Criteria of device ECGs on the context's stream (targets uploaded from the host).
+void criteria_device(Ctx& ctx, const ekg_criteria_desc& c, const double* d_ecg, int batch, int n_leads,
+                     int64_t n_steps, double* d_out);
+void simulate_criteria(Heart& h, int batch, const ekg_sim_desc* descs, const ekg_criteria_desc& c,
+                       double* criteria);
 
 void excite(Heart& h, bool is_int, const double* transfer, int rows, int cols,
             const uint8_t* delay);

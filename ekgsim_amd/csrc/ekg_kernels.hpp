@@ -148,4 +148,10 @@ hipError_t launch_class_chunks(const GroupArgs& a, hipStream_t s);
 hipError_t launch_class_sums(const GroupArgs& a, hipStream_t s);
 hipError_t launch_ecg_class(const GroupArgs& a, hipStream_t s);
 
+// ---- criteria -----------------------------------------------------------------
+
+hipError_t launch_criteria(int mode, int peaks, int raw, const double* ecg, int B, int L, int64_t T,
+                           const double* targets, const double* offsets, int n_crit, int64_t target_len,
+                           double* out, hipStream_t s);
+
 }  // namespace ekgk

@@ -300,7 +300,7 @@ void Evaluator::prepare(const std::vector<double>& x, Prepared& p)
 }
 
 void Evaluator::fitness(const Prepared& p, const std::vector<std::vector<double>>* ecg,
-                        std::vector<double>& result) const
+                        std::vector<double>& result, const double* crit) const
 {
     // SimImplementation::calculateFitness (sim.cpp:600-702)
     if (settings_.criteriaMode != SimSettings::mode_every_lead_is_criterium)
@@ -311,15 +311,25 @@ void Evaluator::fitness(const Prepared& p, const std::vector<std::vector<double>
     result[0] = 0;
     const size_t nm = std::min(sim_->numMeasurements(), targets_.size());
     if (settings_.fastApproxIsCriterion) result[nm] = p.approxCriteria;
+    const bool dev = p.simulate && crit != nullptr;
     for (size_t i = 0; i < nm; ++i) {
-        const std::vector<double>& sim = p.simulate ? (*ecg)[i] : p.approx;
-        const double value = p.simulate ? compare(sim, i) : p.approxCriteria;
+        const std::vector<double>& sim = p.simulate && !dev ? (*ecg)[i] : p.approx;
         size_t ci = i;
-        if (settings_.peakPositionIsCriterion) {
-            ci = 2 * i;
-            const long sp = (long)(std::max_element(sim.begin(), sim.end()) - sim.begin());
-            const long tp = (long)(std::max_element(targets_[i].begin(), targets_[i].end()) - targets_[i].begin());
-            result[ci + 1] = (double)std::labs(sp - tp);
+        double value;
+        if (dev) {
+            // the device's values and peak distances (ekg_simulate_criteria)
+            ci = settings_.peakPositionIsCriterion ? 2 * i : i;
+            value = crit[ci];
+            if (settings_.peakPositionIsCriterion) result[ci + 1] = crit[ci + 1];
+        } else {
+            value = p.simulate ? compare(sim, i) : p.approxCriteria;
+            if (settings_.peakPositionIsCriterion) {
+                ci = 2 * i;
+                const long sp = (long)(std::max_element(sim.begin(), sim.end()) - sim.begin());
+                const long tp =
+                    (long)(std::max_element(targets_[i].begin(), targets_[i].end()) - targets_[i].begin());
+                result[ci + 1] = (double)std::labs(sp - tp);
+            }
         }
         result[ci] = value;
         if (result[ci] < 0 || !std::isfinite(result[ci]) || !std::isnormal(result[ci])) {
@@ -421,8 +431,16 @@ void Evaluator::evalBatch(const std::vector<std::vector<double>>& xs, std::vecto
     const size_t L = sim_->numMeasurements(), nl = sim_->requiredAps();
     const int64_t T = ekg_num_steps(ss.simulationLength, ss.simulationTimeStep);
     std::vector<std::vector<std::vector<double>>> ecg(B);
+    // Criteria on the device when every target has the same length: only
+    // the per-lead values and peak distances come back, not the ECGs.
+    bool same_len = !targets_.empty();
+    for (const auto& t : targets_) same_len = same_len && t.size() == targets_[0].size() && !t.empty();
+    const size_t nm = std::min(L, targets_.size());
+    const size_t per = nm * (settings_.peakPositionIsCriterion ? 2 : 1);
+    std::vector<double> crit;
+    std::vector<const double*> crit_of(B, nullptr);
     if (!run.empty()) {
-        std::vector<double> k(run.size() * nl * 9), pos(run.size() * L * 3), out(run.size() * L * (size_t)T);
+        std::vector<double> k(run.size() * nl * 9), pos(run.size() * L * 3);
         std::vector<ekg_sim_desc> descs(run.size());
         for (size_t r = 0; r < run.size(); ++r) {
             const Prepared& p = prep[run[r]];
@@ -439,17 +457,35 @@ void Evaluator::evalBatch(const std::vector<std::vector<double>>& xs, std::vecto
             d.length = ss.simulationLength;
             d.dt = ss.simulationTimeStep;
         }
-        if (ekg_simulate_batch(sim_->heart(), (int)run.size(), descs.data(), out.data()) != EKG_OK)
-            throw std::runtime_error(ekg_last_error());
-        for (size_t r = 0; r < run.size(); ++r) {
-            auto& e = ecg[run[r]];
-            e.resize(L);
-            for (size_t i = 0; i < L; ++i)
-                e[i].assign(out.begin() + (r * L + i) * (size_t)T, out.begin() + (r * L + i + 1) * (size_t)T);
+        if (same_len && nm > 0 && T > 0) {
+            std::vector<double> tflat;
+            for (const auto& t : targets_) tflat.insert(tflat.end(), t.begin(), t.end());
+            ekg_criteria_desc c{};
+            c.mode = (int)settings_.comparisonMode;
+            c.peaks = settings_.peakPositionIsCriterion ? 1 : 0;
+            c.raw = 1;  // the correction and its warning follow in fitness()
+            c.n_targets = (int)targets_.size();
+            c.target_len = (int64_t)targets_[0].size();
+            c.targets = tflat.data();
+            c.target_offsets = target_offsets_.data();
+            crit.assign(run.size() * per, 0.0);
+            if (ekg_simulate_criteria(sim_->heart(), (int)run.size(), descs.data(), &c, crit.data()) != EKG_OK)
+                throw std::runtime_error(ekg_last_error());
+            for (size_t r = 0; r < run.size(); ++r) crit_of[run[r]] = crit.data() + r * per;
+        } else {
+            std::vector<double> out(run.size() * L * (size_t)T);
+            if (ekg_simulate_batch(sim_->heart(), (int)run.size(), descs.data(), out.data()) != EKG_OK)
+                throw std::runtime_error(ekg_last_error());
+            for (size_t r = 0; r < run.size(); ++r) {
+                auto& e = ecg[run[r]];
+                e.resize(L);
+                for (size_t i = 0; i < L; ++i)
+                    e[i].assign(out.begin() + (r * L + i) * (size_t)T, out.begin() + (r * L + i + 1) * (size_t)T);
+            }
         }
     }
     for (size_t b = 0; b < B; ++b) {
-        fitness(prep[b], &ecg[b], results[b]);
+        fitness(prep[b], &ecg[b], results[b], crit_of[b]);
         violations[b] = prep[b].violation;
     }
     evaluations_ += B;

@@ -105,7 +105,10 @@ private:
     void loadTargets(const char* fname);
     void prepare(const std::vector<double>& solution, Prepared& p);
     double compare(const std::vector<double>& a, size_t target) const;
-    void fitness(const Prepared& p, const std::vector<std::vector<double>>* ecg, std::vector<double>& result) const;
+    // crit: per-lead values (and peak distances) computed on the device for a
+    // simulated solution, in ekg_criteria_desc order (raw values); null: from `ecg`.
+    void fitness(const Prepared& p, const std::vector<std::vector<double>>* ecg, std::vector<double>& result,
+                 const double* crit = nullptr) const;
     void writeOutputs(const std::vector<double>& solution, const Prepared& p);
 
     SimSettings settings_;

@@ -270,6 +270,35 @@ int ekg_layer_aps(const ekg_ap_model *m, int n_layers, const double *genes, int 
 int ekg_criterion(int mode, const double *sim, int64_t n_sim, const double *target, int64_t n_target,
                   double target_offset, double *value);
 
+/* ---- comparison on the device, after the ECG reduction ------------------ */
+/* The per-lead part of SimImplementation::calculateFitness (sim.cpp:600-702)
+ * for a batch whose ECGs are on the device: for request r and lead
+ * i < n_crit = min(n_leads, n_targets), the ekg_criterion value of the lead
+ * against target i (bit for bit the host's), then with `peaks` the distance
+ * |argmax(sim) - argmax(target)| of the first maxima (sim.cpp:662-670):
+ * out[r][i] or out[r][2i], out[r][2i+1].  Values that are negative, zero,
+ * subnormal or not finite become 2e10 (sim.cpp:679-685) unless `raw`. */
+typedef struct ekg_criteria_desc {
+    int mode;                      /* EKG_CMP_* */
+    int peaks;                     /* 1: peak distance after each value ("peak position is criterion") */
+    int raw;                       /* 1: values as computed, without the 2e10 correction */
+    int n_targets;
+    int64_t target_len;            /* samples per target */
+    const double *targets;         /* host [n_targets][target_len], prepared as loadTargets does */
+    const double *target_offsets;  /* host [n_targets]: EKG_CMP_DEV_FROM_LINEAR offsets */
+} ekg_criteria_desc;
+/* d_ecg: device [batch][n_leads][n_steps]; d_out: device [batch][n_crit (x2 with peaks)];
+ * runs on the context's stream. */
+int ekg_criteria_device(ekg_ctx *ctx, const ekg_criteria_desc *c, const double *d_ecg, int batch, int n_leads,
+                        int64_t n_steps, double *d_out);
+/* Simulate a batch (one launch group, as ekg_simulate_batch; every request
+ * with the same lead count and steps) and return only its criteria (host
+ * [batch][n_crit (x2 with peaks)]): the ECGs stay on the device.  Replaces
+ * the simulate-then-compare sequence of SimImplementation::eval
+ * (sim.cpp:443-491, 600-702) for batched evaluations. */
+int ekg_simulate_criteria(ekg_heart *h, int batch, const ekg_sim_desc *descs, const ekg_criteria_desc *c,
+                          double *criteria);
+
 /* Target resampling of SimImplementation::loadTargets (sim.cpp:1013-1022,
  * vectorMath.h:209-227): dest[i] = linear interpolation of src at
  * i / factor + start for 1 + ceil((n - 1 - start) * factor) samples; a factor
