@@ -321,6 +321,20 @@ def main():
         ref_win_err = math.sqrt(num / den)
         log(f"rel-L2 vs FP64 oracle {errs}; vs {cpu['kind']} on windows {ref_win_err:.2e}")
 
+    # ---- criteria on the device for the whole batch (correlation, peaks) ----
+    tg = ecg_gpu[0] / (ecg_gpu[0].max(axis=1, keepdims=True) - ecg_gpu[0].min(axis=1, keepdims=True))
+    offs = 1.0 - tg.min(axis=1) / (tg.max(axis=1) - tg.min(axis=1))
+    d_c = torch.empty((B, 2 * L), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    ctx.criteria_device("correlation", tg, offs, d_e.data_ptr(), B, L, T, d_c.data_ptr(), peaks=True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        ctx.criteria_device("correlation", tg, offs, d_e.data_ptr(), B, L, T, d_c.data_ptr(), peaks=True)
+    ctx.synchronize()
+    crit_ms = (time.perf_counter() - t0) * 1e3 / 10
+    log(f"criteria on the device: {crit_ms:.3f} ms per batch of {B} x {L} leads")
+
     # ---- the same batch with activation classes off: every voxel's AP ----
     general = None
     if not args.no_general and args.kernel != "valu" and n_classes > 0:
@@ -422,6 +436,7 @@ def main():
         "rel_l2_max_vs_fp64": rel_max,
         "rel_l2_vs_reference_windows": ref_win_err,
         "excitation_ms": round(ca_ms, 2),
+        "criteria_ms_per_batch": round(crit_ms, 4),
         "roofline": {
             "bound": "mfma",
             "kernel": dom,

@@ -1028,16 +1028,33 @@ int crit_count(const ekg_criteria_desc& c, int n_leads)
     return std::min(n_leads, c.n_targets);
 }
 
-// Targets and offsets into the context's buffers on its stream.
-void upload_targets(Ctx& ctx, const ekg_criteria_desc& c)
+// Targets and offsets into the context's buffers on its stream, and their
+// statistics over the comparison length for ECGs of n_steps samples -- only
+// when these differ from the last call (an optimiser keeps its targets).
+void prepare_targets(Ctx& ctx, const ekg_criteria_desc& c, int64_t n_steps)
 {
     const size_t nt = (size_t)c.n_targets * (size_t)c.target_len;
+    const int64_t len = std::min<int64_t>(n_steps, c.target_len);
+    std::vector<double> h(c.targets, c.targets + nt);
+    h.insert(h.end(), c.target_offsets, c.target_offsets + c.n_targets);
+    h.push_back((double)c.n_targets);
+    h.push_back((double)len);
+    void* b0 = ctx.crit_tg.p;
+    void* b1 = ctx.crit_off.p;
+    void* b2 = ctx.crit_stats.p;
     ctx.crit_tg.reserve(nt * 8);
     ctx.crit_off.reserve((size_t)c.n_targets * 8);
+    ctx.crit_stats.reserve((size_t)c.n_targets * 4 * 8);
+    if (ctx.crit_tg.p != b0 || ctx.crit_off.p != b1 || ctx.crit_stats.p != b2) ctx.crit_h.clear();
+    if (h == ctx.crit_h) return;
     hip_check(hipMemcpyAsync(ctx.crit_tg.p, c.targets, nt * 8, hipMemcpyHostToDevice, ctx.stream), "upload targets");
     hip_check(hipMemcpyAsync(ctx.crit_off.p, c.target_offsets, (size_t)c.n_targets * 8, hipMemcpyHostToDevice,
                              ctx.stream),
               "upload target offsets");
+    hip_check(ekgk::launch_target_stats(ctx.crit_tg.as<double>(), c.n_targets, c.target_len, len,
+                                        ctx.crit_stats.as<double>(), ctx.stream),
+              "target statistics launch");
+    ctx.crit_h.swap(h);
 }
 
 }  // namespace
@@ -1051,10 +1068,10 @@ void criteria_device(Ctx& ctx, const ekg_criteria_desc& c, const double* d_ecg, 
     const int n_crit = crit_count(c, n_leads);
     if (batch == 0) return;
     ctx.bind();
-    upload_targets(ctx, c);
+    prepare_targets(ctx, c, n_steps);
     hip_check(ekgk::launch_criteria(c.mode, c.peaks ? 1 : 0, c.raw ? 1 : 0, d_ecg, batch, n_leads, n_steps,
-                                    ctx.crit_tg.as<double>(), ctx.crit_off.as<double>(), n_crit, c.target_len, d_out,
-                                    ctx.stream),
+                                    ctx.crit_tg.as<double>(), ctx.crit_off.as<double>(), ctx.crit_stats.as<double>(),
+                                    n_crit, c.target_len, d_out, ctx.stream),
               "criteria launch");
 }
 
@@ -1078,11 +1095,11 @@ void simulate_criteria(Heart& h, int batch, const ekg_sim_desc* descs, const ekg
     ctx.bind();
     GroupPlan g = plan_group(ctx, std::move(specs));
     enqueue_group(ctx, g, ctx.ws, ctx.stream, nullptr, nullptr, nullptr, ctx.profiling);
-    upload_targets(ctx, c);
+    prepare_targets(ctx, c, T);
     ctx.crit_out.reserve((size_t)batch * per * 8);
     hip_check(ekgk::launch_criteria(c.mode, c.peaks ? 1 : 0, c.raw ? 1 : 0, ctx.ws.ecg.as<double>(), batch, L, T,
-                                    ctx.crit_tg.as<double>(), ctx.crit_off.as<double>(), n_crit, c.target_len,
-                                    ctx.crit_out.as<double>(), ctx.stream),
+                                    ctx.crit_tg.as<double>(), ctx.crit_off.as<double>(), ctx.crit_stats.as<double>(),
+                                    n_crit, c.target_len, ctx.crit_out.as<double>(), ctx.stream),
               "criteria launch");
     hip_check(hipMemcpyAsync(criteria, ctx.crit_out.p, (size_t)batch * per * 8, hipMemcpyDeviceToHost, ctx.stream),
               "download criteria");

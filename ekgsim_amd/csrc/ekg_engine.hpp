@@ -62,7 +62,8 @@ struct Ctx {
     int64_t fused_launches = 0;
     double fused_ms = 0.0;
     DevBuf nb_table;  // [4][26][4] int8 {d0, d1, d2, cube bit} of the four dipole neighbourhoods
-    DevBuf crit_tg, crit_off, crit_out;  // criteria: targets, offsets, output of ekg_simulate_criteria
+    DevBuf crit_tg, crit_off, crit_stats, crit_out;  // criteria: targets, offsets, target statistics, output
+    std::vector<double> crit_h;          // targets, offsets and comparison length as last prepared
     std::mutex mu;    // serialises ABI calls on this context
     void bind() const;
     hipEvent_t get_event();

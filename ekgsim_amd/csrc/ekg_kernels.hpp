@@ -150,8 +150,11 @@ hipError_t launch_ecg_class(const GroupArgs& a, hipStream_t s);
 
 // ---- criteria -----------------------------------------------------------------
 
+// stats: per target {mean, variance, b . b, first maximum} over the first len samples
+hipError_t launch_target_stats(const double* targets, int n_targets, int64_t target_len, int64_t len, double* stats,
+                               hipStream_t s);
 hipError_t launch_criteria(int mode, int peaks, int raw, const double* ecg, int B, int L, int64_t T,
-                           const double* targets, const double* offsets, int n_crit, int64_t target_len,
-                           double* out, hipStream_t s);
+                           const double* targets, const double* offsets, const double* stats, int n_crit,
+                           int64_t target_len, double* out, hipStream_t s);
 
 }  // namespace ekgk

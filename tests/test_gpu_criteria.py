@@ -10,7 +10,9 @@ contraction, so its values must equal the host's ekg_criterion bit for bit:
 * on simulated batches (ekg_simulate_criteria, ECGs kept on the device),
   with targets longer and shorter than the ECG;
 * on device ECGs given directly (ekg_criteria_device), including constant
-  leads whose correlation is 0/0 (2e10, or NaN when raw output is asked for).
+  leads whose correlation is 0/0 (2e10, or NaN when raw output is asked for),
+  on short and long rows, and with the cached target statistics reused
+  across calls and recomputed when the targets or the ECG length change.
 """
 import numpy as np
 import pytest
@@ -75,11 +77,12 @@ def test_simulate_criteria_equals_host(ctx, tlen):
     g.close()
 
 
-def test_device_criteria_degenerate_leads(ctx):
+@pytest.mark.parametrize("T", [200, 6000])
+def test_device_criteria_degenerate_leads(ctx, T):
     import torch
     import ekgsim_amd as E
     rng = np.random.default_rng(5)
-    B, L, T = 3, 3, 200
+    B, L = 3, 3
     ecg = rng.standard_normal((B, L, T))
     ecg[1, 0, :] = 0.25            # constant: correlation 0/0, RMS still finite
     ecg[2, 2, :] = ecg[2, 2, 0]    # constant
