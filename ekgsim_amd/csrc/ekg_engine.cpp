@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <unordered_map>
 
 namespace ekg {
 
@@ -492,7 +493,11 @@ void ensure_cells(Heart& h, int nhood)
     int8_t dif[26][3];
     const int nd = nhood_offsets(nhood, dif);
     // cell -> slot of the current chunk (-1: none); reset after each chunk
-    std::vector<int> slot((size_t)h.nvox, -1);
+    // cell -> slot: a dense array over the grid, or a hash map for grids
+    // above 2^27 cells (the array would exceed 512 MB)
+    const bool dense = h.nvox <= ((int64_t)1 << 27);
+    std::vector<int> slot(dense ? (size_t)h.nvox : 0, -1);
+    std::unordered_map<int64_t, int> slot_map;
     std::vector<int64_t> touched;
     std::vector<int> acc;  // D per slot, three ints
     std::vector<uint2> cells;
@@ -500,9 +505,14 @@ void ensure_cells(Heart& h, int nhood)
     auto bit = [](int d0, int d1, int d2) { return (d0 + 1) * 9 + (d1 + 1) * 3 + (d2 + 1); };
     auto add = [&](int z, int y, int x, int d0, int d1, int d2) {
         const int64_t gi = ((int64_t)z * h.n1 + y) * h.n2 + x;
-        int sl = slot[gi];
-        if (sl < 0) {
-            sl = slot[gi] = (int)touched.size();
+        int sl;
+        if (dense) {
+            sl = slot[gi];
+            if (sl < 0) sl = slot[gi] = (int)touched.size();
+        } else {
+            sl = slot_map.emplace(gi, (int)touched.size()).first->second;
+        }
+        if (sl == (int)touched.size()) {
             touched.push_back(gi);
             acc.insert(acc.end(), {0, 0, 0});
         }
@@ -528,9 +538,10 @@ void ensure_cells(Heart& h, int nhood)
             add(z, y, x, s0, s1, s2);
         }
         // cells in first-touch order (the voxel order of the chunk)
+        slot_map.clear();
         for (size_t sl = 0; sl < touched.size(); ++sl) {
             const int64_t gi = touched[sl];
-            slot[gi] = -1;
+            if (dense) slot[gi] = -1;
             const int D0 = acc[3 * sl], D1 = acc[3 * sl + 1], D2 = acc[3 * sl + 2];
             if (!(D0 || D1 || D2)) continue;
             if (std::abs(D0) > 127 || std::abs(D1) > 127 || std::abs(D2) > 127)

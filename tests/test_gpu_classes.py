@@ -9,7 +9,8 @@ rel-L2 1e-5 of the FP64 oracle whatever the threshold:
   thousands of voxels), with classes at the default threshold, with every
   voxel in a class (class_min 1: layers with no chunk outside the classes),
   with a low threshold that leaves small runs outside, and with classes off;
-  on both neighbourhoods, 16 leads and six layers;
+  on both 3-D neighbourhoods, 16 leads and six layers, and on a one-plane
+  heart with both 2-D neighbourhoods;
 * a heart from the FP64 transfer-matrix excitation, whose times are mostly
   distinct, gives the same result with classes on or off;
 * a request gives the same bits alone, in a uniform batch and in a server
@@ -65,6 +66,26 @@ def test_integer_heart_every_threshold(ctx, nhood):
     assert g.get_tune()["class_min"] == -1
     g.set_tune()
     assert g.get_tune()["class_min"] == 0
+    g.close()
+
+
+@pytest.mark.parametrize("nhood", ["2D4", "2D8"])
+def test_two_dimensional_heart(ctx, nhood):
+    from ekgsim_amd import Heart
+    h = S.make_heart(64, seed=2, dims=(1, 64, 64))
+    if h.seed.sum() == 0:
+        h.seed.ravel()[np.flatnonzero(h.layer.ravel())[0]] = 1
+    g = Heart(ctx, h.layer, h.seed)
+    g.excite_delay(h.delay)
+    act = O.excite_int(h.layer, h.seed, h.delay).astype(np.float64)
+    k, pos = _req(64, 4, 77)
+    ref = O.run(h.layer, act, k, pos, nhood, 0.0, 300.0, 1.0)
+    for cm in (0, 1, -1):
+        g.set_tune(class_min=cm)
+        err = rel_l2(g.simulate(k, pos, nhood, 0.0, 300.0, 1.0), ref)
+        print(f"2-D {nhood} class_min {cm}: rel-L2 {err:.3e}")
+        assert err <= TOL
+    g.set_tune()
     g.close()
 
 
