@@ -1094,6 +1094,7 @@ __global__ void ecg_reduce_kernel(const HeartDev* __restrict__ hearts, const Req
 // ---------------------------------------------------------------------------
 
 constexpr int CC_THREADS = 256;
+constexpr int CC_CELLS = 2;  // cells per thread and iteration of the class-chunk kernel
 
 // Class chunks: one block per (request, class chunk).  Every voxel of a class
 // has the same AP, so only the sum of the chunk's G rows is needed, and by
@@ -1105,7 +1106,7 @@ constexpr int CC_THREADS = 256;
 // e(w) . D(w) is exact in FP64 (FP32 components times small integers), so a
 // cell's contributions here and in chunk-kernel G rows cancel across every
 // dipole as they do between G rows.  Each thread adds its cells t, t + CC_THREADS, ...
-// (two at a time) in registers in that order, the lanes of a wave are added
+// (CC_CELLS at a time) in registers in that order, the lanes of a wave are added
 // by an xor butterfly and the waves in order, and sums[(c_off + n_chunks +
 // k)*3*LP] receives {sum G, P' sum G, Q' sum G} with the class's
 // P' = k2(1-k3) e^{(k4+k5)(a - t0)} and Q' = k2 k3 e^{k5 (a - t0)}.
@@ -1148,43 +1149,38 @@ __global__ __launch_bounds__(CC_THREADS) void ecg_class_chunk_kernel(
 #pragma unroll
     for (int i = 0; i < LP; ++i) acc[i] = 0.0;
     const int cb = H.ccell_off[k], ce = H.ccell_off[k + 1];
-    for (int q = cb + t; q < ce; q += 2 * CC_THREADS) {
+    for (int q = cb + t; q < ce; q += CC_CELLS * CC_THREADS) {
         // reload the electrodes from LDS in every iteration rather than keep
         // all of them in registers (which would halve the waves per SIMD)
         asm volatile("" ::: "memory");
-        const bool two = q + CC_THREADS < ce;
-        const uint2 c0 = H.ccells[q];
-        const uint2 c1 = two ? H.ccells[q + CC_THREADS] : c0;
-        // packed coordinates z | y << 10 | x << 20: component c is axis c
-        float f0[3], f1[3];
-        double d0[3], d1[3];
+        // packed coordinates z | y << 10 | x << 20: component c is axis c;
+        // cells past the chunk's end repeat the first with D = 0
+        float f[CC_CELLS][3];
+        double d[CC_CELLS][3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            f0[c] = (float)((int)((c0.x >> (10 * c)) & 1023u) + 1);
-            f1[c] = (float)((int)((c1.x >> (10 * c)) & 1023u) + 1);
-            d0[c] = (double)(int)(int8_t)((c0.y >> (8 * c)) & 255u);
-            d1[c] = two ? (double)(int)(int8_t)((c1.y >> (8 * c)) & 255u) : 0.0;
+        for (int u = 0; u < CC_CELLS; ++u) {
+            const int qu = q + u * CC_THREADS;
+            const bool ok = qu < ce;
+            const uint2 cu = H.ccells[ok ? qu : q];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                f[u][c] = (float)((int)((cu.x >> (10 * c)) & 1023u) + 1);
+                d[u][c] = ok ? (double)(int)(int8_t)((cu.y >> (8 * c)) & 255u) : 0.0;
+            }
         }
 #pragma unroll
         for (int i = 0; i < LP; ++i) {
             const float4 ph = s_ph[i], pl = s_pl[i];
-            {
-                const float e0 = (ph.x - f0[0]) + pl.x, e1 = (ph.y - f0[1]) + pl.y, e2 = (ph.z - f0[2]) + pl.z;
+#pragma unroll
+            for (int u = 0; u < CC_CELLS; ++u) {
+                const float e0 = (ph.x - f[u][0]) + pl.x, e1 = (ph.y - f[u][1]) + pl.y, e2 = (ph.z - f[u][2]) + pl.z;
                 float l = e0 * e0;
                 l = __builtin_fmaf(e1, e1, l);
                 l = __builtin_fmaf(e2, e2, l);
                 const float rr = __builtin_amdgcn_rsqf(l);
                 // D . e exactly: FP32 components times small integers
-                const double eD = __builtin_fma(d0[2], (double)e2, __builtin_fma(d0[1], (double)e1, d0[0] * (double)e0));
-                acc[i] = __builtin_fma((double)(rr * rr * rr), eD, acc[i]);
-            }
-            {
-                const float e0 = (ph.x - f1[0]) + pl.x, e1 = (ph.y - f1[1]) + pl.y, e2 = (ph.z - f1[2]) + pl.z;
-                float l = e0 * e0;
-                l = __builtin_fmaf(e1, e1, l);
-                l = __builtin_fmaf(e2, e2, l);
-                const float rr = __builtin_amdgcn_rsqf(l);
-                const double eD = __builtin_fma(d1[2], (double)e2, __builtin_fma(d1[1], (double)e1, d1[0] * (double)e0));
+                const double eD =
+                    __builtin_fma(d[u][2], (double)e2, __builtin_fma(d[u][1], (double)e1, d[u][0] * (double)e0));
                 acc[i] = __builtin_fma((double)(rr * rr * rr), eD, acc[i]);
             }
         }
