@@ -677,13 +677,16 @@ constexpr int CH_LEADS = 2;   // leads per pass of the chunk kernel's lead field
 // neighbours in the dipole neighbourhood N, CH_LEADS leads per pass:
 //   G_i(u) = s(u) (e(u) . s_u) + sum_{d in N, u+d myo} s(u+d) (e(u+d) . d),
 // s_u the sum of those d.  A cell's weight m = e s is a function of the cell
-// alone: e = (p_hi - c) + p_lo and s = |e|^-3 (rsq cubed) in FP32, then exact
-// FP64 products, so its rounding cancels between the two ends of each dipole
-// and only V differences across the dipole see it (a weight derived per
-// evaluating voxel, e.g. from e(u) - d, does not cancel: ~1e-5).
+// alone: s = |e|^-3 (rsq cubed) from e = (p_hi - c) + p_lo in FP32, and e in
+// the dipole products is the FP64 offset p - c, so the weight's rounding
+// cancels between the two ends of each dipole (and with the class sums, which
+// use the same s and offsets) and only V differences across the dipole see it
+// (a weight derived per evaluating voxel, e.g. from e(u) - d in FP32, does not
+// cancel: ~1e-5).
 template <int NH>
 struct VoxelField {
     float fc[3];          // grid coordinate + 1 per axis (the reference's voxel centre)
+    double cd[3][3];       // coordinate + 1 + (k - 1) per axis and offset k, in FP64
     double sd0, sd1, sd2;  // s_u
     uint32_t mk;           // 3x3x3 myocardium mask
 
@@ -706,11 +709,16 @@ struct VoxelField {
         fc[0] = (float)(gz + 1);
         fc[1] = (float)(gy + 1);
         fc[2] = (float)(gx + 1);
+#pragma unroll
+        for (int c3 = 0; c3 < 3; ++c3)
+#pragma unroll
+            for (int k3 = 0; k3 < 3; ++k3) cd[c3][k3] = (double)(fc[c3] + (float)(k3 - 1));
     }
 
     // g[w] = G_{i0+w}(u); leads at or past L give 0.  s_p[i][0|1][c]: the
-    // electrode's FP32 hi/lo per component.
-    __device__ __forceinline__ void leads(int i0, int L, const float (*s_p)[2][3], double g[CH_LEADS]) const
+    // electrode's FP32 hi/lo per component; s_pd[i][c]: the same in FP64.
+    __device__ __forceinline__ void leads(int i0, int L, const float (*s_p)[2][3], const double (*s_pd)[3],
+                                          double g[CH_LEADS]) const
     {
         using N = Nhood<NH>;
 #pragma unroll
@@ -733,7 +741,7 @@ struct VoxelField {
 #pragma unroll
                 for (int k3 = 0; k3 < 3; ++k3) {
                     E[w2][c3][k3] = (ph - (fc[c3] + (float)(k3 - 1))) + pl;
-                    E64[w2][c3][k3] = (double)E[w2][c3][k3];
+                    E64[w2][c3][k3] = s_pd[ii][c3] - cd[c3][k3];
                 }
             }
 #pragma unroll
@@ -756,7 +764,7 @@ struct VoxelField {
                     float l = __builtin_fmaf(E[w2][1][k1], E[w2][1][k1], SQ0[w2][k0]);
                     l = __builtin_fmaf(E[w2][2][k2], E[w2][2][k2], l);
                     const float r = __builtin_amdgcn_rsqf(l);
-                    // d . e exactly in FP64 (d has entries -1, 0, 1)
+                    // d . e in FP64 (d has entries -1, 0, 1)
                     double de = 0.0;
                     if (N::d[qn][0]) de += E64[w2][0][k0] * N::d[qn][0];
                     if (N::d[qn][1]) de += E64[w2][1][k1] * N::d[qn][1];
@@ -811,6 +819,7 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
     __shared__ double2 s_pq[NMAX];          // {P', Q'} relative to the layer's t0
     __shared__ double s_part[CHUNK_SEG][3 * LP];
     __shared__ float s_p[16][2][3];         // electrode {hi, lo} per component
+    __shared__ double s_pd[16][3];          // electrode in FP64
     __shared__ int s_tb[2];
     const int t = threadIdx.x, lane = t & 63;
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -844,6 +853,7 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
         const float hi = (float)p;
         s_p[t / 3][0][t % 3] = hi;
         s_p[t / 3][1][t % 3] = (float)(p - (double)hi);
+        s_pd[t / 3][t % 3] = p;
     }
     // operands of the bound-tile tests (16-step sub-tiles), loaded now and
     // used after the G rows: one probe per lane when there are at most 64
@@ -868,7 +878,7 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
 #pragma unroll 1
         for (int i0 = 0; i0 < LP; i0 += CH_LEADS) {
             double g[CH_LEADS];
-            vf.leads(i0, R.L, s_p, g);
+            vf.leads(i0, R.L, s_p, s_pd, g);
 #pragma unroll
             for (int w2 = 0; w2 < CH_LEADS; ++w2) s_G[q * LP + i0 + w2] = g[w2];
         }
@@ -1094,30 +1104,38 @@ __global__ void ecg_reduce_kernel(const HeartDev* __restrict__ hearts, const Req
 // ---------------------------------------------------------------------------
 
 constexpr int CC_THREADS = 256;
-constexpr int CC_CELLS = 2;  // cells per thread and iteration of the class-chunk kernel
 
 // Class chunks: one block per (request, class chunk).  Every voxel of a class
 // has the same AP, so only the sum of the chunk's G rows is needed, and by
 // regrouping the dipoles per cell it is
 //   sum_u G(u) = sum_w s(w) (e(w) . D(w))
 // over the chunk's cells w with a non-zero integer vector D(w) (host-built
-// per neighbourhood; cells inside the chunk cancel).  s(w) and the FP32 e(w)
-// are the chunk kernel's bit for bit (same operands and order), and
-// e(w) . D(w) is exact in FP64 (FP32 components times small integers), so a
-// cell's contributions here and in chunk-kernel G rows cancel across every
-// dipole as they do between G rows.  Each thread adds its cells t, t + CC_THREADS, ...
-// (CC_CELLS at a time) in registers in that order, the lanes of a wave are added
-// by an xor butterfly and the waves in order, and sums[(c_off + n_chunks +
-// k)*3*LP] receives {sum G, P' sum G, Q' sum G} with the class's
-// P' = k2(1-k3) e^{(k4+k5)(a - t0)} and Q' = k2 k3 e^{k5 (a - t0)}.
+// per neighbourhood; cells inside the chunk cancel).  With e = p - c (c the
+// cell's coordinate + 1) this is linear in the electrode p:
+//   sum_w s(w) (e . D) = p . T + T3,  T_c = sum_w s(w) D_c(w),  T3 = -sum_w s(w) D(w) . c(w),
+// a product of the weights s (leads x cells) with four integer coefficients
+// per cell, which runs on v_mfma_f64_4x4x4f64 (4 blocks of 4 leads x 4 cells x
+// 4 coefficients).  s(w) is the chunk kernel's FP32 weight bit for bit (same
+// operands and order), and the chunk kernel's dipole products use the same
+// FP64 offsets, so a cell's contributions here and in G rows cancel across
+// every dipole to FP64 rounding.  gfx950 lane layout (checked by a one-hot
+// probe): A(block, row, k) in lane 16k + 4 block + row, B(block, k, col) in
+// lane 16k + 4 block + col, D(block, row, col) in lane 16 row + 4 block + col.
+// So a lane owns one cell of each group of 16 (k, block) and one lead of each
+// group of 4 (row) for A and one coefficient (col) for B.  Each wave takes
+// groups of 16 cells (strided over the waves), the blocks' partial sums are
+// added in block order, the waves' in wave order, and
+// sums[(c_off + n_chunks + k)*3*LP] receives {sum G, P' sum G, Q' sum G} with
+// the class's P' = k2(1-k3) e^{(k4+k5)(a - t0)} and Q' = k2 k3 e^{k5 (a - t0)}.
 template <int LP>
 __global__ __launch_bounds__(CC_THREADS) void ecg_class_chunk_kernel(
     const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
     const int64_t* __restrict__ cc_start, int cc_uniform, const double* __restrict__ lead_pos,
     const double* __restrict__ layer_k, double* __restrict__ sums)
 {
-    __shared__ float4 s_ph[LP], s_pl[LP];  // electrode FP32 hi / lo per component
-    __shared__ double s_w[CC_THREADS / 64][LP];
+    constexpr int NG = LP / 4;                 // groups of 4 leads
+    constexpr int NW = CC_THREADS / 64;
+    __shared__ double s_t[NW][LP][4];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     // chunk-major when every request has the same class chunks (one heart)
     int r, k;
@@ -1131,73 +1149,72 @@ __global__ __launch_bounds__(CC_THREADS) void ecg_class_chunk_kernel(
     const ReqDev R = reqs[r];
     const HeartDev& H = hearts[R.heart];
     const int4 ch = H.cchunks[k];
-    if (t < LP) {
-        // leads past the request's count repeat lead 0 (their sums are zeroed)
-        const int i = t < R.L ? t : 0;
-        float hi[3], lo[3];
+    const int rc = lane & 3;                                  // A row (lead of a group) = B column
+    const int cl = 4 * ((lane >> 2) & 3) + (lane >> 4);       // the lane's cell of a group of 16
+    // electrodes of the lane's leads 4g + rc (leads past the request's count
+    // repeat lead 0; their sums are zeroed)
+    float ph[NG][3], pl[NG][3];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        const int i = 4 * g + rc < R.L ? 4 * g + rc : 0;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const double p = lead_pos[R.pos_off + 3 * i + c];
-            hi[c] = (float)p;
-            lo[c] = (float)(p - (double)hi[c]);
+            ph[g][c] = (float)p;
+            pl[g][c] = (float)(p - (double)ph[g][c]);
         }
-        s_ph[t] = float4{hi[0], hi[1], hi[2], 0.0f};
-        s_pl[t] = float4{lo[0], lo[1], lo[2], 0.0f};
     }
-    __syncthreads();
-    double acc[LP];
+    double acc[NG];
 #pragma unroll
-    for (int i = 0; i < LP; ++i) acc[i] = 0.0;
+    for (int g = 0; g < NG; ++g) acc[g] = 0.0;
     const int cb = H.ccell_off[k], ce = H.ccell_off[k + 1];
-    for (int q = cb + t; q < ce; q += CC_CELLS * CC_THREADS) {
-        // reload the electrodes from LDS in every iteration rather than keep
-        // all of them in registers (which would halve the waves per SIMD)
-        asm volatile("" ::: "memory");
-        // packed coordinates z | y << 10 | x << 20: component c is axis c;
-        // cells past the chunk's end repeat the first with D = 0
-        float f[CC_CELLS][3];
-        double d[CC_CELLS][3];
+    for (int q0 = cb + 16 * w; q0 < ce; q0 += 16 * NW) {
+        const int q = q0 + cl;
+        const bool ok = q < ce;
+        const uint4 cw = H.ccells[ok ? q : q0];
+        const float f0 = __uint_as_float(cw.x), f1 = __uint_as_float(cw.y), f2 = __uint_as_float(cw.z);
+        const int D0 = (int)(int8_t)(cw.w & 255u), D1 = (int)(int8_t)((cw.w >> 8) & 255u),
+                  D2 = (int)(int8_t)((cw.w >> 16) & 255u);
+        // B: the lane's coefficient of its cell, D_c (byte c of the packed
+        // vector) or -D . c (exact integers); cells past the chunk's end add
+        // nothing.  Both are formed on every lane and one is selected.
+        const int dc = (int)(int8_t)((cw.w >> (8 * rc)) & 255u);
+        const int dk = -(D0 * (int)f0 + D1 * (int)f1 + D2 * (int)f2);
+        const double bv = (double)((rc == 3 ? dk : dc) & -(int)ok);
 #pragma unroll
-        for (int u = 0; u < CC_CELLS; ++u) {
-            const int qu = q + u * CC_THREADS;
-            const bool ok = qu < ce;
-            const uint2 cu = H.ccells[ok ? qu : q];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                f[u][c] = (float)((int)((cu.x >> (10 * c)) & 1023u) + 1);
-                d[u][c] = ok ? (double)(int)(int8_t)((cu.y >> (8 * c)) & 255u) : 0.0;
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < LP; ++i) {
-            const float4 ph = s_ph[i], pl = s_pl[i];
-#pragma unroll
-            for (int u = 0; u < CC_CELLS; ++u) {
-                const float e0 = (ph.x - f[u][0]) + pl.x, e1 = (ph.y - f[u][1]) + pl.y, e2 = (ph.z - f[u][2]) + pl.z;
-                float l = e0 * e0;
-                l = __builtin_fmaf(e1, e1, l);
-                l = __builtin_fmaf(e2, e2, l);
-                const float rr = __builtin_amdgcn_rsqf(l);
-                // D . e exactly: FP32 components times small integers
-                const double eD =
-                    __builtin_fma(d[u][2], (double)e2, __builtin_fma(d[u][1], (double)e1, d[u][0] * (double)e0));
-                acc[i] = __builtin_fma((double)(rr * rr * rr), eD, acc[i]);
-            }
+        for (int g = 0; g < NG; ++g) {
+            const float e0 = (ph[g][0] - f0) + pl[g][0], e1 = (ph[g][1] - f1) + pl[g][1],
+                        e2 = (ph[g][2] - f2) + pl[g][2];
+            float l = e0 * e0;
+            l = __builtin_fmaf(e1, e1, l);
+            l = __builtin_fmaf(e2, e2, l);
+            const float rr = __builtin_amdgcn_rsqf(l);
+            acc[g] = __builtin_amdgcn_mfma_f64_4x4x4f64((double)(rr * rr * rr), bv, acc[g], 0, 0, 0);
         }
     }
+    // acc[g] in lane 16 row + 4 block + col: lead 4g + row, coefficient col,
+    // this wave's cells of one block; the four blocks in order
+    const int base = lane & ~12;
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1)
-#pragma unroll
-        for (int i = 0; i < LP; ++i) acc[i] += __shfl_xor(acc[i], off);
-    if (lane == 0)
-#pragma unroll
-        for (int i = 0; i < LP; ++i) s_w[w][i] = acc[i];
+    for (int g = 0; g < NG; ++g) {
+        const double v0 = __shfl(acc[g], base), v1 = __shfl(acc[g], base | 4), v2 = __shfl(acc[g], base | 8),
+                     v3 = __shfl(acc[g], base | 12);
+        if ((lane & 12) == 0) s_t[w][4 * g + (lane >> 4)][lane & 3] = ((v0 + v1) + v2) + v3;
+    }
     __syncthreads();
     if (t < LP) {
-        double sg = 0.0;
+        double T[4];
 #pragma unroll
-        for (int ww = 0; ww < CC_THREADS / 64; ++ww) sg += s_w[ww][t];
-        if (t >= R.L) sg = 0.0;
+        for (int c = 0; c < 4; ++c) {
+            T[c] = 0.0;
+#pragma unroll
+            for (int ww = 0; ww < NW; ++ww) T[c] += s_t[ww][t][c];
+        }
+        double sg = 0.0;
+        if (t < R.L) {
+            const double* p = lead_pos + R.pos_off + 3 * t;
+            sg = __builtin_fma(p[2], T[2], __builtin_fma(p[1], T[1], __builtin_fma(p[0], T[0], T[3])));
+        }
         const int l = ch.x;
         const double* kk = layer_k + R.k_off + 9 * (l - 1);
         const double da = H.vox_a64[ch.y] - H.layer_t0[l];

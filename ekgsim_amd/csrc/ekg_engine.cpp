@@ -484,7 +484,8 @@ void ensure_layout(Heart& h, bool mfma)
 //   sum_{u in S} G(u) = sum_w m(w) . D(w),
 //   D(w) = [w in S] s_w + sum_{d in N, w - d in S, w myocardial} d,
 // an integer vector per cell; cells inside S cancel (D = 0) and are dropped.
-// Each remaining cell's weight is evaluated once per class chunk.
+// Each remaining cell's weight is evaluated once per class chunk.  A cell is
+// stored as {grid coordinates + 1 as three floats, D as three int8}.
 void ensure_cells(Heart& h, int nhood)
 {
     EcgLayout& L = h.layout;
@@ -500,9 +501,15 @@ void ensure_cells(Heart& h, int nhood)
     std::unordered_map<int64_t, int> slot_map;
     std::vector<int64_t> touched;
     std::vector<int> acc;  // D per slot, three ints
-    std::vector<uint2> cells;
+    std::vector<uint4> cells;
     std::vector<int> off(L.n_cchunks + 1, 0);
     auto bit = [](int d0, int d1, int d2) { return (d0 + 1) * 9 + (d1 + 1) * 3 + (d2 + 1); };
+    auto fbits = [](int v) {
+        const float f = (float)v;
+        uint32_t b;
+        std::memcpy(&b, &f, 4);
+        return b;
+    };
     auto add = [&](int z, int y, int x, int d0, int d1, int d2) {
         const int64_t gi = ((int64_t)z * h.n1 + y) * h.n2 + x;
         int sl;
@@ -547,7 +554,8 @@ void ensure_cells(Heart& h, int nhood)
             if (std::abs(D0) > 127 || std::abs(D1) > 127 || std::abs(D2) > 127)
                 fail(EKG_EINVAL, "dipole vector out of range");
             const int x = (int)(gi % h.n2), y = (int)((gi / h.n2) % h.n1), z = (int)(gi / ((int64_t)h.n2 * h.n1));
-            cells.push_back(make_uint2((uint32_t)z | ((uint32_t)y << 10) | ((uint32_t)x << 20),
+            // grid coordinates + 1 (the reference's voxel centre) as floats, and D
+            cells.push_back(make_uint4(fbits(z + 1), fbits(y + 1), fbits(x + 1),
                                        (uint32_t)(uint8_t)(int8_t)D0 | ((uint32_t)(uint8_t)(int8_t)D1 << 8) |
                                            ((uint32_t)(uint8_t)(int8_t)D2 << 16)));
         }
@@ -555,11 +563,11 @@ void ensure_cells(Heart& h, int nhood)
         off[k + 1] = (int)cells.size();
     }
     L.n_ccells[nhood] = (int64_t)cells.size();
-    if (cells.empty()) cells.push_back(make_uint2(0, 0));
+    if (cells.empty()) cells.push_back(make_uint4(0, 0, 0, 0));
     h.ctx->bind();
-    L.ccells[nhood].reserve(cells.size() * sizeof(uint2));
+    L.ccells[nhood].reserve(cells.size() * sizeof(uint4));
     L.ccell_off[nhood].reserve(off.size() * sizeof(int));
-    hip_check(hipMemcpy(L.ccells[nhood].p, cells.data(), cells.size() * sizeof(uint2), hipMemcpyHostToDevice),
+    hip_check(hipMemcpy(L.ccells[nhood].p, cells.data(), cells.size() * sizeof(uint4), hipMemcpyHostToDevice),
               "upload class cells");
     hip_check(hipMemcpy(L.ccell_off[nhood].p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice),
               "upload class cell offsets");
@@ -653,7 +661,7 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
             hi = (int)hs.size();
             hs.push_back(&h);
             ekgk::HeartDev d = h.layout.dev;
-            d.ccells = g.mfma ? h.layout.ccells[g.nhood].as<uint2>() : nullptr;
+            d.ccells = g.mfma ? h.layout.ccells[g.nhood].as<uint4>() : nullptr;
             d.ccell_off = g.mfma ? h.layout.ccell_off[g.nhood].as<int>() : nullptr;
             hdev.push_back(d);
         }
