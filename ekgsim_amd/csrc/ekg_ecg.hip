@@ -1105,29 +1105,32 @@ __global__ void ecg_reduce_kernel(const HeartDev* __restrict__ hearts, const Req
 
 constexpr int CC_THREADS = 256;
 
-// Class chunks: one block per (request, class chunk).  Every voxel of a class
+// Class chunks: one block per (class chunk, request), or per (class chunk,
+// pair of requests) when every request of the group uses the same heart (the
+// cells are then loaded and unpacked once for both).  Every voxel of a class
 // has the same AP, so only the sum of the chunk's G rows is needed, and by
 // regrouping the dipoles per cell it is
 //   sum_u G(u) = sum_w s(w) (e(w) . D(w))
 // over the chunk's cells w with a non-zero integer vector D(w) (host-built
 // per neighbourhood; cells inside the chunk cancel).  With e = p - c (c the
 // cell's coordinate + 1) this is linear in the electrode p:
-//   sum_w s(w) (e . D) = p . T + T3,  T_c = sum_w s(w) D_c(w),  T3 = -sum_w s(w) D(w) . c(w),
-// a product of the weights s (leads x cells) with four integer coefficients
-// per cell, which runs on v_mfma_f64_4x4x4f64 (4 blocks of 4 leads x 4 cells x
-// 4 coefficients).  s(w) is the chunk kernel's FP32 weight bit for bit (same
-// operands and order), and the chunk kernel's dipole products use the same
-// FP64 offsets, so a cell's contributions here and in G rows cancel across
-// every dipole to FP64 rounding.  gfx950 lane layout (checked by a one-hot
-// probe): A(block, row, k) in lane 16k + 4 block + row, B(block, k, col) in
-// lane 16k + 4 block + col, D(block, row, col) in lane 16 row + 4 block + col.
-// So a lane owns one cell of each group of 16 (k, block) and one lead of each
-// group of 4 (row) for A and one coefficient (col) for B.  Each wave takes
-// groups of 16 cells (strided over the waves), the blocks' partial sums are
-// added in block order, the waves' in wave order, and
-// sums[(c_off + n_chunks + k)*3*LP] receives {sum G, P' sum G, Q' sum G} with
-// the class's P' = k2(1-k3) e^{(k4+k5)(a - t0)} and Q' = k2 k3 e^{k5 (a - t0)}.
-template <int LP>
+//   sum_w s(w) (e . D) = p . T + T3,  T_c = sum_w s(w) D_c(w),  T3 = sum_w s(w) K(w),
+// K = -D . c (host-computed), a product of the weights s (leads x cells) with
+// four integer coefficients per cell, which runs on v_mfma_f64_4x4x4f64 (4
+// blocks of 4 leads x 4 cells x 4 coefficients).  s(w) is the chunk kernel's
+// FP32 weight bit for bit (same operands and order), and the chunk kernel's
+// dipole products use the same FP64 offsets, so a cell's contributions here
+// and in G rows cancel across every dipole to FP64 rounding.  gfx950 lane
+// layout (checked by a one-hot probe): A(block, row, k) in lane
+// 16k + 4 block + row, B(block, k, col) in lane 16k + 4 block + col,
+// D(block, row, col) in lane 16 row + 4 block + col.  So a lane owns one cell
+// of each group of 16 (k, block) and one lead of each group of 4 (row) for A
+// and one coefficient (col) for B.  Each wave takes groups of 16 cells
+// (strided over the waves), the blocks' partial sums are added in block
+// order, the waves' in wave order, and sums[(c_off + n_chunks + k)*3*LP]
+// receives {sum G, P' sum G, Q' sum G} with the class's
+// P' = k2(1-k3) e^{(k4+k5)(a - t0)} and Q' = k2 k3 e^{k5 (a - t0)}.
+template <int LP, int RQ>
 __global__ __launch_bounds__(CC_THREADS) void ecg_class_chunk_kernel(
     const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
     const int64_t* __restrict__ cc_start, int cc_uniform, const double* __restrict__ lead_pos,
@@ -1135,84 +1138,100 @@ __global__ __launch_bounds__(CC_THREADS) void ecg_class_chunk_kernel(
 {
     constexpr int NG = LP / 4;                 // groups of 4 leads
     constexpr int NW = CC_THREADS / 64;
-    __shared__ double s_t[NW][LP][4];
+    __shared__ double s_t[RQ][NW][LP][4];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    // chunk-major when every request has the same class chunks (one heart)
-    int r, k;
-    if (cc_uniform > 0) {
-        r = (int)(blockIdx.x % (unsigned)B);
+    // chunk-major: over pairs of requests (RQ = 2, one heart; an odd last
+    // request is paired with itself), else over requests when every request
+    // has the same class chunks
+    int rq[RQ], k;
+    if (RQ == 2) {
+        const int np = (B + 1) / 2;
+        k = (int)(blockIdx.x / (unsigned)np);
+        rq[0] = 2 * (int)(blockIdx.x % (unsigned)np);
+        rq[RQ - 1] = min(rq[0] + 1, B - 1);
+    } else if (cc_uniform > 0) {
+        rq[0] = (int)(blockIdx.x % (unsigned)B);
         k = (int)(blockIdx.x / (unsigned)B);
     } else {
-        r = find_req(cc_start, B, blockIdx.x);
-        k = (int)(blockIdx.x - cc_start[r]);
+        rq[0] = find_req(cc_start, B, blockIdx.x);
+        k = (int)(blockIdx.x - cc_start[rq[0]]);
     }
-    const ReqDev R = reqs[r];
-    const HeartDev& H = hearts[R.heart];
+    const HeartDev& H = hearts[reqs[rq[0]].heart];
     const int4 ch = H.cchunks[k];
     const int rc = lane & 3;                                  // A row (lead of a group) = B column
     const int cl = 4 * ((lane >> 2) & 3) + (lane >> 4);       // the lane's cell of a group of 16
     // electrodes of the lane's leads 4g + rc (leads past the request's count
     // repeat lead 0; their sums are zeroed)
-    float ph[NG][3], pl[NG][3];
+    float ph[RQ][NG][3], pl[RQ][NG][3];
 #pragma unroll
-    for (int g = 0; g < NG; ++g) {
-        const int i = 4 * g + rc < R.L ? 4 * g + rc : 0;
+    for (int j = 0; j < RQ; ++j) {
+        const ReqDev& R = reqs[rq[j]];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const double p = lead_pos[R.pos_off + 3 * i + c];
-            ph[g][c] = (float)p;
-            pl[g][c] = (float)(p - (double)ph[g][c]);
+        for (int g = 0; g < NG; ++g) {
+            const int i = 4 * g + rc < R.L ? 4 * g + rc : 0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double p = lead_pos[R.pos_off + 3 * i + c];
+                ph[j][g][c] = (float)p;
+                pl[j][g][c] = (float)(p - (double)ph[j][g][c]);
+            }
         }
     }
-    double acc[NG];
+    double acc[RQ][NG];
 #pragma unroll
-    for (int g = 0; g < NG; ++g) acc[g] = 0.0;
+    for (int j = 0; j < RQ; ++j)
+#pragma unroll
+        for (int g = 0; g < NG; ++g) acc[j][g] = 0.0;
     const int cb = H.ccell_off[k], ce = H.ccell_off[k + 1];
     for (int q0 = cb + 16 * w; q0 < ce; q0 += 16 * NW) {
         const int q = q0 + cl;
         const bool ok = q < ce;
         const uint4 cw = H.ccells[ok ? q : q0];
+        const int kc = H.ccellk[ok ? q : q0];
         const float f0 = __uint_as_float(cw.x), f1 = __uint_as_float(cw.y), f2 = __uint_as_float(cw.z);
-        const int D0 = (int)(int8_t)(cw.w & 255u), D1 = (int)(int8_t)((cw.w >> 8) & 255u),
-                  D2 = (int)(int8_t)((cw.w >> 16) & 255u);
         // B: the lane's coefficient of its cell, D_c (byte c of the packed
-        // vector) or -D . c (exact integers); cells past the chunk's end add
-        // nothing.  Both are formed on every lane and one is selected.
+        // vector) or K; cells past the chunk's end add nothing
         const int dc = (int)(int8_t)((cw.w >> (8 * rc)) & 255u);
-        const int dk = -(D0 * (int)f0 + D1 * (int)f1 + D2 * (int)f2);
-        const double bv = (double)((rc == 3 ? dk : dc) & -(int)ok);
+        const double bv = (double)((rc == 3 ? kc : dc) & -(int)ok);
 #pragma unroll
-        for (int g = 0; g < NG; ++g) {
-            const float e0 = (ph[g][0] - f0) + pl[g][0], e1 = (ph[g][1] - f1) + pl[g][1],
-                        e2 = (ph[g][2] - f2) + pl[g][2];
-            float l = e0 * e0;
-            l = __builtin_fmaf(e1, e1, l);
-            l = __builtin_fmaf(e2, e2, l);
-            const float rr = __builtin_amdgcn_rsqf(l);
-            acc[g] = __builtin_amdgcn_mfma_f64_4x4x4f64((double)(rr * rr * rr), bv, acc[g], 0, 0, 0);
-        }
+        for (int j = 0; j < RQ; ++j)
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                const float e0 = (ph[j][g][0] - f0) + pl[j][g][0], e1 = (ph[j][g][1] - f1) + pl[j][g][1],
+                            e2 = (ph[j][g][2] - f2) + pl[j][g][2];
+                float l = e0 * e0;
+                l = __builtin_fmaf(e1, e1, l);
+                l = __builtin_fmaf(e2, e2, l);
+                const float rr = __builtin_amdgcn_rsqf(l);
+                acc[j][g] = __builtin_amdgcn_mfma_f64_4x4x4f64((double)(rr * rr * rr), bv, acc[j][g], 0, 0, 0);
+            }
     }
-    // acc[g] in lane 16 row + 4 block + col: lead 4g + row, coefficient col,
-    // this wave's cells of one block; the four blocks in order
+    // acc[j][g] in lane 16 row + 4 block + col: lead 4g + row, coefficient
+    // col, this wave's cells of one block; the four blocks in order
     const int base = lane & ~12;
 #pragma unroll
-    for (int g = 0; g < NG; ++g) {
-        const double v0 = __shfl(acc[g], base), v1 = __shfl(acc[g], base | 4), v2 = __shfl(acc[g], base | 8),
-                     v3 = __shfl(acc[g], base | 12);
-        if ((lane & 12) == 0) s_t[w][4 * g + (lane >> 4)][lane & 3] = ((v0 + v1) + v2) + v3;
-    }
+    for (int j = 0; j < RQ; ++j)
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            const double v0 = __shfl(acc[j][g], base), v1 = __shfl(acc[j][g], base | 4),
+                         v2 = __shfl(acc[j][g], base | 8), v3 = __shfl(acc[j][g], base | 12);
+            if ((lane & 12) == 0) s_t[j][w][4 * g + (lane >> 4)][lane & 3] = ((v0 + v1) + v2) + v3;
+        }
     __syncthreads();
-    if (t < LP) {
+    if (t < RQ * LP) {
+        const int j = t / LP, i = t - j * LP;
+        if (j > 0 && rq[RQ - 1] == rq[0]) return;  // the odd last request's copy
+        const ReqDev& R = reqs[rq[j]];
         double T[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             T[c] = 0.0;
 #pragma unroll
-            for (int ww = 0; ww < NW; ++ww) T[c] += s_t[ww][t][c];
+            for (int ww = 0; ww < NW; ++ww) T[c] += s_t[j][ww][i][c];
         }
         double sg = 0.0;
-        if (t < R.L) {
-            const double* p = lead_pos + R.pos_off + 3 * t;
+        if (i < R.L) {
+            const double* p = lead_pos + R.pos_off + 3 * i;
             sg = __builtin_fma(p[2], T[2], __builtin_fma(p[1], T[1], __builtin_fma(p[0], T[0], T[3])));
         }
         const int l = ch.x;
@@ -1221,9 +1240,9 @@ __global__ __launch_bounds__(CC_THREADS) void ecg_class_chunk_kernel(
         const double pp = kk[2] * (1.0 - kk[3]) * exp((kk[4] + kk[5]) * da);
         const double qq = kk[2] * kk[3] * exp(kk[5] * da);
         double* o = sums + (R.c_off + H.n_chunks + k) * 3 * LP;
-        o[t] = sg;
-        o[LP + t] = pp * sg;
-        o[2 * LP + t] = qq * sg;
+        o[i] = sg;
+        o[LP + i] = pp * sg;
+        o[2 * LP + i] = qq * sg;
     }
 }
 
@@ -1481,8 +1500,12 @@ hipError_t launch_ecg_chunk(const GroupArgs& a, hipStream_t s)
 template <int LP>
 static void class_chunk_launch(const GroupArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL((ecg_class_chunk_kernel<LP>), dim3((unsigned)a.n_cc), dim3(CC_THREADS), 0, s, a.hearts,
-                       a.reqs, a.B, a.cc_start, a.cc_uniform, a.lead_pos, a.layer_k, a.sums);
+    if (a.cc_rpb == 2)
+        hipLaunchKernelGGL((ecg_class_chunk_kernel<LP, 2>), dim3((unsigned)a.n_cc), dim3(CC_THREADS), 0, s, a.hearts,
+                           a.reqs, a.B, a.cc_start, a.cc_uniform, a.lead_pos, a.layer_k, a.sums);
+    else
+        hipLaunchKernelGGL((ecg_class_chunk_kernel<LP, 1>), dim3((unsigned)a.n_cc), dim3(CC_THREADS), 0, s, a.hearts,
+                           a.reqs, a.B, a.cc_start, a.cc_uniform, a.lead_pos, a.layer_k, a.sums);
 }
 
 hipError_t launch_class_chunks(const GroupArgs& a, hipStream_t s)

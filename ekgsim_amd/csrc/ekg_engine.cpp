@@ -502,6 +502,7 @@ void ensure_cells(Heart& h, int nhood)
     std::vector<int64_t> touched;
     std::vector<int> acc;  // D per slot, three ints
     std::vector<uint4> cells;
+    std::vector<int> kc;  // K = -D . c per cell
     std::vector<int> off(L.n_cchunks + 1, 0);
     auto bit = [](int d0, int d1, int d2) { return (d0 + 1) * 9 + (d1 + 1) * 3 + (d2 + 1); };
     auto fbits = [](int v) {
@@ -558,15 +559,22 @@ void ensure_cells(Heart& h, int nhood)
             cells.push_back(make_uint4(fbits(z + 1), fbits(y + 1), fbits(x + 1),
                                        (uint32_t)(uint8_t)(int8_t)D0 | ((uint32_t)(uint8_t)(int8_t)D1 << 8) |
                                            ((uint32_t)(uint8_t)(int8_t)D2 << 16)));
+            kc.push_back(-(D0 * (z + 1) + D1 * (y + 1) + D2 * (x + 1)));
         }
         if (cells.size() > 0x7fffffffu) fail(EKG_EINVAL, "too many class cells");
         off[k + 1] = (int)cells.size();
     }
     L.n_ccells[nhood] = (int64_t)cells.size();
-    if (cells.empty()) cells.push_back(make_uint4(0, 0, 0, 0));
+    if (cells.empty()) {
+        cells.push_back(make_uint4(0, 0, 0, 0));
+        kc.push_back(0);
+    }
     h.ctx->bind();
     L.ccells[nhood].reserve(cells.size() * sizeof(uint4));
     L.ccell_off[nhood].reserve(off.size() * sizeof(int));
+    L.ccellk[nhood].reserve(kc.size() * sizeof(int));
+    hip_check(hipMemcpy(L.ccellk[nhood].p, kc.data(), kc.size() * sizeof(int), hipMemcpyHostToDevice),
+              "upload class cell moments");
     hip_check(hipMemcpy(L.ccells[nhood].p, cells.data(), cells.size() * sizeof(uint4), hipMemcpyHostToDevice),
               "upload class cells");
     hip_check(hipMemcpy(L.ccell_off[nhood].p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice),
@@ -663,6 +671,7 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
             ekgk::HeartDev d = h.layout.dev;
             d.ccells = g.mfma ? h.layout.ccells[g.nhood].as<uint4>() : nullptr;
             d.ccell_off = g.mfma ? h.layout.ccell_off[g.nhood].as<int>() : nullptr;
+            d.ccellk = g.mfma ? h.layout.ccellk[g.nhood].as<int>() : nullptr;
             hdev.push_back(d);
         }
         int64_t t_off = -1;
@@ -810,6 +819,15 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     a.cc_uniform = B > 0 ? (int)(st_cc[1] - st_cc[0]) : 0;
     for (int r = 1; r < B && a.cc_uniform; ++r)
         if (st_cc[r + 1] - st_cc[r] != a.cc_uniform) a.cc_uniform = 0;
+    // one heart: each class-chunk block takes a chunk for two requests
+    a.cc_rpb = 1;
+    bool one_heart = B > 1 && a.cc_uniform > 0;
+    for (int r = 1; r < B && one_heart; ++r)
+        if (g.reqs[r].h != g.reqs[0].h) one_heart = false;
+    if (one_heart) {
+        a.cc_rpb = 2;
+        a.n_cc = (int64_t)a.cc_uniform * ((B + 1) / 2);
+    }
     a.n_xy = st_xy[B];
     a.n_lay = st_lay[B];
     a.n_out = st_out[B];

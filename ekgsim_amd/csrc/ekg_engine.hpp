@@ -93,7 +93,7 @@ struct EcgLayout {
     // ensure_cells), cells [ccell_off[k], ccell_off[k+1]) for class chunk k.
     bool cells_built[4] = {false, false, false, false};
     int64_t n_ccells[4] = {0, 0, 0, 0};
-    DevBuf ccells[4], ccell_off[4];
+    DevBuf ccells[4], ccellk[4], ccell_off[4];
     std::vector<int> h_layer_chunk;
     std::vector<double> h_layer_t0;
     DevBuf vox_chunk, vox_a64, vox_a, vox_geo, chunks, bricks, brick_off, brick_vox, halo_off, halo_cells,

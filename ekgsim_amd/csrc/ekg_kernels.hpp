@@ -74,6 +74,7 @@ struct HeartDev {
     // x + 1 as float bits, D as three int8 (bits 0-7, 8-15, 16-23)}; chunk k
     // owns [ccell_off[k], ccell_off[k+1])
     const uint4* ccells;
+    const int* ccellk;           // per cell: K = -D . (z + 1, y + 1, x + 1)
     const int* ccell_off;
     int n0, n1, n2;
     int V, n_chunks, n_bricks, max_layer;
@@ -117,6 +118,7 @@ struct GroupArgs {
     int64_t n_lf, n_ch, n_xy, n_lay, n_out, n_cc, n_cs, n_cl;  // totals
     int ch_uniform;            // chunks per request when all requests have the same count, else 0
     int cc_uniform;            // the same for class chunks
+    int cc_rpb;                // requests per class-chunk block: 2 when every request uses one heart, else 1
     const double* layer_k;
     const double* lead_pos;
     const double* tgrid;       // FP64 sample times
