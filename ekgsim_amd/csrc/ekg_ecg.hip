@@ -1144,11 +1144,12 @@ __global__ __launch_bounds__(CC_THREADS) void ecg_class_chunk_kernel(
     // request is paired with itself), else over requests when every request
     // has the same class chunks
     int rq[RQ], k;
-    if (RQ == 2) {
-        const int np = (B + 1) / 2;
+    if (RQ > 1) {
+        const int np = (B + RQ - 1) / RQ;
         k = (int)(blockIdx.x / (unsigned)np);
-        rq[0] = 2 * (int)(blockIdx.x % (unsigned)np);
-        rq[RQ - 1] = min(rq[0] + 1, B - 1);
+        rq[0] = RQ * (int)(blockIdx.x % (unsigned)np);
+#pragma unroll
+        for (int j = 1; j < RQ; ++j) rq[j] = min(rq[0] + j, B - 1);
     } else if (cc_uniform > 0) {
         rq[0] = (int)(blockIdx.x % (unsigned)B);
         k = (int)(blockIdx.x / (unsigned)B);
@@ -1220,7 +1221,7 @@ __global__ __launch_bounds__(CC_THREADS) void ecg_class_chunk_kernel(
     __syncthreads();
     if (t < RQ * LP) {
         const int j = t / LP, i = t - j * LP;
-        if (j > 0 && rq[RQ - 1] == rq[0]) return;  // the odd last request's copy
+        if (j > 0 && rq[j] == rq[j - 1]) return;  // a copy of the last request
         const ReqDev& R = reqs[rq[j]];
         double T[4];
 #pragma unroll
@@ -1500,6 +1501,8 @@ hipError_t launch_ecg_chunk(const GroupArgs& a, hipStream_t s)
 template <int LP>
 static void class_chunk_launch(const GroupArgs& a, hipStream_t s)
 {
+    // two requests per block (timed against one and four at 256 x 12 leads:
+    // four needs 116 VGPRs, 4 waves per SIMD, and is slower)
     if (a.cc_rpb == 2)
         hipLaunchKernelGGL((ecg_class_chunk_kernel<LP, 2>), dim3((unsigned)a.n_cc), dim3(CC_THREADS), 0, s, a.hearts,
                            a.reqs, a.B, a.cc_start, a.cc_uniform, a.lead_pos, a.layer_k, a.sums);

@@ -826,7 +826,7 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
         if (g.reqs[r].h != g.reqs[0].h) one_heart = false;
     if (one_heart) {
         a.cc_rpb = 2;
-        a.n_cc = (int64_t)a.cc_uniform * ((B + 1) / 2);
+        a.n_cc = (int64_t)a.cc_uniform * ((B + a.cc_rpb - 1) / a.cc_rpb);
     }
     a.n_xy = st_xy[B];
     a.n_lay = st_lay[B];
