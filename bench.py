@@ -70,14 +70,15 @@ def lead_field_cells(layer):
 
 # Algorithmic work, counted per unit.  Class-chunk kernel (the dominant one
 # when the heart has activation classes), per (class cell, lead): 14 FP32
-# flops (offset e 6, |e|^2 5, rsq 1, |e|^-3 2) and 7 FP64 flops (D . e 5, the
-# accumulating FMA 2).  Chunk kernel (dominant without classes):
+# flops for the weight (offset e 6, |e|^2 5, rsq 1, |e|^-3 2) and 8 FP64
+# matrix-core flops (an FMA for each of the coefficients D0, D1, D2, K).
+# Chunk kernel (dominant without classes):
 #   per lead-weight evaluation (voxel, lead, cell): 14 FP32 flops (electrode
 #     offset 6, |e|^2 5, rsq 1, r^3 2) and 4 FP64 flops (d . e, one FMA);
 #   per window voxel-step (the transition tiles): 2L FP64 MFMA flops and 14
 #     FP32 ops (10 AP assembly + 4 transcendental).
 # FP32 counts half against the FP64 peak (the FP32 vector rate is twice it).
-FP32_PER_CLASS_CELL, FP64_PER_CLASS_CELL = 14, 7
+FP32_PER_CLASS_CELL, FP64_PER_CLASS_CELL = 14, 8
 FP32_PER_CELL, FP64_PER_CELL = 14, 4
 FP32_PER_WINDOW_STEP = 14
 # Naive per-voxel-step model: every voxel at every step, 38 VALU lane-cycles
@@ -89,7 +90,7 @@ NAIVE_LANE_CYCLES, LANE_PEAK = 38, 256 * 64 * 2.4e9
 # by its reading for a 256 MiB calibration copy: with activation classes
 # (profiles/ecg_class_chunk_kernel_128cubed_batch256_hbm_traffic.md) and
 # without (profiles/ecg_chunk_kernel_128cubed_batch256_hbm_traffic.md).
-TRAFFIC_BYTES = {"ecg_class_chunk_kernel": 121094073, "ecg_chunk_kernel": 1396124157}
+TRAFFIC_BYTES = {"ecg_class_chunk_kernel": 280036622, "ecg_chunk_kernel": 1396124157}
 
 
 def log(*a):

@@ -1184,28 +1184,55 @@ __global__ __launch_bounds__(CC_THREADS) void ecg_class_chunk_kernel(
 #pragma unroll
         for (int g = 0; g < NG; ++g) acc[j][g] = 0.0;
     const int cb = H.ccell_off[k], ce = H.ccell_off[k + 1];
-    for (int q0 = cb + 16 * w; q0 < ce; q0 += 16 * NW) {
-        const int q = q0 + cl;
-        const bool ok = q < ce;
-        const uint4 cw = H.ccells[ok ? q : q0];
-        const int kc = H.ccellk[ok ? q : q0];
-        const float f0 = __uint_as_float(cw.x), f1 = __uint_as_float(cw.y), f2 = __uint_as_float(cw.z);
-        // B: the lane's coefficient of its cell, D_c (byte c of the packed
-        // vector) or K; cells past the chunk's end add nothing
-        const int dc = (int)(int8_t)((cw.w >> (8 * rc)) & 255u);
-        const double bv = (double)((rc == 3 ? kc : dc) & -(int)ok);
+    // one request: U groups of 16 cells per iteration, their records loaded
+    // together (that path is bound by load latency, not vector work)
+    constexpr int U = RQ > 1 ? 1 : 4;
+    constexpr int step = 16 * NW;
+    for (int q0 = cb + 16 * w; q0 < ce; q0 += step * U) {
+        uint4 cwu[U];
+        int kcu[U];
+        bool oku[U];
 #pragma unroll
-        for (int j = 0; j < RQ; ++j)
+        for (int u = 0; u < U; ++u) {
+            const int q = q0 + u * step + cl;
+            oku[u] = q < ce;
+            const int qs = oku[u] ? q : q0;
+            cwu[u] = H.ccells[qs];
+            if (RQ > 1) kcu[u] = H.ccellk[qs];
+        }
 #pragma unroll
-            for (int g = 0; g < NG; ++g) {
-                const float e0 = (ph[j][g][0] - f0) + pl[j][g][0], e1 = (ph[j][g][1] - f1) + pl[j][g][1],
-                            e2 = (ph[j][g][2] - f2) + pl[j][g][2];
-                float l = e0 * e0;
-                l = __builtin_fmaf(e1, e1, l);
-                l = __builtin_fmaf(e2, e2, l);
-                const float rr = __builtin_amdgcn_rsqf(l);
-                acc[j][g] = __builtin_amdgcn_mfma_f64_4x4x4f64((double)(rr * rr * rr), bv, acc[j][g], 0, 0, 0);
+        for (int u = 0; u < U; ++u) {
+            const uint4 cw = cwu[u];
+            const bool ok = oku[u];
+            const float f0 = __uint_as_float(cw.x), f1 = __uint_as_float(cw.y), f2 = __uint_as_float(cw.z);
+            // K: loaded when two requests share the cell (saves vector work),
+            // formed from D and the coordinates for one (saves a load)
+            int kc;
+            if (RQ > 1) {
+                kc = kcu[u];
+            } else {
+                const int D0 = (int)(int8_t)(cw.w & 255u), D1 = (int)(int8_t)((cw.w >> 8) & 255u),
+                          D2 = (int)(int8_t)((cw.w >> 16) & 255u);
+                kc = -(D0 * (int)f0 + D1 * (int)f1 + D2 * (int)f2);
             }
+            // B: the lane's coefficient of its cell, D_c (byte c of the packed
+            // vector) or K; cells past the chunk's end add nothing
+            const int dc = (int)(int8_t)((cw.w >> (8 * rc)) & 255u);
+            const double bv = (double)((rc == 3 ? kc : dc) & -(int)ok);
+#pragma unroll
+            for (int j = 0; j < RQ; ++j)
+#pragma unroll
+                for (int g = 0; g < NG; ++g) {
+                    const float e0 = (ph[j][g][0] - f0) + pl[j][g][0], e1 = (ph[j][g][1] - f1) + pl[j][g][1],
+                                e2 = (ph[j][g][2] - f2) + pl[j][g][2];
+                    float l = e0 * e0;
+                    l = __builtin_fmaf(e1, e1, l);
+                    l = __builtin_fmaf(e2, e2, l);
+                    const float rr = __builtin_amdgcn_rsqf(l);
+                    acc[j][g] =
+                        __builtin_amdgcn_mfma_f64_4x4x4f64((double)(rr * rr * rr), bv, acc[j][g], 0, 0, 0);
+                }
+        }
     }
     // acc[j][g] in lane 16 row + 4 block + col: lead 4g + row, coefficient
     // col, this wave's cells of one block; the four blocks in order
