@@ -148,7 +148,9 @@ def test_same_bits_alone_in_batch_and_in_mixed_groups(ctx):
     reqs2 = [_req(28, 3, 600 + i) for i in range(5)]
     alone1 = [g1.simulate(k, p, "3D4", 0.0, 250.0, 1.0) for k, p in reqs1]
     alone2 = [g2.simulate(k, p, "3D4", 0.0, 250.0, 1.0) for k, p in reqs2]
-    # a uniform batch of one heart runs chunk-major with classes
+    # a uniform batch of one heart runs chunk-major with classes, two requests
+    # per class-chunk block (five: the last is paired with itself); alone, one
+    # request per block with four cell groups per iteration
     batch = g1.simulate_batch(np.stack([k for k, _ in reqs1]), np.stack([p for _, p in reqs1]),
                               "3D4", 0.0, 250.0, 1.0)
     for i in range(5):
