@@ -1311,17 +1311,28 @@ constexpr int CL_WAVES = 4;
 template <int LP>
 __global__ __launch_bounds__(64 * CL_WAVES) void ecg_class_kernel(
     const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
-    const int64_t* __restrict__ cl_start, const double* __restrict__ layer_k,
+    const int64_t* __restrict__ cl_start, int cl_per_req, const double* __restrict__ layer_k,
     const double2* __restrict__ xy, const double* __restrict__ tgrid, const double* __restrict__ csums,
     double* __restrict__ ecg)
 {
     __shared__ double s_acc[CL_WAVES - 1][LP][64];
-    const int r = find_req(cl_start, B, blockIdx.x);
+    // Blocks go round-robin over the 8 XCDs.  With cl_per_req blocks for
+    // every request and B a multiple of 8, XCD x takes requests x, x + 8, ...
+    // whole, so each request's class sums are fetched into one L2.
+    int r, jb;
+    if (cl_per_req > 0) {
+        const int x = (int)(blockIdx.x & 7u), slot = (int)(blockIdx.x >> 3);
+        r = 8 * (slot / cl_per_req) + x;
+        jb = slot % cl_per_req;
+    } else {
+        r = find_req(cl_start, B, blockIdx.x);
+        jb = (int)(blockIdx.x - cl_start[r]);
+    }
     const ReqDev R = reqs[r];
     const HeartDev& H = hearts[R.heart];
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-    const int j0 = (int)(blockIdx.x - cl_start[r]) * 64;
+    const int j0 = jb * 64;
     if (j0 >= R.Tn) return;  // the whole block
     const int j = j0 + lane;
     const bool live = j < R.Tn;
@@ -1580,7 +1591,7 @@ hipError_t launch_ecg_class(const GroupArgs& a, hipStream_t s)
     case LPV:                                                                                           \
         hipLaunchKernelGGL(ecg_class_kernel<LPV>, dim3((unsigned)a.n_cl), dim3(64 * CL_WAVES), 0, s, a.hearts,  \
                            a.reqs,                                                                        \
-                           a.B, a.cl_start, a.layer_k, a.xy, a.tgrid, a.csums, a.ecg);                   \
+                           a.B, a.cl_start, a.cl_per_req, a.layer_k, a.xy, a.tgrid, a.csums, a.ecg);     \
         break;
         EKG_CLASS(4)
         EKG_CLASS(8)

@@ -819,6 +819,10 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     a.cc_uniform = B > 0 ? (int)(st_cc[1] - st_cc[0]) : 0;
     for (int r = 1; r < B && a.cc_uniform; ++r)
         if (st_cc[r + 1] - st_cc[r] != a.cc_uniform) a.cc_uniform = 0;
+    // class-kernel blocks: XCD-aware order when every request has the same count
+    a.cl_per_req = (B > 0 && B % 8 == 0) ? (int)(st_cl[1] - st_cl[0]) : 0;
+    for (int r = 1; r < B && a.cl_per_req; ++r)
+        if (st_cl[r + 1] - st_cl[r] != a.cl_per_req) a.cl_per_req = 0;
     // one heart: each class-chunk block takes a chunk for two requests
     a.cc_rpb = 1;
     bool one_heart = B > 1 && a.cc_uniform > 0;

@@ -118,6 +118,7 @@ struct GroupArgs {
     int64_t n_lf, n_ch, n_xy, n_lay, n_out, n_cc, n_cs, n_cl;  // totals
     int ch_uniform;            // chunks per request when all requests have the same count, else 0
     int cc_uniform;            // the same for class chunks
+    int cl_per_req;            // class-kernel blocks per request when equal for all and B % 8 == 0, else 0
     int cc_rpb;                // requests per class-chunk block: 2 when every request uses one heart, else 1
     const double* layer_k;
     const double* lead_pos;

@@ -170,3 +170,18 @@ def test_same_bits_alone_in_batch_and_in_mixed_groups(ctx):
     assert rel_l2(alone1[0], ref) <= TOL
     g1.close()
     g2.close()
+
+
+def test_batch_of_eight_same_bits_as_alone(ctx):
+    """A uniform batch whose size is a multiple of 8 takes the XCD-ordered
+    class-kernel blocks (each request's blocks on one XCD) and the paired
+    class-chunk blocks; every request equals its single call bitwise."""
+    h, g, act = _int_heart(ctx, 24, 4)
+    reqs = [_req(24, 5, 700 + i) for i in range(16)]
+    batch = g.simulate_batch(np.stack([k for k, _ in reqs]), np.stack([p for _, p in reqs]),
+                             "3D4", 0.0, 300.0, 1.0)
+    for i, (k, p) in enumerate(reqs):
+        assert np.array_equal(batch[i], g.simulate(k, p, "3D4", 0.0, 300.0, 1.0)), i
+    ref = O.run(h.layer, act, reqs[9][0], reqs[9][1], "3D4", 0.0, 300.0, 1.0)
+    assert rel_l2(batch[9], ref) <= TOL
+    g.close()
