@@ -185,3 +185,19 @@ def test_batch_of_eight_same_bits_as_alone(ctx):
     ref = O.run(h.layer, act, reqs[9][0], reqs[9][1], "3D4", 0.0, 300.0, 1.0)
     assert rel_l2(batch[9], ref) <= TOL
     g.close()
+
+
+@pytest.mark.parametrize("L", [2, 7, 16])
+def test_batch_lead_packs_same_bits_as_alone(ctx, L):
+    """Paired class-chunk blocks at 4, 8 and 16 lead slots (2, 7 and 16
+    leads; an odd batch, so the last request is paired with itself) give each
+    request the bits of its single call, within 1e-5 of the FP64 oracle."""
+    h, g, act = _int_heart(ctx, 22, 8)
+    reqs = [_req(22, L, 40 * L + i) for i in range(3)]
+    batch = g.simulate_batch(np.stack([k for k, _ in reqs]), np.stack([p for _, p in reqs]),
+                             "3D8", 0.0, 250.0, 1.0)
+    for i, (k, p) in enumerate(reqs):
+        assert np.array_equal(batch[i], g.simulate(k, p, "3D8", 0.0, 250.0, 1.0)), i
+    ref = O.run(h.layer, act, reqs[2][0], reqs[2][1], "3D8", 0.0, 250.0, 1.0)
+    assert rel_l2(batch[2], ref) <= TOL
+    g.close()
