@@ -484,8 +484,9 @@ void ensure_layout(Heart& h, bool mfma)
 //   sum_{u in S} G(u) = sum_w m(w) . D(w),
 //   D(w) = [w in S] s_w + sum_{d in N, w - d in S, w myocardial} d,
 // an integer vector per cell; cells inside S cancel (D = 0) and are dropped.
-// Each remaining cell's weight is evaluated once per class chunk.  A cell is
-// stored as {grid coordinates + 1 as three floats, D as three int8}.
+// Each remaining cell's weight is evaluated once per class chunk and request.
+// A cell is stored as {grid coordinates + 1 as three floats, D as three int8},
+// with its moment K = -D . (grid coordinates + 1) in a second array.
 void ensure_cells(Heart& h, int nhood)
 {
     EcgLayout& L = h.layout;
