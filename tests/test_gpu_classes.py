@@ -201,3 +201,39 @@ def test_batch_lead_packs_same_bits_as_alone(ctx, L):
     ref = O.run(h.layer, act, reqs[2][0], reqs[2][1], "3D8", 0.0, 250.0, 1.0)
     assert rel_l2(batch[2], ref) <= TOL
     g.close()
+
+
+def test_class_cells_above_dense_grid_limit(ctx):
+    """Grids above 2^27 cells build the class cells through a hash map
+    instead of the dense cell->slot array.  The same heart in a 40^3 grid and
+    at the far corner of a 520^3 grid (electrodes moved with it) has the same
+    activation times, the same classes and class cells, and the same ECG up
+    to the FP32 rounding of the electrode offsets."""
+    from ekgsim_amd import Heart
+    n, N = 40, 520
+    o = N - n - 1
+    h = S.make_heart(n, seed=2)
+    big = {}
+    for name in ("layer", "seed", "delay"):
+        a = np.zeros((N, N, N), np.uint8)
+        a[o:o + n, o:o + n, o:o + n] = getattr(h, name)
+        big[name] = a
+    g1 = Heart(ctx, h.layer, h.seed)
+    g1.excite_delay(h.delay)
+    g2 = Heart(ctx, big["layer"], big["seed"])
+    g2.excite_delay(big["delay"])
+    a1 = g1.activation()
+    a2 = g2.activation()[o:o + n, o:o + n, o:o + n]
+    assert np.array_equal(a1, a2)
+    i1, i2 = g1.layout_info("3D4"), g2.layout_info("3D4")
+    assert i2["classes"] > 0
+    for key in ("voxels", "class_voxels", "class_cells", "classes", "class_chunks"):
+        assert i1[key] == i2[key], key
+    k, pos = _req(n, 6, 99)
+    e1 = g1.simulate(k, pos, "3D4", 0.0, 300.0, 1.0)
+    e2 = g2.simulate(k, pos + o, "3D4", 0.0, 300.0, 1.0)
+    err = rel_l2(e2, e1)
+    print(f"40^3 heart in a 520^3 grid: rel-L2 {err:.3e} against the 40^3 grid")
+    assert err <= 1e-6
+    g1.close()
+    g2.close()
