@@ -245,3 +245,54 @@ def test_mixed_shapes_one_group_equal_alone(ctx):
         assert np.array_equal(got, alone)
         ref = O.run(h.layer, act, k, pos, "3D4", start, length, dt)
         assert rel_l2(got, ref) <= TOL
+
+
+# --- degenerate sizes ---------------------------------------------------------
+
+@pytest.mark.parametrize("length,dt", [(0.0, 1.0), (0.5, 1.0), (1.0, 1.0), (3.0, 1.0)])
+def test_ecg_parity_zero_one_and_few_samples(ctx, length, dt):
+    """Durations of zero, one and a few samples: the same number of samples
+    as the reference's rule and the oracle's values; one lead."""
+    h, act, k, pos = _case(20, 4)
+    pos = pos[:1]
+    ref = O.run(h.layer, act, k, pos, "3D4", 0.0, length, dt)
+    g = _heart(ctx, h)
+    g.set_activation(act)
+    got = g.simulate(k, pos, "3D4", 0.0, length, dt)
+    assert got.shape == ref.shape == (1, O.num_steps(length, dt))
+    if ref.size:
+        assert np.abs(got - ref).max() <= TOL * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.parametrize("nhood", ["3D4", "3D8"])
+def test_two_voxel_and_single_layer_hearts(ctx, nhood):
+    """A heart of two voxels (one dipole; at class threshold 1 each voxel is
+    a class of its own) and a one-layer heart, with activation classes on
+    and off."""
+    from ekgsim_amd import Heart
+    lay = np.zeros((5, 5, 5), np.uint8)
+    lay[2, 2, 2] = lay[3, 3, 3] = 1  # corner neighbours: a dipole in both 3-D neighbourhoods
+    seed = np.zeros_like(lay)
+    seed[2, 2, 2] = 1
+    act = O.excite_int(lay, seed, np.ones_like(lay)).astype(np.float64)
+    x = S.random_params(3, 2, 11)
+    k = S.layer_k_from_params(x, 3)
+    pos = np.array([[2.0, 9.5, 4.0], [-3.0, 1.5, 7.25]])
+    ref = O.run(lay, act, k, pos, nhood, 0.0, 300.0, 1.0)
+    assert np.abs(ref).max() > 0
+    g = Heart(ctx, lay, seed)
+    g.excite_delay(np.ones_like(lay))
+    for cm in (0, 1, -1):
+        g.set_tune(class_min=cm)
+        got = g.simulate(k, pos, nhood, 0.0, 300.0, 1.0)
+        assert np.abs(got - ref).max() <= TOL * max(1.0, np.abs(ref).max()), cm
+    g.close()
+    h1 = S.make_heart(20, seed=6, n_layers=1)
+    act1 = O.excite_int(h1.layer, h1.seed, h1.delay).astype(np.float64)
+    k1 = S.layer_k_from_params(S.random_params(1, 3, 12), 1)
+    pos1 = S.lead_positions(20, 3)
+    ref1 = O.run(h1.layer, act1, k1, pos1, nhood, 0.0, 300.0, 1.0)
+    g1 = Heart(ctx, h1.layer, h1.seed)
+    g1.excite_delay(h1.delay)
+    assert rel_l2(g1.simulate(k1, pos1, nhood, 0.0, 300.0, 1.0), ref1) <= TOL
+    g1.close()
