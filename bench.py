@@ -366,6 +366,18 @@ def main():
             lat.append(time.perf_counter() - t1)
     plain = len(lat) / sum(lat)
     log(f"plain drop-in: {plain:.1f} ECGs/s (p50 {np.percentile(lat, 50) * 1e3:.2f} ms)")
+    # the same calls with the context's graph mode on (off by default): the
+    # launches of the single-request group replay from a captured hipGraph
+    ctx.set_graphs(True)
+    lat_g = []
+    for i in range(args.plain_calls + 3):
+        t1 = time.perf_counter()
+        heart.simulate(ks[i % B], pos[i % B], "3D4", 0.0, args.length, 1.0)
+        if i >= 3:
+            lat_g.append(time.perf_counter() - t1)
+    ctx.set_graphs(False)
+    plain_g = len(lat_g) / sum(lat_g)
+    log(f"plain drop-in, graph mode: {plain_g:.1f} ECGs/s (p50 {np.percentile(lat_g, 50) * 1e3:.2f} ms)")
 
     # ---- serving burst: 4 cached hearts, Poisson arrivals, mixed shapes ----
     serve = None
@@ -431,6 +443,8 @@ def main():
         "plain_ecgs_per_s": round(plain, 2),
         "plain_p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
         "plain_p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+        "plain_graph_ecgs_per_s": round(plain_g, 2),
+        "plain_graph_p50_ms": round(float(np.percentile(lat_g, 50)) * 1e3, 3),
         "batched_vs_plain": round(value / plain, 2),
         "serving": serve,
         "p99_ms": None if serve is None else round(serve["p99_ms"], 3),
