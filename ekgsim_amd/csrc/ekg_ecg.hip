@@ -1133,7 +1133,8 @@ constexpr int CC_THREADS = 256;
 template <int LP, int RQ>
 __global__ __launch_bounds__(CC_THREADS) void ecg_class_chunk_kernel(
     const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
-    const int64_t* __restrict__ cc_start, int cc_uniform, const double* __restrict__ lead_pos,
+    const int64_t* __restrict__ cc_start, int cc_uniform, const int2* __restrict__ cc_units,
+    const int64_t* __restrict__ cc_ustart, int n_units, const double* __restrict__ lead_pos,
     const double* __restrict__ layer_k, double* __restrict__ sums)
 {
     constexpr int NG = LP / 4;                 // groups of 4 leads
@@ -1144,7 +1145,14 @@ __global__ __launch_bounds__(CC_THREADS) void ecg_class_chunk_kernel(
     // request is paired with itself), else over requests when every request
     // has the same class chunks
     int rq[RQ], k;
-    if (RQ > 1) {
+    if (RQ > 1 && cc_units != nullptr) {
+        // pairs of one heart in a group of several hearts
+        const int u = find_req(cc_ustart, n_units, blockIdx.x);
+        k = (int)(blockIdx.x - cc_ustart[u]);
+        const int2 pr = cc_units[u];
+        rq[0] = pr.x;
+        rq[RQ - 1] = pr.y;
+    } else if (RQ > 1) {
         const int np = (B + RQ - 1) / RQ;
         k = (int)(blockIdx.x / (unsigned)np);
         rq[0] = RQ * (int)(blockIdx.x % (unsigned)np);
@@ -1543,10 +1551,12 @@ static void class_chunk_launch(const GroupArgs& a, hipStream_t s)
     // four needs 116 VGPRs, 4 waves per SIMD, and is slower)
     if (a.cc_rpb == 2)
         hipLaunchKernelGGL((ecg_class_chunk_kernel<LP, 2>), dim3((unsigned)a.n_cc), dim3(CC_THREADS), 0, s, a.hearts,
-                           a.reqs, a.B, a.cc_start, a.cc_uniform, a.lead_pos, a.layer_k, a.sums);
+                           a.reqs, a.B, a.cc_start, a.cc_uniform, a.cc_units, a.cc_ustart, a.n_units, a.lead_pos,
+                           a.layer_k, a.sums);
     else
         hipLaunchKernelGGL((ecg_class_chunk_kernel<LP, 1>), dim3((unsigned)a.n_cc), dim3(CC_THREADS), 0, s, a.hearts,
-                           a.reqs, a.B, a.cc_start, a.cc_uniform, a.lead_pos, a.layer_k, a.sums);
+                           a.reqs, a.B, a.cc_start, a.cc_uniform, a.cc_units, a.cc_ustart, a.n_units, a.lead_pos,
+                           a.layer_k, a.sums);
 }
 
 hipError_t launch_class_chunks(const GroupArgs& a, hipStream_t s)

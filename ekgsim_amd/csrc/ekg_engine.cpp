@@ -748,6 +748,25 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
         ws.ecg.reserve((size_t)std::max<int64_t>(1, st_out[B]) * 8);
         d_ecg = ws.ecg.as<double>();
     }
+    // A group of several hearts: neighbouring requests of one heart form
+    // units of two for the class-chunk kernel (each cell loaded and unpacked
+    // once for both); a request without such a neighbour is a unit alone.
+    std::vector<int2> units;
+    std::vector<int64_t> st_u(1, 0);
+    bool several_hearts = false;
+    for (int r = 1; r < B && !several_hearts; ++r) several_hearts = g.reqs[r].h != g.reqs[0].h;
+    if (g.mfma && several_hearts && st_cc[B] > 0) {
+        for (int r = 0; r < B;) {
+            const bool pair = r + 1 < B && g.reqs[r + 1].h == g.reqs[r].h;
+            units.push_back(make_int2(r, pair ? r + 1 : r));
+            st_u.push_back(st_u.back() + (st_cc[r + 1] - st_cc[r]));
+            r += pair ? 2 : 1;
+        }
+        if (units.size() == (size_t)B) {  // no pairs: one request per block
+            units.clear();
+            st_u.assign(1, 0);
+        }
+    }
     // tables: uploaded only when they changed since the last group on this
     // workspace (a repeated group replays without host traffic)
     std::vector<char> tables;
@@ -757,6 +776,10 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     const size_t o_st = tables.size();
     for (auto* st : {&st_lf, &st_ch, &st_xy, &st_lay, &st_out, &st_cc, &st_cs, &st_cl})
         append(tables, st->data(), st->size());
+    const size_t o_ust = tables.size();
+    append(tables, st_u.data(), st_u.size());
+    const size_t o_units = tables.size();
+    append(tables, units.data(), units.size());
     void* before = ws.hearts.p;
     ws.hearts.reserve(tables.size());
     if (ws.hearts.p != before) ws.h_tables.clear();
@@ -832,6 +855,12 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     if (one_heart) {
         a.cc_rpb = 2;
         a.n_cc = (int64_t)a.cc_uniform * ((B + a.cc_rpb - 1) / a.cc_rpb);
+    } else if (!units.empty()) {
+        a.cc_rpb = 2;
+        a.cc_units = reinterpret_cast<const int2*>(tb + o_units);
+        a.cc_ustart = reinterpret_cast<const int64_t*>(tb + o_ust);
+        a.n_units = (int)units.size();
+        a.n_cc = st_u.back();
     }
     a.n_xy = st_xy[B];
     a.n_lay = st_lay[B];

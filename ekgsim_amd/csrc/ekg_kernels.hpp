@@ -119,7 +119,14 @@ struct GroupArgs {
     int ch_uniform;            // chunks per request when all requests have the same count, else 0
     int cc_uniform;            // the same for class chunks
     int cl_per_req;            // class-kernel blocks per request when equal for all and B % 8 == 0, else 0
-    int cc_rpb;                // requests per class-chunk block: 2 when every request uses one heart, else 1
+    int cc_rpb;                // requests per class-chunk block: 2 when requests are paired, else 1
+    // Pairs of requests of one heart in a group of several hearts (cc_rpb 2,
+    // cc_uniform 0): unit u covers requests cc_units[u] = {r0, r1} (r1 == r0
+    // for a request without a partner), class-chunk blocks
+    // [cc_ustart[u], cc_ustart[u+1]).  Null when the group has one heart.
+    const int2* cc_units;
+    const int64_t* cc_ustart;
+    int n_units;
     const double* layer_k;
     const double* lead_pos;
     const double* tgrid;       // FP64 sample times

@@ -174,6 +174,26 @@ void Server::run()
                         plan = plan_group(ctx, specs);
                         bytes = group_device_bytes(plan);
                     }
+                    // requests of one heart side by side (in arrival order):
+                    // the launch pairs neighbours of one heart
+                    std::vector<size_t> ord(group.size());
+                    for (size_t i = 0; i < ord.size(); ++i) ord[i] = i;
+                    std::stable_sort(ord.begin(), ord.end(),
+                                     [&](size_t x, size_t y) { return group[x]->heart_id < group[y]->heart_id; });
+                    bool moved = false;
+                    for (size_t i = 0; i < ord.size(); ++i) moved = moved || ord[i] != i;
+                    if (moved) {
+                        std::vector<Request*> g2;
+                        std::vector<ReqSpec> s2;
+                        for (size_t i : ord) {
+                            g2.push_back(group[i]);
+                            s2.push_back(specs[i]);
+                        }
+                        group.swap(g2);
+                        specs.swap(s2);
+                        plan = plan_group(ctx, specs);
+                        bytes = group_device_bytes(plan);
+                    }
                 } catch (const Error& e) {
                     std::vector<Request*> one{head};
                     pending.pop_front();
