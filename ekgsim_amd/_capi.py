@@ -89,6 +89,7 @@ class ServerOpts(C.Structure):
         ("max_inflight", C.c_int),
         ("max_batch", C.c_int),
         ("n_streams", C.c_int),
+        ("batch_window_us", C.c_int),
     ]
 
 
@@ -521,10 +522,12 @@ class Heart:
 
 
 class Server:
-    def __init__(self, ctx: Context, hbm_budget_bytes=0, max_inflight=0, max_batch=0, n_streams=0):
+    def __init__(self, ctx: Context, hbm_budget_bytes=0, max_inflight=0, max_batch=0, n_streams=0,
+                 batch_window_us=0):
         self.ctx = ctx
         self.h = C.c_void_p()
-        o = ServerOpts(int(hbm_budget_bytes), int(max_inflight), int(max_batch), int(n_streams))
+        o = ServerOpts(int(hbm_budget_bytes), int(max_inflight), int(max_batch), int(n_streams),
+                       int(batch_window_us))
         check(lib().ekg_server_create(ctx.h, C.byref(o), C.byref(self.h)))
         self._hearts = []
         self._shapes = {}

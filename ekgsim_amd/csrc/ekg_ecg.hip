@@ -1613,6 +1613,43 @@ hipError_t launch_ecg_class(const GroupArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
+// Every kernel of the matrix-core path resolved once by an attribute query,
+// so that the first launch of an instantiation (a lead-slot width or
+// neighbourhood a long-lived server has not used yet) does not pay the HIP
+// runtime's lazy per-kernel load inside a request's latency.
+template <int LP, int NH>
+static void prime_nh(hipFuncAttributes& at)
+{
+    (void)hipFuncGetAttributes(&at, (const void*)ecg_chunk_kernel<LP, NH, 128>);
+    (void)hipFuncGetAttributes(&at, (const void*)ecg_chunk_kernel<LP, NH, 256>);
+    (void)hipFuncGetAttributes(&at, (const void*)ecg_chunk_kernel<LP, NH, 512>);
+}
+
+template <int LP>
+static void prime_lp(hipFuncAttributes& at)
+{
+    prime_nh<LP, 0>(at);
+    prime_nh<LP, 1>(at);
+    prime_nh<LP, 2>(at);
+    prime_nh<LP, 3>(at);
+    (void)hipFuncGetAttributes(&at, (const void*)ecg_class_chunk_kernel<LP, 1>);
+    (void)hipFuncGetAttributes(&at, (const void*)ecg_class_chunk_kernel<LP, 2>);
+    (void)hipFuncGetAttributes(&at, (const void*)class_sums_kernel<LP>);
+    (void)hipFuncGetAttributes(&at, (const void*)layer_prefix_kernel<LP>);
+    (void)hipFuncGetAttributes(&at, (const void*)ecg_reduce_kernel<LP>);
+    (void)hipFuncGetAttributes(&at, (const void*)ecg_class_kernel<LP>);
+}
+
+void prime_kernels()
+{
+    hipFuncAttributes at;
+    (void)hipFuncGetAttributes(&at, (const void*)ap_tables_kernel);
+    prime_lp<4>(at);
+    prime_lp<8>(at);
+    prime_lp<12>(at);
+    prime_lp<16>(at);
+}
+
 hipError_t launch_ecg_reduce(const GroupArgs& a, hipStream_t s)
 {
     if (a.n_out == 0) return hipSuccess;

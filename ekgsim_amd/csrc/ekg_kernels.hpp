@@ -152,6 +152,8 @@ hipError_t launch_ecg_fused(const GroupArgs& a, hipStream_t s);
 // Matrix-core path: lead field, chunk sums and transition tiles in one kernel.
 hipError_t launch_ecg_chunk(const GroupArgs& a, hipStream_t s);
 hipError_t launch_ecg_reduce(const GroupArgs& a, hipStream_t s);
+// Resolves every matrix-core-path kernel once (see ekg_ecg.hip).
+void prime_kernels();
 // Activation classes: the lead-weight sums of the class chunks, their sums
 // per class, then the classes' terms added to the reduced ECG.
 hipError_t launch_class_chunks(const GroupArgs& a, hipStream_t s);

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <mutex>
 #include <cstring>
 #include <deque>
 #include <memory>
@@ -145,7 +146,17 @@ void Server::run()
                 break;
             }
         bool launched = false;
-        if (free_slot && !pending.empty() && !stop) {
+        // Batching window: while another group runs (the GPU is busy anyway),
+        // a group below max_batch waits for more requests, up to
+        // batch_window_us from its oldest request; an idle GPU starts at once.
+        bool hold = false;
+        if (free_slot && !pending.empty() && !stop && any_busy && opts.batch_window_us > 0 &&
+            (int)pending.size() < opts.max_batch) {
+            const auto age = std::chrono::duration_cast<std::chrono::microseconds>(Clock::now() -
+                                                                                  pending.front()->t_submit);
+            hold = age.count() < opts.batch_window_us;
+        }
+        if (free_slot && !pending.empty() && !stop && !hold) {
             Request* head = pending.front();
             std::vector<Request*> group;
             const int cap_inflight = opts.max_inflight - inflight;
@@ -294,7 +305,23 @@ Server* server_create(Ctx& ctx, const ekg_server_opts& o)
     if (s->opts.max_batch <= 0) s->opts.max_batch = 256;
     if (s->opts.n_streams <= 0) s->opts.n_streams = 2;
     if (s->opts.n_streams > 8) fail(EKG_EINVAL, "at most 8 streams");
+    if (s->opts.batch_window_us == 0) s->opts.batch_window_us = 1000;
     ctx.bind();
+    // a server's requests may use any lead-slot width: resolve every kernel
+    // on this device now rather than at its first launch inside a request's
+    // latency (once per device and process)
+    {
+        static std::mutex prime_mu;
+        static std::vector<int> primed;
+        int dev = -1;
+        if (hipGetDevice(&dev) == hipSuccess) {
+            std::lock_guard<std::mutex> g(prime_mu);
+            if (std::find(primed.begin(), primed.end(), dev) == primed.end()) {
+                ekgk::prime_kernels();
+                primed.push_back(dev);
+            }
+        }
+    }
     for (int i = 0; i < s->opts.n_streams; ++i) {
         s->slots.push_back(std::make_unique<Slot>());
         Slot& sl = *s->slots.back();

@@ -319,6 +319,9 @@ typedef struct ekg_server_opts {
     int max_inflight;         /* requests on the GPU at once (0 = 1024) */
     int max_batch;            /* requests per launch group (0 = 256) */
     int n_streams;            /* concurrent launch groups (0 = 2) */
+    int batch_window_us;      /* while another group runs, a group below max_batch waits up to
+                                 this long (from its oldest request) for more requests
+                                 (0 = 1000 us; < 0 = launch at once) */
 } ekg_server_opts;
 
 int ekg_server_create(ekg_ctx *ctx, const ekg_server_opts *opts, ekg_server **out);
