@@ -33,7 +33,7 @@ EXPORTS = (
     "ekg_ctx_synchronize", "ekg_ctx_set_kernel", "ekg_ap_value", "ekg_ap_num_genes", "ekg_layer_aps",
     "ekg_criterion", "ekg_resample", "ekg_heart_autotune", "ekg_heart_set_tune", "ekg_heart_get_tune",
     "ekg_ctx_set_graphs", "ekg_ctx_graph_stats", "ekg_heart_layout_info", "ekg_criteria_device",
-    "ekg_simulate_criteria",
+    "ekg_simulate_criteria", "ekg_ap_apd90",
 )
 KERNELS = {"auto": 0, "valu": 1, "mfma": 2}
 
@@ -145,6 +145,8 @@ def lib():
                                    C.POINTER(C.c_int), C.POINTER(C.c_size_t)]
     L.ekg_ap_value.argtypes = [vp, C.c_double]
     L.ekg_ap_value.restype = C.c_double
+    L.ekg_ap_apd90.argtypes = [vp]
+    L.ekg_ap_apd90.restype = C.c_double
     L.ekg_ap_num_genes.argtypes = [C.POINTER(ApModel), C.POINTER(C.c_int)]
     L.ekg_layer_aps.argtypes = [C.POINTER(ApModel), C.c_int, vp, C.c_int, vp, pd]
     L.ekg_criterion.argtypes = [C.c_int, vp, C.c_int64, vp, C.c_int64, C.c_double, pd]
@@ -217,6 +219,13 @@ def ap_value(k, t):
     """WohlfartPlus W(t) in FP64 (host)."""
     k = np.ascontiguousarray(k, np.float64)
     return float(lib().ekg_ap_value(_p(k), float(t)))
+
+
+def ap_apd90(k):
+    """WohlfartPlus::apd90 of coefficients k (host, FP64; -1 if the curve never
+    falls through k0 + k2/10 within 1000 ms)."""
+    k = np.ascontiguousarray(k, np.float64)
+    return float(lib().ekg_ap_apd90(_p(k)))
 
 
 def _ap_model(base_k, k_min, k_max, free_k, interpolation, mid_position):

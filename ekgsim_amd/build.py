@@ -30,8 +30,11 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HOST = ["-ffp-contract=off", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include")]
 
 KERNEL_SRC = ["ekg_ca.hip", "ekg_ecg.hip", "ekg_crit.hip"]
-HOST_SRC = ["ekg_engine.cpp", "ekg_capi.cpp", "ekg_server.cpp", "ekg_model.cpp"]
-CLI_SRC = ["ini.cpp", "sim_lib.cpp", "ekgsim_app.cpp", "ekgsim_main.cpp"]
+# The C++ facade (sim_lib.cpp with its ini reader) is part of the library, so
+# a caller written against the reference's SimLib::EkgSim links libekgsim.so
+# alone.
+HOST_SRC = ["ekg_engine.cpp", "ekg_capi.cpp", "ekg_server.cpp", "ekg_model.cpp", "ini.cpp", "sim_lib.cpp"]
+CLI_SRC = ["ekgsim_app.cpp", "ekgsim_main.cpp"]
 
 
 def _run(cmd):

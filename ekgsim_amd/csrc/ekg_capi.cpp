@@ -451,6 +451,8 @@ int ekg_ctx_graph_stats(ekg_ctx* ctx, int64_t* captures, int64_t* launches)
 
 double ekg_ap_value(const double k[9], double t) { return ekg::ap_value(k, t); }
 
+double ekg_ap_apd90(const double k[9]) { return ekg::ap_apd90(k); }
+
 int ekg_ap_num_genes(const ekg_ap_model* m, int* n_genes)
 {
     return guard([&] {

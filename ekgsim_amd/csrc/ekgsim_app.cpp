@@ -112,32 +112,8 @@ void SimSettings::load(const char* fname)
 
 std::vector<std::vector<double>> readColumns(const std::string& fname)
 {
-    const size_t dot = fname.rfind('.');
-    if (dot == std::string::npos || fname.substr(dot) != ".column")
-        throw std::runtime_error("ColumnFile::load only works on .column files");
-    std::ifstream in(fname.c_str());
-    if (!in.is_open()) throw std::runtime_error("could not open " + fname);
-    while (in.peek() == '#' || in.peek() == '%') in.ignore(10000, '\n');
-    std::string line;
-    std::getline(in, line);
-    std::vector<double> row;
-    {
-        std::istringstream ls(line);
-        double d;
-        while (ls >> d) row.push_back(d);
-    }
-    if (row.empty()) throw std::runtime_error("File has 0 columns");
-    std::vector<std::vector<double>> cols(row.size());
-    bool ok = (bool)in;
-    while (ok) {
-        for (size_t i = 0; i < cols.size(); ++i) cols[i].push_back(row[i]);
-        std::getline(in, line);
-        if (!in) break;
-        std::istringstream ls(line);
-        for (size_t i = 0; i < row.size(); ++i) ls >> row[i];
-        ok = (bool)ls;
-    }
-    return cols;
+    ColumnFile f(fname.c_str());
+    return f.columns();
 }
 
 void resample(const std::vector<double>& src, std::vector<double>& dest, double factor, int startOffset)
@@ -444,7 +420,7 @@ void Evaluator::evalBatch(const std::vector<std::vector<double>>& xs, std::vecto
         std::vector<ekg_sim_desc> descs(run.size());
         for (size_t r = 0; r < run.size(); ++r) {
             const Prepared& p = prep[run[r]];
-            for (size_t l = 0; l < nl; ++l) std::copy(p.layerAps[l].k, p.layerAps[l].k + 9, &k[(r * nl + l) * 9]);
+            for (size_t l = 0; l < nl; ++l) std::copy(p.layerAps[l].getK(), p.layerAps[l].getK() + 9, &k[(r * nl + l) * 9]);
             for (size_t i = 0; i < L; ++i)
                 for (int c = 0; c < 3; ++c) pos[(r * L + i) * 3 + c] = p.points[i][c];
             ekg_sim_desc& d = descs[r];

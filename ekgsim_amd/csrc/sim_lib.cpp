@@ -92,12 +92,15 @@ double char_value(char ch)
 
 void ActionPotential::init(const double* wohlK, double newAt)
 {
+    // ActionPotential::init (simulator.cpp:154-160): k8 moves with the
+    // activation time, so the curve is read at t - at
+    double* k = wohl.getK();
     for (int i = 0; i < 9; ++i) k[i] = wohlK[i];
     k[8] -= newAt;
     at = newAt;
 }
 
-void ActionPotential::init(const ActionPotential& ap, double newAt) { init(ap.k, newAt); }
+void ActionPotential::init(const ActionPotential& ap, double newAt) { init(ap.getK(), newAt); }
 
 // ---- Settings --------------------------------------------------------------
 
@@ -265,7 +268,7 @@ void generateTestShape(ModelShape& s)
     exportShape(s, "autoGeneratedShape.matrix", false);
 }
 
-void exportVectors(const std::vector<ColumnVec>& vec, const std::string& fname, double startTime, double timeStep,
+void exportVectors(const std::vector<saveVecElement>& vec, const std::string& fname, double startTime, double timeStep,
                    const std::string& comment)
 {
     // exportVectors (simulator.h:194-236); the stream keeps the precision
@@ -275,7 +278,7 @@ void exportVectors(const std::vector<ColumnVec>& vec, const std::string& fname, 
         throw std::runtime_error("saving vectors to .column files only for the time being");
     std::ofstream out(fname.c_str());
     out << "#Comment: " << comment << '\n';
-    for (const ColumnVec& v : vec)
+    for (const saveVecElement& v : vec)
         if (!v.comment.empty()) out << "# " << v.name << " : " << v.comment << '\n';
     out << "# \n";
     out << (timeStep > 0 ? "#Time[ms]\t" : "# ");
@@ -494,7 +497,7 @@ std::vector<double> EkgSim::layerCoefficients() const
     const std::vector<ActionPotential>& src = aps_are_cells_ ? layer_aps_ : aps_;
     std::vector<double> k(9 * src.size());
     for (size_t l = 0; l < src.size(); ++l)
-        for (int i = 0; i < 9; ++i) k[9 * l + i] = src[l].k[i];
+        for (int i = 0; i < 9; ++i) k[9 * l + i] = src[l][i];
     return k;
 }
 
@@ -573,3 +576,38 @@ void EkgSim::saveMeasurement(const std::string& comment) { saveTo(settings_.outp
 void EkgSim::saveMeasurementAs(const char* fname, const std::string& comment) { saveTo(fname, comment); }
 
 }  // namespace SimLib
+
+// ---- ColumnFile --------------------------------------------------------------
+
+void ColumnFile::load(const char* fname)
+{
+    // ColumnFile::load (columnFile.h:124-170): skip '#'/'%' lines, the first
+    // data line fixes the column count, then one row per line until a line
+    // holds fewer numbers or the file ends
+    const std::string f(fname);
+    const size_t dot = f.rfind('.');
+    if (dot == std::string::npos || f.substr(dot) != ".column")
+        throw std::runtime_error("ColumnFile::load only works on .column files");
+    std::ifstream in(fname);
+    if (!in.is_open()) throw std::runtime_error("could not open " + f);
+    while (in.peek() == '#' || in.peek() == '%') in.ignore(10000, '\n');
+    std::string line;
+    std::getline(in, line);
+    std::vector<double> row;
+    {
+        std::istringstream ls(line);
+        double d;
+        while (ls >> d) row.push_back(d);
+    }
+    if (row.empty()) throw std::runtime_error("File has 0 columns");
+    cols_.assign(row.size(), Column());
+    bool ok = (bool)in;
+    while (ok) {
+        for (size_t i = 0; i < cols_.size(); ++i) cols_[i].push_back(row[i]);
+        std::getline(in, line);
+        if (!in) break;
+        std::istringstream ls(line);
+        for (size_t i = 0; i < row.size(); ++i) ls >> row[i];
+        ok = (bool)ls;
+    }
+}

@@ -31,27 +31,87 @@ namespace ekgapp {
 class IniFile;
 }
 
+// WohlfartPlus (simlib/Wohlfart.h:167-230): the nine-coefficient AP curve.
+// Values come from ekg_ap_value, which keeps the reference's FP64 operation
+// order; apd90 from ekg_ap_apd90.  Declared at global scope, where the
+// reference's callers name it (sim.cpp:521, 540).
+class WohlfartPlus {
+public:
+    static const size_t numParams = 9;
+
+    WohlfartPlus() {}
+    explicit WohlfartPlus(const double newK[numParams]) { setK(newK); }
+    void setK(const double newK[numParams])
+    {
+        for (size_t i = 0; i < numParams; ++i) k_[i] = newK[i];
+    }
+    void setK(const std::vector<double>& newK)
+    {
+        for (size_t i = 0; i < newK.size() && i < numParams; ++i) k_[i] = newK[i];
+    }
+    const double* getK() const { return k_; }
+    double* getK() { return k_; }
+    // value of the curve at time t (ms after activation)
+    double operator[](double t) const { return ekg_ap_value(k_, t); }
+    // time at which the curve last falls through k0 + k2/10 over 0..999 ms
+    // (-1 if it never does)
+    double apd90() const { return ekg_ap_apd90(k_); }
+
+private:
+    double k_[numParams] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+// .column reader (simlib/columnFile.h:84-186, reached through the
+// reference's simlib headers): equal-length columns of doubles.  Comment lines
+// start with '#' or '%'; the first data line fixes the column count; reading
+// stops at the first line with fewer numbers.  The guard macro is the
+// reference header's, so a caller that also includes that header directly
+// (main.cpp:82) gets one definition.
+#ifndef COLUMNFILEL_H_INCLUDED
+#define COLUMNFILEL_H_INCLUDED
+typedef std::vector<double> Column;
+typedef std::vector<Column> Columns;
+
+class ColumnFile {
+public:
+    ColumnFile() {}
+    explicit ColumnFile(const char* fname) { load(fname); }
+    Columns& columns() { return cols_; }
+    const Columns& columns() const { return cols_; }
+    Column& operator[](size_t p) { return cols_[p]; }
+    const Column& operator[](size_t p) const { return cols_[p]; }
+    size_t numColumns() const { return cols_.size(); }
+    size_t columnsLength() const { return cols_.empty() ? 0 : cols_[0].size(); }
+    // throws std::runtime_error for other extensions, unreadable files and
+    // files without a data line
+    void load(const char* fname);
+
+private:
+    Columns cols_;
+};
+#endif
+
 namespace SimLib {
 
 // (z, y, x) position or displacement, the storage order of the reference's
 // measuring points (simulator.cpp:377-381).
 typedef std::array<double, 3> PositionVec;
 
-// ActionPotential (simulator.h:467-480): WohlfartPlus coefficients with
-// k8 already reduced by the activation time `at`; the value at time t is
+// ActionPotential (simulator.h:467-480): a WohlfartPlus curve whose k8 is
+// already reduced by the activation time `at`; the value at time t is
 // W(t - at).
 struct ActionPotential {
-    double k[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    WohlfartPlus wohl;
     double at = 0;
 
     void init(const double* wohlK, double newAt);
     void init(const ActionPotential& ap, double newAt);
-    double operator()(double time) const { return ekg_ap_value(k, time - at); }
-    double& operator[](size_t i) { return k[i]; }
-    double operator[](size_t i) const { return k[i]; }
-    const double* getK() const { return k; }
-    double* getK() { return k; }
-    size_t size() const { return 9; }
+    double operator()(double time) const { return wohl[time - at]; }
+    double& operator[](size_t i) { return wohl.getK()[i]; }
+    double operator[](size_t i) const { return wohl.getK()[i]; }
+    const double* getK() const { return wohl.getK(); }
+    double* getK() { return wohl.getK(); }
+    size_t size() const { return WohlfartPlus::numParams; }
 };
 
 // Simulator settings (simulator.h:60-146), read from the [model],
@@ -90,6 +150,7 @@ struct ModelShape {
 
 class EkgSim {
 public:
+    typedef SimLib::PositionVec PositionVec;  // sim_lib.h:95
     // device < 0: $EKG_DEVICE, else GPU 0.
     explicit EkgSim(int device = -1);
     ~EkgSim();
@@ -162,11 +223,15 @@ private:
 // .column writer of the reference (simulator.h:186-236): "#Comment: <c>",
 // one "# name : comment" line per commented vector, "# ", a header line and
 // one row per sample, time "%10.5f" when timeStep > 0, values in "%10e".
-struct ColumnVec {
+// saveVecElement is the reference's element type (simulator.h:184-197).
+struct saveVecElement {
     std::string name, comment;
     const std::vector<double>* data = nullptr;
+    size_t size() const { return data->size(); }
+    double operator[](size_t n) const { return (*data)[n]; }
 };
-void exportVectors(const std::vector<ColumnVec>& vec, const std::string& filename, double startTime,
+typedef saveVecElement ColumnVec;
+void exportVectors(const std::vector<saveVecElement>& vec, const std::string& filename, double startTime,
                    double timeStep, const std::string& comment = "");
 
 // .matrix readers/writers (matrix.h:124-248, simulator.cpp:71-106).
@@ -179,3 +244,6 @@ void exportShape(const ModelShape& shape, const std::string& filename, bool exci
 void generateTestShape(ModelShape& shape);
 
 }  // namespace SimLib
+
+// The reference makes the class visible at global scope too (sim_lib.h:290).
+typedef SimLib::EkgSim EkgSim;

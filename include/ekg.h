@@ -241,6 +241,12 @@ typedef struct ekg_ap_model {
  * Wohlfart.h:195-203), FP64 in the reference's operation order. */
 double ekg_ap_value(const double k[9], double t);
 
+/* APD90 of a WohlfartPlus curve (WohlfartPlus::apd90, Wohlfart.h:206-223):
+ * the last time on the 1 ms grid 0..999 at which the curve falls through
+ * k0 + k2/10, placed between the two samples by linear interpolation; -1 when
+ * it never does. */
+double ekg_ap_apd90(const double k[9]);
+
 /* Number of parameter-vector entries the layer APs take: distinct free
  * coefficients x 2 (endo-epi) or x 3 (endo-mid-epi) (sim.cpp:492-509). */
 int ekg_ap_num_genes(const ekg_ap_model *m, int *n_genes);

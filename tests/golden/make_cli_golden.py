@@ -8,7 +8,8 @@ and `expected/`: the files the reference wrote and `criteria.txt` with the
 criteria and violation it printed.  Nothing of the reference's source is
 stored.  Regenerate with:
 
-    python tests/golden/make_cli_golden.py
+    python tests/golden/make_cli_golden.py           # every case
+    python tests/golden/make_cli_golden.py model24   # the shipped 24-layer heart only
 """
 import os
 import re
@@ -148,9 +149,87 @@ def case(name, *, n=20, n_layers=3, heart_seed=3, n_leads=2, shape=True, start=0
     print(f"{name}: criteria <{m.group(1)}> violation {m.group(2)}; {size} bytes")
 
 
+TESTRUN = "/root/reference/testRun"
+# The parameter vector the reference's README gives for its shipped heart
+# (README.md:108): endo, mid and epi k5..k8, then two displacements per lead.
+TESTRUN_X = ("0.00035813,0.0890636,0.0632915,226.183,0.000369406,0.0965625,0.0523254,232.278,"
+             "0.000710767,0.0720323,0.0187579,200.93,23,22,15,13")
+
+
+def read_reference_matrix(path):
+    """(layer, seed) of a reference .matrix heart written with the 'space'
+    separator: '3d X x Y x Z space', then Z blocks of Y rows of X integers; a
+    negative value marks an excitation start of layer -v."""
+    with open(path) as f:
+        f.readline()
+        hdr = f.readline().split()
+        assert hdr[0] in ("3d", "3D") and hdr[-1] == "space", hdr
+        nx, ny, nz = int(hdr[1]), int(hdr[3]), int(hdr[5])
+        vals = np.array(f.read().split(), np.int64)
+    assert vals.size == nx * ny * nz
+    vals = vals.reshape(nz, ny, nx)
+    return np.abs(vals).astype(np.uint8), (vals < 0).astype(np.uint8)
+
+
+def case_testrun(name="model24"):
+    """The reference's shipped heart (testRun/: model_24.matrix, 24 layers,
+    conduction_24.matrix, two measuring points, target_ecg_v2_v5.column) with
+    the simulation settings of testRun/simulator.ini (start 100, length 400,
+    3D4, 1 ms, endo-mid-epi at 0.6, 1 - Pearson, moved electrodes) and the
+    README vector.  The heart is stored as heart.npz (layer, seed); the tests
+    write heart.matrix from it.  expected/activation.txt holds the SHA-256 of
+    the reference's FP64 activation times (Simulation::calculateExcitationSequence
+    through oracle/_ref/libekgref.so) on that heart, in (z, y, x) order."""
+    import hashlib
+    d = os.path.join(OUT, name)
+    shutil.rmtree(d, ignore_errors=True)
+    os.makedirs(os.path.join(d, "expected"))
+    layer, seed = read_reference_matrix(os.path.join(TESTRUN, "model_24.matrix"))
+    np.savez_compressed(os.path.join(d, "heart.npz"), layer=layer, seed=seed)
+    S.write_matrix(os.path.join(d, "heart.matrix"), layer, seed)
+    shutil.copy(os.path.join(TESTRUN, "conduction_24.matrix"), os.path.join(d, "conduction.matrix"))
+    shutil.copy(os.path.join(TESTRUN, "model_24_measuring_pos.txt"), os.path.join(d, "points.txt"))
+    shutil.copy(os.path.join(TESTRUN, "target_ecg_v2_v5.column"), os.path.join(d, "targets.column"))
+    with open(os.path.join(d, "simulator.ini"), "w") as f:
+        f.write(INI.format(
+            shape_line="shape = heart.matrix", excitation_line="", excitation_out="", start=100, length=400,
+            nhood="3D4", dt=1, interp="endo-mid-epi", mid=0.6, base=BASE, kmin=KMIN, kmax=KMAX, cmp=2,
+            endo_epi=-0.5, peak=0, move=1, fa=0, fa_delay=10.0, fa_limit=2))
+    args = ["ignored", "-sim", TESTRUN_X, "-out", "result", "-out", "layer_aps"]
+    with open(os.path.join(d, "args.txt"), "w") as f:
+        f.write("\n".join(args) + "\n")
+    # activation times of the reference's Dijkstra on this heart
+    with open(os.path.join(d, "conduction.matrix")) as f:
+        f.readline()
+        hdr = f.readline().split()
+        T = np.array(f.read().split(), float).reshape(int(hdr[3]), int(hdr[1]))
+    act = O.ref_excite(layer, seed, T, os.path.join(d, "tmp_transfer.matrix"))
+    os.remove(os.path.join(d, "tmp_transfer.matrix"))
+    myo = layer > 0
+    with open(os.path.join(d, "expected", "activation.txt"), "w") as f:
+        f.write(hashlib.sha256(np.ascontiguousarray(act, np.float64).tobytes()).hexdigest() + "\n")
+        f.write(f"{int(myo.sum())} {int((act[myo] == 0).sum())} {act.max()!r}\n")
+    before = set(os.listdir(d))
+    r = subprocess.run([O.REF_CLI] + args, cwd=d, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                       timeout=7200)
+    m = re.search(r"criteria = <([^>]*)>, violation = (\S+)", r.stdout)
+    if not m:
+        raise SystemExit(f"{name}: reference run failed:\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+    with open(os.path.join(d, "expected", "criteria.txt"), "w") as f:
+        f.write(m.group(1) + "\n" + m.group(2) + "\n")
+    for fn in sorted(set(os.listdir(d)) - before):
+        shutil.move(os.path.join(d, fn), os.path.join(d, "expected", fn))
+    os.remove(os.path.join(d, "heart.matrix"))
+    size = sum(os.path.getsize(os.path.join(dp, f)) for dp, _, fs in os.walk(d) for f in fs)
+    print(f"{name}: criteria <{m.group(1)}> violation {m.group(2)}; {size} bytes")
+
+
 def main():
     if not os.path.exists(O.REF_CLI):
         raise SystemExit("oracle/_ref/ekgSim is not built (make -C oracle ref)")
+    if sys.argv[1:] == ["model24"]:
+        case_testrun()
+        return
     shutil.rmtree(OUT, ignore_errors=True)
     # 3 layers, endo-mid-epi, moved electrodes, 1 - Pearson, start 100 (the
     # targets are not shifted: resampling factor 1), excitation exported
@@ -175,6 +254,8 @@ def main():
     # fast approximation limit below the approximation's criterion: the
     # simulation is skipped and the approximation is reported
     case("heart3_approx_only", n=20, heart_seed=6, fa_limit=0.05, out=("result",), x_seed=1004)
+    if os.path.isdir(TESTRUN):
+        case_testrun()
 
 
 if __name__ == "__main__":

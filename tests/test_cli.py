@@ -81,7 +81,37 @@ def read_ini(path):
 
 
 def test_cli_fixtures_present():
-    assert {"heart3", "heart6", "ring2d", "heart3_excitation_file", "heart3_approx_only"} <= set(CASES)
+    assert {"heart3", "heart6", "ring2d", "heart3_excitation_file", "heart3_approx_only", "model24"} <= set(CASES)
+
+
+def _model24():
+    d = _case_dir("model24")
+    z = np.load(os.path.join(d, "heart.npz"), allow_pickle=False)
+    with open(os.path.join(d, "conduction.matrix")) as f:
+        f.readline()
+        hdr = f.readline().split()
+        T = np.array(f.read().split(), float).reshape(int(hdr[3]), int(hdr[1]))
+    with open(os.path.join(d, "expected", "activation.txt")) as f:
+        sha, counts = f.read().split("\n")[:2]
+    return z["layer"], z["seed"], T, sha, counts.split()
+
+
+def test_model24_fixture_is_the_shipped_heart(tmp_path):
+    """The stored heart is the reference's testRun/model_24.matrix (93 x 124 x
+    124, 24 layers) and survives the round trip through the .matrix writer;
+    the conduction matrix is the 26 x 26 FP64 conduction_24 (0.166667 on the
+    diagonal, |i - j| off it, -1 for no tissue)."""
+    layer, seed, T, sha, counts = _model24()
+    assert layer.shape == (124, 124, 93)
+    assert layer.max() == 24 and int(counts[0]) == int((layer > 0).sum())
+    assert seed.sum() > 0 and np.all(layer[seed > 0] > 0)
+    assert T.shape == (26, 26)
+    i = np.arange(1, 26)
+    assert np.allclose(np.diag(T)[1:], 0.166667)
+    assert np.array_equal(T[1:, 1:][~np.eye(25, dtype=bool)], np.abs(i[:, None] - i[None, :])[~np.eye(25, dtype=bool)])
+    _materialise("model24", tmp_path)
+    back, bseed = read_matrix(tmp_path / "heart.matrix")
+    assert np.array_equal(back, layer) and np.array_equal(bseed, seed)
 
 
 @pytest.mark.parametrize("case", ["heart3", "heart3_excitation_file"])
@@ -144,13 +174,23 @@ def test_cli_help_runs_without_gpu():
 # ---- GPU: this build's ekgSim on the same inputs ------------------------------
 
 
-def _run_cli(case, tmp_path, args=None):
+def _materialise(case, tmp_path):
+    """Copy a case's inputs; a heart stored as heart.npz (layer, seed) is
+    written out as heart.matrix in the reference's text format."""
     src = _case_dir(case)
     for fn in os.listdir(src):
         if fn != "expected":
             shutil.copy(os.path.join(src, fn), tmp_path / fn)
-    print(f"ekgSim case {case}", flush=True)
-    r = subprocess.run([CLI] + (args if args is not None else _args(case)[1:]), cwd=tmp_path,
+    if os.path.exists(os.path.join(src, "heart.npz")):
+        from ekgsim_amd import synth as S
+        z = np.load(os.path.join(src, "heart.npz"), allow_pickle=False)
+        S.write_matrix(str(tmp_path / "heart.matrix"), z["layer"], z["seed"])
+
+
+def _run_cli(case, tmp_path, args=None, binary=CLI):
+    _materialise(case, tmp_path)
+    print(f"{os.path.basename(binary)} case {case}", flush=True)
+    r = subprocess.run([binary] + (args if args is not None else _args(case)[1:]), cwd=tmp_path,
                        stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
     return r
@@ -166,10 +206,7 @@ def _close(a, b):
     return np.all(np.abs(a - b) <= 2e-5 * np.abs(b) + 1e-6)
 
 
-@pytest.mark.gpu
-@pytest.mark.parametrize("case", CASES)
-def test_cli_matches_reference_program(case, tmp_path):
-    r = _run_cli(case, tmp_path)
+def _check_outputs(case, tmp_path, r):
     crit, viol = _criteria_line(r.stdout)
     exp_crit, exp_viol = _expected_criteria(case)
     assert crit.shape == exp_crit.shape
@@ -196,6 +233,71 @@ def test_cli_matches_reference_program(case, tmp_path):
         else:
             with open(got_path) as f1, open(os.path.join(exp_dir, fn)) as f0:
                 assert f1.read() == f0.read(), fn
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", CASES)
+def test_cli_matches_reference_program(case, tmp_path):
+    _check_outputs(case, tmp_path, _run_cli(case, tmp_path))
+
+
+# The reference's own main.cpp and sim.cpp, unmodified, compiled against this
+# build's simlib/sim_lib.h and linked with libekgsim.so (oracle/Makefile,
+# target _ref/ekgSim_facade; built where the reference sources are present).
+FACADE_CLI = os.path.join(ROOT, "oracle", "_ref", "ekgSim_facade")
+REF_SRC = "/root/reference"
+
+
+def test_reference_driver_compiles_against_facade():
+    """sim.cpp (SimImplementation: ActionPotential::wohl, WohlfartPlus,
+    EkgSim::PositionVec, ColumnFile, saveVecElement) and main.cpp compile
+    against the facade header and link with the library alone."""
+    if not os.path.isdir(REF_SRC):
+        pytest.skip("reference sources absent")
+    if not os.path.exists(os.path.join(ROOT, "ekgsim_amd", "lib", "libekgsim.so")):
+        pytest.skip("libekgsim.so not built")
+    r = subprocess.run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle"), "_ref/ekgSim_facade"],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-4000:]
+    needed = subprocess.run(["readelf", "-d", FACADE_CLI], stdout=subprocess.PIPE, text=True).stdout
+    assert "libekgsim.so" in needed
+    # nothing of the reference's simulation core is linked in
+    syms = subprocess.run(["nm", "-C", FACADE_CLI], stdout=subprocess.PIPE, text=True).stdout
+    assert "SimLib::Simulation::run" not in syms
+    assert "calculateExcitationSequence" not in syms
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", CASES)
+def test_reference_driver_on_facade_matches_reference_program(case, tmp_path):
+    """The reference's command-line program, running its own evaluation code
+    (sim.cpp) over this build's EkgSim, writes the files the compiled
+    reference wrote for the same case."""
+    if not os.path.exists(FACADE_CLI):
+        pytest.skip("ekgSim_facade not built (needs the reference sources)")
+    _check_outputs(case, tmp_path, _run_cli(case, tmp_path, binary=FACADE_CLI))
+
+
+@pytest.mark.gpu
+def test_model24_activation_bit_exact():
+    """FP64 excitation of the 24-layer shipped heart on the GPU against the
+    SHA-256 of the compiled reference's activation times: every voxel's
+    time, bit for bit (rounding-sensitive sums over 24 conduction layers)."""
+    import hashlib
+
+    import ekgsim_amd as E
+    layer, seed, T, sha, counts = _model24()
+    ctx = E.Context(0)
+    h = E.Heart(ctx, layer, seed)
+    h.excite_transfer(T)
+    act = h.activation()
+    myo = layer > 0
+    print(f"model24: {int(myo.sum())} myocardial voxels, max activation {act.max()!r}", flush=True)
+    assert int((act[myo] == 0).sum()) == int(counts[1])
+    assert act.max() == float(counts[2])
+    assert hashlib.sha256(np.ascontiguousarray(act, np.float64).tobytes()).hexdigest() == sha
+    h.close()
+    ctx.close()
 
 
 @pytest.mark.gpu

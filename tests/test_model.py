@@ -130,3 +130,30 @@ def test_ap_value_matches_reference_golden():
     for k, ts, vs in zip(g["k"], g["t"], g["v"]):
         got = np.array([A.ap_value(k, t) for t in ts])
         assert np.array_equal(_bits(got), _bits(vs))
+
+
+@pytest.mark.skipif(not O.have_refapp(), reason="reference application harness not built")
+def test_apd90_matches_live_reference_bitwise():
+    """ekg_ap_apd90 (the facade's WohlfartPlus::apd90) against the compiled
+    reference's WohlfartPlus::apd90 over the free-coefficient bounds."""
+    rng = np.random.default_rng(90)
+    for _ in range(40):
+        k = BASE.copy()
+        k[5:9] = rng.uniform([3e-4, 0.01, 0.01, 200], [1e-3, 0.1, 0.1, 450])
+        assert np.array_equal(_bits([A.ap_apd90(k)]), _bits([O.ref_apd90(k)])), k
+
+
+def test_apd90_special_cases():
+    # a flat curve (k2 = 0) never falls through k0 + k2/10: -1
+    k = BASE.copy()
+    k[2] = 0.0
+    assert A.ap_apd90(k) == -1.0
+    # the crossing lies between the last sample above and the first at or
+    # below k0 + k2/10, and later repolarisation (larger k8) moves it later
+    k1, k2 = BASE.copy(), BASE.copy()
+    k2[8] += 50.0
+    a1, a2 = A.ap_apd90(k1), A.ap_apd90(k2)
+    t = np.floor(a1)
+    target = k1[0] + 0.1 * k1[2]
+    assert A.ap_value(k1, t) > target >= A.ap_value(k1, t + 1)
+    assert a2 > a1 + 40.0
