@@ -190,6 +190,8 @@ def case_testrun(name="model24"):
     shutil.copy(os.path.join(TESTRUN, "conduction_24.matrix"), os.path.join(d, "conduction.matrix"))
     shutil.copy(os.path.join(TESTRUN, "model_24_measuring_pos.txt"), os.path.join(d, "points.txt"))
     shutil.copy(os.path.join(TESTRUN, "target_ecg_v2_v5.column"), os.path.join(d, "targets.column"))
+    for fn in ("conduction.matrix", "points.txt", "targets.column"):
+        os.chmod(os.path.join(d, fn), 0o644)
     with open(os.path.join(d, "simulator.ini"), "w") as f:
         f.write(INI.format(
             shape_line="shape = heart.matrix", excitation_line="", excitation_out="", start=100, length=400,
@@ -208,7 +210,7 @@ def case_testrun(name="model24"):
     myo = layer > 0
     with open(os.path.join(d, "expected", "activation.txt"), "w") as f:
         f.write(hashlib.sha256(np.ascontiguousarray(act, np.float64).tobytes()).hexdigest() + "\n")
-        f.write(f"{int(myo.sum())} {int((act[myo] == 0).sum())} {act.max()!r}\n")
+        f.write(f"{int(myo.sum())} {int((act[myo] == 0).sum())} {float(act.max())!r}\n")
     before = set(os.listdir(d))
     r = subprocess.run([O.REF_CLI] + args, cwd=d, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
                        timeout=7200)
