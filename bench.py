@@ -34,7 +34,8 @@ from ekgsim_amd import synth as S  # noqa: E402  (no GPU use)
 
 METRIC = ("simulated ECGs/sec at 128^3, 600 ms, 12 leads (batched vs plain) at "
           "≤1e-5 rel-L2; p99")
-# MI355X dense FP64 matrix-core peak (the lead contraction's precision), FLOP/s
+# MI355X dense FP64 matrix-core peak (the lead contraction's precision), FLOP/s;
+# also the FP64 vector and the non-packed FP32 vector rate
 MFMA_FP64_PEAK = 78.6e12
 # Sigmoid window of the transition tiles in the exp2 domain x = c1 (t - a),
 # c1 = -k1 log2(e): outside (SATC_X, SKIP_X) a voxel's AP takes the FP64
@@ -69,17 +70,19 @@ def lead_field_cells(layer):
 
 
 # Algorithmic work, counted per unit.  Class-chunk kernel (the dominant one
-# when the heart has activation classes), per (class cell, lead): 14 FP32
-# flops for the weight (offset e 6, |e|^2 5, rsq 1, |e|^-3 2) and 8 FP64
-# matrix-core flops (an FMA for each of the coefficients D0, D1, D2, K).
+# when the heart has activation classes), per (class cell, lead): 16 FP64
+# matrix-core flops (|e|^2 as a 4-term product, 4 FMAs; then an FMA for each
+# of the coefficients D0, D1, D2, K) and 3 FP32 flops (rsq 1, |e|^-3 2).
 # Chunk kernel (dominant without classes):
-#   per lead-weight evaluation (voxel, lead, cell): 14 FP32 flops (electrode
-#     offset 6, |e|^2 5, rsq 1, r^3 2) and 4 FP64 flops (d . e, one FMA);
+#   per lead-weight evaluation (voxel, lead, cell): 8 FP64 flops (|e|^2 2
+#     FMAs, d . e and its FMA into G) and 3 FP32 flops (rsq 1, r^3 2);
 #   per window voxel-step (the transition tiles): 2L FP64 MFMA flops and 14
 #     FP32 ops (10 AP assembly + 4 transcendental).
-# FP32 counts half against the FP64 peak (the FP32 vector rate is twice it).
-FP32_PER_CLASS_CELL, FP64_PER_CLASS_CELL = 14, 8
-FP32_PER_CELL, FP64_PER_CELL = 14, 4
+# Both count at full weight against 78.6 TFLOP/s: the build disables packed
+# FP32, so an FP32 vector operation issues at the FP64 vector rate, which on
+# MI355X equals the FP64 matrix rate.
+FP32_PER_CLASS_CELL, FP64_PER_CLASS_CELL = 3, 16
+FP32_PER_CELL, FP64_PER_CELL = 3, 8
 FP32_PER_WINDOW_STEP = 14
 # Naive per-voxel-step model: every voxel at every step, 38 VALU lane-cycles
 # at L = 12, against 256 CU x 64 lanes x 2.4 GHz.
@@ -106,6 +109,134 @@ def workload(n, L, B, pseed0=1000):
         ks.append(S.layer_k_from_params(x, 3))
         ps.append((x, base))
     return h, ks, ps
+
+
+# The reference's shipped heart (testRun/model_24: 93 x 124 x 124, 24 layers,
+# FP64 conduction_24, two measuring points), stored as a test fixture.
+TESTRUN_DIR = os.path.join(ROOT, "tests", "golden", "cli", "model24")
+TESTRUN_BASE = [0, 2.5, 100, 0.90, 0.1, 0.0010, 0.10, 0.10, 250]
+TESTRUN_KMIN = [0, 1.5, 100, 0.85, 0.05, 0.0003, 0.01, 0.01, 300]
+TESTRUN_KMAX = [0, 3.5, 100, 0.95, 0.2, 0.0010, 0.10, 0.10, 450]
+
+
+def testrun_workload(B):
+    """(layer, seed, conduction, layer_k [B][24][9], lead positions [B][2][3])
+    for B parameter vectors drawn within the ini bounds (endo-mid-epi
+    k5..k8, then two displacements per lead in [-50, 50])."""
+    from ekgsim_amd import _capi as A
+    z = np.load(os.path.join(TESTRUN_DIR, "heart.npz"), allow_pickle=False)
+    with open(os.path.join(TESTRUN_DIR, "conduction.matrix")) as f:
+        f.readline()
+        hdr = f.readline().split()
+        cond = np.array(f.read().split(), float).reshape(int(hdr[3]), int(hdr[1]))
+    pts = []
+    with open(os.path.join(TESTRUN_DIR, "points.txt")) as f:
+        for ln in f:
+            if ln.strip():
+                x, y, zz = (float(v) for v in ln.split(","))
+                pts.append([zz, y, x])
+    pts = np.array(pts)
+    rng = np.random.default_rng(24)
+    ks, ps = [], []
+    for _ in range(B):
+        genes = rng.uniform([3e-4, 0.01, 0.01, 300] * 3, [1e-3, 0.1, 0.1, 450] * 3)
+        disp = rng.uniform(-50, 50, size=(len(pts), 2))
+        k, _ = A.layer_aps(np.stack([TESTRUN_BASE] * 3), TESTRUN_KMIN, TESTRUN_KMAX, [5, 6, 7, 8], 2, 0.6, 24,
+                           genes)
+        ks.append(k)
+        ps.append(S.move_leads(pts, disp))
+    return z["layer"], z["seed"], cond, np.stack(ks), np.stack(ps)
+
+
+def poisson_burst(srv, reqs, rate, seed):
+    """Submit reqs [(heart id, k, pos, start, length, dt)] at Poisson arrival
+    times of the given rate; returns (ECGs/s, latencies in ms, results).  The
+    descriptors and output arrays are built before the clock starts and the
+    C entry points are called directly, so the client costs a few
+    microseconds per request instead of bounding the measured rate."""
+    import ctypes as C
+    from ekgsim_amd import _capi as A
+    L = A.lib()
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, size=len(reqs))
+    descs = [A._Desc(k, p, "3D4", st, ln, dt) for (_, k, p, st, ln, dt) in reqs]
+    outs = [np.empty((d.p.shape[0], A.num_steps(ln, dt))) for d, (_, _, _, _, ln, dt) in zip(descs, reqs)]
+    tickets = [C.c_uint64() for _ in reqs]
+    lats = [C.c_double() for _ in reqs]
+    args_s = [(srv.h, int(r[0]), C.byref(d.s), C.byref(t)) for r, d, t in zip(reqs, descs, tickets)]
+    submit, wait = L.ekg_server_submit, L.ekg_server_wait
+    t0 = time.perf_counter()
+    nxt = t0
+    for a, gap in zip(args_s, gaps):
+        nxt += gap
+        while time.perf_counter() < nxt:
+            pass
+        A.check(submit(*a))
+    for t, o, lt in zip(tickets, outs, lats):
+        A.check(wait(srv.h, t, o.ctypes.data_as(C.c_void_p), o.size, C.byref(lt)))
+    el = time.perf_counter() - t0
+    return len(reqs) / el, [lt.value for lt in lats], outs
+
+
+def stress(args):
+    """The large configuration: a 256^3 heart (2,964,088 myocardial voxels),
+    12 leads, 600 ms at dt 0.25 (2400 samples), 1024 parameter vectors served
+    through the continuous-batching server in waves admitted by its device
+    budget.  Prints one JSON line; two ECGs are checked against the FP64
+    oracle on an activation window and a repolarisation window."""
+    import torch
+    import ekgsim_amd as E
+    n, L, R, dt, length = 256, 12, args.stress_requests, 0.25, 600.0
+    h = S.make_heart(n, seed=1)
+    reqs = []
+    for r in range(R):
+        x = S.random_params(3, L, 4000 + r)
+        reqs.append((S.layer_k_from_params(x, 3), S.move_leads(S.lead_positions(n, L), S.displacements(x, 3, L))))
+    ctx = E.Context(0)
+    H = E.Heart(ctx, h.layer, h.seed)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    H.excite_delay(h.delay)
+    ca_ms = (time.perf_counter() - t0) * 1e3
+    budget = args.stress_budget_gib << 30
+    srv = E.Server(ctx, hbm_budget_bytes=budget, max_batch=256, n_streams=2)
+    hid = srv.add_heart(H)
+    # one warm-up request (kernel loading), then the timed stream
+    srv.wait(srv.submit(hid, reqs[0][0], reqs[0][1], "3D4", 0.0, length, dt))
+    t0 = time.perf_counter()
+    tickets = [srv.submit(hid, k, p, "3D4", 0.0, length, dt) for k, p in reqs]
+    got = [srv.wait(t) for t in tickets]
+    el = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lat = [g[1] for g in got]
+    log(f"stress: {R} requests in {el:.2f} s, {st}")
+    err = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        act = O.excite_int(h.layer, h.seed, h.delay).astype(np.float64)
+        num = den = 0.0
+        for r in (0, R - 1):
+            k, pos = reqs[r]
+            for (st_ms, ln) in ((40.0, 10.0), (300.0, 10.0)):
+                ref = O.run(h.layer, act, k, pos, "3D4", st_ms, ln, dt)
+                j0 = int(st_ms / dt)
+                seg = got[r][0][:, j0:j0 + ref.shape[1]]
+                num += float(np.sum((seg - ref) ** 2))
+                den += float(np.sum(ref ** 2))
+        err = math.sqrt(num / den)
+        log(f"stress: rel-L2 vs FP64 oracle on the windows {err:.2e}")
+    print(json.dumps({
+        "metric": "simulated ECGs/sec at 256^3, 600 ms at 0.25 ms (2400 samples), 12 leads, served in "
+                  "HBM-admitted waves",
+        "value": round(R / el, 2), "unit": "ECGs/s", "n_gpus": 1, "requests": R,
+        "seconds": round(el, 3), "p50_ms": round(float(np.percentile(lat, 50)), 2),
+        "p99_ms": round(float(np.percentile(lat, 99)), 2), "groups": st["groups"],
+        "peak_device_bytes": st["peak_bytes"], "hbm_budget_bytes": budget, "excitation_ms": round(ca_ms, 2),
+        "rel_l2_vs_fp64_windows": err,
+        "config": {"workload": f"{n}^3 ellipsoid heart ({h.n_myo} myocardial voxels), {L} leads, "
+                               f"{int(length / dt)} steps of {dt} ms, 3D4 dipoles, {R} parameter vectors"},
+    }), flush=True)
 
 
 # ---------------------------------------------------------------------------
@@ -178,6 +309,11 @@ def main():
     ap.add_argument("--no-general", action="store_true",
                     help="skip the timing with activation classes off")
     ap.add_argument("--no-graph", action="store_true", help="launch the timed steps eagerly")
+    ap.add_argument("--stress", action="store_true",
+                    help="time the large configuration (256^3, dt 0.25, 1024 vectors) instead")
+    ap.add_argument("--stress-requests", type=int, default=1024)
+    ap.add_argument("--stress-budget-gib", type=int, default=16)
+    ap.add_argument("--no-testrun", action="store_true", help="skip the shipped 24-layer heart")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="one device-to-device copy of MIB MiB before the steps (scale for HBM counters)")
     args = ap.parse_args()
@@ -186,6 +322,9 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(requests would shard across GPUs with no exchange; see DESIGN.md).", file=sys.stderr)
         sys.exit(2)
+    if args.stress:
+        stress(args)
+        return
 
     n, L, B = args.grid, args.leads, args.batch
     T = int(math.ceil(args.length / 1.0))
@@ -296,7 +435,7 @@ def main():
         cells = B * L * lead_field_cells(h.layer) if args.kernel != "valu" else 0
         f64 = 2.0 * L * win + FP64_PER_CELL * cells
         f32 = FP32_PER_WINDOW_STEP * win + FP32_PER_CELL * cells
-    flops = f64 + 0.5 * f32
+    flops = f64 + f32
     achieved = flops / avg_kernel_s
     naive_rate = value * V * T
     log(f"batched: {ms_step:.2f} ms/step, {value:.1f} ECGs/s; {dom} {avg_kernel_s * 1e3:.2f} ms; "
@@ -399,24 +538,84 @@ def main():
             reqs.append((hid, S.layer_k_from_params(x, 3),
                          E.move_leads(S.lead_positions(g, nl), S.displacements(x, 3, nl)), dur))
         # Poisson arrivals at a rate above the service rate (saturating)
-        rate = 4.0 * value
-        gaps = rng.exponential(1.0 / rate, size=len(reqs))
-        tickets = []
-        t0 = time.perf_counter()
-        nxt = t0
-        for (hid, k, p, dur), gap in zip(reqs, gaps):
-            nxt += gap
-            while time.perf_counter() < nxt:
-                pass
-            tickets.append(srv.submit(hid, k, p, "3D4", 0.0, dur, 1.0))
-        lats = [srv.wait(t)[1] for t in tickets]
-        el_s = time.perf_counter() - t0
+        rate_m, lats, _ = poisson_burst(srv, [(hid, k, p, 0.0, dur, 1.0) for hid, k, p, dur in reqs],
+                                        4.0 * value, 7)
         st = srv.stats()
         srv.close()
-        serve = {"requests": len(reqs), "ecgs_per_s": len(reqs) / el_s,
+        serve = {"requests": len(reqs), "ecgs_per_s": rate_m,
                  "p50_ms": float(np.percentile(lats, 50)), "p99_ms": float(np.percentile(lats, 99)),
                  "groups": st["groups"], "peak_inflight": st["peak_inflight"]}
         log(f"serving: {serve}")
+
+    # ---- serving on the batch's own heart and shape, like for like with the
+    # plain drop-in: every request 128^3 x 12 leads x 600 ms ----
+    same = None
+    if args.serve_requests > 0:
+        srv = E.Server(ctx, hbm_budget_bytes=48 << 30, max_batch=256, n_streams=3)
+        hid = srv.add_heart(heart)
+        sreqs = [(hid, ks[r % B], pos[r % B], 0.0, args.length, 1.0) for r in range(2 * B)]
+        srv.wait(srv.submit(hid, ks[0], pos[0], "3D4", 0.0, args.length, 1.0))
+        rate_s, lats_s, outs = poisson_burst(srv, sreqs, 4.0 * value, 11)
+        st = srv.stats()
+        srv.close()
+        same_bits = all(np.array_equal(outs[r], ecg_gpu[r % B]) for r in (0, 1, B - 1, B, 2 * B - 1))
+        same = {"requests": len(sreqs), "ecgs_per_s": rate_s, "p50_ms": float(np.percentile(lats_s, 50)),
+                "p99_ms": float(np.percentile(lats_s, 99)), "groups": st["groups"],
+                "vs_plain": rate_s / plain, "served_equal_batched_bitwise": same_bits}
+        log(f"serving, one heart and shape: {same}")
+
+    # ---- the reference's shipped 24-layer heart (FP64 conduction: distinct
+    # activation times, so no activation classes) ----
+    testrun = None
+    if not args.no_testrun and os.path.exists(os.path.join(TESTRUN_DIR, "heart.npz")):
+        tl, tsd, tcond, tks, tps = testrun_workload(B)
+        th = E.Heart(ctx, tl, tsd)
+        th.excite_transfer(tcond)
+        tlen, tstart = 400.0, 100.0
+        tT = int(tlen)
+        dk = torch.from_numpy(tks).to(dev)
+        dp = torch.from_numpy(tps).to(dev)
+        de = torch.empty((B, tps.shape[1], tT), dtype=torch.float64, device=dev)
+        ttune = None
+        if not args.no_autotune:
+            ttune = th.autotune(tks[0], tps[0], "3D4", tstart, tlen, 1.0, batch=B)
+
+        def tstep():
+            th.simulate_batch_device(dk.data_ptr(), dp.data_ptr(), de.data_ptr(), B, 24, tps.shape[1], "3D4",
+                                     tstart, tlen, 1.0)
+
+        ctx.set_graphs(not args.no_graph)
+        tstep()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            tstep()
+        ctx.synchronize()
+        tel = time.perf_counter() - t0
+        ctx.set_graphs(False)
+        tecg = de.cpu().numpy()
+        tlat = []
+        for i in range(23):
+            t1 = time.perf_counter()
+            th.simulate(tks[i], tps[i], "3D4", tstart, tlen, 1.0)
+            if i >= 3:
+                tlat.append(time.perf_counter() - t1)
+        terr = tbits = None
+        if not args.no_cpu:
+            from oracle import oracle as O
+            tact = O.excite_fp64(tl, tsd, tcond)
+            tbits = bool(np.array_equal(tact.view(np.uint64), th.activation().view(np.uint64)))
+            tref = O.run(tl, tact, tks[0], tps[0], "3D4", tstart, tlen, 1.0)
+            terr = float(np.linalg.norm(tecg[0] - tref) / np.linalg.norm(tref))
+        tinfo = th.layout_info("3D4")
+        testrun = {"ecgs_per_s": round(B * args.steps / tel, 2), "ms_per_batch": round(tel * 1e3 / args.steps, 3),
+                   "plain_ecgs_per_s": round(len(tlat) / sum(tlat), 2),
+                   "myocardial_voxels": int((tl > 0).sum()), "layers": 24, "leads": int(tps.shape[1]),
+                   "steps": tT, "class_voxels": tinfo["class_voxels"], "autotuned": ttune,
+                   "rel_l2_vs_fp64": terr, "activation_bit_exact_vs_oracle": tbits,
+                   "reference_published_s_per_ecg": 439.131}
+        log(f"shipped 24-layer heart: {testrun}")
+        th.close()
 
     line = {
         "metric": METRIC,
@@ -447,7 +646,9 @@ def main():
         "plain_graph_p50_ms": round(float(np.percentile(lat_g, 50)) * 1e3, 3),
         "batched_vs_plain": round(value / plain, 2),
         "serving": serve,
+        "serving_same_heart": same,
         "p99_ms": None if serve is None else round(serve["p99_ms"], 3),
+        "testrun_model24": testrun,
         "rel_l2_max_vs_fp64": rel_max,
         "rel_l2_vs_reference_windows": ref_win_err,
         "excitation_ms": round(ca_ms, 2),
@@ -469,7 +670,7 @@ def main():
             "per_unit": {"class_cell_lead": {"fp32": FP32_PER_CLASS_CELL, "fp64": FP64_PER_CLASS_CELL},
                          "lead_weight_evaluation": {"fp32": FP32_PER_CELL, "fp64": FP64_PER_CELL},
                          "window_voxel_step": {"fp32": FP32_PER_WINDOW_STEP, "fp64": 2 * L},
-                         "fp32_weight": 0.5},
+                         "fp32_weight": 1.0},
             "naive_voxel_steps_per_s": naive_rate,
             "naive_model_frac": round(naive_rate * NAIVE_LANE_CYCLES / LANE_PEAK, 3),
             "kernel_ms": round(avg_kernel_s * 1e3, 4),

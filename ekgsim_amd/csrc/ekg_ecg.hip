@@ -677,12 +677,13 @@ constexpr int CH_LEADS = 2;   // leads per pass of the chunk kernel's lead field
 // neighbours in the dipole neighbourhood N, CH_LEADS leads per pass:
 //   G_i(u) = s(u) (e(u) . s_u) + sum_{d in N, u+d myo} s(u+d) (e(u+d) . d),
 // s_u the sum of those d.  A cell's weight m = e s is a function of the cell
-// alone: s = |e|^-3 (rsq cubed) from e = (p_hi - c) + p_lo in FP32, and e in
-// the dipole products is the FP64 offset p - c, so the weight's rounding
-// cancels between the two ends of each dipole (and with the class sums, which
-// use the same s and offsets) and only V differences across the dipole see it
-// (a weight derived per evaluating voxel, e.g. from e(u) - d in FP32, does not
-// cancel: ~1e-5).
+// alone: s = rsq(|e|^2 rounded to FP32)^3 with |e|^2 summed in FP64 from the
+// exact FP64 offsets e = p - c, which are also the offsets of the dipole
+// products, so the weight's rounding cancels between the two ends of each
+// dipole (and with the class sums, which round their own FP64 |e|^2 to the
+// same FP32 value but where two FP64 roundings straddle an FP32 boundary) and
+// only V differences across the dipole see it (a weight derived per
+// evaluating voxel, e.g. from e(u) - d in FP32, does not cancel: ~1e-5).
 template <int NH>
 struct VoxelField {
     float fc[3];          // grid coordinate + 1 per axis (the reference's voxel centre)
@@ -715,44 +716,37 @@ struct VoxelField {
             for (int k3 = 0; k3 < 3; ++k3) cd[c3][k3] = (double)(fc[c3] + (float)(k3 - 1));
     }
 
-    // g[w] = G_{i0+w}(u); leads at or past L give 0.  s_p[i][0|1][c]: the
-    // electrode's FP32 hi/lo per component; s_pd[i][c]: the same in FP64.
-    __device__ __forceinline__ void leads(int i0, int L, const float (*s_p)[2][3], const double (*s_pd)[3],
-                                          double g[CH_LEADS]) const
+    // g[w] = G_{i0+w}(u); leads at or past L give 0.  s_pd[i][c]: the
+    // electrode in FP64.
+    __device__ __forceinline__ void leads(int i0, int L, const double (*s_pd)[3], double g[CH_LEADS]) const
     {
         using N = Nhood<NH>;
 #pragma unroll
         for (int w2 = 0; w2 < CH_LEADS; ++w2) g[w2] = 0.0;
         if (i0 >= L) return;
-        // Per lead, the electrode offset of a cell is (p_hi - c) + p_lo per
-        // axis, and the voxel and its neighbours span only three values of c
-        // per axis: E[axis][d + 1] in FP32 and FP64, and the first term of
-        // |e|^2 per x-offset.  Each cell's weight is then formed from exactly
-        // the operands (and in the order) its own evaluation uses.
-        float E[CH_LEADS][3][3], SQ0[CH_LEADS][3];
-        double E64[CH_LEADS][3][3];
+        // Per lead, the electrode offset of a cell is p - c per axis, exact
+        // in FP64, and the voxel and its neighbours span only three values of
+        // c per axis: E64[axis][d + 1], and the first term of |e|^2 per
+        // x-offset.  Each cell's weight is then formed from exactly the
+        // operands (and in the order) its own evaluation uses.
+        double E64[CH_LEADS][3][3], SQ0[CH_LEADS][3];
 #pragma unroll
         for (int w2 = 0; w2 < CH_LEADS; ++w2) {
             // leads past the request's count repeat its first lead of the pass
             const int ii = i0 + w2 < L ? i0 + w2 : i0;
 #pragma unroll
-            for (int c3 = 0; c3 < 3; ++c3) {
-                const float ph = s_p[ii][0][c3], pl = s_p[ii][1][c3];
+            for (int c3 = 0; c3 < 3; ++c3)
 #pragma unroll
-                for (int k3 = 0; k3 < 3; ++k3) {
-                    E[w2][c3][k3] = (ph - (fc[c3] + (float)(k3 - 1))) + pl;
-                    E64[w2][c3][k3] = s_pd[ii][c3] - cd[c3][k3];
-                }
-            }
+                for (int k3 = 0; k3 < 3; ++k3) E64[w2][c3][k3] = s_pd[ii][c3] - cd[c3][k3];
 #pragma unroll
-            for (int k3 = 0; k3 < 3; ++k3) SQ0[w2][k3] = E[w2][0][k3] * E[w2][0][k3];
+            for (int k3 = 0; k3 < 3; ++k3) SQ0[w2][k3] = E64[w2][0][k3] * E64[w2][0][k3];
         }
 #pragma unroll
         for (int w2 = 0; w2 < CH_LEADS; ++w2) {
             // own cell: m(u) . s_u
-            float l = __builtin_fmaf(E[w2][1][1], E[w2][1][1], SQ0[w2][1]);
-            l = __builtin_fmaf(E[w2][2][1], E[w2][2][1], l);
-            const float r = __builtin_amdgcn_rsqf(l);
+            const double l = __builtin_fma(E64[w2][2][1], E64[w2][2][1],
+                                           __builtin_fma(E64[w2][1][1], E64[w2][1][1], SQ0[w2][1]));
+            const float r = __builtin_amdgcn_rsqf((float)l);
             g[w2] = (double)(r * r * r) * (E64[w2][0][1] * sd0 + E64[w2][1][1] * sd1 + E64[w2][2][1] * sd2);
         }
 #pragma unroll
@@ -761,9 +755,9 @@ struct VoxelField {
                 const int k0 = N::d[qn][0] + 1, k1 = N::d[qn][1] + 1, k2 = N::d[qn][2] + 1;
 #pragma unroll
                 for (int w2 = 0; w2 < CH_LEADS; ++w2) {
-                    float l = __builtin_fmaf(E[w2][1][k1], E[w2][1][k1], SQ0[w2][k0]);
-                    l = __builtin_fmaf(E[w2][2][k2], E[w2][2][k2], l);
-                    const float r = __builtin_amdgcn_rsqf(l);
+                    const double l = __builtin_fma(E64[w2][2][k2], E64[w2][2][k2],
+                                                   __builtin_fma(E64[w2][1][k1], E64[w2][1][k1], SQ0[w2][k0]));
+                    const float r = __builtin_amdgcn_rsqf((float)l);
                     // d . e in FP64 (d has entries -1, 0, 1)
                     double de = 0.0;
                     if (N::d[qn][0]) de += E64[w2][0][k0] * N::d[qn][0];
@@ -818,7 +812,6 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
     __shared__ float4 s_aux[NMAX];          // {P, Q, a_hi, a_lo} relative to the chunk's t_ref
     __shared__ double2 s_pq[NMAX];          // {P', Q'} relative to the layer's t0
     __shared__ double s_part[CHUNK_SEG][3 * LP];
-    __shared__ float s_p[16][2][3];         // electrode {hi, lo} per component
     __shared__ double s_pd[16][3];          // electrode in FP64
     __shared__ int s_tb[2];
     const int t = threadIdx.x, lane = t & 63;
@@ -848,13 +841,7 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
     const double c1d = -kk[1] * LOG2E_D;
     const double* tg = tgrid + R.t_off;
     const float2* th = thl + R.t_off;
-    if (t < 3 * R.L) {
-        const double p = lead_pos[R.pos_off + t];
-        const float hi = (float)p;
-        s_p[t / 3][0][t % 3] = hi;
-        s_p[t / 3][1][t % 3] = (float)(p - (double)hi);
-        s_pd[t / 3][t % 3] = p;
-    }
+    if (t < 3 * R.L) s_pd[t / 3][t % 3] = lead_pos[R.pos_off + t];
     // operands of the bound-tile tests (16-step sub-tiles), loaded now and
     // used after the G rows: one probe per lane when there are at most 64
     constexpr int BT = 1 << BT_SHIFT_MFMA;
@@ -878,7 +865,7 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
 #pragma unroll 1
         for (int i0 = 0; i0 < LP; i0 += CH_LEADS) {
             double g[CH_LEADS];
-            vf.leads(i0, R.L, s_p, s_pd, g);
+            vf.leads(i0, R.L, s_pd, g);
 #pragma unroll
             for (int w2 = 0; w2 < CH_LEADS; ++w2) s_G[q * LP + i0 + w2] = g[w2];
         }
@@ -1115,21 +1102,30 @@ constexpr int CC_THREADS = 256;
 // per neighbourhood; cells inside the chunk cancel).  With e = p - c (c the
 // cell's coordinate + 1) this is linear in the electrode p:
 //   sum_w s(w) (e . D) = p . T + T3,  T_c = sum_w s(w) D_c(w),  T3 = sum_w s(w) K(w),
-// K = -D . c (host-computed), a product of the weights s (leads x cells) with
-// four integer coefficients per cell, which runs on v_mfma_f64_4x4x4f64 (4
-// blocks of 4 leads x 4 cells x 4 coefficients).  s(w) is the chunk kernel's
-// FP32 weight bit for bit (same operands and order), and the chunk kernel's
-// dipole products use the same FP64 offsets, so a cell's contributions here
-// and in G rows cancel across every dipole to FP64 rounding.  gfx950 lane
-// layout (checked by a one-hot probe): A(block, row, k) in lane
+// K = -D . c (host-computed).  Both the squared distance and the products run
+// on v_mfma_f64_4x4x4f64 (4 blocks of 4 x 4 x 4), leaving the vector ALUs
+// only the conversions, the reciprocal square root and the cube:
+//  1. |e|^2 = |p|^2 - 2 p . c + |c|^2 as an FP64 matrix product: rows are the
+//     16 cells of a group (4 per block), columns the 4 leads of a group, the
+//     inner dimension {c_z, c_y, c_x, 1} x {-2 p_z, -2 p_y, -2 p_x, |p|^2},
+//     with |c|^2 (exact) as the accumulator input;
+//  2. s = rsq(|e|^2 rounded to FP32)^3 in FP32 -- the chunk kernel's weight:
+//     it rounds its own FP64 |e|^2 (from the exact FP64 offsets) the same way,
+//     so the two kernels give a cell the same weight except where the two
+//     FP64 values (both within ~1e-14 relative of |e|^2) straddle an FP32
+//     rounding boundary;
+//  3. T = s x {D_z, D_y, D_x, K}: rows the leads, inner dimension the cells.
+// gfx950 lane layout (checked by a one-hot probe): A(block, row, k) in lane
 // 16k + 4 block + row, B(block, k, col) in lane 16k + 4 block + col,
-// D(block, row, col) in lane 16 row + 4 block + col.  So a lane owns one cell
-// of each group of 16 (k, block) and one lead of each group of 4 (row) for A
-// and one coefficient (col) for B.  Each wave takes groups of 16 cells
-// (strided over the waves), the blocks' partial sums are added in block
-// order, the waves' in wave order, and sums[(c_off + n_chunks + k)*3*LP]
-// receives {sum G, P' sum G, Q' sum G} with the class's
-// P' = k2(1-k3) e^{(k4+k5)(a - t0)} and Q' = k2 k3 e^{k5 (a - t0)}.
+// D(block, row, col) in lane 16 row + 4 block + col.  Product 1's result for
+// (cell r of block b, lead col) lands in lane 16r + 4b + col, which is where
+// product 3 takes its operand (block b, lead row, cell k): so a lane owns one
+// cell of each group of 16 and one lead of each group of 4 throughout, and
+// feeds product 1 coordinate (lane >> 4) of cell (b, lane & 3).  Each wave
+// takes groups of 16 cells (strided over the waves), the blocks' partial sums
+// are added in block order, the waves' in wave order, and
+// sums[(c_off + n_chunks + k)*3*LP] receives {sum G, P' sum G, Q' sum G} with
+// the class's P' = k2(1-k3) e^{(k4+k5)(a - t0)} and Q' = k2 k3 e^{k5 (a - t0)}.
 template <int LP, int RQ>
 __global__ __launch_bounds__(CC_THREADS) void ecg_class_chunk_kernel(
     const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
@@ -1150,8 +1146,8 @@ __global__ __launch_bounds__(CC_THREADS) void ecg_class_chunk_kernel(
         const int u = find_req(cc_ustart, n_units, blockIdx.x);
         k = (int)(blockIdx.x - cc_ustart[u]);
         const int2 pr = cc_units[u];
-        rq[0] = pr.x;
-        rq[RQ - 1] = pr.y;
+#pragma unroll
+        for (int j = 0; j < RQ; ++j) rq[j] = j == 0 ? pr.x : pr.y;
     } else if (RQ > 1) {
         const int np = (B + RQ - 1) / RQ;
         k = (int)(blockIdx.x / (unsigned)np);
@@ -1167,23 +1163,23 @@ __global__ __launch_bounds__(CC_THREADS) void ecg_class_chunk_kernel(
     }
     const HeartDev& H = hearts[reqs[rq[0]].heart];
     const int4 ch = H.cchunks[k];
-    const int rc = lane & 3;                                  // A row (lead of a group) = B column
-    const int cl = 4 * ((lane >> 2) & 3) + (lane >> 4);       // the lane's cell of a group of 16
-    // electrodes of the lane's leads 4g + rc (leads past the request's count
-    // repeat lead 0; their sums are zeroed)
-    float ph[RQ][NG][3], pl[RQ][NG][3];
+    const int rc = lane & 3;                                  // lead of a group (and product-3 column)
+    const int kq = lane >> 4;                                 // product-1 inner index: coordinate kq
+    const int cl = 4 * ((lane >> 2) & 3) + kq;                // the lane's cell of a group of 16
+    const int ca = 4 * ((lane >> 2) & 3) + rc;                // the cell whose coordinate kq it feeds
+    // product-1 right operand: {-2 p_z, -2 p_y, -2 p_x, |p|^2} of the lane's
+    // leads 4g + rc (leads past the request's count repeat lead 0; their sums
+    // are zeroed)
+    double pe[RQ][NG];
 #pragma unroll
     for (int j = 0; j < RQ; ++j) {
         const ReqDev& R = reqs[rq[j]];
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
             const int i = 4 * g + rc < R.L ? 4 * g + rc : 0;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const double p = lead_pos[R.pos_off + 3 * i + c];
-                ph[j][g][c] = (float)p;
-                pl[j][g][c] = (float)(p - (double)ph[j][g][c]);
-            }
+            const double* p = lead_pos + R.pos_off + 3 * i;
+            const double pp = __builtin_fma(p[2], p[2], __builtin_fma(p[1], p[1], p[0] * p[0]));
+            pe[j][g] = kq < 3 ? -2.0 * p[kq < 3 ? kq : 0] : pp;
         }
     }
     double acc[RQ][NG];
@@ -1199,14 +1195,19 @@ __global__ __launch_bounds__(CC_THREADS) void ecg_class_chunk_kernel(
     for (int q0 = cb + 16 * w; q0 < ce; q0 += step * U) {
         uint4 cwu[U];
         int kcu[U];
+        float cau[U];
         bool oku[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int q = q0 + u * step + cl;
+            const int q = q0 + u * step + cl, qa = q0 + u * step + ca;
             oku[u] = q < ce;
             const int qs = oku[u] ? q : q0;
             cwu[u] = H.ccells[qs];
             if (RQ > 1) kcu[u] = H.ccellk[qs];
+            // coordinate kq of cell ca (kq = 3: the constant 1); a cell past
+            // the chunk's end reads the group's first cell (its weight is
+            // multiplied by 0)
+            cau[u] = kq < 3 ? reinterpret_cast<const float*>(H.ccells + (qa < ce ? qa : q0))[kq] : 1.0f;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -1223,20 +1224,20 @@ __global__ __launch_bounds__(CC_THREADS) void ecg_class_chunk_kernel(
                           D2 = (int)(int8_t)((cw.w >> 16) & 255u);
                 kc = -(D0 * (int)f0 + D1 * (int)f1 + D2 * (int)f2);
             }
-            // B: the lane's coefficient of its cell, D_c (byte c of the packed
-            // vector) or K; cells past the chunk's end add nothing
+            // product-3 right operand: the lane's coefficient of its cell, D_c
+            // (byte c of the packed vector) or K; cells past the chunk's end
+            // add nothing
             const int dc = (int)(int8_t)((cw.w >> (8 * rc)) & 255u);
             const double bv = (double)((rc == 3 ? kc : dc) & -(int)ok);
+            // |c|^2 of the lane's cell: integers below 2^22, exact in FP32
+            const double cc = (double)__builtin_fmaf(f2, f2, __builtin_fmaf(f1, f1, f0 * f0));
+            const double av = (double)cau[u];
 #pragma unroll
             for (int j = 0; j < RQ; ++j)
 #pragma unroll
                 for (int g = 0; g < NG; ++g) {
-                    const float e0 = (ph[j][g][0] - f0) + pl[j][g][0], e1 = (ph[j][g][1] - f1) + pl[j][g][1],
-                                e2 = (ph[j][g][2] - f2) + pl[j][g][2];
-                    float l = e0 * e0;
-                    l = __builtin_fmaf(e1, e1, l);
-                    l = __builtin_fmaf(e2, e2, l);
-                    const float rr = __builtin_amdgcn_rsqf(l);
+                    const double e2 = __builtin_amdgcn_mfma_f64_4x4x4f64(av, pe[j][g], cc, 0, 0, 0);
+                    const float rr = __builtin_amdgcn_rsqf((float)e2);
                     acc[j][g] =
                         __builtin_amdgcn_mfma_f64_4x4x4f64((double)(rr * rr * rr), bv, acc[j][g], 0, 0, 0);
                 }
@@ -1547,9 +1548,11 @@ hipError_t launch_ecg_chunk(const GroupArgs& a, hipStream_t s)
 template <int LP>
 static void class_chunk_launch(const GroupArgs& a, hipStream_t s)
 {
-    // two requests per block (timed against one and four at 256 x 12 leads:
-    // four needs 116 VGPRs, 4 waves per SIMD, and is slower)
-    if (a.cc_rpb == 2)
+    if (a.cc_rpb == 4 && a.cc_units == nullptr)
+        hipLaunchKernelGGL((ecg_class_chunk_kernel<LP, 4>), dim3((unsigned)a.n_cc), dim3(CC_THREADS), 0, s, a.hearts,
+                           a.reqs, a.B, a.cc_start, a.cc_uniform, a.cc_units, a.cc_ustart, a.n_units, a.lead_pos,
+                           a.layer_k, a.sums);
+    else if (a.cc_rpb == 2)
         hipLaunchKernelGGL((ecg_class_chunk_kernel<LP, 2>), dim3((unsigned)a.n_cc), dim3(CC_THREADS), 0, s, a.hearts,
                            a.reqs, a.B, a.cc_start, a.cc_uniform, a.cc_units, a.cc_ustart, a.n_units, a.lead_pos,
                            a.layer_k, a.sums);
@@ -1634,6 +1637,7 @@ static void prime_lp(hipFuncAttributes& at)
     prime_nh<LP, 3>(at);
     (void)hipFuncGetAttributes(&at, (const void*)ecg_class_chunk_kernel<LP, 1>);
     (void)hipFuncGetAttributes(&at, (const void*)ecg_class_chunk_kernel<LP, 2>);
+    (void)hipFuncGetAttributes(&at, (const void*)ecg_class_chunk_kernel<LP, 4>);
     (void)hipFuncGetAttributes(&at, (const void*)class_sums_kernel<LP>);
     (void)hipFuncGetAttributes(&at, (const void*)layer_prefix_kernel<LP>);
     (void)hipFuncGetAttributes(&at, (const void*)ecg_reduce_kernel<LP>);

@@ -184,8 +184,12 @@ GroupPlan plan_group(Ctx& ctx, std::vector<ReqSpec> reqs)
     GroupPlan g;
     g.nhood = reqs[0].nhood;
     int L = 1, kernel = ctx.kernel, waves = 0, phase = 0;
+    const bool path0 = heart_mfma(ctx, *reqs[0].h);
     for (const ReqSpec& r : reqs) {
         if (r.nhood != g.nhood) fail(EKG_EINVAL, "all requests of a launch group must share the neighbourhood");
+        // each heart's layout is the one of its own contraction path
+        if (heart_mfma(ctx, *r.h) != path0)
+            fail(EKG_EINVAL, "all requests of a launch group must take the same contraction path");
         L = std::max(L, r.L);
         if (r.h->tune_kernel) kernel = r.h->tune_kernel;
         if (r.h->tune_waves) waves = std::max(waves, r.h->tune_waves);
@@ -210,6 +214,9 @@ constexpr int DEFAULT_CLASS_MIN = 64;
 // Largest class chunk (voxels): the class-chunk kernel sums its lead weights
 // cell by cell in registers (see ensure_cells).
 constexpr int64_t CLASS_CHUNK = 4096;
+// Requests per class-chunk block when a group uses one heart (each cell's
+// record is loaded and unpacked once for all of them).
+constexpr int CC_RPB_ONE_HEART = 2;
 
 void ensure_layout(Heart& h, bool mfma)
 {
@@ -222,6 +229,12 @@ void ensure_layout(Heart& h, bool mfma)
     // activation time with at least cmin voxels.
     const int cmin = (!mfma || h.tune_class < 0) ? 0 : (h.tune_class > 0 ? h.tune_class : DEFAULT_CLASS_MIN);
     if (L.valid && L.chunk_target == target && L.class_min == cmin) return;
+    // The layout's device buffers are rewritten in place: kernels still
+    // reading the old layout on any stream (a server's slots) finish first.
+    if (L.valid) {
+        h.ctx->bind();
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    }
     L.valid = false;
     std::vector<int64_t> idx;
     idx.reserve(h.n_myo);
@@ -847,13 +860,13 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     a.cl_per_req = (B > 0 && B % 8 == 0) ? (int)(st_cl[1] - st_cl[0]) : 0;
     for (int r = 1; r < B && a.cl_per_req; ++r)
         if (st_cl[r + 1] - st_cl[r] != a.cl_per_req) a.cl_per_req = 0;
-    // one heart: each class-chunk block takes a chunk for two requests
+    // one heart: each class-chunk block takes a chunk for CC_RPB_ONE_HEART requests
     a.cc_rpb = 1;
     bool one_heart = B > 1 && a.cc_uniform > 0;
     for (int r = 1; r < B && one_heart; ++r)
         if (g.reqs[r].h != g.reqs[0].h) one_heart = false;
     if (one_heart) {
-        a.cc_rpb = 2;
+        a.cc_rpb = CC_RPB_ONE_HEART;
         a.n_cc = (int64_t)a.cc_uniform * ((B + a.cc_rpb - 1) / a.cc_rpb);
     } else if (!units.empty()) {
         a.cc_rpb = 2;
@@ -1219,10 +1232,14 @@ void excite(Heart& h, bool is_int, const double* transfer, int rows, int cols, c
     ctx.bind();
     const int nb0 = (h.n0 + 7) / 8, nb1 = (h.n1 + 7) / 8, nb2 = (h.n2 + 7) / 8;
     const int nbricks = nb0 * nb1 * nb2;
+    // Sweeps are enqueued in rounds of growing length (4, 8, 16, ... up to
+    // 64) with one flag per sweep; the host reads the flags once per round.
+    // A sweep after convergence finds no active brick and changes nothing.
+    constexpr int MAX_ROUND = 64;
     DevBuf d_dist, d_wtab, d_delay, d_flags, d_any;
     d_dist.reserve((size_t)h.nvox * (is_int ? 4 : 8));
     d_flags.reserve((size_t)nbricks * 2);
-    d_any.reserve(sizeof(int));
+    d_any.reserve(MAX_ROUND * sizeof(int));
     if (!is_int) {
         d_wtab.reserve(wtab.size() * 8);
         hip_check(hipMemcpy(d_wtab.p, wtab.data(), wtab.size() * 8, hipMemcpyHostToDevice),
@@ -1238,20 +1255,25 @@ void excite(Heart& h, bool is_int, const double* transfer, int rows, int cols, c
     hip_check(ekgk::launch_ca_init(is_int, h.d_layer.as<uint8_t>(), h.d_seed.as<uint8_t>(), d_dist.p,
                                    fa, h.n0, h.n1, h.n2, nb1, nb2, s),
               "ca_init launch");
-    int any = 1;
     int64_t sweeps = 0;
     const int64_t max_sweeps = 64 + 8 * (int64_t)h.nvox;
-    while (any) {
-        hip_check(hipMemsetAsync(fb, 0, nbricks, s), "hipMemsetAsync");
-        hip_check(hipMemsetAsync(d_any.p, 0, sizeof(int), s), "hipMemsetAsync");
-        hip_check(ekgk::launch_ca_sweep(is_int, h.d_layer.as<uint8_t>(), d_delay.as<uint8_t>(),
-                                        d_wtab.as<double>(), cols, d_dist.p, h.n0, h.n1, h.n2, nb0,
-                                        nb1, nb2, fa, fb, d_any.as<int>(), s),
-                  "ca_sweep launch");
-        hip_check(hipMemcpyAsync(&any, d_any.p, sizeof(int), hipMemcpyDeviceToHost, s), "read flag");
+    int flags[MAX_ROUND];
+    for (int round = 4;; round = std::min(2 * round, MAX_ROUND)) {
+        hip_check(hipMemsetAsync(d_any.p, 0, (size_t)round * sizeof(int), s), "hipMemsetAsync");
+        for (int q = 0; q < round; ++q) {
+            hip_check(hipMemsetAsync(fb, 0, nbricks, s), "hipMemsetAsync");
+            hip_check(ekgk::launch_ca_sweep(is_int, h.d_layer.as<uint8_t>(), d_delay.as<uint8_t>(),
+                                            d_wtab.as<double>(), cols, d_dist.p, h.n0, h.n1, h.n2, nb0,
+                                            nb1, nb2, fa, fb, d_any.as<int>() + q, s),
+                      "ca_sweep launch");
+            std::swap(fa, fb);
+        }
+        hip_check(hipMemcpyAsync(flags, d_any.p, (size_t)round * sizeof(int), hipMemcpyDeviceToHost, s),
+                  "read flags");
         hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
-        std::swap(fa, fb);
-        if (++sweeps > max_sweeps) fail(EKG_EDEVICE, "excitation relaxation did not converge");
+        sweeps += round;
+        if (!flags[round - 1]) break;  // the round's last sweep marked no brick: converged
+        if (sweeps > max_sweeps) fail(EKG_EDEVICE, "excitation relaxation did not converge");
     }
     h.d_act.reserve((size_t)h.nvox * 8);
     hip_check(ekgk::launch_ca_finalize(is_int, d_dist.p, h.d_act.as<double>(), h.nvox, s),

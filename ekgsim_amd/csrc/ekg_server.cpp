@@ -5,8 +5,9 @@
 // external evaluator is one process per simulation (ExternalEvaluation.h:95-151).
 // Here one worker thread owns n_streams launch slots.  Requests queue FIFO;
 // whenever a slot is free the worker takes the oldest request and every
-// queued request with the same dipole neighbourhood -- any heart, lead
-// count, duration and time step -- up to max_batch, shrinks the group until
+// queued request with the same dipole neighbourhood and contraction path
+// (the path each heart's tuning selects) -- any heart, lead count, duration
+// and time step -- up to max_batch, shrinks the group until
 // its device bytes fit the budget and the in-flight cap, and enqueues it on
 // the slot's stream as one launch group (one launch per kernel).  Groups on
 // different slots run concurrently, so a request that arrives while others
@@ -32,6 +33,55 @@ using Clock = std::chrono::steady_clock;
 
 namespace {
 
+// Pinned host memory for the ECGs of one launch group.  The group's requests
+// share it: each keeps it alive until its result has been copied out by
+// ekg_server_wait, then it returns to the server's pool for a later group.
+struct HostBuf {
+    double* p = nullptr;
+    size_t cap = 0;  // doubles
+};
+
+class HostPool {
+public:
+    // a buffer of at least n doubles, from the pool or newly pinned
+    std::shared_ptr<HostBuf> take(size_t n)
+    {
+        HostBuf* b = nullptr;
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            for (size_t i = 0; i < free_.size(); ++i)
+                if (free_[i]->cap >= n) {
+                    b = free_[i];
+                    free_.erase(free_.begin() + (std::ptrdiff_t)i);
+                    break;
+                }
+        }
+        if (!b) {
+            std::unique_ptr<HostBuf> nb(new HostBuf);
+            const size_t cap = std::max<size_t>(n, 1);
+            hip_check(hipHostMalloc((void**)&nb->p, cap * 8, hipHostMallocDefault), "hipHostMalloc");
+            nb->cap = cap;
+            b = nb.release();
+        }
+        return std::shared_ptr<HostBuf>(b, [this](HostBuf* q) {
+            std::lock_guard<std::mutex> g(mu_);
+            free_.push_back(q);
+        });
+    }
+    // called once no result refers to a buffer any more
+    ~HostPool()
+    {
+        for (HostBuf* b : free_) {
+            if (b->p) (void)hipHostFree(b->p);
+            delete b;
+        }
+    }
+
+private:
+    std::mutex mu_;
+    std::vector<HostBuf*> free_;
+};
+
 struct Request {
     uint64_t ticket = 0;
     int heart_id = 0;
@@ -41,7 +91,9 @@ struct Request {
     enum State { Pending, Running, Done, Failed } state = Pending;
     int err = EKG_OK;
     std::string msg;
-    std::vector<double> ecg;
+    // the result: n_out doubles at out->p + out_off
+    std::shared_ptr<HostBuf> out;
+    size_t out_off = 0, n_out = 0;
     Clock::time_point t_submit, t_done;
 };
 
@@ -52,16 +104,14 @@ struct Slot {
     bool busy = false;
     std::vector<Request*> reqs;
     std::vector<int64_t> out_off;
-    double* h_out = nullptr;
-    size_t h_out_cap = 0;
+    std::shared_ptr<HostBuf> h_out;
     size_t bytes = 0;
 };
-
-bool same_group(const Request& a, const Request& b) { return a.desc.nhood == b.desc.nhood; }
 
 }  // namespace
 
 struct Server {
+    HostPool pool;  // destroyed last: every result buffer has returned by then
     Ctx& ctx;
     ekg_server_opts opts{};
     std::vector<Heart*> hearts;
@@ -77,6 +127,27 @@ struct Server {
     size_t bytes_in_use = 0, peak_bytes = 0;
 
     explicit Server(Ctx& c) : ctx(c) {}
+    // Requests that may share a launch group: the same dipole neighbourhood
+    // and the same contraction path of their hearts (a group runs one path,
+    // and each heart's layout is the one of its own path).
+    bool same_group(const Request& a, const Request& b) const
+    {
+        return a.desc.nhood == b.desc.nhood &&
+               heart_mfma(ctx, *hearts[a.heart_id]) == heart_mfma(ctx, *hearts[b.heart_id]);
+    }
+    // Requests the group of the oldest pending request could take now: its
+    // group's pending requests up to max_batch and the in-flight cap.
+    int group_room() const
+    {
+        const Request& head = *pending.front();
+        int n = 0;
+        const int cap = std::min(opts.max_batch, opts.max_inflight - inflight);
+        for (const Request* r : pending) {
+            if (n >= cap) break;
+            if (same_group(head, *r)) ++n;
+        }
+        return cap - n;
+    }
     void run();
     void retire(Slot& s);
     void fail_group(std::vector<Request*>& g, int code, const std::string& msg);
@@ -98,7 +169,9 @@ void Server::retire(Slot& s)
 {
     for (size_t b = 0; b < s.reqs.size(); ++b) {
         Request* r = s.reqs[b];
-        r->ecg.assign(s.h_out + s.out_off[b], s.h_out + s.out_off[b + 1]);
+        r->out = s.h_out;  // no copy here: ekg_server_wait copies out of the group's buffer
+        r->out_off = (size_t)s.out_off[b];
+        r->n_out = (size_t)(s.out_off[b + 1] - s.out_off[b]);
         r->state = Request::Done;
         r->t_done = Clock::now();
         ++completed;
@@ -106,6 +179,7 @@ void Server::retire(Slot& s)
     inflight -= (int)s.reqs.size();
     bytes_in_use -= s.bytes;
     s.reqs.clear();
+    s.h_out.reset();
     s.busy = false;
     s.bytes = 0;
     cv_done.notify_all();
@@ -128,6 +202,7 @@ void Server::run()
                 inflight -= (int)s.reqs.size();
                 bytes_in_use -= s.bytes;
                 s.reqs.clear();
+                s.h_out.reset();
                 s.busy = false;
             } else {
                 any_busy = true;
@@ -147,11 +222,12 @@ void Server::run()
             }
         bool launched = false;
         // Batching window: while another group runs (the GPU is busy anyway),
-        // a group below max_batch waits for more requests, up to
+        // a group that could still take more requests (below max_batch and
+        // the in-flight cap, inside the device budget) waits for them, up to
         // batch_window_us from its oldest request; an idle GPU starts at once.
         bool hold = false;
         if (free_slot && !pending.empty() && !stop && any_busy && opts.batch_window_us > 0 &&
-            (int)pending.size() < opts.max_batch) {
+            bytes_in_use < opts.hbm_budget_bytes && group_room() > 0) {
             const auto age = std::chrono::duration_cast<std::chrono::microseconds>(Clock::now() -
                                                                                   pending.front()->t_submit);
             hold = age.count() < opts.batch_window_us;
@@ -256,18 +332,10 @@ void Server::run()
                     for (Heart* h : hl) locks.emplace_back(h->mu);
                     Slot& s = *free_slot;
                     const size_t n_out = (size_t)s.out_off.back();
-                    if (s.h_out_cap < n_out) {
-                        if (s.h_out) (void)hipHostFree(s.h_out);
-                        s.h_out = nullptr;
-                        s.h_out_cap = 0;
-                        hip_check(hipHostMalloc((void**)&s.h_out, std::max<size_t>(n_out, 1) * 8,
-                                                hipHostMallocDefault),
-                                  "hipHostMalloc");
-                        s.h_out_cap = n_out;
-                    }
+                    s.h_out = pool.take(n_out);
                     enqueue_group(ctx, plan, s.ws, s.stream, nullptr, nullptr, nullptr, false);
                     if (n_out)
-                        hip_check(hipMemcpyAsync(s.h_out, s.ws.ecg.p, n_out * 8, hipMemcpyDeviceToHost,
+                        hip_check(hipMemcpyAsync(s.h_out->p, s.ws.ecg.p, n_out * 8, hipMemcpyDeviceToHost,
                                                  s.stream),
                                   "download ECGs");
                     hip_check(hipEventRecord(s.done, s.stream), "hipEventRecord");
@@ -281,6 +349,7 @@ void Server::run()
                     inflight -= (int)free_slot->reqs.size();
                     bytes_in_use -= free_slot->bytes;
                     free_slot->reqs.clear();
+                    free_slot->h_out.reset();
                     free_slot->busy = false;
                     free_slot->bytes = 0;
                 }
@@ -345,11 +414,12 @@ void server_destroy(Server* s)
     for (auto& sp : s->slots) {
         Slot& sl = *sp;
         if (sl.stream) (void)hipStreamSynchronize(sl.stream);
-        if (sl.h_out) (void)hipHostFree(sl.h_out);
+        sl.h_out.reset();
         if (sl.done) (void)hipEventDestroy(sl.done);
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
     s->slots.clear();
+    s->reqs.clear();  // results never waited for: their buffers return to the pool
     delete s;
 }
 
@@ -408,21 +478,26 @@ bool server_poll(Server& s, uint64_t ticket)
 
 void server_wait(Server& s, uint64_t ticket, double* ecg, size_t cap, double* latency_ms)
 {
-    std::unique_lock<std::mutex> lk(s.mu);
-    auto it = s.reqs.find(ticket);
-    if (it == s.reqs.end()) fail(EKG_ENOTFOUND, "unknown ticket");
-    Request* r = it->second.get();
-    s.cv_done.wait(lk, [&] { return r->state == Request::Done || r->state == Request::Failed; });
-    std::unique_ptr<Request> own = std::move(it->second);
-    s.reqs.erase(it);
+    std::unique_ptr<Request> own;
+    {
+        std::unique_lock<std::mutex> lk(s.mu);
+        auto it = s.reqs.find(ticket);
+        if (it == s.reqs.end()) fail(EKG_ENOTFOUND, "unknown ticket");
+        Request* r = it->second.get();
+        s.cv_done.wait(lk, [&] { return r->state == Request::Done || r->state == Request::Failed; });
+        own = std::move(it->second);
+        s.reqs.erase(it);
+    }
+    // the copy runs without the server lock (the worker keeps admitting)
+    Request* r = own.get();
     if (latency_ms)
         *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
     if (r->state == Request::Failed) fail(r->err, r->msg);
     if (ecg) {
-        if (cap < r->ecg.size())
+        if (cap < r->n_out)
             fail(EKG_EINVAL, "output buffer holds " + std::to_string(cap) + " doubles, " +
-                                 std::to_string(r->ecg.size()) + " needed");
-        std::memcpy(ecg, r->ecg.data(), r->ecg.size() * sizeof(double));
+                                 std::to_string(r->n_out) + " needed");
+        if (r->n_out) std::memcpy(ecg, r->out->p + r->out_off, r->n_out * sizeof(double));
     }
 }
 

@@ -192,9 +192,10 @@ int ekg_heart_set_tune(ekg_heart *h, const ekg_tune *t);
 int ekg_heart_get_tune(const ekg_heart *h, ekg_tune *t);
 
 /* Layout of the heart's matrix-core path as currently tuned (built on
- * demand): chunks, activation classes and, for dipole neighbourhood `nhood`,
- * the class cells whose lead weights each simulation evaluates once.  No
- * counterpart in the reference (diagnostics of this library's layout). */
+ * demand, so that later calls find it ready): chunks, activation classes and,
+ * for dipole neighbourhood `nhood`, the class cells whose lead weights each
+ * simulation evaluates once.  No counterpart in the reference (diagnostics of
+ * this library's layout). */
 typedef struct ekg_layout_info {
     int64_t voxels;        /* myocardial voxels */
     int64_t class_voxels;  /* of which in activation classes */
@@ -318,14 +319,17 @@ int ekg_resample(const double *src, int64_t n, double factor, int start, double 
 /* Device-resident request loop standing in for AMS-DEMO's
  * ExternalEvaluation / PFramework job queues (ExternalEvaluation.h:95-151,
  * ParallelFramework.h:286-430): requests are admitted while their device
- * memory fits hbm_budget_bytes and a slot is free, grouped per heart into
- * fused launches, and overlap on the GPU with requests already running. */
+ * memory fits hbm_budget_bytes and a slot is free, grouped into fused
+ * launches per dipole neighbourhood and contraction path (requests of several
+ * hearts share a group), and overlap on the GPU with requests already
+ * running. */
 typedef struct ekg_server_opts {
     size_t hbm_budget_bytes;  /* device memory the server may use for requests (0 = 16 GiB) */
     int max_inflight;         /* requests on the GPU at once (0 = 1024) */
     int max_batch;            /* requests per launch group (0 = 256) */
     int n_streams;            /* concurrent launch groups (0 = 2) */
-    int batch_window_us;      /* while another group runs, a group below max_batch waits up to
+    int batch_window_us;      /* while another group runs, a group that could still grow (below
+                                 max_batch and the in-flight cap, inside the budget) waits up to
                                  this long (from its oldest request) for more requests
                                  (0 = 1000 us; < 0 = launch at once) */
 } ekg_server_opts;

@@ -136,3 +136,54 @@ def test_hbm_budget_limits_groups(ctx):
     with pytest.raises(EkgError):
         srv.wait(t)
     srv.close()
+
+
+def test_hearts_on_different_paths_equal_alone():
+    """Two hearts tuned to different contraction paths (matrix cores and
+    vector ALUs) in one server: requests of the two never share a launch
+    group (each heart keeps the layout of its own path), and every served
+    ECG equals the alone call bit for bit."""
+    from ekgsim_amd import Context, Server
+    c = Context(0)  # the context leaves the path to each heart's tuning
+    hearts = _hearts(c, [(24, 11), (28, 12)])
+    hearts[0][1].set_tune(kernel="mfma")
+    hearts[1][1].set_tune(kernel="valu")
+    reqs = _requests(hearts, 24, 3)
+    srv = Server(c, max_batch=8, n_streams=2)
+    ids = [srv.add_heart(H) for _, H in hearts]
+    tickets = [srv.submit(ids[hi], k, pos, "3D4", 0.0, length, 1.0) for hi, k, pos, length in reqs]
+    got = [srv.wait(t)[0] for t in tickets]
+    srv.close()
+    for req, g in zip(reqs, got):
+        assert np.array_equal(g, _alone(hearts, req))
+    for _, H in hearts:
+        H.close()
+    c.close()
+
+
+@pytest.mark.parametrize("window_us", [-1, 0, 3000])
+def test_batching_window_equals_alone(ctx, window_us):
+    """The batching window (off, the 1 ms default, 3 ms) changes only how
+    requests are grouped, never a result; with the window on, a burst does
+    not take more launch groups than without it."""
+    from ekgsim_amd import Server
+    hearts = _hearts(ctx, [(24, 13), (20, 14)])
+    reqs = _requests(hearts, 40, 4)
+    alone = [_alone(hearts, r) for r in reqs]
+
+    def burst(w):
+        srv = Server(ctx, max_batch=16, n_streams=2, batch_window_us=w)
+        ids = [srv.add_heart(H) for _, H in hearts]
+        tickets = [srv.submit(ids[hi], k, pos, "3D4", 0.0, length, 1.0) for hi, k, pos, length in reqs]
+        got = [srv.wait(t)[0] for t in tickets]
+        st = srv.stats()
+        srv.close()
+        return got, st["groups"]
+
+    got, groups = burst(window_us)
+    for g, a in zip(got, alone):
+        assert np.array_equal(g, a)
+    if window_us >= 0:
+        _, groups_off = burst(-1)
+        print(f"window {window_us} us: {groups} groups; no window: {groups_off} groups", flush=True)
+        assert groups <= groups_off + 1
