@@ -33,7 +33,7 @@ EXPORTS = (
     "ekg_ctx_synchronize", "ekg_ctx_set_kernel", "ekg_ap_value", "ekg_ap_num_genes", "ekg_layer_aps",
     "ekg_criterion", "ekg_resample", "ekg_heart_autotune", "ekg_heart_set_tune", "ekg_heart_get_tune",
     "ekg_ctx_set_graphs", "ekg_ctx_graph_stats", "ekg_heart_layout_info", "ekg_criteria_device",
-    "ekg_simulate_criteria", "ekg_ap_apd90",
+    "ekg_simulate_criteria", "ekg_ap_apd90", "ekg_simulate_batch_device_noalloc",
 )
 KERNELS = {"auto": 0, "valu": 1, "mfma": 2}
 
@@ -128,6 +128,7 @@ def lib():
     L.ekg_simulate.argtypes = [vp, C.POINTER(SimDesc), vp]
     L.ekg_simulate_batch.argtypes = [vp, C.c_int, C.POINTER(SimDesc), vp]
     L.ekg_simulate_batch_device.argtypes = [vp, C.POINTER(BatchDev)]
+    L.ekg_simulate_batch_device_noalloc.argtypes = [vp, C.POINTER(BatchDev)]
     L.ekg_ctx_synchronize.argtypes = [vp]
     L.ekg_ctx_set_kernel.argtypes = [vp, C.c_int]
     L.ekg_lead_basis.argtypes = [vp, vp, vp]
@@ -528,6 +529,18 @@ class Heart:
                      NHOOD[nhood] if isinstance(nhood, str) else int(nhood),
                      float(start), float(length), float(dt), C.c_void_p(int(ecg_ptr)))
         check(lib().ekg_simulate_batch_device(self.h, C.byref(d)))
+
+    def simulate_batch_device_noalloc(self, layer_k_ptr, lead_pos_ptr, ecg_ptr, batch, n_layers, n_leads,
+                                      nhood="3D4", start=0.0, length=600.0, dt=1.0):
+        """simulate_batch_device with no device allocation: every group buffer
+        lives in the GPU's static workspace (ekg_simulate_batch_device_noalloc);
+        the heart's layout for `nhood` must be built (an earlier simulation, or
+        layout_info on the matrix-core path)."""
+        d = BatchDev(int(batch), int(n_layers), C.c_void_p(int(layer_k_ptr)), int(n_leads),
+                     C.c_void_p(int(lead_pos_ptr)),
+                     NHOOD[nhood] if isinstance(nhood, str) else int(nhood),
+                     float(start), float(length), float(dt), C.c_void_p(int(ecg_ptr)))
+        check(lib().ekg_simulate_batch_device_noalloc(self.h, C.byref(d)))
 
 
 class Server:

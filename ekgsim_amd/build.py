@@ -33,7 +33,7 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-p
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HOST = ["-ffp-contract=off", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include")]
 
-KERNEL_SRC = ["ekg_ca.hip", "ekg_ecg.hip", "ekg_crit.hip"]
+KERNEL_SRC = ["ekg_ca.hip", "ekg_ecg.hip", "ekg_crit.hip", "ekg_scratch.hip"]
 # The C++ facade (sim_lib.cpp with its ini reader) is part of the library, so
 # a caller written against the reference's SimLib::EkgSim links libekgsim.so
 # alone.

@@ -101,6 +101,12 @@ int ekg_ctx_create(int device, ekg_ctx** out)
             c->c.bind();
             ekg::hip_check(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking),
                            "hipStreamCreate");
+            // the device's static workspace (allocation-free entry point),
+            // resolved once per device with that device current
+            void* scratch = nullptr;
+            ekg::hip_check(ekgk::scratch_ptr(device, &scratch), "resolve the static device workspace");
+            c->c.ws_fixed.arena = static_cast<char*>(scratch);
+            c->c.ws_fixed.arena_bytes = ekgk::SCRATCH_BYTES;
             // per neighbourhood: {d0, d1, d2, index of d among the 26 cube offsets}
             int8_t tab[4][26][4] = {};
             int8_t cube[26][3];
@@ -319,6 +325,14 @@ int ekg_simulate_batch_device(ekg_heart* h, const ekg_batch_dev* d)
     return guard([&] {
         if (!h || !d) ekg::fail(EKG_EINVAL, "NULL argument");
         ekg::simulate_device(h->h, *d);
+    });
+}
+
+int ekg_simulate_batch_device_noalloc(ekg_heart* h, const ekg_batch_dev* d)
+{
+    return guard([&] {
+        if (!h || !d) ekg::fail(EKG_EINVAL, "NULL argument");
+        ekg::simulate_device_fixed(h->h, *d);
     });
 }
 

@@ -42,11 +42,35 @@ void DevBuf::reserve(size_t n)
     bytes = n;
 }
 
+void DevBuf::adopt(void* q, size_t n)
+{
+    release();
+    p = q;
+    bytes = n;
+    owned = false;
+}
+
 void DevBuf::release()
 {
-    if (p) (void)hipFree(p);
+    if (p && owned) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+    owned = true;
+}
+
+void Workspace::need(DevBuf& b, size_t n)
+{
+    if (!arena) {
+        b.reserve(n);
+        return;
+    }
+    const size_t at = (arena_used + 255) & ~(size_t)255;
+    if (at > arena_bytes || n > arena_bytes - at)
+        fail(EKG_ENOMEM, "the group needs more than the " + std::to_string(arena_bytes >> 20) +
+                             " MiB static device workspace of the allocation-free entry point; use fewer "
+                             "requests per call or ekg_simulate_batch_device");
+    b.adopt(arena + at, n);
+    arena_used = at + n;
 }
 
 void Ctx::bind() const { hip_check(hipSetDevice(device), "hipSetDevice"); }
@@ -218,16 +242,32 @@ constexpr int64_t CLASS_CHUNK = 4096;
 // record is loaded and unpacked once for all of them).
 constexpr int CC_RPB_ONE_HEART = 2;
 
+namespace {
+// Chunk size: DEFAULT_CHUNK voxels unless the heart is tuned.  Chunks are
+// layer-pure and span at most CHUNK_SPAN_MS of activation time.
+int64_t chunk_target_of(const Heart& h) { return h.tune_chunk ? h.tune_chunk : DEFAULT_CHUNK; }
+// Activation classes (matrix-core path only): runs of one layer and one
+// activation time with at least this many voxels (0: none).
+int class_min_of(const Heart& h, bool mfma)
+{
+    return (!mfma || h.tune_class < 0) ? 0 : (h.tune_class > 0 ? h.tune_class : DEFAULT_CLASS_MIN);
+}
+}  // namespace
+
+bool layout_ready(const Heart& h, bool mfma, int nhood)
+{
+    const EcgLayout& L = h.layout;
+    if (!(h.excited && L.valid && L.chunk_target == chunk_target_of(h) && L.class_min == class_min_of(h, mfma)))
+        return false;
+    return !mfma || (nhood >= 0 && nhood < 4 && L.cells_built[nhood]);
+}
+
 void ensure_layout(Heart& h, bool mfma)
 {
     if (!h.excited) fail(EKG_ESTATE, "excitation sequence not computed or loaded");
     EcgLayout& L = h.layout;
-    // Chunk size: DEFAULT_CHUNK voxels unless the heart is tuned.  Chunks
-    // are layer-pure and span at most CHUNK_SPAN_MS of activation time.
-    const int64_t target = h.tune_chunk ? h.tune_chunk : DEFAULT_CHUNK;
-    // Activation classes (matrix-core path only): runs of one layer and one
-    // activation time with at least cmin voxels.
-    const int cmin = (!mfma || h.tune_class < 0) ? 0 : (h.tune_class > 0 ? h.tune_class : DEFAULT_CLASS_MIN);
+    const int64_t target = chunk_target_of(h);
+    const int cmin = class_min_of(h, mfma);
     if (L.valid && L.chunk_target == target && L.class_min == cmin) return;
     // The layout's device buffers are rewritten in place: kernels still
     // reading the old layout on any stream (a server's slots) finish first.
@@ -649,9 +689,16 @@ size_t group_device_bytes(const GroupPlan& g)
 }
 
 void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, const double* d_k,
-                   const double* d_pos, double* d_ecg, bool time_fused)
+                   const double* d_pos, double* d_ecg, bool time_fused, bool allow_graph)
 {
     const int B = (int)g.reqs.size();
+    if (ws.arena) {
+        // a fixed workspace is shared by every call on the device: nothing of
+        // an earlier group (tables, time grids) can be assumed still there
+        ws.arena_used = 0;
+        ws.h_tables.clear();
+        ws.h_grids.clear();
+    }
     const int LP = g.LP;
     const bool dev_in = d_k != nullptr;
     // hearts of the group
@@ -747,18 +794,18 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
         st_cl[B] > 0x7fffffffLL)
         fail(EKG_EINVAL, "launch group too large");
     // device buffers (grow-only)
-    ws.lc.reserve((size_t)st_lay[B] * sizeof(ekgk::LayerConst));
-    ws.xy.reserve((size_t)std::max<int64_t>(1, st_xy[B]) * 16);
-    ws.G.reserve((size_t)std::max<int64_t>(1, rows) * 16 * 8);
-    ws.aux.reserve((size_t)std::max<int64_t>(1, rows) * sizeof(float4));
-    ws.sums.reserve((size_t)std::max<int64_t>(1, c_off) * 3 * LP * 8);
-    ws.bnd.reserve((size_t)std::max<int64_t>(1, c_off) * sizeof(int2));
-    ws.pre_rng.reserve((size_t)std::max<int64_t>(1, pre_off) * sizeof(int2));
-    ws.pre_ab.reserve((size_t)std::max<int64_t>(1, pre_off) * 2 * LP * 8);
-    ws.partial.reserve((size_t)std::max<int64_t>(1, p_off) * 8);
-    ws.csums.reserve((size_t)std::max<int64_t>(1, st_cs[B]) * 3 * LP * 8);
+    ws.need(ws.lc, (size_t)st_lay[B] * sizeof(ekgk::LayerConst));
+    ws.need(ws.xy, (size_t)std::max<int64_t>(1, st_xy[B]) * 16);
+    ws.need(ws.G, (size_t)std::max<int64_t>(1, rows) * 16 * 8);
+    ws.need(ws.aux, (size_t)std::max<int64_t>(1, rows) * sizeof(float4));
+    ws.need(ws.sums, (size_t)std::max<int64_t>(1, c_off) * 3 * LP * 8);
+    ws.need(ws.bnd, (size_t)std::max<int64_t>(1, c_off) * sizeof(int2));
+    ws.need(ws.pre_rng, (size_t)std::max<int64_t>(1, pre_off) * sizeof(int2));
+    ws.need(ws.pre_ab, (size_t)std::max<int64_t>(1, pre_off) * 2 * LP * 8);
+    ws.need(ws.partial, (size_t)std::max<int64_t>(1, p_off) * 8);
+    ws.need(ws.csums, (size_t)std::max<int64_t>(1, st_cs[B]) * 3 * LP * 8);
     if (!d_ecg) {
-        ws.ecg.reserve((size_t)std::max<int64_t>(1, st_out[B]) * 8);
+        ws.need(ws.ecg, (size_t)std::max<int64_t>(1, st_out[B]) * 8);
         d_ecg = ws.ecg.as<double>();
     }
     // A group of several hearts: neighbouring requests of one heart form
@@ -794,7 +841,7 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     const size_t o_units = tables.size();
     append(tables, units.data(), units.size());
     void* before = ws.hearts.p;
-    ws.hearts.reserve(tables.size());
+    ws.need(ws.hearts, tables.size());
     if (ws.hearts.p != before) ws.h_tables.clear();
     if (tables != ws.h_tables) {
         hip_check(hipMemcpyAsync(ws.hearts.p, tables.data(), tables.size(), hipMemcpyHostToDevice, stream),
@@ -805,7 +852,7 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     append(grid_bytes, ws.h_tgrid.data(), ws.h_tgrid.size());
     append(grid_bytes, ws.h_thl.data(), ws.h_thl.size());
     before = ws.tgrid.p;
-    ws.tgrid.reserve(grid_bytes.size());
+    ws.need(ws.tgrid, grid_bytes.size());
     if (ws.tgrid.p != before) ws.h_grids.clear();
     if (grid_bytes != ws.h_grids) {
         hip_check(hipMemcpyAsync(ws.tgrid.p, grid_bytes.data(), grid_bytes.size(), hipMemcpyHostToDevice, stream),
@@ -813,8 +860,8 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
         ws.h_grids.swap(grid_bytes);
     }
     if (!dev_in) {
-        ws.layer_k.reserve(ws.h_k.size() * 8);
-        ws.lead_pos.reserve(ws.h_pos.size() * 8);
+        ws.need(ws.layer_k, ws.h_k.size() * 8);
+        ws.need(ws.lead_pos, ws.h_pos.size() * 8);
         hip_check(hipMemcpyAsync(ws.layer_k.p, ws.h_k.data(), ws.h_k.size() * 8, hipMemcpyHostToDevice, stream),
                   "upload AP coefficients");
         hip_check(hipMemcpyAsync(ws.lead_pos.p, ws.h_pos.data(), ws.h_pos.size() * 8, hipMemcpyHostToDevice,
@@ -931,7 +978,7 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
         hip_check(ekgk::launch_ecg_reduce(a, stream), "ecg_reduce launch");
         hip_check(ekgk::launch_ecg_class(a, stream), "ecg_class launch");
     };
-    if (!ctx.graphs || time_fused) {
+    if (!ctx.graphs || time_fused || !allow_graph) {
         launch_all(time_fused);
         return;
     }
@@ -1006,6 +1053,32 @@ void simulate_device(Heart& h, const ekg_batch_dev& d)
     ctx.bind();
     GroupPlan g = plan_group(ctx, std::vector<ReqSpec>(d.batch, one));
     enqueue_group(ctx, g, ctx.ws, ctx.stream, d.layer_k_dev, d.lead_pos_dev, d.ecg_dev, ctx.profiling);
+}
+
+void simulate_device_fixed(Heart& h, const ekg_batch_dev& d)
+{
+    Ctx& ctx = *h.ctx;
+    std::lock_guard<std::mutex> g1(ctx.mu);
+    std::lock_guard<std::mutex> g2(h.mu);
+    if (!d.layer_k_dev || !d.lead_pos_dev || !d.ecg_dev) fail(EKG_EINVAL, "device pointers are required");
+    if (d.batch < 1 || d.batch > (1 << 20)) fail(EKG_EINVAL, "batch must hold 1..2^20 simulations");
+    if (!ctx.ws_fixed.arena) fail(EKG_ESTATE, "the device's static workspace is not resolved");
+    ekg_sim_desc s{};
+    s.n_layers = d.n_layers;
+    s.n_leads = d.n_leads;
+    s.nhood = d.nhood;
+    s.start = d.start;
+    s.length = d.length;
+    s.dt = d.dt;
+    const ReqSpec one = make_spec(h, s, false);
+    if (one.T == 0) return;
+    GroupPlan g = plan_group(ctx, std::vector<ReqSpec>(d.batch, one));
+    // the per-heart layout is device memory built on first use: it must exist
+    if (!layout_ready(h, g.mfma, g.nhood))
+        fail(EKG_ESTATE, "the heart's layout for this neighbourhood and tuning is not built; call "
+                         "ekg_heart_layout_info (or any simulation of the heart) first");
+    ctx.bind();
+    enqueue_group(ctx, g, ctx.ws_fixed, ctx.stream, d.layer_k_dev, d.lead_pos_dev, d.ecg_dev, false, false);
 }
 
 ekg_tune autotune(Heart& h, const ekg_sim_desc& sample, int batch)

@@ -23,11 +23,14 @@ struct Error {
 [[noreturn]] void fail(int code, const std::string& msg);
 void hip_check(hipError_t e, const char* what);
 
-// Grow-only device buffer.
+// Grow-only device buffer.  A buffer may instead point into memory it does
+// not own (adopt): release() then forgets it without freeing.
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    bool owned = true;
     void reserve(size_t n);
+    void adopt(void* q, size_t n);
     void release();
     template <class T> T* as() const { return static_cast<T*>(p); }
     ~DevBuf() { release(); }
@@ -47,12 +50,20 @@ struct Workspace {
     // captured launch sequence of the last group (graph mode)
     hipGraphExec_t graph = nullptr;
     std::vector<char> graph_key;
+    // Fixed workspace: when `arena` is set, every buffer of a group is carved
+    // from it in order (no device allocation); a group that does not fit
+    // fails with EKG_ENOMEM.  Contents are not kept between groups.
+    char* arena = nullptr;
+    size_t arena_bytes = 0, arena_used = 0;
+    // reserve() for an owning workspace, a piece of the arena otherwise
+    void need(DevBuf& b, size_t n);
     ~Workspace();
 };
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     Workspace ws;
+    Workspace ws_fixed;  // the allocation-free entry point's view of the device's static workspace
     bool profiling = false;
     int kernel = EKG_KERNEL_AUTO;  // contraction path of the fused kernel
     bool graphs = false;           // replay repeated launch groups from a captured hipGraph
@@ -146,6 +157,9 @@ GroupPlan plan_group(Ctx& ctx, std::vector<ReqSpec> reqs);
 // Builds (or keeps) the heart's layout for the matrix-core path (mfma, with
 // activation classes as tuned) or the vector-ALU path (no classes).
 void ensure_layout(Heart& h, bool mfma);
+// Whether ensure_layout (and, on the matrix-core path, ensure_cells for
+// nhood) would find everything built already.
+bool layout_ready(const Heart& h, bool mfma, int nhood);
 // Cell lists of the heart's class chunks for dipole neighbourhood `nhood`.
 void ensure_cells(Heart& h, int nhood);
 // Whether the heart's simulations take the matrix-core path (as plan_group decides).
@@ -156,8 +170,9 @@ size_t group_device_bytes(const GroupPlan& g);
 // d_k[r*nl*9 ...] and d_pos[r*L*3 ...] (uniform device batches).  ECGs go to
 // d_ecg (null: the workspace's ecg buffer), request r at the packed offset
 // sum_{q<r} L_q T_q.
+// allow_graph false: eager launches even in graph mode.
 void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, const double* d_k,
-                   const double* d_pos, double* d_ecg, bool time_fused);
+                   const double* d_pos, double* d_ecg, bool time_fused, bool allow_graph = true);
 void simulate(Heart& h, int batch, const ekg_sim_desc* descs, double* ecg);
 // Times the tunable knobs on `batch` copies of `sample` and keeps the fastest
 // on the heart: the contraction path, then for the matrix-core path every
@@ -165,6 +180,9 @@ void simulate(Heart& h, int batch, const ekg_sim_desc* descs, double* ecg);
 // and the lead-field phase width.
 ekg_tune autotune(Heart& h, const ekg_sim_desc& sample, int batch);
 void simulate_device(Heart& h, const ekg_batch_dev& d);
+// simulate_device with every buffer in the device's static workspace: no
+// device allocation, eager launches; the heart's layout must be built.
+void simulate_device_fixed(Heart& h, const ekg_batch_dev& d);
 // Criteria of device ECGs on the context's stream (targets uploaded from the host).
 void criteria_device(Ctx& ctx, const ekg_criteria_desc& c, const double* d_ecg, int batch, int n_leads,
                      int64_t n_steps, double* d_out);

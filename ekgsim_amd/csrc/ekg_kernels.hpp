@@ -160,6 +160,14 @@ hipError_t launch_class_chunks(const GroupArgs& a, hipStream_t s);
 hipError_t launch_class_sums(const GroupArgs& a, hipStream_t s);
 hipError_t launch_ecg_class(const GroupArgs& a, hipStream_t s);
 
+// ---- fixed workspace of the allocation-free entry point ----------------------
+
+// Bytes of the static device workspace (ekg_scratch.hip), per GPU.
+constexpr size_t SCRATCH_BYTES = (size_t)256 << 20;
+// Device address of `device`'s copy of the workspace (resolved once per
+// device with that device current, then cached).
+hipError_t scratch_ptr(int device, void** out);
+
 // ---- criteria -----------------------------------------------------------------
 
 // stats: per target {mean, variance, b . b, first maximum} over the first len samples

@@ -162,6 +162,27 @@ typedef struct ekg_batch_dev {
 } ekg_batch_dev;
 
 int ekg_simulate_batch_device(ekg_heart *h, const ekg_batch_dev *d);
+
+/* ekg_simulate_batch_device without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same kernels and bitwise the same ECGs; every group buffer
+ * (tables, time grids, AP tables, chunk and class sums, prefix tables,
+ * partial rows) lives in a 256 MiB static device workspace that each GPU holds
+ * from library load until the process ends.  Limits and conditions:
+ *   - the heart's layout for d->nhood and its current tuning must already be
+ *     built, by any earlier simulation of the heart on the same contraction
+ *     path or, for the matrix-core path, by ekg_heart_layout_info; else
+ *     EKG_ESTATE;
+ *   - a batch whose buffers exceed the workspace fails with EKG_ENOMEM before
+ *     any kernel runs;
+ *   - launches are eager (no graph capture of this library's own), and no
+ *     kernel timing is taken;
+ *   - every call on one GPU shares that GPU's workspace: calls that overlap on
+ *     one GPU (two contexts, two threads, or this call next to a server) race
+ *     and must be serialised by the caller.  Nothing needs to start at zero:
+ *     each call writes every workspace byte it reads. */
+int ekg_simulate_batch_device_noalloc(ekg_heart *h, const ekg_batch_dev *d);
 int ekg_ctx_synchronize(ekg_ctx *ctx);
 
 /* ---- launch tuning and graph replay (no reference counterpart) ---------- */
