@@ -240,7 +240,7 @@ constexpr int DEFAULT_CLASS_MIN = 64;
 constexpr int64_t CLASS_CHUNK = 4096;
 // Requests per class-chunk block when a group uses one heart (each cell's
 // record is loaded and unpacked once for all of them).
-constexpr int CC_RPB_ONE_HEART = 2;
+constexpr int CC_RPB_ONE_HEART = 4;
 
 namespace {
 // Chunk size: DEFAULT_CHUNK voxels unless the heart is tuned.  Chunks are
