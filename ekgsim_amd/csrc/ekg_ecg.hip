@@ -1093,8 +1093,8 @@ __global__ void ecg_reduce_kernel(const HeartDev* __restrict__ hearts, const Req
 constexpr int CC_THREADS = 256;
 
 // Class chunks: one block per (class chunk, request), or per (class chunk,
-// pair of requests) when every request of the group uses the same heart (the
-// cells are then loaded and unpacked once for both).  Every voxel of a class
+// RQ = 2 or 4 requests) when those requests use the same heart (the cells are
+// then loaded once for all of them).  Every voxel of a class
 // has the same AP, so only the sum of the chunk's G rows is needed, and by
 // regrouping the dipoles per cell it is
 //   sum_u G(u) = sum_w s(w) (e(w) . D(w))
@@ -1121,8 +1121,11 @@ constexpr int CC_THREADS = 256;
 // (cell r of block b, lead col) lands in lane 16r + 4b + col, which is where
 // product 3 takes its operand (block b, lead row, cell k): so a lane owns one
 // cell of each group of 16 and one lead of each group of 4 throughout, and
-// feeds product 1 coordinate (lane >> 4) of cell (b, lane & 3).  Each wave
-// takes groups of 16 cells (strided over the waves), the blocks' partial sums
+// feeds product 1 coordinate (lane >> 4) of cell (b, lane & 3).  The host
+// stores those three per-lane operands of every group lane-major (cgrp), so a
+// lane loads three coalesced words per group and converts them, with no index
+// arithmetic or unpacking.  Each wave takes groups of 16 cells (strided over
+// the waves), the blocks' partial sums
 // are added in block order, the waves' in wave order, and
 // sums[(c_off + n_chunks + k)*3*LP] receives {sum G, P' sum G, Q' sum G} with
 // the class's P' = k2(1-k3) e^{(k4+k5)(a - t0)} and Q' = k2 k3 e^{k5 (a - t0)}.
@@ -1137,9 +1140,9 @@ __global__ __launch_bounds__(CC_THREADS) void ecg_class_chunk_kernel(
     constexpr int NW = CC_THREADS / 64;
     __shared__ double s_t[RQ][NW][LP][4];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    // chunk-major: over pairs of requests (RQ = 2, one heart; an odd last
-    // request is paired with itself), else over requests when every request
-    // has the same class chunks
+    // chunk-major: over runs of RQ requests of one heart (RQ = 2 or 4; a
+    // short last run repeats its last request), else over requests when every
+    // request has the same class chunks
     int rq[RQ], k;
     if (RQ > 1 && cc_units != nullptr) {
         // pairs of one heart in a group of several hearts
@@ -1165,8 +1168,6 @@ __global__ __launch_bounds__(CC_THREADS) void ecg_class_chunk_kernel(
     const int4 ch = H.cchunks[k];
     const int rc = lane & 3;                                  // lead of a group (and product-3 column)
     const int kq = lane >> 4;                                 // product-1 inner index: coordinate kq
-    const int cl = 4 * ((lane >> 2) & 3) + kq;                // the lane's cell of a group of 16
-    const int ca = 4 * ((lane >> 2) & 3) + rc;                // the cell whose coordinate kq it feeds
     // product-1 right operand: {-2 p_z, -2 p_y, -2 p_x, |p|^2} of the lane's
     // leads 4g + rc (leads past the request's count repeat lead 0; their sums
     // are zeroed)
@@ -1187,59 +1188,53 @@ __global__ __launch_bounds__(CC_THREADS) void ecg_class_chunk_kernel(
     for (int j = 0; j < RQ; ++j)
 #pragma unroll
         for (int g = 0; g < NG; ++g) acc[j][g] = 0.0;
-    const int cb = H.ccell_off[k], ce = H.ccell_off[k + 1];
-    // one request: U groups of 16 cells per iteration, their records loaded
-    // together (that path is bound by load latency, not vector work)
+    // this wave's groups of 16 cells: gb + w, gb + w + NW, ...  U groups per
+    // iteration (4 for one request, whose loop is short of vector work to
+    // cover the loads), the next iteration's operands loaded before this
+    // one's products; a group past the chunk's end loads the wave's first
+    // group (in range) and is skipped
+    const int gb = H.cgrp_off[k], ge = H.cgrp_off[k + 1];
     constexpr int U = RQ > 1 ? 1 : 4;
-    constexpr int step = 16 * NW;
-    for (int q0 = cb + 16 * w; q0 < ce; q0 += step * U) {
-        uint4 cwu[U];
-        int kcu[U];
-        float cau[U];
-        bool oku[U];
+    float avu[U], bvu[U], ccu[U];
+    auto load = [&](int q0) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int q = q0 + u * step + cl, qa = q0 + u * step + ca;
-            oku[u] = q < ce;
-            const int qs = oku[u] ? q : q0;
-            cwu[u] = H.ccells[qs];
-            if (RQ > 1) kcu[u] = H.ccellk[qs];
-            // coordinate kq of cell ca (kq = 3: the constant 1); a cell past
-            // the chunk's end reads the group's first cell (its weight is
-            // multiplied by 0)
-            cau[u] = kq < 3 ? reinterpret_cast<const float*>(H.ccells + (qa < ce ? qa : q0))[kq] : 1.0f;
+            const int q = q0 + u * NW < ge ? q0 + u * NW : gb + w;
+            const float* o = H.cgrp + (int64_t)192 * q + lane;
+            avu[u] = o[0];
+            bvu[u] = o[64];
+            ccu[u] = o[128];
         }
+    };
+    // (the empty asm statements keep the compiler from merging the two loads
+    // of each operand into one at the loop head, which would wait for them
+    // right before their use)
+    if (gb + w < ge) load(gb + w);
+    asm volatile("" ::: "memory");
+    for (int q0 = gb + w; q0 < ge; q0 += NW * U) {
+        double av[U], bv[U], cc[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const uint4 cw = cwu[u];
-            const bool ok = oku[u];
-            const float f0 = __uint_as_float(cw.x), f1 = __uint_as_float(cw.y), f2 = __uint_as_float(cw.z);
-            // K: loaded when two requests share the cell (saves vector work),
-            // formed from D and the coordinates for one (saves a load)
-            int kc;
-            if (RQ > 1) {
-                kc = kcu[u];
-            } else {
-                const int D0 = (int)(int8_t)(cw.w & 255u), D1 = (int)(int8_t)((cw.w >> 8) & 255u),
-                          D2 = (int)(int8_t)((cw.w >> 16) & 255u);
-                kc = -(D0 * (int)f0 + D1 * (int)f1 + D2 * (int)f2);
-            }
-            // product-3 right operand: the lane's coefficient of its cell, D_c
-            // (byte c of the packed vector) or K; cells past the chunk's end
-            // add nothing
-            const int dc = (int)(int8_t)((cw.w >> (8 * rc)) & 255u);
-            const double bv = (double)((rc == 3 ? kc : dc) & -(int)ok);
-            // |c|^2 of the lane's cell: integers below 2^22, exact in FP32
-            const double cc = (double)__builtin_fmaf(f2, f2, __builtin_fmaf(f1, f1, f0 * f0));
-            const double av = (double)cau[u];
+            // product-1 left operand (coordinate kq of cell 4 block + rc, or
+            // 1), product-3 right operand (coefficient rc of cell
+            // 4 block + kq: D_rc or K) and |c|^2 of that cell
+            av[u] = (double)avu[u];
+            bv[u] = (double)bvu[u];
+            cc[u] = (double)ccu[u];
+        }
+        load(q0 + NW * U);
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (u > 0 && q0 + u * NW >= ge) break;  // the whole wave
 #pragma unroll
             for (int j = 0; j < RQ; ++j)
 #pragma unroll
                 for (int g = 0; g < NG; ++g) {
-                    const double e2 = __builtin_amdgcn_mfma_f64_4x4x4f64(av, pe[j][g], cc, 0, 0, 0);
+                    const double e2 = __builtin_amdgcn_mfma_f64_4x4x4f64(av[u], pe[j][g], cc[u], 0, 0, 0);
                     const float rr = __builtin_amdgcn_rsqf((float)e2);
                     acc[j][g] =
-                        __builtin_amdgcn_mfma_f64_4x4x4f64((double)(rr * rr * rr), bv, acc[j][g], 0, 0, 0);
+                        __builtin_amdgcn_mfma_f64_4x4x4f64((double)(rr * rr * rr), bv[u], acc[j][g], 0, 0, 0);
                 }
         }
     }

@@ -538,8 +538,20 @@ void ensure_layout(Heart& h, bool mfma)
 //   D(w) = [w in S] s_w + sum_{d in N, w - d in S, w myocardial} d,
 // an integer vector per cell; cells inside S cancel (D = 0) and are dropped.
 // Each remaining cell's weight is evaluated once per class chunk and request.
-// A cell is stored as {grid coordinates + 1 as three floats, D as three int8},
-// with its moment K = -D . (grid coordinates + 1) in a second array.
+// A cell w has grid coordinates + 1 (the reference's voxel centre) c(w), D(w)
+// and its moment K(w) = -D(w) . c(w), all integers.
+//
+// The class-chunk kernel takes a chunk's cells 16 at a time, one group per
+// wave and iteration, and each of the 64 lanes feeds the matrix products three
+// fixed operands of the group (lane = 16 kq + 4 b + rc, cell index within the
+// group cl = 4 b + kq, ca = 4 b + rc):
+//   av = coordinate kq of cell ca (kq = 3: 1),
+//   bv = coefficient rc of cell cl: D_0, D_1, D_2 (rc < 3) or K (rc = 3),
+//   cc = |c|^2 of cell cl.
+// They are stored lane-major, as FP32 (every value is an integer below 2^24,
+// exact), group g at cgrp[192 g]: av[64], bv[64], cc[64].  A chunk's cells
+// are padded to a multiple of 16: a missing cell repeats the group's first
+// cell with coefficients 0, so its weight is finite and adds nothing.
 void ensure_cells(Heart& h, int nhood)
 {
     EcgLayout& L = h.layout;
@@ -555,16 +567,14 @@ void ensure_cells(Heart& h, int nhood)
     std::unordered_map<int64_t, int> slot_map;
     std::vector<int64_t> touched;
     std::vector<int> acc;  // D per slot, three ints
-    std::vector<uint4> cells;
-    std::vector<int> kc;  // K = -D . c per cell
-    std::vector<int> off(L.n_cchunks + 1, 0);
-    auto bit = [](int d0, int d1, int d2) { return (d0 + 1) * 9 + (d1 + 1) * 3 + (d2 + 1); };
-    auto fbits = [](int v) {
-        const float f = (float)v;
-        uint32_t b;
-        std::memcpy(&b, &f, 4);
-        return b;
+    struct Cell {
+        int c[3], D[3], K;
     };
+    std::vector<Cell> cells;       // the current chunk's
+    std::vector<float> grp;        // lane-major groups of all chunks
+    std::vector<int> off(L.n_cchunks + 1, 0);  // in groups
+    int64_t n_cells = 0;
+    auto bit = [](int d0, int d1, int d2) { return (d0 + 1) * 9 + (d1 + 1) * 3 + (d2 + 1); };
     auto add = [&](int z, int y, int x, int d0, int d1, int d2) {
         const int64_t gi = ((int64_t)z * h.n1 + y) * h.n2 + x;
         int sl;
@@ -586,6 +596,7 @@ void ensure_cells(Heart& h, int nhood)
         const int4 cc = L.h_cchunks[k];
         touched.clear();
         acc.clear();
+        cells.clear();
         for (int v = cc.y; v < cc.z; ++v) {
             const uint2 g = L.h_geo[v];
             const int z = (int)(g.x & 1023u), y = (int)((g.x >> 10) & 1023u), x = (int)(g.x >> 20);
@@ -609,30 +620,39 @@ void ensure_cells(Heart& h, int nhood)
             if (std::abs(D0) > 127 || std::abs(D1) > 127 || std::abs(D2) > 127)
                 fail(EKG_EINVAL, "dipole vector out of range");
             const int x = (int)(gi % h.n2), y = (int)((gi / h.n2) % h.n1), z = (int)(gi / ((int64_t)h.n2 * h.n1));
-            // grid coordinates + 1 (the reference's voxel centre) as floats, and D
-            cells.push_back(make_uint4(fbits(z + 1), fbits(y + 1), fbits(x + 1),
-                                       (uint32_t)(uint8_t)(int8_t)D0 | ((uint32_t)(uint8_t)(int8_t)D1 << 8) |
-                                           ((uint32_t)(uint8_t)(int8_t)D2 << 16)));
-            kc.push_back(-(D0 * (z + 1) + D1 * (y + 1) + D2 * (x + 1)));
+            cells.push_back(Cell{{z + 1, y + 1, x + 1}, {D0, D1, D2}, -(D0 * (z + 1) + D1 * (y + 1) + D2 * (x + 1))});
         }
-        if (cells.size() > 0x7fffffffu) fail(EKG_EINVAL, "too many class cells");
-        off[k + 1] = (int)cells.size();
+        n_cells += (int64_t)cells.size();
+        const int ng = (int)((cells.size() + 15) / 16);
+        for (int q = 0; q < ng; ++q) {
+            const size_t base = grp.size();
+            grp.resize(base + 192);
+            float* av = grp.data() + base;
+            float* bv = av + 64;
+            float* sq = av + 128;
+            for (int lane = 0; lane < 64; ++lane) {
+                const int kq = lane >> 4, b = (lane >> 2) & 3, rc = lane & 3;
+                const size_t q0 = (size_t)16 * q, ql = q0 + 4 * b + kq, qa = q0 + 4 * b + rc;
+                const Cell& A = cells[qa < cells.size() ? qa : q0];
+                const bool ok = ql < cells.size();
+                const Cell& C = cells[ok ? ql : q0];
+                av[lane] = kq < 3 ? (float)A.c[kq] : 1.0f;
+                bv[lane] = ok ? (float)(rc < 3 ? C.D[rc] : C.K) : 0.0f;
+                sq[lane] = (float)(C.c[0] * C.c[0] + C.c[1] * C.c[1] + C.c[2] * C.c[2]);
+            }
+        }
+        if (grp.size() / 192 > 0x7fffffffu) fail(EKG_EINVAL, "too many class cells");
+        off[k + 1] = (int)(grp.size() / 192);
     }
-    L.n_ccells[nhood] = (int64_t)cells.size();
-    if (cells.empty()) {
-        cells.push_back(make_uint4(0, 0, 0, 0));
-        kc.push_back(0);
-    }
+    L.n_ccells[nhood] = n_cells;
+    if (grp.empty()) grp.assign(192, 0.0f);
     h.ctx->bind();
-    L.ccells[nhood].reserve(cells.size() * sizeof(uint4));
-    L.ccell_off[nhood].reserve(off.size() * sizeof(int));
-    L.ccellk[nhood].reserve(kc.size() * sizeof(int));
-    hip_check(hipMemcpy(L.ccellk[nhood].p, kc.data(), kc.size() * sizeof(int), hipMemcpyHostToDevice),
-              "upload class cell moments");
-    hip_check(hipMemcpy(L.ccells[nhood].p, cells.data(), cells.size() * sizeof(uint4), hipMemcpyHostToDevice),
-              "upload class cells");
-    hip_check(hipMemcpy(L.ccell_off[nhood].p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice),
-              "upload class cell offsets");
+    L.cgrp[nhood].reserve(grp.size() * sizeof(float));
+    L.cgrp_off[nhood].reserve(off.size() * sizeof(int));
+    hip_check(hipMemcpy(L.cgrp[nhood].p, grp.data(), grp.size() * sizeof(float), hipMemcpyHostToDevice),
+              "upload class cell groups");
+    hip_check(hipMemcpy(L.cgrp_off[nhood].p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice),
+              "upload class cell group offsets");
     L.cells_built[nhood] = true;
 }
 
@@ -730,9 +750,8 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
             hi = (int)hs.size();
             hs.push_back(&h);
             ekgk::HeartDev d = h.layout.dev;
-            d.ccells = g.mfma ? h.layout.ccells[g.nhood].as<uint4>() : nullptr;
-            d.ccell_off = g.mfma ? h.layout.ccell_off[g.nhood].as<int>() : nullptr;
-            d.ccellk = g.mfma ? h.layout.ccellk[g.nhood].as<int>() : nullptr;
+            d.cgrp = g.mfma ? h.layout.cgrp[g.nhood].as<float>() : nullptr;
+            d.cgrp_off = g.mfma ? h.layout.cgrp_off[g.nhood].as<int>() : nullptr;
             hdev.push_back(d);
         }
         int64_t t_off = -1;

@@ -100,11 +100,12 @@ struct EcgLayout {
     std::vector<double> h_class_a;
     std::vector<uint2> h_geo;      // per voxel of the layout: packed grid coordinates, 3x3x3 myocardium mask
     // Per dipole neighbourhood (EKG_NHOOD_*), built on first use: the cells of
-    // each class chunk with a non-zero integer dipole vector D (see
-    // ensure_cells), cells [ccell_off[k], ccell_off[k+1]) for class chunk k.
+    // each class chunk with a non-zero integer dipole vector D, as lane-major
+    // groups of 16 (see ensure_cells), groups [cgrp_off[k], cgrp_off[k+1]) for
+    // class chunk k.
     bool cells_built[4] = {false, false, false, false};
     int64_t n_ccells[4] = {0, 0, 0, 0};
-    DevBuf ccells[4], ccellk[4], ccell_off[4];
+    DevBuf cgrp[4], cgrp_off[4];
     std::vector<int> h_layer_chunk;
     std::vector<double> h_layer_t0;
     DevBuf vox_chunk, vox_a64, vox_a, vox_geo, chunks, bricks, brick_off, brick_vox, halo_off, halo_cells,

@@ -70,12 +70,12 @@ struct HeartDev {
     const int4* cchunks;         // [n_cchunks] {layer, v_begin, v_end, t_ref as float bits}
     const int4* classes;         // [n_classes] {layer, first class chunk, end class chunk, 0}
     const double* class_a;       // [n_classes] activation time of the class
-    // cells of the class chunks for the group's neighbourhood: {z + 1, y + 1,
-    // x + 1 as float bits, D as three int8 (bits 0-7, 8-15, 16-23)}; chunk k
-    // owns [ccell_off[k], ccell_off[k+1])
-    const uint4* ccells;
-    const int* ccellk;           // per cell: K = -D . (z + 1, y + 1, x + 1)
-    const int* ccell_off;
+    // cells of the class chunks for the group's neighbourhood, in groups of
+    // 16: group q holds the matrix-product operands of its 64 lanes,
+    // cgrp[192 q + {0, 64, 128} + lane] = {coordinate, coefficient, |c|^2};
+    // class chunk k owns groups [cgrp_off[k], cgrp_off[k+1])
+    const float* cgrp;
+    const int* cgrp_off;
     int n0, n1, n2;
     int V, n_chunks, n_bricks, max_layer;
     int n_cchunks, n_classes;
