@@ -69,10 +69,13 @@ def lead_field_cells(layer):
     return int(cnt[myo].sum())
 
 
-# Algorithmic work, counted per unit.  Class-chunk kernel (the dominant one
-# when the heart has activation classes), per (class cell, lead): 16 FP64
-# matrix-core flops (|e|^2 as a 4-term product, 4 FMAs; then an FMA for each
-# of the coefficients D0, D1, D2, K) and 3 FP32 flops (rsq 1, |e|^-3 2).
+# Algorithmic work, counted per unit.  Class-window kernel (the dominant one
+# when the heart has activation classes):
+#   per (class-window entry, lead): 8 FP64 matrix-core flops (|e|^2 as a
+#     4-term product, 4 FMAs) and 3 FP32 flops (rsq 1, |e|^-3 2), the lead
+#     weight shared by the entry's classes;
+#   per (class cell, lead): 8 FP64 matrix-core flops (an FMA for each of the
+#     coefficients D0, D1, D2, K).
 # Chunk kernel (dominant without classes):
 #   per lead-weight evaluation (voxel, lead, cell): 8 FP64 flops (|e|^2 2
 #     FMAs, d . e and its FMA into G) and 3 FP32 flops (rsq 1, r^3 2);
@@ -81,7 +84,7 @@ def lead_field_cells(layer):
 # Both count at full weight against 78.6 TFLOP/s: the build disables packed
 # FP32, so an FP32 vector operation issues at the FP64 vector rate, which on
 # MI355X equals the FP64 matrix rate.
-FP32_PER_CLASS_CELL, FP64_PER_CLASS_CELL = 3, 16
+FP32_PER_CLASS_ENTRY, FP64_PER_CLASS_ENTRY, FP64_PER_CLASS_CELL = 3, 8, 8
 FP32_PER_CELL, FP64_PER_CELL = 3, 8
 FP32_PER_WINDOW_STEP = 14
 # Naive per-voxel-step model: every voxel at every step, 38 VALU lane-cycles
@@ -431,13 +434,15 @@ def main():
     act_myo = act_grid[h.layer > 0]
     info = heart.layout_info("3D4") if args.kernel != "valu" else None
     n_classes = info["classes"] if info and heart.get_tune()["class_min"] >= 0 else 0
-    win = cells = ccells = 0
+    win = cells = ccells = centries = 0
     if n_classes > 0:
-        # the class-chunk kernel: every class cell once per (request, lead)
-        dom = "ecg_class_chunk_kernel"
+        # the class-window kernel: every entry's weight and every class cell's
+        # contraction once per (request, lead)
+        dom = "ecg_class_window_kernel"
         ccells = B * L * info["class_cells"]
-        f64 = FP64_PER_CLASS_CELL * ccells
-        f32 = FP32_PER_CLASS_CELL * ccells
+        centries = B * L * info["class_entries"]
+        f64 = FP64_PER_CLASS_CELL * ccells + FP64_PER_CLASS_ENTRY * centries
+        f32 = FP32_PER_CLASS_ENTRY * centries
     else:
         dom = "ecg_chunk_kernel" if args.kernel != "valu" else "ecg_fused_kernel"
         win = sum(window_voxel_steps(act_myo, float(k[0][1]), 0.0, 1.0, T) for k in ks)
@@ -675,9 +680,11 @@ def main():
             "fp64_flops_per_launch": f64,
             "fp32_flops_per_launch": f32,
             "class_cell_leads_per_launch": ccells,
+            "class_entry_leads_per_launch": centries,
             "lead_weight_evaluations_per_launch": cells,
             "window_voxel_steps_per_launch": win,
-            "per_unit": {"class_cell_lead": {"fp32": FP32_PER_CLASS_CELL, "fp64": FP64_PER_CLASS_CELL},
+            "per_unit": {"class_entry_lead": {"fp32": FP32_PER_CLASS_ENTRY, "fp64": FP64_PER_CLASS_ENTRY},
+                         "class_cell_lead": {"fp32": 0, "fp64": FP64_PER_CLASS_CELL},
                          "lead_weight_evaluation": {"fp32": FP32_PER_CELL, "fp64": FP64_PER_CELL},
                          "window_voxel_step": {"fp32": FP32_PER_WINDOW_STEP, "fp64": 2 * L},
                          "fp32_weight": 1.0},

@@ -79,7 +79,8 @@ class LayoutInfo(C.Structure):
         ("chunks", C.c_int),
         ("class_chunks", C.c_int),
         ("classes", C.c_int),
-        ("pad", C.c_int),
+        ("class_pieces", C.c_int),
+        ("class_entries", C.c_int64),
     ]
 
 

@@ -438,6 +438,8 @@ int ekg_heart_layout_info(ekg_heart* h, int nhood, ekg_layout_info* out)
         o.chunks = L.n_chunks;
         o.class_chunks = L.n_cchunks;
         o.classes = L.n_classes;
+        o.class_pieces = L.n_cpieces[nhood];
+        o.class_entries = L.n_centries[nhood];
         *out = o;
     });
 }

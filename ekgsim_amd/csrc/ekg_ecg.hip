@@ -1092,21 +1092,21 @@ __global__ void ecg_reduce_kernel(const HeartDev* __restrict__ hearts, const Req
 
 constexpr int CC_THREADS = 256;
 
-// Class chunks: one block per (class chunk, request), or per (class chunk,
-// RQ = 2 or 4 requests) when those requests use the same heart (the cells are
-// then loaded once for all of them).  Every voxel of a class
-// has the same AP, so only the sum of the chunk's G rows is needed, and by
-// regrouping the dipoles per cell it is
+// Class windows: one block per (piece of a class window, request).  Every
+// voxel of a class has the same AP, so only the sum of the class's G rows is
+// needed, and by regrouping the dipoles per cell it is
 //   sum_u G(u) = sum_w s(w) (e(w) . D(w))
-// over the chunk's cells w with a non-zero integer vector D(w) (host-built
-// per neighbourhood; cells inside the chunk cancel).  With e = p - c (c the
-// cell's coordinate + 1) this is linear in the electrode p:
+// over the cells w with a non-zero integer vector D(w) for the class
+// (host-built per neighbourhood; cells inside the class cancel).  With
+// e = p - c (c the cell's coordinate + 1) this is linear in the electrode p:
 //   sum_w s(w) (e . D) = p . T + T3,  T_c = sum_w s(w) D_c(w),  T3 = sum_w s(w) K(w),
-// K = -D . c (host-computed).  Both the squared distance and the products run
-// on v_mfma_f64_4x4x4f64 (4 blocks of 4 x 4 x 4), leaving the vector ALUs
-// only the conversions, the reciprocal square root and the cube:
+// K = -D . c.  A cell borders several classes, so the host lists each cell
+// once per window of CLASS_WINDOW classes (an entry) with its D and K for
+// each class of the window, and the weight s(w) is evaluated once per entry
+// and lead.  The squared distance and the products run on
+// v_mfma_f64_4x4x4f64 (4 blocks of 4 x 4 x 4):
 //  1. |e|^2 = |p|^2 - 2 p . c + |c|^2 as an FP64 matrix product: rows are the
-//     16 cells of a group (4 per block), columns the 4 leads of a group, the
+//     16 entries of a group (4 per block), columns the 4 leads of a group, the
 //     inner dimension {c_z, c_y, c_x, 1} x {-2 p_z, -2 p_y, -2 p_x, |p|^2},
 //     with |c|^2 (exact) as the accumulator input;
 //  2. s = rsq(|e|^2 rounded to FP32)^3 in FP32 -- the chunk kernel's weight:
@@ -1114,172 +1114,142 @@ constexpr int CC_THREADS = 256;
 //     so the two kernels give a cell the same weight except where the two
 //     FP64 values (both within ~1e-14 relative of |e|^2) straddle an FP32
 //     rounding boundary;
-//  3. T = s x {D_z, D_y, D_x, K}: rows the leads, inner dimension the cells.
+//  3. for each class i of the group's mask, T_i += s x {D_z, D_y, D_x, K}_i:
+//     rows the leads, inner dimension the entries, one accumulator per class
+//     of the window.
 // gfx950 lane layout (checked by a one-hot probe): A(block, row, k) in lane
 // 16k + 4 block + row, B(block, k, col) in lane 16k + 4 block + col,
 // D(block, row, col) in lane 16 row + 4 block + col.  Product 1's result for
-// (cell r of block b, lead col) lands in lane 16r + 4b + col, which is where
-// product 3 takes its operand (block b, lead row, cell k): so a lane owns one
-// cell of each group of 16 and one lead of each group of 4 throughout, and
-// feeds product 1 coordinate (lane >> 4) of cell (b, lane & 3).  The host
-// stores those three per-lane operands of every group lane-major (cgrp), so a
-// lane loads three coalesced words per group and converts them, with no index
-// arithmetic or unpacking.  Each wave takes groups of 16 cells (strided over
-// the waves), the blocks' partial sums
-// are added in block order, the waves' in wave order, and
-// sums[(c_off + n_chunks + k)*3*LP] receives {sum G, P' sum G, Q' sum G} with
-// the class's P' = k2(1-k3) e^{(k4+k5)(a - t0)} and Q' = k2 k3 e^{k5 (a - t0)}.
-template <int LP, int RQ>
-__global__ __launch_bounds__(CC_THREADS) void ecg_class_chunk_kernel(
+// (entry r of block b, lead col) lands in lane 16r + 4b + col, which is where
+// product 3 takes its operand (block b, lead row, entry k): so a lane owns one
+// entry of each group of 16 and one lead of each group of 4 throughout, and
+// feeds product 1 coordinate (lane >> 4) of entry (b, lane & 3).  The host
+// stores those per-lane operands of every group lane-major, so a lane loads
+// coalesced words and converts them, with no index arithmetic or unpacking.
+// Each wave takes groups of the piece strided over the waves, the blocks'
+// partial sums are added in block order, the waves' in wave order, and
+// sums[(c_off + n_chunks + p CLASS_WINDOW + i)*3*LP] receives the piece's
+// {sum G, P' sum G, Q' sum G} of class i of its window, with the class's
+// P' = k2(1-k3) e^{(k4+k5)(a - t0)} and Q' = k2 k3 e^{k5 (a - t0)} (zero for
+// a class past the heart's last).
+template <int LP>
+__global__ __launch_bounds__(CC_THREADS) void ecg_class_window_kernel(
     const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
-    const int64_t* __restrict__ cc_start, int cc_uniform, const int2* __restrict__ cc_units,
-    const int64_t* __restrict__ cc_ustart, int n_units, const double* __restrict__ lead_pos,
+    const int64_t* __restrict__ cc_start, int cc_uniform, const double* __restrict__ lead_pos,
     const double* __restrict__ layer_k, double* __restrict__ sums)
 {
     constexpr int NG = LP / 4;                 // groups of 4 leads
     constexpr int NW = CC_THREADS / 64;
-    __shared__ double s_t[RQ][NW][LP][4];
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    // chunk-major: over runs of RQ requests of one heart (RQ = 2 or 4; a
-    // short last run repeats its last request), else over requests when every
-    // request has the same class chunks
-    int rq[RQ], k;
-    if (RQ > 1 && cc_units != nullptr) {
-        // pairs of one heart in a group of several hearts
-        const int u = find_req(cc_ustart, n_units, blockIdx.x);
-        k = (int)(blockIdx.x - cc_ustart[u]);
-        const int2 pr = cc_units[u];
-#pragma unroll
-        for (int j = 0; j < RQ; ++j) rq[j] = j == 0 ? pr.x : pr.y;
-    } else if (RQ > 1) {
-        const int np = (B + RQ - 1) / RQ;
-        k = (int)(blockIdx.x / (unsigned)np);
-        rq[0] = RQ * (int)(blockIdx.x % (unsigned)np);
-#pragma unroll
-        for (int j = 1; j < RQ; ++j) rq[j] = min(rq[0] + j, B - 1);
-    } else if (cc_uniform > 0) {
-        rq[0] = (int)(blockIdx.x % (unsigned)B);
+    constexpr int CW = CLASS_WINDOW;
+    __shared__ double s_t[CW][NW][LP][4];
+    const int t = threadIdx.x, lane = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    // piece-major when every request has the same pieces (one heart): the
+    // requests of a piece run back to back and read its groups from L2
+    int r, k;
+    if (cc_uniform > 0) {
+        r = (int)(blockIdx.x % (unsigned)B);
         k = (int)(blockIdx.x / (unsigned)B);
     } else {
-        rq[0] = find_req(cc_start, B, blockIdx.x);
-        k = (int)(blockIdx.x - cc_start[rq[0]]);
+        r = find_req(cc_start, B, blockIdx.x);
+        k = (int)(blockIdx.x - cc_start[r]);
     }
-    const HeartDev& H = hearts[reqs[rq[0]].heart];
-    const int4 ch = H.cchunks[k];
+    const ReqDev& R = reqs[r];
+    const HeartDev& H = hearts[R.heart];
+    const int4 pc4 = H.cpieces[k];
+    const int pw = __builtin_amdgcn_readfirstlane(pc4.x), g0 = __builtin_amdgcn_readfirstlane(pc4.y),
+              g1 = __builtin_amdgcn_readfirstlane(pc4.z);
     const int rc = lane & 3;                                  // lead of a group (and product-3 column)
     const int kq = lane >> 4;                                 // product-1 inner index: coordinate kq
     // product-1 right operand: {-2 p_z, -2 p_y, -2 p_x, |p|^2} of the lane's
     // leads 4g + rc (leads past the request's count repeat lead 0; their sums
     // are zeroed)
-    double pe[RQ][NG];
+    double pe[NG];
 #pragma unroll
-    for (int j = 0; j < RQ; ++j) {
-        const ReqDev& R = reqs[rq[j]];
+    for (int g = 0; g < NG; ++g) {
+        const int i = 4 * g + rc < R.L ? 4 * g + rc : 0;
+        const double* p = lead_pos + R.pos_off + 3 * i;
+        const double pp = __builtin_fma(p[2], p[2], __builtin_fma(p[1], p[1], p[0] * p[0]));
+        pe[g] = kq < 3 ? -2.0 * p[kq < 3 ? kq : 0] : pp;
+    }
+    double acc[CW][NG];
+#pragma unroll
+    for (int i = 0; i < CW; ++i)
+#pragma unroll
+        for (int g = 0; g < NG; ++g) acc[i][g] = 0.0;
+    // each group's rows and class mask (wave-uniform), read one group ahead
+    int2 crn = g0 + w < g1 ? H.crow[g0 + w] : int2{0, 0};
+    for (int q = g0 + w; q < g1; q += NW) {
+        const int2 cr = crn;
+        if (q + NW < g1) crn = H.crow[q + NW];
+        const int row = __builtin_amdgcn_readfirstlane(cr.x), mask = __builtin_amdgcn_readfirstlane(cr.y);
+        const float* o = H.cgrp + (int64_t)64 * row + lane;
+        // product-1 left operand (coordinate kq of entry 4 block + rc, or 1)
+        // and |c|^2 of entry 4 block + kq
+        const double av = (double)o[0], cc = (double)o[64];
+        float bf[CW];
+#pragma unroll
+        for (int i = 0, n = 2; i < CW; ++i)
+            if ((mask >> i) & 1) bf[i] = o[64 * n++];
+        double sw[NG];
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
-            const int i = 4 * g + rc < R.L ? 4 * g + rc : 0;
-            const double* p = lead_pos + R.pos_off + 3 * i;
-            const double pp = __builtin_fma(p[2], p[2], __builtin_fma(p[1], p[1], p[0] * p[0]));
-            pe[j][g] = kq < 3 ? -2.0 * p[kq < 3 ? kq : 0] : pp;
+            const double e2 = __builtin_amdgcn_mfma_f64_4x4x4f64(av, pe[g], cc, 0, 0, 0);
+            const float rr = __builtin_amdgcn_rsqf((float)e2);
+            sw[g] = (double)(rr * rr * rr);
         }
+#pragma unroll
+        for (int i = 0; i < CW; ++i)
+            if ((mask >> i) & 1) {
+                // product-3 right operand: coefficient rc of entry 4 block + kq for class i
+                const double bv = (double)bf[i];
+#pragma unroll
+                for (int g = 0; g < NG; ++g) acc[i][g] = __builtin_amdgcn_mfma_f64_4x4x4f64(sw[g], bv, acc[i][g], 0, 0, 0);
+            }
     }
-    double acc[RQ][NG];
-#pragma unroll
-    for (int j = 0; j < RQ; ++j)
-#pragma unroll
-        for (int g = 0; g < NG; ++g) acc[j][g] = 0.0;
-    // this wave's groups of 16 cells: gb + w, gb + w + NW, ...  U groups per
-    // iteration (4 for one request, whose loop is short of vector work to
-    // cover the loads), the next iteration's operands loaded before this
-    // one's products; a group past the chunk's end loads the wave's first
-    // group (in range) and is skipped
-    const int gb = H.cgrp_off[k], ge = H.cgrp_off[k + 1];
-    constexpr int U = RQ > 1 ? 1 : 4;
-    float avu[U], bvu[U], ccu[U];
-    auto load = [&](int q0) {
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int q = q0 + u * NW < ge ? q0 + u * NW : gb + w;
-            const float* o = H.cgrp + (int64_t)192 * q + lane;
-            avu[u] = o[0];
-            bvu[u] = o[64];
-            ccu[u] = o[128];
-        }
-    };
-    // (the empty asm statements keep the compiler from merging the two loads
-    // of each operand into one at the loop head, which would wait for them
-    // right before their use)
-    if (gb + w < ge) load(gb + w);
-    asm volatile("" ::: "memory");
-    for (int q0 = gb + w; q0 < ge; q0 += NW * U) {
-        double av[U], bv[U], cc[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            // product-1 left operand (coordinate kq of cell 4 block + rc, or
-            // 1), product-3 right operand (coefficient rc of cell
-            // 4 block + kq: D_rc or K) and |c|^2 of that cell
-            av[u] = (double)avu[u];
-            bv[u] = (double)bvu[u];
-            cc[u] = (double)ccu[u];
-        }
-        load(q0 + NW * U);
-        asm volatile("" ::: "memory");
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            if (u > 0 && q0 + u * NW >= ge) break;  // the whole wave
-#pragma unroll
-            for (int j = 0; j < RQ; ++j)
-#pragma unroll
-                for (int g = 0; g < NG; ++g) {
-                    const double e2 = __builtin_amdgcn_mfma_f64_4x4x4f64(av[u], pe[j][g], cc[u], 0, 0, 0);
-                    const float rr = __builtin_amdgcn_rsqf((float)e2);
-                    acc[j][g] =
-                        __builtin_amdgcn_mfma_f64_4x4x4f64((double)(rr * rr * rr), bv[u], acc[j][g], 0, 0, 0);
-                }
-        }
-    }
-    // acc[j][g] in lane 16 row + 4 block + col: lead 4g + row, coefficient
-    // col, this wave's cells of one block; the four blocks in order
+    // acc[i][g] in lane 16 row + 4 block + col: lead 4g + row, coefficient
+    // col, this wave's entries of one block; the four blocks in order
     const int base = lane & ~12;
 #pragma unroll
-    for (int j = 0; j < RQ; ++j)
+    for (int i = 0; i < CW; ++i)
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
-            const double v0 = __shfl(acc[j][g], base), v1 = __shfl(acc[j][g], base | 4),
-                         v2 = __shfl(acc[j][g], base | 8), v3 = __shfl(acc[j][g], base | 12);
-            if ((lane & 12) == 0) s_t[j][w][4 * g + (lane >> 4)][lane & 3] = ((v0 + v1) + v2) + v3;
+            const double v0 = __shfl(acc[i][g], base), v1 = __shfl(acc[i][g], base | 4),
+                         v2 = __shfl(acc[i][g], base | 8), v3 = __shfl(acc[i][g], base | 12);
+            if ((lane & 12) == 0) s_t[i][w][4 * g + (lane >> 4)][lane & 3] = ((v0 + v1) + v2) + v3;
         }
     __syncthreads();
-    if (t < RQ * LP) {
-        const int j = t / LP, i = t - j * LP;
-        if (j > 0 && rq[j] == rq[j - 1]) return;  // a copy of the last request
-        const ReqDev& R = reqs[rq[j]];
+    for (int u = t; u < CW * LP; u += CC_THREADS) {
+        const int i = u / LP, ld = u - i * LP;
+        const int kc = CW * pw + i;
         double T[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             T[c] = 0.0;
 #pragma unroll
-            for (int ww = 0; ww < NW; ++ww) T[c] += s_t[j][ww][i][c];
+            for (int ww = 0; ww < NW; ++ww) T[c] += s_t[i][ww][ld][c];
         }
-        double sg = 0.0;
-        if (i < R.L) {
-            const double* p = lead_pos + R.pos_off + 3 * i;
-            sg = __builtin_fma(p[2], T[2], __builtin_fma(p[1], T[1], __builtin_fma(p[0], T[0], T[3])));
+        double sg = 0.0, pp = 0.0, qq = 0.0;
+        if (kc < H.n_classes) {
+            if (ld < R.L) {
+                const double* p = lead_pos + R.pos_off + 3 * ld;
+                sg = __builtin_fma(p[2], T[2], __builtin_fma(p[1], T[1], __builtin_fma(p[0], T[0], T[3])));
+            }
+            const int l = H.classes[kc].x;
+            const double* kk = layer_k + R.k_off + 9 * (l - 1);
+            const double da = H.class_a[kc] - H.layer_t0[l];
+            pp = kk[2] * (1.0 - kk[3]) * exp((kk[4] + kk[5]) * da);
+            qq = kk[2] * kk[3] * exp(kk[5] * da);
         }
-        const int l = ch.x;
-        const double* kk = layer_k + R.k_off + 9 * (l - 1);
-        const double da = H.vox_a64[ch.y] - H.layer_t0[l];
-        const double pp = kk[2] * (1.0 - kk[3]) * exp((kk[4] + kk[5]) * da);
-        const double qq = kk[2] * kk[3] * exp(kk[5] * da);
-        double* o = sums + (R.c_off + H.n_chunks + k) * 3 * LP;
-        o[i] = sg;
-        o[LP + i] = pp * sg;
-        o[2 * LP + i] = qq * sg;
+        double* os = sums + (R.c_off + H.n_chunks + (int64_t)k * CW + i) * 3 * LP;
+        os[ld] = sg;
+        os[LP + ld] = pp * sg;
+        os[2 * LP + ld] = qq * sg;
     }
 }
 
-// csums[(cls_off + k)*3*LP + v]: the sums of class k's chunks (sum G, sum G P',
-// sum G Q' per lead), added in chunk order.
+// csums[(cls_off + k)*3*LP + v]: the sums of class k (sum G, sum G P',
+// sum G Q' per lead), its window's pieces added in order.
 template <int LP>
 __global__ void class_sums_kernel(const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
                                   const int64_t* __restrict__ cs_start, int64_t n_cs,
@@ -1293,9 +1263,11 @@ __global__ void class_sums_kernel(const HeartDev* __restrict__ hearts, const Req
         const int r = find_req(cs_start, B, item);
         const ReqDev R = reqs[r];
         const HeartDev& H = hearts[R.heart];
-        const int4 cd = H.classes[item - cs_start[r]];
+        const int k = (int)(item - cs_start[r]);
+        const int wi = k / CLASS_WINDOW, i = k - wi * CLASS_WINDOW;
         double s = 0.0;
-        for (int q = cd.y; q < cd.z; ++q) s += sums[(R.c_off + H.n_chunks + q) * 3 * LP + v];
+        for (int p = H.cwin[wi]; p < H.cwin[wi + 1]; ++p)
+            s += sums[(R.c_off + H.n_chunks + (int64_t)p * CLASS_WINDOW + i) * 3 * LP + v];
         csums[item * 3 * LP + v] = s;
     }
 }
@@ -1540,32 +1512,21 @@ hipError_t launch_ecg_chunk(const GroupArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
-template <int LP>
-static void class_chunk_launch(const GroupArgs& a, hipStream_t s)
-{
-    if (a.cc_rpb == 4 && a.cc_units == nullptr)
-        hipLaunchKernelGGL((ecg_class_chunk_kernel<LP, 4>), dim3((unsigned)a.n_cc), dim3(CC_THREADS), 0, s, a.hearts,
-                           a.reqs, a.B, a.cc_start, a.cc_uniform, a.cc_units, a.cc_ustart, a.n_units, a.lead_pos,
-                           a.layer_k, a.sums);
-    else if (a.cc_rpb == 2)
-        hipLaunchKernelGGL((ecg_class_chunk_kernel<LP, 2>), dim3((unsigned)a.n_cc), dim3(CC_THREADS), 0, s, a.hearts,
-                           a.reqs, a.B, a.cc_start, a.cc_uniform, a.cc_units, a.cc_ustart, a.n_units, a.lead_pos,
-                           a.layer_k, a.sums);
-    else
-        hipLaunchKernelGGL((ecg_class_chunk_kernel<LP, 1>), dim3((unsigned)a.n_cc), dim3(CC_THREADS), 0, s, a.hearts,
-                           a.reqs, a.B, a.cc_start, a.cc_uniform, a.cc_units, a.cc_ustart, a.n_units, a.lead_pos,
-                           a.layer_k, a.sums);
-}
-
-hipError_t launch_class_chunks(const GroupArgs& a, hipStream_t s)
+hipError_t launch_class_windows(const GroupArgs& a, hipStream_t s)
 {
     if (a.n_cc == 0) return hipSuccess;
     if (a.n_cc > 0x7fffffffLL) return hipErrorInvalidValue;
     switch (a.LP) {
-    case 4: class_chunk_launch<4>(a, s); break;
-    case 8: class_chunk_launch<8>(a, s); break;
-    case 12: class_chunk_launch<12>(a, s); break;
-    case 16: class_chunk_launch<16>(a, s); break;
+#define EKG_CWIN(LPV)                                                                                        \
+    case LPV:                                                                                                \
+        hipLaunchKernelGGL(ecg_class_window_kernel<LPV>, dim3((unsigned)a.n_cc), dim3(CC_THREADS), 0, s, a.hearts, \
+                           a.reqs, a.B, a.cc_start, a.cc_uniform, a.lead_pos, a.layer_k, a.sums);            \
+        break;
+        EKG_CWIN(4)
+        EKG_CWIN(8)
+        EKG_CWIN(12)
+        EKG_CWIN(16)
+#undef EKG_CWIN
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1630,9 +1591,7 @@ static void prime_lp(hipFuncAttributes& at)
     prime_nh<LP, 1>(at);
     prime_nh<LP, 2>(at);
     prime_nh<LP, 3>(at);
-    (void)hipFuncGetAttributes(&at, (const void*)ecg_class_chunk_kernel<LP, 1>);
-    (void)hipFuncGetAttributes(&at, (const void*)ecg_class_chunk_kernel<LP, 2>);
-    (void)hipFuncGetAttributes(&at, (const void*)ecg_class_chunk_kernel<LP, 4>);
+    (void)hipFuncGetAttributes(&at, (const void*)ecg_class_window_kernel<LP>);
     (void)hipFuncGetAttributes(&at, (const void*)class_sums_kernel<LP>);
     (void)hipFuncGetAttributes(&at, (const void*)layer_prefix_kernel<LP>);
     (void)hipFuncGetAttributes(&at, (const void*)ecg_reduce_kernel<LP>);

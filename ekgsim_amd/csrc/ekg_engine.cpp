@@ -235,12 +235,12 @@ constexpr double CHUNK_SPAN_MS = 16.0;
 // Smallest activation class by default: a run of at least this many voxels of
 // one layer with one activation time takes the class path.
 constexpr int DEFAULT_CLASS_MIN = 64;
-// Largest class chunk (voxels): the class-chunk kernel sums its lead weights
-// cell by cell in registers (see ensure_cells).
+// Largest class chunk (voxels): a class is kept as a run of class chunks
+// (layout bookkeeping; the class-window kernel takes whole classes).
 constexpr int64_t CLASS_CHUNK = 4096;
-// Requests per class-chunk block when a group uses one heart (each cell's
-// record is loaded and unpacked once for all of them).
-constexpr int CC_RPB_ONE_HEART = 4;
+// Classes per class window and groups of 16 entries per piece (see ensure_cells).
+constexpr int CLASS_WINDOW = ekgk::CLASS_WINDOW;
+constexpr int CLASS_PIECE = ekgk::CLASS_PIECE;
 
 namespace {
 // Chunk size: DEFAULT_CHUNK voxels unless the heart is tuned.  Chunks are
@@ -536,22 +536,39 @@ void ensure_layout(Heart& h, bool mfma)
 // m(u + d) . d for each such d, so
 //   sum_{u in S} G(u) = sum_w m(w) . D(w),
 //   D(w) = [w in S] s_w + sum_{d in N, w - d in S, w myocardial} d,
-// an integer vector per cell; cells inside S cancel (D = 0) and are dropped.
-// Each remaining cell's weight is evaluated once per class chunk and request.
-// A cell w has grid coordinates + 1 (the reference's voxel centre) c(w), D(w)
-// and its moment K(w) = -D(w) . c(w), all integers.
+// an integer vector per cell; cells inside S cancel (D = 0).  With S an
+// activation class, every voxel of S has the same AP, so the class needs only
+// this sum; a cell w has grid coordinates + 1 (the reference's voxel centre)
+// c(w), D(w) and its moment K(w) = -D(w) . c(w), all integers.
 //
-// The class-chunk kernel takes a chunk's cells 16 at a time, one group per
-// wave and iteration, and each of the 64 lanes feeds the matrix products three
-// fixed operands of the group (lane = 16 kq + 4 b + rc, cell index within the
-// group cl = 4 b + kq, ca = 4 b + rc):
-//   av = coordinate kq of cell ca (kq = 3: 1),
-//   bv = coefficient rc of cell cl: D_0, D_1, D_2 (rc < 3) or K (rc = 3),
-//   cc = |c|^2 of cell cl.
+// A cell borders several classes (4 to 5 on average on a heart with delays of
+// 1 to 3 ms per voxel), and its lead weight s(w) = |p - c(w)|^-3 is the same
+// for all of them.  So the classes are taken CLASS_WINDOW at a time (in layer
+// and time order): an entry is a cell with a non-zero D for at least one
+// class of the window, and the class-window kernel evaluates each entry's
+// weight once per lead and request and adds it, times the entry's D and K for
+// class i of the window, to that class's sums.  Entries are sorted by their
+// class mask and cut into groups of 16; a group lists, for every class of its
+// mask (the union of its entries'), the 64 lanes' coefficients, and the
+// kernel runs one matrix product per class of the mask.
+//
+// Each of the 64 lanes feeds the matrix products fixed operands of its group
+// (lane = 16 kq + 4 b + rc, entry index within the group el = 4 b + kq,
+// ea = 4 b + rc):
+//   av = coordinate kq of entry ea (kq = 3: 1),
+//   cc = |c|^2 of entry el,
+//   bv = coefficient rc of entry el for class i: D_0, D_1, D_2 (rc < 3) or K.
 // They are stored lane-major, as FP32 (every value is an integer below 2^24,
-// exact), group g at cgrp[192 g]: av[64], bv[64], cc[64].  A chunk's cells
-// are padded to a multiple of 16: a missing cell repeats the group's first
-// cell with coefficients 0, so its weight is finite and adds nothing.
+// exact): group q occupies rows crow[q].x, ... of 64 floats in cgrp: av, cc,
+// then bv for each class of its mask in order; crow[q].y is the mask.  A
+// group's missing entries (past the window's last) repeat its first entry
+// with coefficients 0, so their weights are finite and add nothing.
+//
+// For parallelism a window's groups are cut into pieces of at most
+// CLASS_PIECE groups, one block per (piece, request); each writes its sums
+// for all CLASS_WINDOW classes, and class_sums_kernel adds a class's pieces
+// in order.  The cut depends on the heart only, so a request's ECG does not
+// depend on the batch it runs in.
 void ensure_cells(Heart& h, int nhood)
 {
     EcgLayout& L = h.layout;
@@ -559,30 +576,37 @@ void ensure_cells(Heart& h, int nhood)
     if (L.cells_built[nhood]) return;
     int8_t dif[26][3];
     const int nd = nhood_offsets(nhood, dif);
-    // cell -> slot of the current chunk (-1: none); reset after each chunk
-    // cell -> slot: a dense array over the grid, or a hash map for grids
-    // above 2^27 cells (the array would exceed 512 MB)
+    // cell -> entry of the current window (-1: none), reset after each
+    // window: a dense array over the grid, or a hash map for grids above
+    // 2^27 cells (the array would exceed 512 MB)
     const bool dense = h.nvox <= ((int64_t)1 << 27);
     std::vector<int> slot(dense ? (size_t)h.nvox : 0, -1);
     std::unordered_map<int64_t, int> slot_map;
-    std::vector<int64_t> touched;
-    std::vector<int> acc;  // D per slot, three ints
-    struct Cell {
-        int c[3], D[3], K;
+    std::vector<int64_t> touched;   // cells of the current class, first-touch order
+    std::vector<int> acc;           // D per touched cell, three ints
+    std::vector<int> tslot(dense ? (size_t)h.nvox : 0, -1);  // cell -> touched index of the current class
+    std::unordered_map<int64_t, int> tslot_map;
+    struct Entry {
+        int c[3];
+        int mask;
+        int first;  // first touch (order within a mask)
+        int coef[CLASS_WINDOW][4];
     };
-    std::vector<Cell> cells;       // the current chunk's
-    std::vector<float> grp;        // lane-major groups of all chunks
-    std::vector<int> off(L.n_cchunks + 1, 0);  // in groups
-    int64_t n_cells = 0;
+    std::vector<Entry> ent;
+    std::vector<float> grp;          // rows of 64 floats
+    std::vector<int2> crow;          // per group: first row, class mask
+    std::vector<int4> pieces;        // {window, first group, end group, 0}
+    std::vector<int> wpiece(1, 0);   // pieces of window w: [wpiece[w], wpiece[w+1])
+    int64_t n_cells = 0, n_entries = 0;
     auto bit = [](int d0, int d1, int d2) { return (d0 + 1) * 9 + (d1 + 1) * 3 + (d2 + 1); };
     auto add = [&](int z, int y, int x, int d0, int d1, int d2) {
         const int64_t gi = ((int64_t)z * h.n1 + y) * h.n2 + x;
         int sl;
         if (dense) {
-            sl = slot[gi];
-            if (sl < 0) sl = slot[gi] = (int)touched.size();
+            sl = tslot[gi];
+            if (sl < 0) sl = tslot[gi] = (int)touched.size();
         } else {
-            sl = slot_map.emplace(gi, (int)touched.size()).first->second;
+            sl = tslot_map.emplace(gi, (int)touched.size()).first->second;
         }
         if (sl == (int)touched.size()) {
             touched.push_back(gi);
@@ -592,67 +616,136 @@ void ensure_cells(Heart& h, int nhood)
         acc[3 * sl + 1] += d1;
         acc[3 * sl + 2] += d2;
     };
-    for (int k = 0; k < L.n_cchunks; ++k) {
-        const int4 cc = L.h_cchunks[k];
-        touched.clear();
-        acc.clear();
-        cells.clear();
-        for (int v = cc.y; v < cc.z; ++v) {
-            const uint2 g = L.h_geo[v];
-            const int z = (int)(g.x & 1023u), y = (int)((g.x >> 10) & 1023u), x = (int)(g.x >> 20);
-            int s0 = 0, s1 = 0, s2 = 0;
-            for (int q = 0; q < nd; ++q) {
-                if (!((g.y >> bit(dif[q][0], dif[q][1], dif[q][2])) & 1u)) continue;
-                add(z + dif[q][0], y + dif[q][1], x + dif[q][2], dif[q][0], dif[q][1], dif[q][2]);
-                s0 += dif[q][0];
-                s1 += dif[q][1];
-                s2 += dif[q][2];
+    const int n_win = (L.n_classes + CLASS_WINDOW - 1) / CLASS_WINDOW;
+    for (int w = 0; w < n_win; ++w) {
+        ent.clear();
+        const int k0 = w * CLASS_WINDOW, k1 = std::min(L.n_classes, k0 + CLASS_WINDOW);
+        for (int k = k0; k < k1; ++k) {
+            // the class's voxels: those of its class chunks, which are consecutive
+            const int4 cl = L.h_classes[k];
+            const int v_begin = L.h_cchunks[cl.y].y, v_end = L.h_cchunks[cl.z - 1].z;
+            touched.clear();
+            acc.clear();
+            for (int v = v_begin; v < v_end; ++v) {
+                const uint2 g = L.h_geo[v];
+                const int z = (int)(g.x & 1023u), y = (int)((g.x >> 10) & 1023u), x = (int)(g.x >> 20);
+                int s0 = 0, s1 = 0, s2 = 0;
+                for (int q = 0; q < nd; ++q) {
+                    if (!((g.y >> bit(dif[q][0], dif[q][1], dif[q][2])) & 1u)) continue;
+                    add(z + dif[q][0], y + dif[q][1], x + dif[q][2], dif[q][0], dif[q][1], dif[q][2]);
+                    s0 += dif[q][0];
+                    s1 += dif[q][1];
+                    s2 += dif[q][2];
+                }
+                add(z, y, x, s0, s1, s2);
             }
-            add(z, y, x, s0, s1, s2);
+            tslot_map.clear();
+            for (size_t t = 0; t < touched.size(); ++t) {
+                const int64_t gi = touched[t];
+                if (dense) tslot[gi] = -1;
+                const int D0 = acc[3 * t], D1 = acc[3 * t + 1], D2 = acc[3 * t + 2];
+                if (!(D0 || D1 || D2)) continue;
+                if (std::abs(D0) > 127 || std::abs(D1) > 127 || std::abs(D2) > 127)
+                    fail(EKG_EINVAL, "dipole vector out of range");
+                ++n_cells;
+                int e;
+                if (dense) {
+                    e = slot[gi];
+                    if (e < 0) e = slot[gi] = (int)ent.size();
+                } else {
+                    e = slot_map.emplace(gi, (int)ent.size()).first->second;
+                }
+                if (e == (int)ent.size()) {
+                    const int x = (int)(gi % h.n2), y = (int)((gi / h.n2) % h.n1),
+                              z = (int)(gi / ((int64_t)h.n2 * h.n1));
+                    Entry en{};
+                    en.c[0] = z + 1;
+                    en.c[1] = y + 1;
+                    en.c[2] = x + 1;
+                    en.first = e;
+                    ent.push_back(en);
+                }
+                Entry& en = ent[e];
+                const int i = k - k0;
+                en.mask |= 1 << i;
+                en.coef[i][0] = D0;
+                en.coef[i][1] = D1;
+                en.coef[i][2] = D2;
+                en.coef[i][3] = -(D0 * en.c[0] + D1 * en.c[1] + D2 * en.c[2]);
+            }
         }
-        // cells in first-touch order (the voxel order of the chunk)
-        slot_map.clear();
-        for (size_t sl = 0; sl < touched.size(); ++sl) {
-            const int64_t gi = touched[sl];
-            if (dense) slot[gi] = -1;
-            const int D0 = acc[3 * sl], D1 = acc[3 * sl + 1], D2 = acc[3 * sl + 2];
-            if (!(D0 || D1 || D2)) continue;
-            if (std::abs(D0) > 127 || std::abs(D1) > 127 || std::abs(D2) > 127)
-                fail(EKG_EINVAL, "dipole vector out of range");
-            const int x = (int)(gi % h.n2), y = (int)((gi / h.n2) % h.n1), z = (int)(gi / ((int64_t)h.n2 * h.n1));
-            cells.push_back(Cell{{z + 1, y + 1, x + 1}, {D0, D1, D2}, -(D0 * (z + 1) + D1 * (y + 1) + D2 * (x + 1))});
-        }
-        n_cells += (int64_t)cells.size();
-        const int ng = (int)((cells.size() + 15) / 16);
-        for (int q = 0; q < ng; ++q) {
+        // reset the cell -> entry map for the next window
+        if (dense)
+            for (const Entry& en : ent)
+                slot[((int64_t)(en.c[0] - 1) * h.n1 + (en.c[1] - 1)) * h.n2 + (en.c[2] - 1)] = -1;
+        else
+            slot_map.clear();
+        // entries by (lowest class, mask, first touch): groups of like masks
+        std::sort(ent.begin(), ent.end(), [](const Entry& x, const Entry& y) {
+            const int fx = __builtin_ctz((unsigned)x.mask), fy = __builtin_ctz((unsigned)y.mask);
+            if (fx != fy) return fx < fy;
+            if (x.mask != y.mask) return x.mask < y.mask;
+            return x.first < y.first;
+        });
+        const int g_begin = (int)crow.size();
+        const size_t ne = ent.size();
+        n_entries += (int64_t)ne;
+        for (size_t q0 = 0; q0 < ne; q0 += 16) {
+            int mask = 0;
+            for (size_t e = q0; e < std::min(ne, q0 + 16); ++e) mask |= ent[e].mask;
+            const int nrow = 2 + __builtin_popcount((unsigned)mask);
+            if (grp.size() / 64 + nrow > 0x7fffffffu) fail(EKG_EINVAL, "too many class cells");
+            crow.push_back(make_int2((int)(grp.size() / 64), mask));
             const size_t base = grp.size();
-            grp.resize(base + 192);
+            grp.resize(base + (size_t)64 * nrow);
             float* av = grp.data() + base;
-            float* bv = av + 64;
-            float* sq = av + 128;
+            float* sq = av + 64;
             for (int lane = 0; lane < 64; ++lane) {
                 const int kq = lane >> 4, b = (lane >> 2) & 3, rc = lane & 3;
-                const size_t q0 = (size_t)16 * q, ql = q0 + 4 * b + kq, qa = q0 + 4 * b + rc;
-                const Cell& A = cells[qa < cells.size() ? qa : q0];
-                const bool ok = ql < cells.size();
-                const Cell& C = cells[ok ? ql : q0];
+                const size_t el = q0 + 4 * b + kq, ea = q0 + 4 * b + rc;
+                const Entry& A = ent[ea < ne ? ea : q0];
+                const bool ok = el < ne;
+                const Entry& C = ent[ok ? el : q0];
                 av[lane] = kq < 3 ? (float)A.c[kq] : 1.0f;
-                bv[lane] = ok ? (float)(rc < 3 ? C.D[rc] : C.K) : 0.0f;
                 sq[lane] = (float)(C.c[0] * C.c[0] + C.c[1] * C.c[1] + C.c[2] * C.c[2]);
+                int r = 2;
+                for (int i = 0; i < CLASS_WINDOW; ++i) {
+                    if (!((mask >> i) & 1)) continue;
+                    av[64 * r + lane] = ok ? (float)C.coef[i][rc] : 0.0f;
+                    ++r;
+                }
             }
         }
-        if (grp.size() / 192 > 0x7fffffffu) fail(EKG_EINVAL, "too many class cells");
-        off[k + 1] = (int)(grp.size() / 192);
+        const int g_end = (int)crow.size();
+        // at least one piece per window, so every class has sums (0 without entries)
+        int g = g_begin;
+        do {
+            const int ge = std::min(g_end, g + CLASS_PIECE);
+            pieces.push_back(int4{w, g, ge, 0});
+            g = ge;
+        } while (g < g_end);
+        wpiece.push_back((int)pieces.size());
     }
     L.n_ccells[nhood] = n_cells;
-    if (grp.empty()) grp.assign(192, 0.0f);
+    L.n_centries[nhood] = n_entries;
+    L.n_cgroups[nhood] = (int64_t)crow.size();
+    L.n_cpieces[nhood] = (int)pieces.size();
+    if (grp.empty()) grp.assign(64, 0.0f);
+    if (crow.empty()) crow.push_back(make_int2(0, 0));
+    if (pieces.empty()) pieces.push_back(int4{0, 0, 0, 0});
     h.ctx->bind();
     L.cgrp[nhood].reserve(grp.size() * sizeof(float));
-    L.cgrp_off[nhood].reserve(off.size() * sizeof(int));
+    L.crow[nhood].reserve(crow.size() * sizeof(int2));
+    L.cpieces[nhood].reserve(pieces.size() * sizeof(int4));
+    L.cwin[nhood].reserve(wpiece.size() * sizeof(int));
     hip_check(hipMemcpy(L.cgrp[nhood].p, grp.data(), grp.size() * sizeof(float), hipMemcpyHostToDevice),
               "upload class cell groups");
-    hip_check(hipMemcpy(L.cgrp_off[nhood].p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice),
-              "upload class cell group offsets");
+    hip_check(hipMemcpy(L.crow[nhood].p, crow.data(), crow.size() * sizeof(int2), hipMemcpyHostToDevice),
+              "upload class cell group rows");
+    hip_check(hipMemcpy(L.cpieces[nhood].p, pieces.data(), pieces.size() * sizeof(int4), hipMemcpyHostToDevice),
+              "upload class window pieces");
+    hip_check(hipMemcpy(L.cwin[nhood].p, wpiece.data(), wpiece.size() * sizeof(int), hipMemcpyHostToDevice),
+              "upload class window piece offsets");
     L.cells_built[nhood] = true;
 }
 
@@ -667,9 +760,11 @@ ReqSizes req_sizes(const ReqSpec& r, bool mfma)
 {
     const EcgLayout& L = r.h->layout;
     // the chunk kernel (matrix-core path) keeps G rows on chip: no records;
-    // per-chunk sums for every chunk, partial rows for chunks outside classes
-    return ReqSizes{mfma ? 0 : L.V + ekgk::MFMA_PAD, (int64_t)L.n_chunks + L.n_cchunks, L.n_chunks, L.n_cchunks,
-                    L.n_bricks, L.n_classes};
+    // per-chunk sums for the chunks outside classes and per (class-window
+    // piece, class of the window), partial rows for chunks outside classes
+    const int64_t pieces = mfma ? L.n_cpieces[r.nhood] : 0;
+    return ReqSizes{mfma ? 0 : L.V + ekgk::MFMA_PAD, (int64_t)L.n_chunks + pieces * CLASS_WINDOW, L.n_chunks,
+                    pieces, L.n_bricks, L.n_classes};
 }
 
 template <class T>
@@ -692,7 +787,21 @@ size_t group_device_bytes(const GroupPlan& g)
         const bool fits = L.valid && (L.class_min == 0 || g.mfma);
         const int64_t est = std::max<int64_t>(1, V / 64 + r.h->max_layer + 4);
         const int64_t nmix = fits ? L.n_chunks : est;
-        const int64_t nall = fits ? (int64_t)L.n_chunks + L.n_cchunks : est;
+        // sums rows: the chunks outside classes and, on the matrix-core path,
+        // CLASS_WINDOW per class-window piece (exact once the cells are
+        // built; before that bounded by a piece per window and one per
+        // CLASS_PIECE groups of at most 9 entries per class voxel)
+        int64_t pieces = 0;
+        if (g.mfma && fits && L.n_classes > 0) {
+            if (L.cells_built[g.nhood]) {
+                pieces = L.n_cpieces[g.nhood];
+            } else {
+                int64_t cv = 0;
+                for (const int4& cc : L.h_cchunks) cv += cc.z - cc.y;
+                pieces = (L.n_classes + CLASS_WINDOW - 1) / CLASS_WINDOW + (9 * cv / 16 + CLASS_PIECE - 1) / CLASS_PIECE;
+            }
+        }
+        const int64_t nall = fits ? (int64_t)L.n_chunks + pieces * CLASS_WINDOW : est;
         const int64_t ncls = fits ? L.n_classes : 0;
         const int64_t rows = g.mfma ? 0 : V + ekgk::MFMA_PAD;
         s += (size_t)r.nl * 9 * 8 + (size_t)r.L * 3 * 8 + (size_t)r.Tp * 16;
@@ -751,7 +860,9 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
             hs.push_back(&h);
             ekgk::HeartDev d = h.layout.dev;
             d.cgrp = g.mfma ? h.layout.cgrp[g.nhood].as<float>() : nullptr;
-            d.cgrp_off = g.mfma ? h.layout.cgrp_off[g.nhood].as<int>() : nullptr;
+            d.crow = g.mfma ? h.layout.crow[g.nhood].as<int2>() : nullptr;
+            d.cpieces = g.mfma ? h.layout.cpieces[g.nhood].as<int4>() : nullptr;
+            d.cwin = g.mfma ? h.layout.cwin[g.nhood].as<int>() : nullptr;
             hdev.push_back(d);
         }
         int64_t t_off = -1;
@@ -827,25 +938,6 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
         ws.need(ws.ecg, (size_t)std::max<int64_t>(1, st_out[B]) * 8);
         d_ecg = ws.ecg.as<double>();
     }
-    // A group of several hearts: neighbouring requests of one heart form
-    // units of two for the class-chunk kernel (each cell loaded and unpacked
-    // once for both); a request without such a neighbour is a unit alone.
-    std::vector<int2> units;
-    std::vector<int64_t> st_u(1, 0);
-    bool several_hearts = false;
-    for (int r = 1; r < B && !several_hearts; ++r) several_hearts = g.reqs[r].h != g.reqs[0].h;
-    if (g.mfma && several_hearts && st_cc[B] > 0) {
-        for (int r = 0; r < B;) {
-            const bool pair = r + 1 < B && g.reqs[r + 1].h == g.reqs[r].h;
-            units.push_back(make_int2(r, pair ? r + 1 : r));
-            st_u.push_back(st_u.back() + (st_cc[r + 1] - st_cc[r]));
-            r += pair ? 2 : 1;
-        }
-        if (units.size() == (size_t)B) {  // no pairs: one request per block
-            units.clear();
-            st_u.assign(1, 0);
-        }
-    }
     // tables: uploaded only when they changed since the last group on this
     // workspace (a repeated group replays without host traffic)
     std::vector<char> tables;
@@ -855,10 +947,6 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     const size_t o_st = tables.size();
     for (auto* st : {&st_lf, &st_ch, &st_xy, &st_lay, &st_out, &st_cc, &st_cs, &st_cl})
         append(tables, st->data(), st->size());
-    const size_t o_ust = tables.size();
-    append(tables, st_u.data(), st_u.size());
-    const size_t o_units = tables.size();
-    append(tables, units.data(), units.size());
     void* before = ws.hearts.p;
     ws.need(ws.hearts, tables.size());
     if (ws.hearts.p != before) ws.h_tables.clear();
@@ -926,21 +1014,6 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     a.cl_per_req = (B > 0 && B % 8 == 0) ? (int)(st_cl[1] - st_cl[0]) : 0;
     for (int r = 1; r < B && a.cl_per_req; ++r)
         if (st_cl[r + 1] - st_cl[r] != a.cl_per_req) a.cl_per_req = 0;
-    // one heart: each class-chunk block takes a chunk for CC_RPB_ONE_HEART requests
-    a.cc_rpb = 1;
-    bool one_heart = B > 1 && a.cc_uniform > 0;
-    for (int r = 1; r < B && one_heart; ++r)
-        if (g.reqs[r].h != g.reqs[0].h) one_heart = false;
-    if (one_heart) {
-        a.cc_rpb = CC_RPB_ONE_HEART;
-        a.n_cc = (int64_t)a.cc_uniform * ((B + a.cc_rpb - 1) / a.cc_rpb);
-    } else if (!units.empty()) {
-        a.cc_rpb = 2;
-        a.cc_units = reinterpret_cast<const int2*>(tb + o_units);
-        a.cc_ustart = reinterpret_cast<const int64_t*>(tb + o_ust);
-        a.n_units = (int)units.size();
-        a.n_cc = st_u.back();
-    }
     a.n_xy = st_xy[B];
     a.n_lay = st_lay[B];
     a.n_out = st_out[B];
@@ -960,9 +1033,9 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     a.csums = ws.csums.as<double>();
     a.ecg = d_ecg;
     auto launch_all = [&](bool timed) {
-        // The timed kernel is the dominant one of each path: the class-chunk
-        // kernel, or without class chunks the chunk kernel (matrix-core
-        // path), or the fused transition-tile kernel (vector ALUs).
+        // The timed kernel is the dominant one of each path: the class-window
+        // kernel, or without classes the chunk kernel (matrix-core path), or
+        // the fused transition-tile kernel (vector ALUs).
         hipEvent_t e0 = nullptr, e1 = nullptr;
         auto tic = [&]() {
             if (!timed) return;
@@ -977,12 +1050,12 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
         };
         hip_check(ekgk::launch_ap_tables(a, stream), "ap_tables launch");
         if (g.mfma) {
-            // the class-chunk kernel is timed when the group has class chunks
+            // the class-window kernel is timed when the group has classes
             if (a.n_cc == 0) tic();
             hip_check(ekgk::launch_ecg_chunk(a, stream), "ecg_chunk launch");
             if (a.n_cc == 0) toc();
             if (a.n_cc > 0) tic();
-            hip_check(ekgk::launch_class_chunks(a, stream), "class_chunk launch");
+            hip_check(ekgk::launch_class_windows(a, stream), "class_window launch");
             if (a.n_cc > 0) toc();
             hip_check(ekgk::launch_class_sums(a, stream), "class_sums launch");
             hip_check(ekgk::launch_layer_prefix(a, stream), "layer_prefix launch");

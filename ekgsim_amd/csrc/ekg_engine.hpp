@@ -99,13 +99,14 @@ struct EcgLayout {
     std::vector<int4> h_chunks, h_cchunks, h_classes;
     std::vector<double> h_class_a;
     std::vector<uint2> h_geo;      // per voxel of the layout: packed grid coordinates, 3x3x3 myocardium mask
-    // Per dipole neighbourhood (EKG_NHOOD_*), built on first use: the cells of
-    // each class chunk with a non-zero integer dipole vector D, as lane-major
-    // groups of 16 (see ensure_cells), groups [cgrp_off[k], cgrp_off[k+1]) for
-    // class chunk k.
+    // Per dipole neighbourhood (EKG_NHOOD_*), built on first use: the class
+    // windows' entries as lane-major groups of 16 and their pieces (see
+    // ensure_cells).  n_ccells counts (class, cell) pairs with a non-zero
+    // dipole vector, n_centries (window, cell) pairs.
     bool cells_built[4] = {false, false, false, false};
-    int64_t n_ccells[4] = {0, 0, 0, 0};
-    DevBuf cgrp[4], cgrp_off[4];
+    int64_t n_ccells[4] = {0, 0, 0, 0}, n_centries[4] = {0, 0, 0, 0}, n_cgroups[4] = {0, 0, 0, 0};
+    int n_cpieces[4] = {0, 0, 0, 0};
+    DevBuf cgrp[4], crow[4], cpieces[4], cwin[4];
     std::vector<int> h_layer_chunk;
     std::vector<double> h_layer_t0;
     DevBuf vox_chunk, vox_a64, vox_a, vox_geo, chunks, bricks, brick_off, brick_vox, halo_off, halo_cells,

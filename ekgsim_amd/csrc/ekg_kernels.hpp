@@ -22,6 +22,10 @@ constexpr int CHUNK_MAX = 512;
 // step per lane of a wave).
 constexpr int BT_SHIFT_MFMA = 4;
 constexpr int BT_SHIFT_VALU = 6;
+// Activation classes per class window, and groups of 16 window entries per
+// piece (one class-window block per piece and request).
+constexpr int CLASS_WINDOW = 8;
+constexpr int CLASS_PIECE = 512;
 
 // ---- excitation sequence ---------------------------------------------------
 
@@ -70,12 +74,17 @@ struct HeartDev {
     const int4* cchunks;         // [n_cchunks] {layer, v_begin, v_end, t_ref as float bits}
     const int4* classes;         // [n_classes] {layer, first class chunk, end class chunk, 0}
     const double* class_a;       // [n_classes] activation time of the class
-    // cells of the class chunks for the group's neighbourhood, in groups of
-    // 16: group q holds the matrix-product operands of its 64 lanes,
-    // cgrp[192 q + {0, 64, 128} + lane] = {coordinate, coefficient, |c|^2};
-    // class chunk k owns groups [cgrp_off[k], cgrp_off[k+1])
+    // class windows for the group's neighbourhood: CLASS_WINDOW classes each
+    // (window w: classes [w CLASS_WINDOW, (w+1) CLASS_WINDOW)), their entries
+    // (cells with a non-zero dipole vector for a class of the window) in
+    // groups of 16, group q in rows crow[q].x, ... of 64 floats of cgrp
+    // {coordinate, |c|^2, then a coefficient row per class of the mask
+    // crow[q].y}; the groups cut into pieces {window, first group, end group},
+    // those of window w being [cwin[w], cwin[w+1])
     const float* cgrp;
-    const int* cgrp_off;
+    const int2* crow;
+    const int4* cpieces;
+    const int* cwin;
     int n0, n1, n2;
     int V, n_chunks, n_bricks, max_layer;
     int n_cchunks, n_classes;
@@ -112,21 +121,13 @@ struct GroupArgs {
     const int64_t* xy_start;   // nl * Tp per request
     const int64_t* lay_start;  // nl per request
     const int64_t* out_start;  // L * T per request
-    const int64_t* cc_start;   // class chunks per request
+    const int64_t* cc_start;   // class-window pieces per request
     const int64_t* cs_start;   // activation classes per request
     const int64_t* cl_start;   // 64-step tiles of the class kernel per request (0 without classes)
     int64_t n_lf, n_ch, n_xy, n_lay, n_out, n_cc, n_cs, n_cl;  // totals
     int ch_uniform;            // chunks per request when all requests have the same count, else 0
-    int cc_uniform;            // the same for class chunks
+    int cc_uniform;            // the same for class-window pieces
     int cl_per_req;            // class-kernel blocks per request when equal for all and B % 8 == 0, else 0
-    int cc_rpb;                // requests per class-chunk block: 2 when requests are paired, else 1
-    // Pairs of requests of one heart in a group of several hearts (cc_rpb 2,
-    // cc_uniform 0): unit u covers requests cc_units[u] = {r0, r1} (r1 == r0
-    // for a request without a partner), class-chunk blocks
-    // [cc_ustart[u], cc_ustart[u+1]).  Null when the group has one heart.
-    const int2* cc_units;
-    const int64_t* cc_ustart;
-    int n_units;
     const double* layer_k;
     const double* lead_pos;
     const double* tgrid;       // FP64 sample times
@@ -156,7 +157,7 @@ hipError_t launch_ecg_reduce(const GroupArgs& a, hipStream_t s);
 void prime_kernels();
 // Activation classes: the lead-weight sums of the class chunks, their sums
 // per class, then the classes' terms added to the reduced ECG.
-hipError_t launch_class_chunks(const GroupArgs& a, hipStream_t s);
+hipError_t launch_class_windows(const GroupArgs& a, hipStream_t s);
 hipError_t launch_class_sums(const GroupArgs& a, hipStream_t s);
 hipError_t launch_ecg_class(const GroupArgs& a, hipStream_t s);
 

@@ -214,17 +214,18 @@ int ekg_heart_get_tune(const ekg_heart *h, ekg_tune *t);
 
 /* Layout of the heart's matrix-core path as currently tuned (built on
  * demand, so that later calls find it ready): chunks, activation classes and,
- * for dipole neighbourhood `nhood`, the class cells whose lead weights each
- * simulation evaluates once.  No counterpart in the reference (diagnostics of
- * this library's layout). */
+ * for dipole neighbourhood `nhood`, the class cells and the class-window
+ * entries whose lead weights each simulation evaluates once.  No counterpart
+ * in the reference (diagnostics of this library's layout). */
 typedef struct ekg_layout_info {
     int64_t voxels;        /* myocardial voxels */
     int64_t class_voxels;  /* of which in activation classes */
-    int64_t class_cells;   /* cells of the class chunks with a non-zero dipole vector */
+    int64_t class_cells;   /* (class, cell) pairs with a non-zero dipole vector */
     int chunks;            /* chunks of the voxels outside classes */
     int class_chunks;
     int classes;
-    int pad;
+    int class_pieces;      /* class-window pieces (blocks per request of the class-window kernel) */
+    int64_t class_entries; /* (class window, cell) pairs: lead weights evaluated per lead and request */
 } ekg_layout_info;
 int ekg_heart_layout_info(ekg_heart *h, int nhood, ekg_layout_info *out);
 
