@@ -23,8 +23,10 @@ constexpr int CHUNK_MAX = 512;
 constexpr int BT_SHIFT_MFMA = 4;
 constexpr int BT_SHIFT_VALU = 6;
 // Activation classes per class window, and groups of 16 window entries per
-// piece (one class-window block per piece and request).
-constexpr int CLASS_WINDOW = 8;
+// piece (one class-window block per piece and request).  Six classes keep the
+// kernel's accumulators within 64 VGPRs (eight waves per SIMD); eight share
+// more lead weights but leave five waves and run slower.
+constexpr int CLASS_WINDOW = 6;
 constexpr int CLASS_PIECE = 512;
 
 // ---- excitation sequence ---------------------------------------------------
