@@ -1134,8 +1134,8 @@ constexpr int CC_THREADS = 256;
 // a class past the heart's last).
 template <int LP>
 __global__ __launch_bounds__(CC_THREADS) void ecg_class_window_kernel(
-    const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
-    const int64_t* __restrict__ cc_start, int cc_uniform, const double* __restrict__ lead_pos,
+    const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, const int4* __restrict__ runs,
+    const int64_t* __restrict__ run_start, int n_runs, const double* __restrict__ lead_pos,
     const double* __restrict__ layer_k, double* __restrict__ sums)
 {
     constexpr int NG = LP / 4;                 // groups of 4 leads
@@ -1144,16 +1144,14 @@ __global__ __launch_bounds__(CC_THREADS) void ecg_class_window_kernel(
     __shared__ double s_t[CW][NW][LP][4];
     const int t = threadIdx.x, lane = t & 63;
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
-    // piece-major when every request has the same pieces (one heart): the
-    // requests of a piece run back to back and read its groups from L2
-    int r, k;
-    if (cc_uniform > 0) {
-        r = (int)(blockIdx.x % (unsigned)B);
-        k = (int)(blockIdx.x / (unsigned)B);
-    } else {
-        r = find_req(cc_start, B, blockIdx.x);
-        k = (int)(blockIdx.x - cc_start[r]);
-    }
+    // Piece-major within each run of requests of one heart (a run
+    // {first request, requests, pieces}): the requests of a piece run back to
+    // back and read its groups from L2 (blocks go round-robin over the XCDs,
+    // so each XCD fetches a piece once per run).
+    const int u = find_req(run_start, n_runs, blockIdx.x);
+    const int4 run = runs[u];
+    const int local = (int)(blockIdx.x - run_start[u]);
+    const int k = local / run.y, r = run.x + (local - k * run.y);
     const ReqDev& R = reqs[r];
     const HeartDev& H = hearts[R.heart];
     const int4 pc4 = H.cpieces[k];
@@ -1520,7 +1518,7 @@ hipError_t launch_class_windows(const GroupArgs& a, hipStream_t s)
 #define EKG_CWIN(LPV)                                                                                        \
     case LPV:                                                                                                \
         hipLaunchKernelGGL(ecg_class_window_kernel<LPV>, dim3((unsigned)a.n_cc), dim3(CC_THREADS), 0, s, a.hearts, \
-                           a.reqs, a.B, a.cc_start, a.cc_uniform, a.lead_pos, a.layer_k, a.sums);            \
+                           a.reqs, a.cc_runs, a.cc_rstart, a.n_runs, a.lead_pos, a.layer_k, a.sums);       \
         break;
         EKG_CWIN(4)
         EKG_CWIN(8)

@@ -938,6 +938,20 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
         ws.need(ws.ecg, (size_t)std::max<int64_t>(1, st_out[B]) * 8);
         d_ecg = ws.ecg.as<double>();
     }
+    // class-window blocks: runs of consecutive requests of one heart (a
+    // server group is ordered by heart), piece-major within a run
+    std::vector<int4> runs;
+    std::vector<int64_t> st_run(1, 0);
+    if (g.mfma && st_cc[B] > 0) {
+        for (int r = 0; r < B;) {
+            int r1 = r + 1;
+            while (r1 < B && g.reqs[r1].h == g.reqs[r].h) ++r1;
+            const int64_t np = st_cc[r + 1] - st_cc[r];
+            runs.push_back(int4{r, r1 - r, (int)np, 0});
+            st_run.push_back(st_run.back() + np * (r1 - r));
+            r = r1;
+        }
+    }
     // tables: uploaded only when they changed since the last group on this
     // workspace (a repeated group replays without host traffic)
     std::vector<char> tables;
@@ -947,6 +961,11 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     const size_t o_st = tables.size();
     for (auto* st : {&st_lf, &st_ch, &st_xy, &st_lay, &st_out, &st_cc, &st_cs, &st_cl})
         append(tables, st->data(), st->size());
+    const size_t o_rst = tables.size();
+    append(tables, st_run.data(), st_run.size());
+    tables.resize((tables.size() + 15) & ~(size_t)15);  // int4 alignment
+    const size_t o_runs = tables.size();
+    append(tables, runs.data(), runs.size());
     void* before = ws.hearts.p;
     ws.need(ws.hearts, tables.size());
     if (ws.hearts.p != before) ws.h_tables.clear();
@@ -995,7 +1014,6 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     a.xy_start = stp + 2 * (B + 1);
     a.lay_start = stp + 3 * (B + 1);
     a.out_start = stp + 4 * (B + 1);
-    a.cc_start = stp + 5 * (B + 1);
     a.cs_start = stp + 6 * (B + 1);
     a.cl_start = stp + 7 * (B + 1);
     a.n_cc = st_cc[B];
@@ -1007,9 +1025,9 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     a.ch_uniform = B > 0 ? (int)(st_ch[1] - st_ch[0]) : 0;
     for (int r = 1; r < B && a.ch_uniform; ++r)
         if (st_ch[r + 1] - st_ch[r] != a.ch_uniform) a.ch_uniform = 0;
-    a.cc_uniform = B > 0 ? (int)(st_cc[1] - st_cc[0]) : 0;
-    for (int r = 1; r < B && a.cc_uniform; ++r)
-        if (st_cc[r + 1] - st_cc[r] != a.cc_uniform) a.cc_uniform = 0;
+    a.cc_runs = reinterpret_cast<const int4*>(tb + o_runs);
+    a.cc_rstart = reinterpret_cast<const int64_t*>(tb + o_rst);
+    a.n_runs = (int)runs.size();
     // class-kernel blocks: XCD-aware order when every request has the same count
     a.cl_per_req = (B > 0 && B % 8 == 0) ? (int)(st_cl[1] - st_cl[0]) : 0;
     for (int r = 1; r < B && a.cl_per_req; ++r)

@@ -123,12 +123,16 @@ struct GroupArgs {
     const int64_t* xy_start;   // nl * Tp per request
     const int64_t* lay_start;  // nl per request
     const int64_t* out_start;  // L * T per request
-    const int64_t* cc_start;   // class-window pieces per request
     const int64_t* cs_start;   // activation classes per request
     const int64_t* cl_start;   // 64-step tiles of the class kernel per request (0 without classes)
     int64_t n_lf, n_ch, n_xy, n_lay, n_out, n_cc, n_cs, n_cl;  // totals
     int ch_uniform;            // chunks per request when all requests have the same count, else 0
-    int cc_uniform;            // the same for class-window pieces
+    // class-window blocks: runs of consecutive requests of one heart
+    // {first request, requests, pieces of the heart}, blocks of run u
+    // [cc_rstart[u], cc_rstart[u+1]) = pieces x requests
+    const int4* cc_runs;
+    const int64_t* cc_rstart;
+    int n_runs;
     int cl_per_req;            // class-kernel blocks per request when equal for all and B % 8 == 0, else 0
     const double* layer_k;
     const double* lead_pos;
