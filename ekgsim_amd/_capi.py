@@ -29,7 +29,8 @@ EXPORTS = (
     "ekg_heart_excite_delay", "ekg_heart_set_activation", "ekg_heart_get_activation",
     "ekg_num_steps", "ekg_simulate", "ekg_simulate_batch", "ekg_lead_basis", "ekg_move_lead",
     "ekg_server_create", "ekg_server_destroy", "ekg_server_add_heart", "ekg_server_submit",
-    "ekg_server_poll", "ekg_server_wait", "ekg_server_stats", "ekg_simulate_batch_device",
+    "ekg_server_poll", "ekg_server_wait", "ekg_server_wait_view", "ekg_server_release", "ekg_server_stats",
+    "ekg_simulate_batch_device",
     "ekg_ctx_synchronize", "ekg_ctx_set_kernel", "ekg_ap_value", "ekg_ap_num_genes", "ekg_layer_aps",
     "ekg_criterion", "ekg_resample", "ekg_heart_autotune", "ekg_heart_set_tune", "ekg_heart_get_tune",
     "ekg_ctx_set_graphs", "ekg_ctx_graph_stats", "ekg_heart_layout_info", "ekg_criteria_device",
@@ -143,6 +144,8 @@ def lib():
     L.ekg_server_submit.argtypes = [vp, C.c_int, C.POINTER(SimDesc), C.POINTER(C.c_uint64)]
     L.ekg_server_poll.argtypes = [vp, C.c_uint64, C.POINTER(C.c_int)]
     L.ekg_server_wait.argtypes = [vp, C.c_uint64, vp, C.c_size_t, pd]
+    L.ekg_server_wait_view.argtypes = [vp, C.c_uint64, C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.c_size_t), pd]
+    L.ekg_server_release.argtypes = [vp, C.c_uint64]
     L.ekg_server_stats.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                    C.POINTER(C.c_int), C.POINTER(C.c_size_t)]
     L.ekg_ap_value.argtypes = [vp, C.c_double]
@@ -590,6 +593,24 @@ class Server:
         lat = C.c_double()
         check(lib().ekg_server_wait(self.h, C.c_uint64(ticket), _p(out), out.size, C.byref(lat)))
         return out, lat.value
+
+    def wait_view(self, ticket):
+        """Waits for the ticket and returns (ECG, latency): the ECG is a
+        read-only view of the server's pinned buffer, valid until
+        release(ticket)."""
+        L, T = self._shapes.pop(ticket)
+        ptr = C.POINTER(C.c_double)()
+        n = C.c_size_t()
+        lat = C.c_double()
+        check(lib().ekg_server_wait_view(self.h, C.c_uint64(ticket), C.byref(ptr), C.byref(n), C.byref(lat)))
+        if n.value != L * T:
+            raise EkgError(6, f"result holds {n.value} doubles, {L * T} expected")
+        view = np.ctypeslib.as_array(ptr, shape=(L, T)) if n.value else np.zeros((L, T))
+        view.flags.writeable = False
+        return view, lat.value
+
+    def release(self, ticket):
+        check(lib().ekg_server_release(self.h, C.c_uint64(ticket)))
 
     def stats(self):
         c = C.c_uint64()

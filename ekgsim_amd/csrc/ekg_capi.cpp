@@ -558,6 +558,22 @@ int ekg_server_wait(ekg_server* s, uint64_t ticket, double* ecg, size_t cap, dou
     });
 }
 
+int ekg_server_wait_view(ekg_server* s, uint64_t ticket, const double** ecg, size_t* n, double* latency_ms)
+{
+    return guard([&] {
+        if (!s || !ecg) ekg::fail(EKG_EINVAL, "NULL argument");
+        *ecg = ekg::server_wait_view(*reinterpret_cast<ekg::Server*>(s), ticket, n, latency_ms);
+    });
+}
+
+int ekg_server_release(ekg_server* s, uint64_t ticket)
+{
+    return guard([&] {
+        if (!s) ekg::fail(EKG_EINVAL, "NULL argument");
+        ekg::server_release(*reinterpret_cast<ekg::Server*>(s), ticket);
+    });
+}
+
 int ekg_server_stats(ekg_server* s, uint64_t* completed, uint64_t* groups, int* peak_inflight,
                      size_t* peak_bytes)
 {

@@ -61,7 +61,10 @@ void DevBuf::release()
 void Workspace::need(DevBuf& b, size_t n)
 {
     if (!arena) {
-        b.reserve(n);
+        // grow by at least half the current size: a workspace that serves
+        // groups of varying size reallocates (and synchronises the device
+        // in hipFree) a few times, not at every larger group
+        if (n > b.bytes) b.reserve(std::max(n, b.bytes + b.bytes / 2));
         return;
     }
     const size_t at = (arena_used + 255) & ~(size_t)255;

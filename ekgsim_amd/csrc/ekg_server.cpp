@@ -43,22 +43,26 @@ struct HostBuf {
 
 class HostPool {
 public:
-    // a buffer of at least n doubles, from the pool or newly pinned
+    // a buffer of at least n doubles: the smallest free one that fits, or a
+    // newly pinned one of the next power of two (at least 64 Ki doubles), so
+    // that groups of varying size soon find pinned buffers to reuse
     std::shared_ptr<HostBuf> take(size_t n)
     {
         HostBuf* b = nullptr;
         {
             std::lock_guard<std::mutex> g(mu_);
+            size_t best = free_.size();
             for (size_t i = 0; i < free_.size(); ++i)
-                if (free_[i]->cap >= n) {
-                    b = free_[i];
-                    free_.erase(free_.begin() + (std::ptrdiff_t)i);
-                    break;
-                }
+                if (free_[i]->cap >= n && (best == free_.size() || free_[i]->cap < free_[best]->cap)) best = i;
+            if (best < free_.size()) {
+                b = free_[best];
+                free_.erase(free_.begin() + (std::ptrdiff_t)best);
+            }
         }
         if (!b) {
             std::unique_ptr<HostBuf> nb(new HostBuf);
-            const size_t cap = std::max<size_t>(n, 1);
+            size_t cap = (size_t)1 << 16;
+            while (cap < n) cap <<= 1;
             hip_check(hipHostMalloc((void**)&nb->p, cap * 8, hipHostMallocDefault), "hipHostMalloc");
             nb->cap = cap;
             b = nb.release();
@@ -94,6 +98,7 @@ struct Request {
     // the result: n_out doubles at out->p + out_off
     std::shared_ptr<HostBuf> out;
     size_t out_off = 0, n_out = 0;
+    bool viewed = false;  // handed out by server_wait_view, kept until server_release
     Clock::time_point t_submit, t_done;
 };
 
@@ -484,6 +489,7 @@ void server_wait(Server& s, uint64_t ticket, double* ecg, size_t cap, double* la
         auto it = s.reqs.find(ticket);
         if (it == s.reqs.end()) fail(EKG_ENOTFOUND, "unknown ticket");
         Request* r = it->second.get();
+        if (r->viewed) fail(EKG_ESTATE, "ticket already viewed; release it");
         s.cv_done.wait(lk, [&] { return r->state == Request::Done || r->state == Request::Failed; });
         own = std::move(it->second);
         s.reqs.erase(it);
@@ -499,6 +505,40 @@ void server_wait(Server& s, uint64_t ticket, double* ecg, size_t cap, double* la
                                  std::to_string(r->n_out) + " needed");
         if (r->n_out) std::memcpy(ecg, r->out->p + r->out_off, r->n_out * sizeof(double));
     }
+}
+
+const double* server_wait_view(Server& s, uint64_t ticket, size_t* n, double* latency_ms)
+{
+    std::unique_lock<std::mutex> lk(s.mu);
+    auto it = s.reqs.find(ticket);
+    if (it == s.reqs.end()) fail(EKG_ENOTFOUND, "unknown ticket");
+    Request* r = it->second.get();
+    if (r->viewed) fail(EKG_ESTATE, "ticket already viewed; release it");
+    s.cv_done.wait(lk, [&] { return r->state == Request::Done || r->state == Request::Failed; });
+    if (latency_ms)
+        *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+    if (r->state == Request::Failed) {
+        std::unique_ptr<Request> own = std::move(it->second);
+        s.reqs.erase(it);
+        lk.unlock();
+        fail(own->err, own->msg);
+    }
+    // the request (and with it the pinned buffer of its group) stays until
+    // server_release
+    r->viewed = true;
+    if (n) *n = r->n_out;
+    return r->n_out ? r->out->p + r->out_off : nullptr;
+}
+
+void server_release(Server& s, uint64_t ticket)
+{
+    std::unique_ptr<Request> own;
+    std::lock_guard<std::mutex> g(s.mu);
+    auto it = s.reqs.find(ticket);
+    if (it == s.reqs.end()) fail(EKG_ENOTFOUND, "unknown ticket");
+    if (!it->second->viewed) fail(EKG_ESTATE, "ticket not viewed: use ekg_server_wait");
+    own = std::move(it->second);
+    s.reqs.erase(it);
 }
 
 void server_stats(Server& s, uint64_t* completed, uint64_t* groups, int* peak_inflight,

@@ -370,6 +370,14 @@ int ekg_server_poll(ekg_server *s, uint64_t ticket, int *done);
  * doubles available) and release it.  latency_ms (optional) receives
  * submit-to-completion time. */
 int ekg_server_wait(ekg_server *s, uint64_t ticket, double *ecg, size_t cap, double *latency_ms);
+/* Block until the ticket is done and hand out its ECG in place: *ecg points
+ * at n doubles ([n_leads][T] FP64) in the server's pinned host memory, valid
+ * until ekg_server_release(ticket).  Saves the copy of ekg_server_wait for
+ * callers that read the result once (a holder keeps its whole launch
+ * group's buffer from reuse).  No counterpart in the reference (the
+ * request loop's result hand-off, like ekg_server_wait). */
+int ekg_server_wait_view(ekg_server *s, uint64_t ticket, const double **ecg, size_t *n, double *latency_ms);
+int ekg_server_release(ekg_server *s, uint64_t ticket);
 /* Counters: completed requests, launch groups, peak in-flight requests,
  * peak device bytes. */
 int ekg_server_stats(ekg_server *s, uint64_t *completed, uint64_t *groups, int *peak_inflight,

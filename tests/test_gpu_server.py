@@ -187,3 +187,35 @@ def test_batching_window_equals_alone(ctx, window_us):
         _, groups_off = burst(-1)
         print(f"window {window_us} us: {groups} groups; no window: {groups_off} groups", flush=True)
         assert groups <= groups_off + 1
+
+
+def test_result_views_equal_copies(ctx):
+    """ekg_server_wait_view hands out a request's ECG in the server's pinned
+    buffer without copying: the view equals the plain simulation bit for bit
+    and stays valid while later groups run (its group's buffer is held until
+    release), copies and views mix freely, and a ticket is released once."""
+    from ekgsim_amd import EkgError, Server
+    hearts = _hearts(ctx, [(24, 21), (20, 22)])
+    reqs = _requests(hearts, 24, 5)
+    alone = [_alone(hearts, r) for r in reqs]
+    srv = Server(ctx, max_batch=6, n_streams=2)
+    ids = [srv.add_heart(H) for _, H in hearts]
+    tickets = [srv.submit(ids[hi], k, pos, "3D4", 0.0, length, 1.0) for hi, k, pos, length in reqs]
+    views = {}
+    for i, t in enumerate(tickets):
+        if i % 2 == 0:
+            views[i] = (t, srv.wait_view(t)[0])
+        else:
+            assert np.array_equal(srv.wait(t)[0], alone[i]), i
+    # a second burst reuses the pinned pool while the first burst's views are held
+    more = [srv.submit(ids[hi], k, pos, "3D4", 0.0, length, 1.0) for hi, k, pos, length in reqs[:8]]
+    for i, t in enumerate(more):
+        assert np.array_equal(srv.wait(t)[0], alone[i]), ("second burst", i)
+    for i, (t, v) in views.items():
+        assert not v.flags.writeable
+        assert np.array_equal(v, alone[i]), i
+        srv.release(t)
+    with pytest.raises(EkgError) as ei:
+        srv.release(views[0][0])
+    assert ei.value.code == 7  # EKG_ENOTFOUND: released already
+    srv.close()
