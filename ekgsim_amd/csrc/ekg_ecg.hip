@@ -1175,20 +1175,24 @@ __global__ __launch_bounds__(CC_THREADS) void ecg_class_window_kernel(
     for (int i = 0; i < CW; ++i)
 #pragma unroll
         for (int g = 0; g < NG; ++g) acc[i][g] = 0.0;
-    // each group's rows and class mask (wave-uniform), read one group ahead
-    int2 crn = g0 + w < g1 ? H.crow[g0 + w] : int2{0, 0};
+    // Each group's rows and class mask are wave-uniform: read through the
+    // constant address space they become scalar loads, and the group's rows
+    // through the global one take a scalar base and the lane's offset, so
+    // neither needs vector address arithmetic.
+    using ci64 = __attribute__((address_space(4))) const long long;
+    using gfloat = __attribute__((address_space(1))) const float;
+    ci64* const crow = (ci64*)H.crow;
     for (int q = g0 + w; q < g1; q += NW) {
-        const int2 cr = crn;
-        if (q + NW < g1) crn = H.crow[q + NW];
-        const int row = __builtin_amdgcn_readfirstlane(cr.x), mask = __builtin_amdgcn_readfirstlane(cr.y);
-        const float* o = H.cgrp + (int64_t)64 * row + lane;
+        const long long cr = crow[q];  // {first row, class mask}
+        const int row = (int)(cr & 0xffffffffLL), mask = (int)(cr >> 32);
+        gfloat* o = (gfloat*)H.cgrp + (int64_t)64 * row;
         // product-1 left operand (coordinate kq of entry 4 block + rc, or 1)
         // and |c|^2 of entry 4 block + kq
-        const double av = (double)o[0], cc = (double)o[64];
+        const double av = (double)o[lane], cc = (double)o[64 + lane];
         float bf[CW];
 #pragma unroll
         for (int i = 0, n = 2; i < CW; ++i)
-            if ((mask >> i) & 1) bf[i] = o[64 * n++];
+            if ((mask >> i) & 1) bf[i] = o[64 * n++ + lane];
         double sw[NG];
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
