@@ -1180,19 +1180,24 @@ __global__ __launch_bounds__(CC_THREADS) void ecg_class_window_kernel(
     // through the global one take a scalar base and the lane's offset, so
     // neither needs vector address arithmetic.
     using ci64 = __attribute__((address_space(4))) const long long;
-    using gfloat = __attribute__((address_space(1))) const float;
     ci64* const crow = (ci64*)H.crow;
     for (int q = g0 + w; q < g1; q += NW) {
         const long long cr = crow[q];  // {first row, class mask}
         const int row = (int)(cr & 0xffffffffLL), mask = (int)(cr >> 32);
-        gfloat* o = (gfloat*)H.cgrp + (int64_t)64 * row;
+        // the group's rows as a buffer resource: the lane's offset is a
+        // fixed vector operand and each row's offset a scalar one (reads
+        // past the group's rows would return 0, never fault)
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(H.cgrp + (int64_t)64 * row), 0, 256 * (2 + __builtin_popcount(mask)), 0x00020000);
+        const int lo = 4 * lane;
         // product-1 left operand (coordinate kq of entry 4 block + rc, or 1)
         // and |c|^2 of entry 4 block + kq
-        const double av = (double)o[lane], cc = (double)o[64 + lane];
+        const double av = (double)__int_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, lo, 0, 0));
+        const double cc = (double)__int_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, lo, 256, 0));
         float bf[CW];
 #pragma unroll
         for (int i = 0, n = 2; i < CW; ++i)
-            if ((mask >> i) & 1) bf[i] = o[64 * n++ + lane];
+            if ((mask >> i) & 1) bf[i] = __int_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, lo, 256 * n++, 0));
         double sw[NG];
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
