@@ -89,6 +89,47 @@ def test_two_dimensional_heart(ctx, nhood):
     g.close()
 
 
+def _window_counts(h, act, class_min, window):
+    """(class, cell) pairs with a non-zero 3D4 dipole vector and (window,
+    cell) entries, restated on the grid with numpy: classes are the runs of
+    one layer and one activation time with at least class_min voxels, in
+    layer and time order, taken `window` at a time; for a class S the vector
+    of cell w is D(w) = [w in S] s_w + sum over the corners d with w - d in S
+    and w myocardial of d, s_w the sum of the corners d with w + d
+    myocardial."""
+    lay = h.layer
+    myo = lay > 0
+    keys = {}
+    for l in range(1, int(lay.max()) + 1):
+        times, cnt = np.unique(act[lay == l], return_counts=True)
+        for t, c in zip(times, cnt):
+            if c >= class_min:
+                keys[(l, t)] = len(keys)
+    cls = np.full(lay.shape, -1, np.int64)
+    for (l, t), i in keys.items():
+        cls[(lay == l) & (act == t)] = i
+    P = np.pad(myo, 1)
+    C = np.pad(cls, 1, constant_values=-1)
+    n0, n1, n2 = lay.shape
+    corners = [(a, b, c) for a in (-1, 1) for b in (-1, 1) for c in (-1, 1)]
+    s_w = np.zeros(lay.shape + (3,), np.int64)
+    for d in corners:
+        nb = P[1 + d[0]:1 + d[0] + n0, 1 + d[1]:1 + d[1] + n1, 1 + d[2]:1 + d[2] + n2]
+        s_w += nb[..., None] * np.array(d)
+    pairs = 0
+    per_window = {}
+    for i in range(len(keys)):
+        D = np.where((cls == i)[..., None], s_w, 0)
+        for d in corners:
+            src = C[1 - d[0]:1 - d[0] + n0, 1 - d[1]:1 - d[1] + n1, 1 - d[2]:1 - d[2] + n2] == i
+            D += (src & myo)[..., None] * np.array(d)
+        nz = np.any(D != 0, axis=-1)
+        pairs += int(nz.sum())
+        per_window.setdefault(i // window, np.zeros(lay.shape, bool))
+        per_window[i // window] |= nz
+    return pairs, int(sum(m.sum() for m in per_window.values()))
+
+
 def test_layout_info(ctx):
     h, g, act = _int_heart(ctx, 40, 9)
     runs = _class_runs(h, act)
@@ -99,6 +140,10 @@ def test_layout_info(ctx):
     assert info["classes"] == int((runs >= 64).sum())
     assert 0 < info["class_cells"] <= 9 * info["class_voxels"]
     assert info["class_chunks"] >= info["classes"] and info["chunks"] > 0
+    pairs, entries = _window_counts(h, act, 64, 6)
+    assert info["class_cells"] == pairs
+    assert info["class_entries"] == entries
+    assert -(-info["classes"] // 6) <= info["class_pieces"] <= entries // 16 + info["classes"]
     g.set_tune(class_min=-1)
     off = g.layout_info("3D8")
     assert off["classes"] == 0 and off["class_voxels"] == 0 and off["class_cells"] == 0
