@@ -112,6 +112,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def max_abs_err(ecg, refs):
+    """Largest |GPU - FP64 oracle| over the checked ECGs, absolute and
+    relative to the largest |oracle value| of the same ECG (None when no ECG
+    was checked)."""
+    if not refs:
+        return None
+    ab = max(float(np.max(np.abs(ecg[r] - ref))) for r, ref in refs.items())
+    rel = max(float(np.max(np.abs(ecg[r] - ref)) / np.max(np.abs(ref))) for r, ref in refs.items())
+    return {"abs": ab, "rel_to_peak": rel}
+
+
 def workload(n, L, B, pseed0=1000):
     h = S.make_heart(n, seed=1)
     ks, ps = [], []
@@ -478,6 +489,7 @@ def main():
     for r, ref in ref_full.items():
         errs[r] = float(np.linalg.norm(ecg_gpu[r] - ref) / np.linalg.norm(ref))
     rel_max = max(errs.values()) if errs else None
+    max_err = max_abs_err(ecg_gpu, ref_full)
     ref_win_err = None
     if cpu is not None:
         num = den = 0.0
@@ -517,7 +529,8 @@ def main():
         gecg = d_e.cpu().numpy()
         gerr = {r: float(np.linalg.norm(gecg[r] - ref) / np.linalg.norm(ref)) for r, ref in ref_full.items()}
         general = {"ecgs_per_s": round(B * args.steps / gel, 2), "ms_per_step": round(gel * 1e3 / args.steps, 3),
-                   "autotuned": gtune, "rel_l2_max_vs_fp64": max(gerr.values()) if gerr else None}
+                   "autotuned": gtune, "rel_l2_max_vs_fp64": max(gerr.values()) if gerr else None,
+                   "max_err_vs_fp64": max_abs_err(gecg, ref_full)}
         log(f"activation classes off: {general}")
         heart.set_tune(saved["chunk_voxels"], saved["mfma_waves"], saved["lead_phase"], saved["kernel"],
                        saved["class_min"])
@@ -686,6 +699,7 @@ def main():
         "p99_ms": None if serve is None else round(serve["p99_ms"], 3),
         "testrun_model24": testrun,
         "rel_l2_max_vs_fp64": rel_max,
+        "max_err_vs_fp64": max_err,
         "rel_l2_vs_reference_windows": ref_win_err,
         "excitation_ms": round(ca_ms, 2),
         "criteria_ms_per_batch": round(crit_ms, 4),
