@@ -107,6 +107,10 @@ int ekg_ctx_create(int device, ekg_ctx** out)
             ekg::hip_check(ekgk::scratch_ptr(device, &scratch), "resolve the static device workspace");
             c->c.ws_fixed.arena = static_cast<char*>(scratch);
             c->c.ws_fixed.arena_bytes = ekgk::SCRATCH_BYTES;
+            // second streams of the launch groups, made now so that no later
+            // call (the allocation-free one included) creates them
+            c->c.ws.ensure_side();
+            c->c.ws_fixed.ensure_side();
             // per neighbourhood: {d0, d1, d2, index of d among the 26 cube offsets}
             int8_t tab[4][26][4] = {};
             int8_t cube[26][3];

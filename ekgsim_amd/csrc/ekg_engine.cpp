@@ -81,6 +81,20 @@ void Ctx::bind() const { hip_check(hipSetDevice(device), "hipSetDevice"); }
 Workspace::~Workspace()
 {
     if (graph) (void)hipGraphExecDestroy(graph);
+    if (side) {
+        (void)hipStreamSynchronize(side);
+        (void)hipStreamDestroy(side);
+    }
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+}
+
+void Workspace::ensure_side()
+{
+    if (side) return;
+    hip_check(hipStreamCreateWithFlags(&side, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming), "hipEventCreate");
+    hip_check(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming), "hipEventCreate");
 }
 
 hipEvent_t Ctx::get_event()
@@ -1069,6 +1083,27 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
             hip_check(hipEventRecord(e1, stream), "hipEventRecord");
             ctx.pending.emplace_back(e0, e1);
         };
+        if (g.mfma && a.n_cc > 0 && ws.side) {
+            // Two branches: the class-window kernel and the class sums on
+            // `stream`, beside the AP tables, chunk, prefix and reduction
+            // kernels on the side stream (they share no buffer: the chunk
+            // and class-window kernels write disjoint rows of the sums); the
+            // class ECG kernel adds to the reduction's output after both.
+            hip_check(hipEventRecord(ws.ev_fork, stream), "hipEventRecord");
+            hip_check(hipStreamWaitEvent(ws.side, ws.ev_fork, 0), "hipStreamWaitEvent");
+            hip_check(ekgk::launch_ap_tables(a, ws.side), "ap_tables launch");
+            hip_check(ekgk::launch_ecg_chunk(a, ws.side), "ecg_chunk launch");
+            hip_check(ekgk::launch_layer_prefix(a, ws.side), "layer_prefix launch");
+            hip_check(ekgk::launch_ecg_reduce(a, ws.side), "ecg_reduce launch");
+            hip_check(hipEventRecord(ws.ev_join, ws.side), "hipEventRecord");
+            tic();
+            hip_check(ekgk::launch_class_windows(a, stream), "class_window launch");
+            toc();
+            hip_check(ekgk::launch_class_sums(a, stream), "class_sums launch");
+            hip_check(hipStreamWaitEvent(stream, ws.ev_join, 0), "hipStreamWaitEvent");
+            hip_check(ekgk::launch_ecg_class(a, stream), "ecg_class launch");
+            return;
+        }
         hip_check(ekgk::launch_ap_tables(a, stream), "ap_tables launch");
         if (g.mfma) {
             // the class-window kernel is timed when the group has classes

@@ -50,6 +50,12 @@ struct Workspace {
     // captured launch sequence of the last group (graph mode)
     hipGraphExec_t graph = nullptr;
     std::vector<char> graph_key;
+    // second stream of a group's launches (the chunk, prefix and reduction
+    // kernels beside the class-window kernel) and its fork and join events;
+    // created by ensure_side, on the device that is current
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    void ensure_side();
     // Fixed workspace: when `arena` is set, every buffer of a group is carved
     // from it in order (no device allocation); a group that does not fit
     // fails with EKG_ENOMEM.  Contents are not kept between groups.

@@ -401,6 +401,7 @@ Server* server_create(Ctx& ctx, const ekg_server_opts& o)
         Slot& sl = *s->slots.back();
         hip_check(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming), "hipEventCreate");
+        sl.ws.ensure_side();
     }
     Server* raw = s.release();
     raw->worker = std::thread([raw] { raw->run(); });
