@@ -7,6 +7,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -335,10 +336,17 @@ class _Desc:
 class Context:
     def __init__(self, device: int = 0):
         self.h = C.c_void_p()
+        self._servers = weakref.WeakSet()
+        self._hearts = weakref.WeakSet()
         check(lib().ekg_ctx_create(int(device), C.byref(self.h)))
 
     def close(self):
+        """Closes the context's servers and hearts, then the context."""
         if self.h:
+            for s in list(self._servers):
+                s.close()
+            for h in list(self._hearts):
+                h.close()
             lib().ekg_ctx_destroy(self.h)
             self.h = C.c_void_p()
 
@@ -402,6 +410,7 @@ class Heart:
         n0, n1, n2 = self.layer.shape
         check(lib().ekg_heart_create(ctx.h, n0, n1, n2, _p(self.layer),
                                      None if self.seed is None else _p(self.seed), C.byref(self.h)))
+        ctx._hearts.add(self)
 
     def close(self):
         if self.h:
@@ -555,6 +564,7 @@ class Server:
         o = ServerOpts(int(hbm_budget_bytes), int(max_inflight), int(max_batch), int(n_streams),
                        int(batch_window_us))
         check(lib().ekg_server_create(ctx.h, C.byref(o), C.byref(self.h)))
+        ctx._servers.add(self)
         self._hearts = []
         self._shapes = {}
 

@@ -213,6 +213,7 @@ int ekg_heart_create(ekg_ctx* ctx, int n0, int n1, int n2, const uint8_t* layer,
         try {
             ekg::Heart& h = H->h;
             h.ctx = &ctx->c;
+            h.device = ctx->c.device;
             h.n0 = n0;
             h.n1 = n1;
             h.n2 = n2;
@@ -242,7 +243,7 @@ int ekg_heart_create(ekg_ctx* ctx, int n0, int n1, int n2, const uint8_t* layer,
 void ekg_heart_destroy(ekg_heart* h)
 {
     if (!h) return;
-    (void)hipSetDevice(h->h.ctx->device);
+    (void)hipSetDevice(h->h.device);
     delete h;
 }
 

@@ -838,6 +838,9 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
                    const double* d_pos, double* d_ecg, bool time_fused, bool allow_graph)
 {
     const int B = (int)g.reqs.size();
+    // the launches below report hipGetLastError: drop an error that an
+    // unchecked call (a destroy, a best-effort synchronise) left before them
+    (void)hipGetLastError();
     if (ws.arena) {
         // a fixed workspace is shared by every call on the device: nothing of
         // an earlier group (tables, time grids) can be assumed still there

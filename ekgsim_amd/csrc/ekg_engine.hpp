@@ -121,6 +121,7 @@ struct EcgLayout {
 };
 struct Heart {
     Ctx* ctx = nullptr;
+    int device = 0;  // ctx->device, kept so that the heart can be destroyed after its context
     int n0 = 0, n1 = 0, n2 = 0;
     int64_t nvox = 0, n_myo = 0;
     int max_layer = 0;

@@ -60,6 +60,9 @@ const char *ekg_version(void);
 /* One GPU, one stream.  No reference counterpart (the reference is CPU-only);
  * it replaces the process-wide state of SimLib::EkgSim. */
 int ekg_ctx_create(int device, ekg_ctx **out);
+/* Destroy the context's servers first.  A heart may be destroyed before or
+ * after its context (it frees only its own device memory); no other call on
+ * a heart is valid once its context is destroyed. */
 void ekg_ctx_destroy(ekg_ctx *ctx);
 /* Kernel timing on the context stream: when enabled, every launch of the
  * fused AP+lead kernel is bracketed by HIP events; ekg_ctx_kernel_time

@@ -177,6 +177,31 @@ def test_heart_without_seed_rejected(ctx):
         g2.excite_delay(h.delay)
 
 
+def test_heart_destroyed_after_its_context(ctx):
+    """A heart destroyed after its context frees its memory without touching
+    the context (no stale device error reaches the next launch), and closing a
+    Context closes its hearts first."""
+    from ekgsim_amd import Context, Heart
+    from ekgsim_amd._capi import lib
+    h, act, k, pos = _case(16, 2)
+    ref = O.run(h.layer, act, k, pos, "3D4", 0, 300, 1.0)
+    c1 = Context(0)
+    g1 = Heart(c1, h.layer, h.seed)
+    g1.set_activation(act)
+    g1.simulate(k, pos, "3D4", 0, 300, 1.0)
+    raw, g1.h = g1.h, type(g1.h)()          # keep the handle past Context.close
+    c1.close()
+    lib().ekg_heart_destroy(raw)
+    g2 = Heart(ctx, h.layer, h.seed)
+    g2.set_activation(act)
+    assert rel_l2(g2.simulate(k, pos, "3D4", 0, 300, 1.0), ref) <= TOL
+    c3 = Context(0)
+    g3 = Heart(c3, h.layer, h.seed)
+    c3.close()
+    assert not g3.h
+    g2.close()
+
+
 def test_simulate_before_excitation_rejected(ctx):
     from ekgsim_amd import EkgError
     h, act, k, pos = _case(16, 2)
