@@ -670,24 +670,29 @@ typedef double mfma_f64x4 __attribute__((ext_vector_type(4)));
 // Bit of neighbour offset d in a voxel's 3x3x3 myocardium mask.
 __device__ constexpr int cube_bit(int d0, int d1, int d2) { return (d0 + 1) * 9 + (d1 + 1) * 3 + (d2 + 1); }
 
-constexpr int CHUNK_SEG = 8;  // fixed summation segments of the chunk sums
 constexpr int CH_LEADS = 2;   // leads per pass of the chunk kernel's lead field (divides LP)
 
 // G row of one myocardial voxel u from its own cell and its myocardial
 // neighbours in the dipole neighbourhood N, CH_LEADS leads per pass:
 //   G_i(u) = s(u) (e(u) . s_u) + sum_{d in N, u+d myo} s(u+d) (e(u+d) . d),
-// s_u the sum of those d.  A cell's weight m = e s is a function of the cell
-// alone: s = rsq(|e|^2 rounded to FP32)^3 with |e|^2 summed in FP64 from the
-// exact FP64 offsets e = p - c, which are also the offsets of the dipole
-// products, so the weight's rounding cancels between the two ends of each
-// dipole (and with the class sums, which round their own FP64 |e|^2 to the
-// same FP32 value but where two FP64 roundings straddle an FP32 boundary) and
-// only V differences across the dipole see it (a weight derived per
-// evaluating voxel, e.g. from e(u) - d in FP32, does not cancel: ~1e-5).
+// s_u the sum of those d.  A cell's weight is s = rsq(|e|^2 rounded to
+// FP32)^3 with |e|^2 in FP64; the offsets e = p - c are exact in FP64 (c is an
+// integer coordinate + 1).  The voxel's own |e(u)|^2 is summed from its
+// offsets; a neighbour cell's follows from it and the dipole sum
+// sigma = e(u) . d, which the dipole product needs anyway:
+//   e(u+d) . d = sigma - |d|^2,   |e(u+d)|^2 = |e(u)|^2 + |d|^2 - 2 sigma,
+// one FMA per cell.  The FP64 value differs from the cell's directly summed
+// |e|^2 (its own evaluation, or the class kernel's matrix product) by a few
+// 1e-16 relative, so after rounding to FP32 a cell gets the same weight from
+// every voxel that evaluates it except where those FP64 values straddle an
+// FP32 rounding boundary (a few per 1e8 evaluations, each a one-ulp weight
+// difference on one dipole); the weight's rounding therefore cancels between
+// the two ends of each dipole and only V differences across the dipole see
+// it (a weight derived per evaluating voxel in FP32 arithmetic would not
+// cancel: ~1e-5).
 template <int NH>
 struct VoxelField {
-    float fc[3];          // grid coordinate + 1 per axis (the reference's voxel centre)
-    double cd[3][3];       // coordinate + 1 + (k - 1) per axis and offset k, in FP64
+    double cc[3];          // grid coordinate + 1 per axis (the reference's voxel centre), FP64
     double sd0, sd1, sd2;  // s_u
     uint32_t mk;           // 3x3x3 myocardium mask
 
@@ -707,13 +712,9 @@ struct VoxelField {
         sd0 = (double)i0;
         sd1 = (double)i1;
         sd2 = (double)i2;
-        fc[0] = (float)(gz + 1);
-        fc[1] = (float)(gy + 1);
-        fc[2] = (float)(gx + 1);
-#pragma unroll
-        for (int c3 = 0; c3 < 3; ++c3)
-#pragma unroll
-            for (int k3 = 0; k3 < 3; ++k3) cd[c3][k3] = (double)(fc[c3] + (float)(k3 - 1));
+        cc[0] = (double)(gz + 1);
+        cc[1] = (double)(gy + 1);
+        cc[2] = (double)(gx + 1);
     }
 
     // g[w] = G_{i0+w}(u); leads at or past L give 0.  s_pd[i][c]: the
@@ -724,46 +725,33 @@ struct VoxelField {
 #pragma unroll
         for (int w2 = 0; w2 < CH_LEADS; ++w2) g[w2] = 0.0;
         if (i0 >= L) return;
-        // Per lead, the electrode offset of a cell is p - c per axis, exact
-        // in FP64, and the voxel and its neighbours span only three values of
-        // c per axis: E64[axis][d + 1], and the first term of |e|^2 per
-        // x-offset.  Each cell's weight is then formed from exactly the
-        // operands (and in the order) its own evaluation uses.
-        double E64[CH_LEADS][3][3], SQ0[CH_LEADS][3];
+        double e[CH_LEADS][3], lo[CH_LEADS];
 #pragma unroll
         for (int w2 = 0; w2 < CH_LEADS; ++w2) {
             // leads past the request's count repeat its first lead of the pass
             const int ii = i0 + w2 < L ? i0 + w2 : i0;
 #pragma unroll
-            for (int c3 = 0; c3 < 3; ++c3)
-#pragma unroll
-                for (int k3 = 0; k3 < 3; ++k3) E64[w2][c3][k3] = s_pd[ii][c3] - cd[c3][k3];
-#pragma unroll
-            for (int k3 = 0; k3 < 3; ++k3) SQ0[w2][k3] = E64[w2][0][k3] * E64[w2][0][k3];
-        }
-#pragma unroll
-        for (int w2 = 0; w2 < CH_LEADS; ++w2) {
+            for (int c3 = 0; c3 < 3; ++c3) e[w2][c3] = s_pd[ii][c3] - cc[c3];
+            lo[w2] = __builtin_fma(e[w2][2], e[w2][2], __builtin_fma(e[w2][1], e[w2][1], e[w2][0] * e[w2][0]));
             // own cell: m(u) . s_u
-            const double l = __builtin_fma(E64[w2][2][1], E64[w2][2][1],
-                                           __builtin_fma(E64[w2][1][1], E64[w2][1][1], SQ0[w2][1]));
-            const float r = __builtin_amdgcn_rsqf((float)l);
-            g[w2] = (double)(r * r * r) * (E64[w2][0][1] * sd0 + E64[w2][1][1] * sd1 + E64[w2][2][1] * sd2);
+            const float r = __builtin_amdgcn_rsqf((float)lo[w2]);
+            g[w2] = (double)(r * r * r) * (e[w2][0] * sd0 + e[w2][1] * sd1 + e[w2][2] * sd2);
         }
 #pragma unroll
         for (int qn = 0; qn < N::n; ++qn) {
             if ((mk >> cube_bit(N::d[qn][0], N::d[qn][1], N::d[qn][2])) & 1u) {
-                const int k0 = N::d[qn][0] + 1, k1 = N::d[qn][1] + 1, k2 = N::d[qn][2] + 1;
+                const int d0 = N::d[qn][0], d1 = N::d[qn][1], d2 = N::d[qn][2];
+                const double dd = (double)(d0 * d0 + d1 * d1 + d2 * d2);
 #pragma unroll
                 for (int w2 = 0; w2 < CH_LEADS; ++w2) {
-                    const double l = __builtin_fma(E64[w2][2][k2], E64[w2][2][k2],
-                                                   __builtin_fma(E64[w2][1][k1], E64[w2][1][k1], SQ0[w2][k0]));
+                    // sigma = e(u) . d (d has entries -1, 0, 1)
+                    double sg = 0.0;
+                    if (d0) sg = d0 > 0 ? e[w2][0] : -e[w2][0];
+                    if (d1) sg = d1 > 0 ? sg + e[w2][1] : sg - e[w2][1];
+                    if (d2) sg = d2 > 0 ? sg + e[w2][2] : sg - e[w2][2];
+                    const double l = __builtin_fma(-2.0, sg, lo[w2] + dd);
                     const float r = __builtin_amdgcn_rsqf((float)l);
-                    // d . e in FP64 (d has entries -1, 0, 1)
-                    double de = 0.0;
-                    if (N::d[qn][0]) de += E64[w2][0][k0] * N::d[qn][0];
-                    if (N::d[qn][1]) de += E64[w2][1][k1] * N::d[qn][1];
-                    if (N::d[qn][2]) de += E64[w2][2][k2] * N::d[qn][2];
-                    g[w2] = __builtin_fma((double)(r * r * r), de, g[w2]);
+                    g[w2] = __builtin_fma((double)(r * r * r), sg - dd, g[w2]);
                 }
             }
         }
@@ -785,19 +773,25 @@ struct VoxelField {
 //     predicates at that granularity): before z every voxel is skipped, from
 //     f on every voxel is saturated and the reduction takes the closed form
 //     X_l(t) sum G P' + Y_l(t) sum G Q'.
-//  2. Chunk sums sum G, sum G P', sum G Q' in a fixed order (CHUNK_SEG
-//     strided segments, then the segments in order), to `sums`.
-//  3. Transition sub-tiles [z, f), spread over the waves: the chunk's voxels
-//     split as in ecg_fused_kernel into [0, s1) saturated, [s1, s2) general
-//     and skipped from s2 on, and the lead contraction runs on v_mfma_f64_16x16x4_f64,
-//     D[lead i][step j] += sum_k G[voxel k][lead i] * V[voxel k][step j],
-//     one MFMA per group of 4 voxels below s2.  gfx950 lane layout (checked
-//     by a one-hot probe): A(i,k) in lane 16k+i, B(k,j) in lane 16k+j,
-//     D(i,j) in lane 16(i%4)+j, register i/4.  LDS rows past the chunk end
-//     (to the next multiple of 16) hold zeros and contribute nothing.  Four
-//     accumulator chains per sub-tile (group g into
-//     chain g % 4) keep the matrix cores busy; their sum is taken in a fixed
-//     order.
+//     Per group of 16 voxels (chunk order) wave 0 also finds the quads (4
+//     steps) of [4z, 4f) in which the group is evaluated, [qs, qe): before
+//     qs its earliest voxel is skipped, from qe on its latest is saturated.
+//  2. Group sums sum G, sum G P', sum G Q' (voxels in order) and their
+//     exclusive prefix over the groups in order; the last is the chunk's,
+//     to `sums`.
+//  3. Transition tiles [z, f) of 16 steps, spread over the waves.  In quad
+//     Q the groups saturated by then take the FP64 closed form X_l(t) (group
+//     prefix of sum G P') + Y_l(t) (... sum G Q'); each group with
+//     qs <= Q < qe is evaluated: a lane computes the FP32 AP of one (voxel,
+//     step) pair, S (P X + Q Y), and the lead contraction
+//     D[lead i][step j] += sum_k G[voxel k][lead i] V[voxel k][step j] runs
+//     on v_mfma_f64_4x4x4f64 with the four blocks over the group's voxels
+//     (block b takes voxels 4b .. 4b+3), one product per group of 4 leads.
+//     gfx950 lane layout (checked by one-hot probes): A(block, row, k) in
+//     lane 16k + 4 block + row, B(block, k, col) in lane 16k + 4 block + col,
+//     D(block, row, col) in lane 16 row + 4 block + col.  LDS rows past the
+//     chunk end (to the next multiple of 16) hold zeros and contribute
+//     nothing; the blocks' sums are added in a fixed order.
 // The host keeps every chunk of a group at most NMAX voxels.
 template <int LP, int NH, int NMAX>
 __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
@@ -811,7 +805,8 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
     __shared__ double s_G[NMAX * LP];       // G rows of the chunk's voxels
     __shared__ float4 s_aux[NMAX];          // {P, Q, a_hi, a_lo} relative to the chunk's t_ref
     __shared__ double2 s_pq[NMAX];          // {P', Q'} relative to the layer's t0
-    __shared__ double s_part[CHUNK_SEG][3 * LP];
+    __shared__ double s_grp[NMAX / 16 + 1][3 * LP];  // group sums: exclusive prefix over the groups
+    __shared__ int2 s_qb[NMAX / 16];        // per group: first evaluated and first saturated quad
     __shared__ double s_pd[16][3];          // electrode in FP64
     __shared__ int s_tb[2];
     const int t = threadIdx.x, lane = t & 63;
@@ -833,6 +828,7 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
     const HeartDev& H = hearts[R.heart];
     const int4 ch = H.chunks[c];
     const int l = ch.x, v0 = ch.y, n = ch.z - ch.y;
+    const int ng = (n + 15) >> 4;  // groups of 16 voxels
     const double* kk = layer_k + R.k_off + 9 * (l - 1);
     const LayerConst kc = lc[R.lay_off + l - 1];
     const double t0 = H.layer_t0[l];
@@ -905,128 +901,139 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
                     return !(kc.c1 * tau_of(tl.x, tl.y, a_min.x, a_min.y) < SKIP_X);
                 });
         }
+        const int zf = z < f ? z : f;
         if (lane == 0) {
-            s_tb[0] = z < f ? z : f;
+            s_tb[0] = zf;
             s_tb[1] = f;
-            bnd[R.c_off + c] = int2{z < f ? z : f, f};
+            bnd[R.c_off + c] = int2{zf, f};
+        }
+        // per group of 16 voxels (lane g): the quads of [4 zf, 4 f) in which
+        // it is evaluated, [qs, qe).  qs - 4 zf counts the quads whose last
+        // step still skips the group's earliest voxel, qe - 4 zf those whose
+        // first step does not yet saturate its latest voxel (the tile
+        // predicates above per group; both hold on a prefix of the quads).
+        if (lane < ng) {
+            const float2 af = H.vox_a[v0 + 16 * lane];
+            const double al = H.vox_a64[v0 + (16 * lane + 15 < n ? 16 * lane + 15 : n - 1)];
+            int qs = 4 * zf, qe = 4 * zf;
+            for (int Q = 4 * zf; Q < 4 * f; ++Q) {
+                const float2 tl = th[4 * Q + 3];
+                qs += (skip_ok && !(kc.c1 * tau_of(tl.x, tl.y, af.x, af.y) < SKIP_X)) ? 1 : 0;
+                qe += (sat_ok && c1d * (tg[4 * Q] - al) <= SATC_X) ? 0 : 1;
+            }
+            s_qb[lane] = int2{qs, qe};
         }
     }
     __syncthreads();
 
-    // 2. chunk sums: lead i = t % 16, segment t / 16 takes voxels seg, seg + CHUNK_SEG, ...
-    {
-        const int i = t & 15, sg = t >> 4;
-        if (sg < CHUNK_SEG && i < LP) {
-            double sG = 0.0, sP = 0.0, sQ = 0.0;
-            int q = sg;
-            // four rows' loads in flight, summed in row order
-            for (; q + 3 * CHUNK_SEG < n; q += 4 * CHUNK_SEG) {
-                double gq[4];
-                double2 pq[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    gq[u] = s_G[(q + u * CHUNK_SEG) * LP + i];
-                    pq[u] = s_pq[q + u * CHUNK_SEG];
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    sG += gq[u];
-                    sP = __builtin_fma(gq[u], pq[u].x, sP);
-                    sQ = __builtin_fma(gq[u], pq[u].y, sQ);
-                }
-            }
-            for (; q < n; q += CHUNK_SEG) {
-                const double g = s_G[q * LP + i];
-                const double2 pq = s_pq[q];
-                sG += g;
-                sP = __builtin_fma(g, pq.x, sP);
-                sQ = __builtin_fma(g, pq.y, sQ);
-            }
-            s_part[sg][i] = sG;
-            s_part[sg][LP + i] = sP;
-            s_part[sg][2 * LP + i] = sQ;
+    // 2. group and chunk sums.  Per group of 16 voxels (chunk order) and lead:
+    // sum G, sum G P', sum G Q' over the group's voxels in order; then per
+    // (sum, lead) the exclusive prefix over the groups in order: s_grp[g]
+    // holds the sums of groups 0 .. g-1, s_grp[ng] the chunk's.
+    for (int u = t; u < ng * LP; u += blockDim.x) {
+        const int g = u / LP, i = u - g * LP;
+        const int qa = 16 * g, qz = qa + 16 < n ? qa + 16 : n;
+        double sG = 0.0, sP = 0.0, sQ = 0.0;
+        for (int q = qa; q < qz; ++q) {
+            const double gq = s_G[q * LP + i];
+            const double2 pq = s_pq[q];
+            sG += gq;
+            sP = __builtin_fma(gq, pq.x, sP);
+            sQ = __builtin_fma(gq, pq.y, sQ);
         }
+        s_grp[g][i] = sG;
+        s_grp[g][LP + i] = sP;
+        s_grp[g][2 * LP + i] = sQ;
     }
     __syncthreads();
     if (t < 3 * LP) {
-        double sum = 0.0;
-#pragma unroll
-        for (int sg = 0; sg < CHUNK_SEG; ++sg) sum += s_part[sg][t];
-        sums[(R.c_off + c) * 3 * LP + t] = sum;
+        double run = 0.0;
+        for (int g = 0; g < ng; ++g) {
+            const double x = s_grp[g][t];
+            s_grp[g][t] = run;
+            run += x;
+        }
+        s_grp[ng][t] = run;
+        sums[(R.c_off + c) * 3 * LP + t] = run;
     }
     __syncthreads();
 
-    // 3. transition tiles [z, f)
+    // 3. transition tiles [z, f), 16 steps each, spread over the waves.  A
+    // tile is four quads of 4 steps; in quad Q group g is skipped (Q < qs),
+    // evaluated (qs <= Q < qe) or saturated (Q >= qe), and the saturated
+    // groups 0 .. nsq(Q)-1 take the FP64 closed form X_l s_grp[nsq].P' +
+    // Y_l s_grp[nsq].Q'.  Evaluated groups run on v_mfma_f64_4x4x4f64 with
+    // the blocks over the group's voxels: block b, inner index k holds voxel
+    // 4b + k, so a lane computes the AP of one (voxel, step) pair of the quad
+    // and feeds it to the LP/4 products of the lead groups; the blocks'
+    // partial sums are added in block order.
     const int tz = s_tb[0], tf = s_tb[1];
-    if (tf - tz <= w) return;  // no sub-tile for this wave
+    if (tf - tz <= w) return;  // no tile for this wave
     const double ex = exp(k45 * (tref - t0)), ey = exp(k5 * (tref - t0));
-    const int jl = lane & 15;   // step column
-    const int kv = lane >> 4;   // voxel slot of the B operand / lead group of D
-    const int ia = lane & 15;   // lead of the A operand
+    constexpr int NLG = LP / 4;
+    const int col = lane & 3;                             // step of the quad; lead row of the A operand
+    const int vs = 4 * ((lane >> 2) & 3) + (lane >> 4);   // voxel of the group (A and B operands)
+    const int2 qb = lane < ng ? s_qb[lane] : int2{0x7fffffff, 0x7fffffff};
     const double2* XY = xy + R.xy_off + (int64_t)(l - 1) * R.Tp;
     double* out = partial + R.p_off + (int64_t)c * LP * R.Tp;
-
-    // 16-step sub-tiles [z, f) over the waves, each wave owning whole
-    // accumulators.  Every one of them has work: before f the chunk's latest
-    // voxel is not saturated, from z on its earliest is not skipped.
     for (int sb = tz + w; sb < tf; sb += nw) {
-        const int j0 = sb * BT;
-        const int j = j0 + jl;
-        const double2 xyj = XY[j];
-        const float X = (float)fmin(xyj.x * ex, XMAX);
-        const float Y = (float)fmin(xyj.y * ey, XMAX);
-        const float2 tt = th[j];
-        int s1 = 0, s2 = n;
-        if (kc.c1 < 0.0f) {
-            const float2 tf2 = th[j0], tl2 = th[j0 + 15];
-            s1 = first_false(0, n, [&](int q) {
-                const float4 a4 = s_aux[q];
-                return kc.c1 * tau_of(tf2.x, tf2.y, a4.z, a4.w) <= SAT_X;
-            });
-            if (kc.skip != 0.0f)
-                s2 = first_false(s1, n, [&](int q) {
-                    const float4 a4 = s_aux[q];
-                    return kc.c1 * tau_of(tl2.x, tl2.y, a4.z, a4.w) < SKIP_X;
-                });
+        const int Qt = 4 * sb, j0 = 16 * sb;
+        float Xq[4], Yq[4], Th[4], Tl[4];
+#pragma unroll
+        for (int qd = 0; qd < 4; ++qd) {
+            const int j = j0 + 4 * qd + col;
+            const double2 xyj = XY[j];
+            Xq[qd] = (float)fmin(xyj.x * ex, XMAX);
+            Yq[qd] = (float)fmin(xyj.y * ey, XMAX);
+            const float2 tt = th[j];
+            Th[qd] = tt.x;
+            Tl[qd] = tt.y;
         }
-        // s1, s2 are wave-uniform (ballot searches): scalar loop control
-        const int s1u = __builtin_amdgcn_readfirstlane(s1), s2u = __builtin_amdgcn_readfirstlane(s2);
-        // four independent accumulator chains (group g goes to chain g % 4),
-        // summed in a fixed order at the end; each block of 4 groups loads its
-        // operands first, then evaluates and contracts.  Rows past the chunk
-        // end up to the next multiple of 16 are zero (G = P = Q = 0), and
-        // lanes of A rows i >= LP repeat lead LP - 1: rows of D past the lead
-        // count are never stored, so neither needs a select.
-        mfma_f64x4 acc[4];
+        // groups saturated from the tile's first quad on, and groups not
+        // skipped in its last quad: both prefixes of the group order
+        const int g_lo = __popcll(__ballot(qb.y <= Qt));
+        const int g_hi = __popcll(__ballot(qb.x < Qt + 4));
+        double acc[4][NLG];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc[u] = mfma_f64x4{0.0, 0.0, 0.0, 0.0};
-        const int ic = ia < LP ? ia : LP - 1;
-        const double* gcol = s_G + kv * LP + ic;
-        const float4* acol = s_aux + kv;
-        for (int vb = 0; vb < s2u; vb += 16) {
-            double gv[4];
-            float4 qv[4];
+        for (int qd = 0; qd < 4; ++qd)
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                gv[u] = gcol[(vb + 4 * u) * LP];
-                qv[u] = acol[vb + 4 * u];
-            }
+            for (int lg = 0; lg < NLG; ++lg) acc[qd][lg] = 0.0;
+        for (int g = g_lo; g < g_hi; ++g) {
+            const int qs = __builtin_amdgcn_readlane(qb.x, g), qe = __builtin_amdgcn_readlane(qb.y, g);
+            const int v = 16 * g + vs;
+            double ga[NLG];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int vg = vb + 4 * u;
-                if (vg < s2u) {
-                    float ap = __builtin_fmaf(qv[u].x, X, qv[u].y * Y);
-                    if (vg + 4 > s1u)
-                        ap *= fast_rcp(1.0f + fast_exp2(kc.c1 * tau_of(tt.x, tt.y, qv[u].z, qv[u].w)));
-                    acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(gv[u], (double)ap, acc[u], 0, 0, 0);
+            for (int lg = 0; lg < NLG; ++lg) ga[lg] = s_G[v * LP + 4 * lg + col];
+            const float4 ax = s_aux[v];
+#pragma unroll
+            for (int qd = 0; qd < 4; ++qd) {
+                if (Qt + qd >= qs && Qt + qd < qe) {
+                    float ap = __builtin_fmaf(ax.x, Xq[qd], ax.y * Yq[qd]);
+                    ap *= fast_rcp(1.0f + fast_exp2(kc.c1 * tau_of(Th[qd], Tl[qd], ax.z, ax.w)));
+                    const double bv = (double)ap;
+#pragma unroll
+                    for (int lg = 0; lg < NLG; ++lg)
+                        acc[qd][lg] = __builtin_amdgcn_mfma_f64_4x4x4f64(ga[lg], bv, acc[qd][lg], 0, 0, 0);
                 }
             }
         }
-        // D(i, j) sits in lane 16(i%4)+j, register i/4: lead i = 4q + lane/16
+        // D(block, row, col) in lane 16 row + 4 block + col: lead 4 lg + row,
+        // step col of the quad; blocks (0 + 1) + (2 + 3)
 #pragma unroll
         for (int qd = 0; qd < 4; ++qd) {
-            const int i = 4 * qd + kv;
-            if (i < R.L && i < LP) out[(int64_t)i * R.Tp + j] = (acc[0][qd] + acc[1][qd]) + (acc[2][qd] + acc[3][qd]);
+            const int nsq = __popcll(__ballot(qb.y <= Qt + qd));
+            const int j = j0 + 4 * qd + col;
+            const double2 xyj = XY[j];
+            const double* cp = s_grp[nsq];
+#pragma unroll
+            for (int lg = 0; lg < NLG; ++lg) {
+                double x = acc[qd][lg];
+                x += __shfl_xor(x, 4, 64);
+                x += __shfl_xor(x, 8, 64);
+                const int i = 4 * lg + (lane >> 4);
+                if ((lane & 12) == 0 && i < R.L)
+                    out[(int64_t)i * R.Tp + j] = __builtin_fma(xyj.y, cp[2 * LP + i], __builtin_fma(xyj.x, cp[LP + i], x));
+            }
         }
     }
 }
