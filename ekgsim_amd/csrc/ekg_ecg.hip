@@ -769,17 +769,17 @@ struct VoxelField {
 //     FP64 sum of the exact products
 //       G_i(u) = s(u) (e(u) . s_u) + sum_{d in N, u+d myo} s(u+d) (e(u+d) . d)
 //     into LDS, with the voxel's {P, Q, a} and FP64 {P', Q'}.  Wave 0 then
-//     finds the bounds {z, f} in 16-step bound tiles (the chunk_sums_kernel
+//     finds the bounds {z, f} in bound tiles of BT steps (the chunk_sums_kernel
 //     predicates at that granularity): before z every voxel is skipped, from
 //     f on every voxel is saturated and the reduction takes the closed form
 //     X_l(t) sum G P' + Y_l(t) sum G Q'.
 //     Per group of 16 voxels (chunk order) wave 0 also finds the quads (4
-//     steps) of [4z, 4f) in which the group is evaluated, [qs, qe): before
+//     steps) of the tiles [z, f) in which the group is evaluated, [qs, qe): before
 //     qs its earliest voxel is skipped, from qe on its latest is saturated.
 //  2. Group sums sum G, sum G P', sum G Q' (voxels in order) and their
 //     exclusive prefix over the groups in order; the last is the chunk's,
 //     to `sums`.
-//  3. Transition tiles [z, f) of 16 steps, spread over the waves.  In quad
+//  3. Transition tiles [z, f) of BT steps, spread over the waves.  In quad
 //     Q the groups saturated by then take the FP64 closed form X_l(t) (group
 //     prefix of sum G P') + Y_l(t) (... sum G Q'); each group with
 //     qs <= Q < qe is evaluated: a lane computes the FP32 AP of one (voxel,
@@ -838,17 +838,28 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
     const double* tg = tgrid + R.t_off;
     const float2* th = thl + R.t_off;
     if (t < 3 * R.L) s_pd[t / 3][t % 3] = lead_pos[R.pos_off + t];
-    // operands of the bound-tile tests (16-step sub-tiles), loaded now and
-    // used after the G rows: one probe per lane when there are at most 64
+    // operands of the bound-tile tests (tiles of BT steps), loaded now and
+    // used after the G rows: up to NPROBE probes per lane (64 NPROBE tiles)
     constexpr int BT = 1 << BT_SHIFT_MFMA;
+    constexpr int QPT = BT / 4;  // quads (4 steps) per tile
+    constexpr int NPROBE = BT >= 8 ? 2 : 4;
     const int nbt = R.Tp >> BT_SHIFT_MFMA;
     const double a_max = H.vox_a64[ch.z - 1];
     const float2 a_min = H.vox_a[ch.y];
-    double tg_probe = 0.0;
-    float2 th_probe = float2{0.0f, 0.0f};
-    if (w == 0 && lane < nbt) {
-        tg_probe = tg[lane * BT];
-        th_probe = th[lane * BT + BT - 1];
+    double tg_probe[NPROBE];
+    float2 th_probe[NPROBE];
+#pragma unroll
+    for (int k = 0; k < NPROBE; ++k) {
+        tg_probe[k] = 0.0;
+        th_probe[k] = float2{0.0f, 0.0f};
+    }
+    if (w == 0) {
+#pragma unroll
+        for (int k = 0; k < NPROBE; ++k)
+            if (lane + 64 * k < nbt) {
+                tg_probe[k] = tg[(lane + 64 * k) * BT];
+                th_probe[k] = th[(lane + 64 * k) * BT + BT - 1];
+            }
     }
     // factors from the layer's t0 to the chunk's t_ref: P = P' e^{(k4+k5)(t0 - t_ref)}, Q = Q' e^{k5 (t0 - t_ref)}
     const double eP = exp(k45 * (t0 - tref)), eQ = exp(k5 * (t0 - tref));
@@ -880,18 +891,25 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
     }
     if (w == 0) {
         // tile bounds (the same predicates as chunk_sums_kernel), needed from
-        // the transition tiles on: one ballot over the prefetched probes, or
-        // the wave-parallel search for requests with more than 64 tiles
+        // the transition tiles on: ballots over the prefetched probes, or
+        // the wave-parallel search for requests with more tiles
         const bool sat_ok = c1d < 0.0, skip_ok = kc.c1 < 0.0f && kc.skip != 0.0f;
         int f = nbt, z = 0;
-        if (nbt <= 64) {
-            const bool fp = lane < nbt && sat_ok && c1d * (tg_probe - a_max) <= SATC_X;
-            const bool zp = lane < nbt && skip_ok &&
-                            kc.c1 * tau_of(th_probe.x, th_probe.y, a_min.x, a_min.y) < SKIP_X;
+        if (nbt <= 64 * NPROBE) {
             // both predicates are monotone in the tile: the count of false
             // probes is the first true tile
-            if (sat_ok) f = nbt - __popcll(__ballot(fp));
-            if (skip_ok) z = nbt - __popcll(__ballot(zp));
+            int cf = 0, cz = 0;
+#pragma unroll
+            for (int k = 0; k < NPROBE; ++k) {
+                const bool in = lane + 64 * k < nbt;
+                const bool fp = in && sat_ok && c1d * (tg_probe[k] - a_max) <= SATC_X;
+                const bool zp = in && skip_ok &&
+                                kc.c1 * tau_of(th_probe[k].x, th_probe[k].y, a_min.x, a_min.y) < SKIP_X;
+                cf += __popcll(__ballot(fp));
+                cz += __popcll(__ballot(zp));
+            }
+            if (sat_ok) f = nbt - cf;
+            if (skip_ok) z = nbt - cz;
         } else {
             if (sat_ok)
                 f = first_false(0, nbt, [&](int tile) { return !(c1d * (tg[tile * BT] - a_max) <= SATC_X); });
@@ -907,16 +925,16 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
             s_tb[1] = f;
             bnd[R.c_off + c] = int2{zf, f};
         }
-        // per group of 16 voxels (lane g): the quads of [4 zf, 4 f) in which
-        // it is evaluated, [qs, qe).  qs - 4 zf counts the quads whose last
-        // step still skips the group's earliest voxel, qe - 4 zf those whose
-        // first step does not yet saturate its latest voxel (the tile
+        // per group of 16 voxels (lane g): the quads of the tiles [zf, f) in
+        // which it is evaluated, [qs, qe).  qs - QPT zf counts the quads whose
+        // last step still skips the group's earliest voxel, qe - QPT zf those
+        // whose first step does not yet saturate its latest voxel (the tile
         // predicates above per group; both hold on a prefix of the quads).
         if (lane < ng) {
             const float2 af = H.vox_a[v0 + 16 * lane];
             const double al = H.vox_a64[v0 + (16 * lane + 15 < n ? 16 * lane + 15 : n - 1)];
-            int qs = 4 * zf, qe = 4 * zf;
-            for (int Q = 4 * zf; Q < 4 * f; ++Q) {
+            int qs = QPT * zf, qe = QPT * zf;
+            for (int Q = QPT * zf; Q < QPT * f; ++Q) {
                 const float2 tl = th[4 * Q + 3];
                 qs += (skip_ok && !(kc.c1 * tau_of(tl.x, tl.y, af.x, af.y) < SKIP_X)) ? 1 : 0;
                 qe += (sat_ok && c1d * (tg[4 * Q] - al) <= SATC_X) ? 0 : 1;
@@ -958,8 +976,8 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
     }
     __syncthreads();
 
-    // 3. transition tiles [z, f), 16 steps each, spread over the waves.  A
-    // tile is four quads of 4 steps; in quad Q group g is skipped (Q < qs),
+    // 3. transition tiles [z, f) of BT steps, spread over the waves.  A
+    // tile is QPT quads of 4 steps; in quad Q group g is skipped (Q < qs),
     // evaluated (qs <= Q < qe) or saturated (Q >= qe), and the saturated
     // groups 0 .. nsq(Q)-1 take the FP64 closed form X_l s_grp[nsq].P' +
     // Y_l s_grp[nsq].Q'.  Evaluated groups run on v_mfma_f64_4x4x4f64 with
@@ -977,10 +995,10 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
     const double2* XY = xy + R.xy_off + (int64_t)(l - 1) * R.Tp;
     double* out = partial + R.p_off + (int64_t)c * LP * R.Tp;
     for (int sb = tz + w; sb < tf; sb += nw) {
-        const int Qt = 4 * sb, j0 = 16 * sb;
-        float Xq[4], Yq[4], Th[4], Tl[4];
+        const int Qt = QPT * sb, j0 = BT * sb;
+        float Xq[QPT], Yq[QPT], Th[QPT], Tl[QPT];
 #pragma unroll
-        for (int qd = 0; qd < 4; ++qd) {
+        for (int qd = 0; qd < QPT; ++qd) {
             const int j = j0 + 4 * qd + col;
             const double2 xyj = XY[j];
             Xq[qd] = (float)fmin(xyj.x * ex, XMAX);
@@ -992,10 +1010,10 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
         // groups saturated from the tile's first quad on, and groups not
         // skipped in its last quad: both prefixes of the group order
         const int g_lo = __popcll(__ballot(qb.y <= Qt));
-        const int g_hi = __popcll(__ballot(qb.x < Qt + 4));
-        double acc[4][NLG];
+        const int g_hi = __popcll(__ballot(qb.x < Qt + QPT));
+        double acc[QPT][NLG];
 #pragma unroll
-        for (int qd = 0; qd < 4; ++qd)
+        for (int qd = 0; qd < QPT; ++qd)
 #pragma unroll
             for (int lg = 0; lg < NLG; ++lg) acc[qd][lg] = 0.0;
         for (int g = g_lo; g < g_hi; ++g) {
@@ -1006,7 +1024,7 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
             for (int lg = 0; lg < NLG; ++lg) ga[lg] = s_G[v * LP + 4 * lg + col];
             const float4 ax = s_aux[v];
 #pragma unroll
-            for (int qd = 0; qd < 4; ++qd) {
+            for (int qd = 0; qd < QPT; ++qd) {
                 if (Qt + qd >= qs && Qt + qd < qe) {
                     float ap = __builtin_fmaf(ax.x, Xq[qd], ax.y * Yq[qd]);
                     ap *= fast_rcp(1.0f + fast_exp2(kc.c1 * tau_of(Th[qd], Tl[qd], ax.z, ax.w)));
@@ -1020,7 +1038,7 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
         // D(block, row, col) in lane 16 row + 4 block + col: lead 4 lg + row,
         // step col of the quad; blocks (0 + 1) + (2 + 3)
 #pragma unroll
-        for (int qd = 0; qd < 4; ++qd) {
+        for (int qd = 0; qd < QPT; ++qd) {
             const int nsq = __popcll(__ballot(qb.y <= Qt + qd));
             const int j = j0 + 4 * qd + col;
             const double2 xyj = XY[j];

@@ -17,10 +17,11 @@ constexpr int MFMA_PAD = 8;
 constexpr int CHUNK_MAX = 512;
 
 // Bound tiles: the chunk bounds {z, f}, the prefix tables and the reduction
-// work in tiles of 2^BT_SHIFT steps -- 16 on the matrix-core path (its
-// transition work goes by 16-step sub-tiles), 64 on the vector-ALU path (one
-// step per lane of a wave).
-constexpr int BT_SHIFT_MFMA = 4;
+// work in tiles of 2^BT_SHIFT steps -- 8 on the matrix-core path (its
+// transition tiles are two 4-step quads; measured at 128^3 x 256 requests
+// without classes: 16-step tiles 6.29 ms, 8 5.88 ms, 4 6.40 ms per chunk-kernel
+// launch), 64 on the vector-ALU path (one step per lane of a wave).
+constexpr int BT_SHIFT_MFMA = 3;
 constexpr int BT_SHIFT_VALU = 6;
 // Activation classes per class window, and groups of 16 window entries per
 // piece (one class-window block per piece and request).  Six classes keep the
