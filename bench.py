@@ -77,16 +77,17 @@ def lead_field_cells(layer):
 #   per (class cell, lead): 8 FP64 matrix-core flops (an FMA for each of the
 #     coefficients D0, D1, D2, K).
 # Chunk kernel (dominant without classes):
-#   per lead-weight evaluation (voxel, lead, cell): 8 FP64 flops (|e|^2 2
-#     FMAs, d . e and its FMA into G) and 3 FP32 flops (rsq 1, r^3 2);
-#   per window voxel-step (the transition tiles): 2L FP64 MFMA flops and 14
-#     FP32 ops (10 AP assembly + 4 transcendental).
+#   per lead-weight evaluation (voxel, lead, cell): 6 FP64 flops (|e|^2 as
+#     one FMA from the voxel's own |e|^2 and the dipole sum, the dipole
+#     offset, the FMA into G) and 3 FP32 flops (rsq 1, r^3 2);
+#   per window voxel-step (the transition quads): 2L FP64 MFMA flops and 11
+#     FP32 ops (9 AP assembly + 2 transcendental).
 # Both count at full weight against 78.6 TFLOP/s: the build disables packed
 # FP32, so an FP32 vector operation issues at the FP64 vector rate, which on
 # MI355X equals the FP64 matrix rate.
 FP32_PER_CLASS_ENTRY, FP64_PER_CLASS_ENTRY, FP64_PER_CLASS_CELL = 3, 8, 8
-FP32_PER_CELL, FP64_PER_CELL = 3, 8
-FP32_PER_WINDOW_STEP = 14
+FP32_PER_CELL, FP64_PER_CELL = 3, 6
+FP32_PER_WINDOW_STEP = 11
 # Naive per-voxel-step model: every voxel at every step, 38 VALU lane-cycles
 # at L = 12, against 256 CU x 64 lanes x 2.4 GHz.
 NAIVE_LANE_CYCLES, LANE_PEAK = 38, 256 * 64 * 2.4e9
@@ -96,7 +97,7 @@ NAIVE_LANE_CYCLES, LANE_PEAK = 38, 256 * 64 * 2.4e9
 # 256 MiB calibration copy: with activation classes
 # (profiles/ecg_class_window_kernel_128cubed_batch256_hbm_traffic.md) and
 # without (profiles/ecg_chunk_kernel_128cubed_batch256_hbm_traffic.md).
-TRAFFIC_BYTES = {"ecg_class_window_kernel": 456175974, "ecg_chunk_kernel": 1396132997}
+TRAFFIC_BYTES = {"ecg_class_window_kernel": 456175974, "ecg_chunk_kernel": 1130362818}
 # Issue-pipe occupancy of the dominant kernel on this workload, from one
 # rocprofv3 --pmc pass (SQ_VALU_MFMA_BUSY_CYCLES, SQ_INSTS_VALU minus the MFMA
 # instructions at 4 cycles each, SQ_VALU_MFMA_COEXEC_CYCLES, GRBM_GUI_ACTIVE)
@@ -105,7 +106,7 @@ TRAFFIC_BYTES = {"ecg_class_window_kernel": 456175974, "ecg_chunk_kernel": 13961
 # and the vector ALU never overlap (COEXEC 0), so their sum is the fraction of
 # the kernel's cycles spent issuing.
 PIPES = {"ecg_class_window_kernel": {"mfma_busy": 0.548, "valu_issue": 0.297, "coexec": 0.0},
-         "ecg_chunk_kernel": {"mfma_busy": 0.194, "valu_issue": 0.720, "coexec": 0.0}}
+         "ecg_chunk_kernel": {"mfma_busy": 0.112, "valu_issue": 0.810, "coexec": 0.0}}
 
 
 def log(*a):
