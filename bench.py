@@ -88,6 +88,9 @@ def lead_field_cells(layer):
 FP32_PER_CLASS_ENTRY, FP64_PER_CLASS_ENTRY, FP64_PER_CLASS_CELL = 3, 8, 8
 FP32_PER_CELL, FP64_PER_CELL = 3, 6
 FP32_PER_WINDOW_STEP = 11
+# Serving bursts: warm-up bursts before the measured ones, and measured bursts
+# (the median is reported).
+SERVE_WARM, SERVE_MEASURED = 5, 3
 # Naive per-voxel-step model: every voxel at every step, 38 VALU lane-cycles
 # at L = 12, against 256 CU x 64 lanes x 2.4 GHz.
 NAIVE_LANE_CYCLES, LANE_PEAK = 38, 256 * 64 * 2.4e9
@@ -577,20 +580,24 @@ def main():
             x = S.random_params(3, nl, 5000 + r)
             reqs.append((hid, S.layer_k_from_params(x, 3),
                          E.move_leads(S.lead_positions(g, nl), S.displacements(x, 3, nl)), dur))
-        # Poisson arrivals at a rate above the service rate (saturating); the
-        # first bursts size the server's workspaces and pinned buffers, the
-        # last is the one reported (steady state)
+        # Poisson arrivals at a rate above the service rate (saturating).  The
+        # server's workspaces and pinned buffers grow over the first few
+        # bursts (group compositions differ from burst to burst); after
+        # SERVE_WARM of them, the median of SERVE_MEASURED bursts is reported
+        # (steady state)
         sreq = [(hid, k, p, 0.0, dur, 1.0) for hid, k, p, dur in reqs]
         first = poisson_burst(srv, sreq, 4.0 * value, 6)[0]
-        poisson_burst(srv, sreq, 4.0 * value, 8)
+        for sd in range(SERVE_WARM - 1):
+            poisson_burst(srv, sreq, 4.0 * value, 20 + sd)
         g0 = srv.stats()["groups"]
-        rate_m, lats, _ = poisson_burst(srv, sreq, 4.0 * value, 7)
+        runs = [poisson_burst(srv, sreq, 4.0 * value, 7 + sd) for sd in range(SERVE_MEASURED)]
         st = srv.stats()
         srv.close()
+        rate_m, lats, _ = sorted(runs, key=lambda x: x[0])[SERVE_MEASURED // 2]
         serve = {"requests": len(reqs), "ecgs_per_s": rate_m,
                  "p50_ms": float(np.percentile(lats, 50)), "p99_ms": float(np.percentile(lats, 99)),
-                 "groups": st["groups"] - g0, "peak_inflight": st["peak_inflight"],
-                 "first_burst_ecgs_per_s": first}
+                 "groups": (st["groups"] - g0) / SERVE_MEASURED, "peak_inflight": st["peak_inflight"],
+                 "bursts_ecgs_per_s": [x[0] for x in runs], "first_burst_ecgs_per_s": first}
         log(f"serving: {serve}")
 
     # ---- serving on the batch's own heart and shape, like for like with the
@@ -601,17 +608,19 @@ def main():
         hid = srv.add_heart(heart)
         sreqs = [(hid, ks[r % B], pos[r % B], 0.0, args.length, 1.0) for r in range(2 * B)]
         first = poisson_burst(srv, sreqs, 4.0 * value, 9)[0]
-        poisson_burst(srv, sreqs, 4.0 * value, 10)
+        for sd in range(SERVE_WARM - 1):
+            poisson_burst(srv, sreqs, 4.0 * value, 30 + sd)
         g0 = srv.stats()["groups"]
         chk = (0, 1, B - 1, B, 2 * B - 1)
-        rate_s, lats_s, outs = poisson_burst(srv, sreqs, 4.0 * value, 11, keep=chk)
+        runs = [poisson_burst(srv, sreqs, 4.0 * value, 11 + sd, keep=chk) for sd in range(SERVE_MEASURED)]
         st = srv.stats()
         srv.close()
-        same_bits = all(np.array_equal(outs[r], ecg_gpu[r % B]) for r in chk)
+        same_bits = all(np.array_equal(outs[r], ecg_gpu[r % B]) for _, _, outs in runs for r in chk)
+        rate_s, lats_s, _ = sorted(runs, key=lambda x: x[0])[SERVE_MEASURED // 2]
         same = {"requests": len(sreqs), "ecgs_per_s": rate_s, "p50_ms": float(np.percentile(lats_s, 50)),
-                "p99_ms": float(np.percentile(lats_s, 99)), "groups": st["groups"] - g0,
+                "p99_ms": float(np.percentile(lats_s, 99)), "groups": (st["groups"] - g0) / SERVE_MEASURED,
                 "vs_plain": rate_s / plain, "served_equal_batched_bitwise": same_bits,
-                "first_burst_ecgs_per_s": first}
+                "bursts_ecgs_per_s": [x[0] for x in runs], "first_burst_ecgs_per_s": first}
         log(f"serving, one heart and shape: {same}")
 
     # ---- the reference's shipped 24-layer heart (FP64 conduction: distinct
