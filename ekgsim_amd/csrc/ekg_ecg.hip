@@ -65,9 +65,10 @@ __device__ __forceinline__ int find_req(const int64_t* __restrict__ start, int B
 //   x <= SAT_X for every lane   -> exp2(x) < 2^-24, 1 + exp2(x) rounds to 1,
 //                                  S = rcp(1) = 1 exactly in FP32: no sigmoid;
 //   x <= SATC_X (FP64 test on   -> every voxel's S is within 2^-24.5 of 1, as
-//   the latest voxel of a          in the FP32 AP: the chunk or sub-tile takes
-//   chunk at a tile's first        the exact FP64 closed form X_l sum G P' +
-//   step)                          Y_l sum G Q' instead of FP32 evaluation.
+//   the latest voxel of a          in the FP32 AP: the chunk (or group of 16
+//   chunk or group at a tile's     voxels in a quad) takes the exact FP64
+//   or quad's first step)          closed form X_l sum G P' + Y_l sum G Q'
+//                                  instead of FP32 evaluation.
 constexpr float SKIP_X = 48.0f;
 constexpr float SAT_X = -24.5f;
 constexpr double SATC_X = -24.5;
