@@ -321,3 +321,30 @@ def test_two_voxel_and_single_layer_hearts(ctx, nhood):
     g1.excite_delay(h1.delay)
     assert rel_l2(g1.simulate(k1, pos1, nhood, 0.0, 300.0, 1.0), ref1) <= TOL
     g1.close()
+
+
+@pytest.mark.parametrize("nhood", ["3D4", "3D8"])
+@pytest.mark.parametrize("edge", ["slow_fast_noskip", "flat_sigmoid"])
+def test_edge_layer_coefficients(ctx, nhood, edge):
+    """Layer coefficients at the edges of the AP paths' shortcuts, classes on
+    and off: a slow upstroke (k1 0.3: windows of ~170 steps, many transition
+    tiles), k4 + k5 > k1/4 (no voxel is skipped before its upstroke) and a
+    one-step upstroke (k1 40); or k1 = 0 (S = 1/2 for every voxel at every
+    step: no tile is ever skipped or saturated)."""
+    h, act, k, pos = _case(24, 3, seed=5, pseed=77)
+    k = k.copy()
+    if edge == "slow_fast_noskip":
+        k[0, 1] = 0.3
+        k[1, 4] = k[1, 1]
+        k[2, 1] = 40.0
+    else:
+        k[:, 1] = 0.0
+    ref = O.run(h.layer, act, k, pos, nhood, 0.0, 600.0, 1.0)
+    g = _heart(ctx, h)
+    g.set_activation(act)
+    for cm in (0, -1):
+        g.set_tune(class_min=cm)
+        err = rel_l2(g.simulate(k, pos, nhood, 0.0, 600.0, 1.0), ref)
+        print(f"{edge} {nhood} class_min {cm}: rel-L2 {err:.3e}")
+        assert err <= TOL, cm
+    g.close()
