@@ -1281,12 +1281,14 @@ __global__ __launch_bounds__(CC_THREADS) void ecg_class_window_kernel(
     }
 }
 
-// csums[(cls_off + k)*3*LP + v]: the sums of class k (sum G, sum G P',
-// sum G Q' per lead), its window's pieces added in order.
+// csums[(cls_off + k)*3*LP + v]: the sums of class k (k0 sum G, sum G P',
+// sum G Q' per lead, k0 the class's layer's resting value), its window's
+// pieces added in order.
 template <int LP>
 __global__ void class_sums_kernel(const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
                                   const int64_t* __restrict__ cs_start, int64_t n_cs,
-                                  const double* __restrict__ sums, double* __restrict__ csums)
+                                  const double* __restrict__ layer_k, const double* __restrict__ sums,
+                                  double* __restrict__ csums)
 {
     const int64_t n = n_cs * 3 * LP;
     for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n;
@@ -1301,20 +1303,26 @@ __global__ void class_sums_kernel(const HeartDev* __restrict__ hearts, const Req
         double s = 0.0;
         for (int p = H.cwin[wi]; p < H.cwin[wi + 1]; ++p)
             s += sums[(R.c_off + H.n_chunks + (int64_t)p * CLASS_WINDOW + i) * 3 * LP + v];
+        if (v < LP) s *= layer_k[R.k_off + 9 * (H.classes[k].x - 1)];
         csums[item * 3 * LP + v] = s;
     }
 }
 
 // ecg[r][i][t] += sum over the heart's activation classes of
 //   S(t - a) (X_l(t) sum G P' + Y_l(t) sum G Q') + k0 sum G,
-// with S(tau) = 1 / (1 + exp(-k1 tau)) in FP64, the factor of the AP
-// (Wohlfart.h:195-203) at the cell time tau = t - a (simulator.cpp:154-170).
-// Every voxel of a class has the same layer and activation time, hence the
-// same AP, so the class's dipole terms factor through its sums.  Samples from
-// Tn on receive nothing, as in the reduction.  One block of CL_WAVES waves
-// per (request, 64 steps), one step per lane: wave w adds the classes
-// [w n / CL_WAVES, (w+1) n / CL_WAVES) in order, then the waves' sums are
-// added in wave order.
+// with S(tau) = 1 / (1 + exp(-k1 tau)), the factor of the AP
+// (Wohlfart.h:195-203) at the cell time tau = t - a (simulator.cpp:154-170),
+// in FP32 as in the per-voxel AP (tau = t - a formed in FP64 and rounded once;
+// 1 + 2^(c1 tau) rounds to 1 once the upstroke is 2^-24 from done, so S is
+// exactly 1 from there on), times the FP64 sums.  Every voxel of a class has
+// the same layer and activation time, hence the same AP, so the class's dipole
+// terms factor through its sums.  Samples from Tn on receive nothing, as in
+// the reduction.  One block of CL_WAVES waves per (request, 64 steps), one
+// step per lane: wave w adds the classes [w n / CL_WAVES, (w+1) n / CL_WAVES)
+// in order, two FMAs per lead, then the waves' sums are added in wave order.
+// The time-independent sum over the classes of k0 sum G (csums holds k0 sum G)
+// is formed once per block: thread (part p, lead i) adds classes p, p + NP,
+// ..., then lead i's parts are added in order.
 constexpr int CL_WAVES = 4;
 
 template <int LP>
@@ -1324,7 +1332,9 @@ __global__ __launch_bounds__(64 * CL_WAVES) void ecg_class_kernel(
     const double2* __restrict__ xy, const double* __restrict__ tgrid, const double* __restrict__ csums,
     double* __restrict__ ecg)
 {
+    constexpr int NP = 64 * CL_WAVES / LP;  // parts of the k0 sum per lead
     __shared__ double s_acc[CL_WAVES - 1][LP][64];
+    __shared__ double s_k0[NP][LP];
     // Blocks go round-robin over the 8 XCDs.  With cl_per_req blocks for
     // every request and B a multiple of 8, XCD x takes requests x, x + 8, ...
     // whole, so each request's class sums are fetched into one L2.
@@ -1346,27 +1356,42 @@ __global__ __launch_bounds__(64 * CL_WAVES) void ecg_class_kernel(
     const int j = j0 + lane;
     const bool live = j < R.Tn;
     const double t = tgrid[R.t_off + j];  // j < Tp: in the grid
+    if ((int)threadIdx.x < NP * LP) {
+        const int p = (int)threadIdx.x / LP, i = (int)threadIdx.x - p * LP;
+        double c = 0.0;
+#pragma unroll 4
+        for (int k = p; k < H.n_classes; k += NP) c += csums[(R.cls_off + k) * 3 * LP + i];
+        s_k0[p][i] = c;
+    }
     double acc[LP];
 #pragma unroll
     for (int i = 0; i < LP; ++i) acc[i] = 0.0;
-    const int k0c = (int)((int64_t)H.n_classes * w / CL_WAVES), k1c = (int)((int64_t)H.n_classes * (w + 1) / CL_WAVES);
+    const int k0c = __builtin_amdgcn_readfirstlane((int)((int64_t)H.n_classes * w / CL_WAVES)),
+              k1c = __builtin_amdgcn_readfirstlane((int)((int64_t)H.n_classes * (w + 1) / CL_WAVES));
     int lp = 0;
-    double X = 0.0, Y = 0.0, k0 = 0.0, k1 = 0.0;
-    for (int k = k0c; k < k1c; ++k) {
-        const int l = H.classes[k].x;
+    double X = 0.0, Y = 0.0;
+    float c1 = 0.0f;
+    // the class records through the constant address space: scalar loads
+    using ci32 = __attribute__((address_space(4))) const int;
+    using ci64 = __attribute__((address_space(4))) const long long;
+    ci32* const cls = (ci32*)H.classes;
+    ci64* const cla = (ci64*)H.class_a;
+    for (int kv = k0c; kv < k1c; ++kv) {
+        const int k = __builtin_amdgcn_readfirstlane(kv);
+        const int l = cls[4 * k];
         if (l != lp) {  // classes are ordered by layer
             const double2 xyj = xy[R.xy_off + (int64_t)(l - 1) * R.Tp + j];
             X = xyj.x;
             Y = xyj.y;
-            k0 = layer_k[R.k_off + 9 * (l - 1)];
-            k1 = layer_k[R.k_off + 9 * (l - 1) + 1];
+            c1 = (float)(-layer_k[R.k_off + 9 * (l - 1) + 1] * LOG2E_D);
             lp = l;
         }
-        const double S = 1.0 / (1.0 + exp(-k1 * (t - H.class_a[k])));
+        const float tau = (float)(t - __builtin_bit_cast(double, cla[k]));
+        const double S = (double)fast_rcp(1.0f + fast_exp2(c1 * tau));
+        const double SX = S * X, SY = S * Y;
         const double* cs = csums + (R.cls_off + k) * 3 * LP;
 #pragma unroll
-        for (int i = 0; i < LP; ++i)
-            acc[i] += __builtin_fma(k0, cs[i], S * __builtin_fma(X, cs[LP + i], Y * cs[2 * LP + i]));
+        for (int i = 0; i < LP; ++i) acc[i] = __builtin_fma(SY, cs[2 * LP + i], __builtin_fma(SX, cs[LP + i], acc[i]));
     }
     if (w > 0)
 #pragma unroll
@@ -1378,7 +1403,9 @@ __global__ __launch_bounds__(64 * CL_WAVES) void ecg_class_kernel(
             double v = acc[i];
 #pragma unroll
             for (int ww = 0; ww < CL_WAVES - 1; ++ww) v += s_acc[ww][i][lane];
-            if (i < R.L) ecg[R.ecg_off + (int64_t)i * R.T + j] += v;
+            double c0 = 0.0;
+            for (int p = 0; p < NP; ++p) c0 += s_k0[p][i];
+            if (i < R.L) ecg[R.ecg_off + (int64_t)i * R.T + j] += v + c0;
         }
     }
 }
@@ -1572,7 +1599,7 @@ hipError_t launch_class_sums(const GroupArgs& a, hipStream_t s)
 #define EKG_CSUMS(LPV)                                                                                  \
     case LPV:                                                                                           \
         hipLaunchKernelGGL(class_sums_kernel<LPV>, dim3(grid_for(a.n_cs * 3 * LPV, 256)), dim3(256), 0, s, \
-                           a.hearts, a.reqs, a.B, a.cs_start, a.n_cs, a.sums, a.csums);                  \
+                           a.hearts, a.reqs, a.B, a.cs_start, a.n_cs, a.layer_k, a.sums, a.csums);       \
         break;
         EKG_CSUMS(4)
         EKG_CSUMS(8)
