@@ -797,7 +797,7 @@ struct VoxelField {
 template <int LP, int NH, int NMAX>
 __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
     const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
-    const int64_t* __restrict__ ch_start, int ch_uniform, const double* __restrict__ lead_pos,
+    const int64_t* __restrict__ ch_start, int ch_uniform, int one_heart, const double* __restrict__ lead_pos,
     const double* __restrict__ layer_k, const LayerConst* __restrict__ lc,
     const double2* __restrict__ xy, const double* __restrict__ tgrid, const float2* __restrict__ thl,
     double* __restrict__ sums, int2* __restrict__ bnd, double* __restrict__ partial)
@@ -825,28 +825,52 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
         r = find_req(ch_start, B, blockIdx.x);
         c = (int)(blockIdx.x - ch_start[r]);
     }
+    // With every request on one heart its record is read beside the
+    // request's, and the chunk record right after: one dependent load fewer
+    // before the chunk's voxel records can be requested.
+    const HeartDev* Hp = hearts;
+    if (!one_heart) Hp = hearts + reqs[r].heart;
+    const HeartDev& H = *Hp;
     const ReqDev R = reqs[r];
-    const HeartDev& H = hearts[R.heart];
-    const int4 ch = H.chunks[c];
+    // the chunk record through the constant address space (a scalar load)
+    using cu64 = __attribute__((address_space(4))) const unsigned long long;
+    const unsigned long long cw0 = ((cu64*)H.chunks)[2 * c], cw1 = ((cu64*)H.chunks)[2 * c + 1];
+    const int4 ch = int4{(int)(unsigned)cw0, (int)(cw0 >> 32), (int)(unsigned)cw1, (int)(cw1 >> 32)};
     const int l = ch.x, v0 = ch.y, n = ch.z - ch.y;
     const int ng = (n + 15) >> 4;  // groups of 16 voxels
-    const double* kk = layer_k + R.k_off + 9 * (l - 1);
+    // the records of the thread's first voxel through the global address
+    // space, requested before the request's coefficients (they depend on the
+    // chunk alone) and first used after the barrier below
+    using gu64 = __attribute__((address_space(1))) const unsigned long long;
+    unsigned long long geo0 = 0ull, a0 = 0ull, av0 = 0ull;
+    if (t < n) {
+        geo0 = ((gu64*)H.vox_geo)[v0 + t];
+        a0 = ((gu64*)H.vox_a64)[v0 + t];
+        av0 = ((gu64*)H.vox_a)[v0 + t];
+    }
+    // the layer's coefficients, reference time and the chunk's extreme
+    // activation times as scalar loads, all issued together
+    const cu64* kk = (const cu64*)(layer_k + R.k_off + 9 * (l - 1));
     const LayerConst kc = lc[R.lay_off + l - 1];
-    const double t0 = H.layer_t0[l];
+    const double t0 = __builtin_bit_cast(double, ((cu64*)H.layer_t0)[l]);
     const double tref = (double)__int_as_float(ch.w);
-    const double alpha = kk[2] * (1.0 - kk[3]), beta = kk[2] * kk[3], k45 = kk[4] + kk[5], k5 = kk[5];
-    const double c1d = -kk[1] * LOG2E_D;
+    const double k1v = __builtin_bit_cast(double, kk[1]), k2v = __builtin_bit_cast(double, kk[2]),
+                 k3v = __builtin_bit_cast(double, kk[3]), k4v = __builtin_bit_cast(double, kk[4]),
+                 k5v = __builtin_bit_cast(double, kk[5]);
+    const double alpha = k2v * (1.0 - k3v), beta = k2v * k3v, k45 = k4v + k5v, k5 = k5v;
+    const double c1d = -k1v * LOG2E_D;
     const double* tg = tgrid + R.t_off;
     const float2* th = thl + R.t_off;
-    if (t < 3 * R.L) s_pd[t / 3][t % 3] = lead_pos[R.pos_off + t];
+    if (t < 3 * R.L) s_pd[t / 3][t % 3] = __builtin_bit_cast(double, ((gu64*)lead_pos)[R.pos_off + t]);
     // operands of the bound-tile tests (tiles of BT steps), loaded now and
     // used after the G rows: up to NPROBE probes per lane (64 NPROBE tiles)
     constexpr int BT = 1 << BT_SHIFT_MFMA;
     constexpr int QPT = BT / 4;  // quads (4 steps) per tile
     constexpr int NPROBE = BT >= 8 ? 2 : 4;
     const int nbt = R.Tp >> BT_SHIFT_MFMA;
-    const double a_max = H.vox_a64[ch.z - 1];
-    const float2 a_min = H.vox_a[ch.y];
+    const double a_max = __builtin_bit_cast(double, ((cu64*)H.vox_a64)[ch.z - 1]);
+    const unsigned long long amw = ((cu64*)H.vox_a)[ch.y];
+    const float2 a_min = float2{__int_as_float((int)(unsigned)amw), __int_as_float((int)(amw >> 32))};
     double tg_probe[NPROBE];
     float2 th_probe[NPROBE];
 #pragma unroll
@@ -858,8 +882,9 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
 #pragma unroll
         for (int k = 0; k < NPROBE; ++k)
             if (lane + 64 * k < nbt) {
-                tg_probe[k] = tg[(lane + 64 * k) * BT];
-                th_probe[k] = th[(lane + 64 * k) * BT + BT - 1];
+                tg_probe[k] = __builtin_bit_cast(double, ((gu64*)tg)[(lane + 64 * k) * BT]);
+                const unsigned long long pw = ((gu64*)th)[(lane + 64 * k) * BT + BT - 1];
+                th_probe[k] = float2{__int_as_float((int)(unsigned)pw), __int_as_float((int)(pw >> 32))};
             }
     }
     // factors from the layer's t0 to the chunk's t_ref: P = P' e^{(k4+k5)(t0 - t_ref)}, Q = Q' e^{k5 (t0 - t_ref)}
@@ -869,7 +894,8 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
     // 1. G rows and AP factors, one voxel per thread (strided)
     for (int q = t; q < n; q += blockDim.x) {
         const int v = v0 + q;
-        const VoxelField<NH> vf(H.vox_geo[v]);
+        const bool first = q == t;
+        const VoxelField<NH> vf(first ? uint2{(unsigned)geo0, (unsigned)(geo0 >> 32)} : H.vox_geo[v]);
 #pragma unroll 1
         for (int i0 = 0; i0 < LP; i0 += CH_LEADS) {
             double g[CH_LEADS];
@@ -877,8 +903,9 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
 #pragma unroll
             for (int w2 = 0; w2 < CH_LEADS; ++w2) s_G[q * LP + i0 + w2] = g[w2];
         }
-        const double a = H.vox_a64[v];
-        const float2 av = H.vox_a[v];
+        const double a = first ? __builtin_bit_cast(double, a0) : H.vox_a64[v];
+        const float2 av = first ? float2{__int_as_float((int)(unsigned)av0), __int_as_float((int)(av0 >> 32))}
+                                : H.vox_a[v];
         const double pp = alpha * exp(k45 * (a - t0)), qq = beta * exp(k5 * (a - t0));
         s_aux[q] = float4{(float)(pp * eP), (float)(qq * eQ), av.x, av.y};
         s_pq[q] = double2{pp, qq};
@@ -1528,7 +1555,7 @@ template <int LP, int NH, int NMAX>
 static void chunk_launch3(const GroupArgs& a, hipStream_t s)
 {
     hipLaunchKernelGGL((ecg_chunk_kernel<LP, NH, NMAX>), dim3((unsigned)a.n_ch), dim3(64 * a.mfma_waves), 0, s,
-                       a.hearts, a.reqs, a.B, a.ch_start, a.ch_uniform, a.lead_pos, a.layer_k, a.lc, a.xy,
+                       a.hearts, a.reqs, a.B, a.ch_start, a.ch_uniform, a.one_heart, a.lead_pos, a.layer_k, a.lc, a.xy,
                        a.tgrid, a.thl, a.sums, a.bnd, a.partial);
 }
 

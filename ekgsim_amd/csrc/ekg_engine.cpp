@@ -1045,6 +1045,7 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     a.ch_uniform = B > 0 ? (int)(st_ch[1] - st_ch[0]) : 0;
     for (int r = 1; r < B && a.ch_uniform; ++r)
         if (st_ch[r + 1] - st_ch[r] != a.ch_uniform) a.ch_uniform = 0;
+    a.one_heart = hs.size() == 1 ? 1 : 0;
     a.cc_runs = reinterpret_cast<const int4*>(tb + o_runs);
     a.cc_rstart = reinterpret_cast<const int64_t*>(tb + o_rst);
     a.n_runs = (int)runs.size();

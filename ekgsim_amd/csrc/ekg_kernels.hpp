@@ -128,6 +128,7 @@ struct GroupArgs {
     const int64_t* cl_start;   // 64-step tiles of the class kernel per request (0 without classes)
     int64_t n_lf, n_ch, n_xy, n_lay, n_out, n_cc, n_cs, n_cl;  // totals
     int ch_uniform;            // chunks per request when all requests have the same count, else 0
+    int one_heart;             // every request on one heart (tables hold it as heart 0)
     // class-window blocks: runs of consecutive requests of one heart
     // {first request, requests, pieces of the heart}, blocks of run u
     // [cc_rstart[u], cc_rstart[u+1]) = pieces x requests
