@@ -1251,6 +1251,11 @@ __global__ __launch_bounds__(CC_THREADS) void ecg_class_window_kernel(
 #pragma unroll
         for (int i = 0, n = 2; i < CW; ++i)
             if ((mask >> i) & 1) bf[i] = __int_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, lo, 256 * n++, 0));
+        // A wave past its loads takes issue priority for the products and
+        // weights of the group, and gives it back once its next loads are
+        // due, so that the waves of a SIMD spread over the loop instead of
+        // waiting on their loads together (measured: 1.0% per step).
+        __builtin_amdgcn_s_setprio(1);
         double sw[NG];
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
@@ -1266,6 +1271,7 @@ __global__ __launch_bounds__(CC_THREADS) void ecg_class_window_kernel(
 #pragma unroll
                 for (int g = 0; g < NG; ++g) acc[i][g] = __builtin_amdgcn_mfma_f64_4x4x4f64(sw[g], bv, acc[i][g], 0, 0, 0);
             }
+        __builtin_amdgcn_s_setprio(0);
     }
     // acc[i][g] in lane 16 row + 4 block + col: lead 4g + row, coefficient
     // col, this wave's entries of one block; the four blocks in order
