@@ -683,10 +683,10 @@ constexpr int CH_LEADS = 2;   // leads per pass of the chunk kernel's lead field
 // sigma = e(u) . d, which the dipole product needs anyway:
 //   e(u+d) . d = sigma - |d|^2,   |e(u+d)|^2 = |e(u)|^2 + |d|^2 - 2 sigma,
 // one FMA per cell.  The FP64 value differs from the cell's directly summed
-// |e|^2 (its own evaluation, or the class kernel's matrix product) by a few
+// |e|^2 (its own evaluation, or the class-window kernel's matrix product) by a few
 // 1e-16 relative, so after rounding to FP32 a cell gets the same weight from
 // every voxel that evaluates it except where those FP64 values straddle an
-// FP32 rounding boundary (a few per 1e8 evaluations, each a one-ulp weight
+// FP32 rounding boundary (about one in 1e8 evaluations, each a one-ulp weight
 // difference on one dipole); the weight's rounding therefore cancels between
 // the two ends of each dipole and only V differences across the dipole see
 // it (a weight derived per evaluating voxel in FP32 arithmetic would not
@@ -1163,9 +1163,10 @@ constexpr int CC_THREADS = 256;
 //     inner dimension {c_z, c_y, c_x, 1} x {-2 p_z, -2 p_y, -2 p_x, |p|^2},
 //     with |c|^2 (exact) as the accumulator input;
 //  2. s = rsq(|e|^2 rounded to FP32)^3 in FP32 -- the chunk kernel's weight:
-//     it rounds its own FP64 |e|^2 (from the exact FP64 offsets) the same way,
-//     so the two kernels give a cell the same weight except where the two
-//     FP64 values (both within ~1e-14 relative of |e|^2) straddle an FP32
+//     it rounds its own FP64 |e|^2 (from the exact FP64 offsets, a
+//     neighbour's from the voxel's |e|^2 and dipole sum) the same way, so the
+//     two kernels give a cell the same weight except where the two FP64
+//     values (both within ~1e-14 relative of |e|^2) straddle an FP32
 //     rounding boundary;
 //  3. for each class i of the group's mask, T_i += s x {D_z, D_y, D_x, K}_i:
 //     rows the leads, inner dimension the entries, one accumulator per class
