@@ -1301,6 +1301,14 @@ ekg_tune autotune(Heart& h, const ekg_sim_desc& sample, int batch)
                         best.ms_per_batch = ms;
                     }
                 }
+            // every voxel in a class (no chunks outside classes) when the
+            // default threshold already leaves at most 1% of the voxels out:
+            // the few extra classes are cheaper than the chunk path for them
+            if (best.class_min == 0 && h.layout.valid && h.layout.n_classes > 0) {
+                int64_t cv = 0;
+                for (const int4& cc : h.layout.h_cchunks) cv += cc.z - cc.y;
+                if (h.layout.V - cv <= h.layout.V / 100) sweep(&ekg_tune::class_min, {1});
+            }
         } else {
             std::vector<int> chunks{64, 128, 256, 512};
             chunks.erase(std::remove(chunks.begin(), chunks.end(), def_chunk), chunks.end());
