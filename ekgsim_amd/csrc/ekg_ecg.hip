@@ -86,6 +86,15 @@ __device__ __forceinline__ float tau_of(float th, float tl, float ah, float al)
     return (th - ah) + (tl - al);
 }
 
+// Offset of (lead i, step j) in a chunk's partial rows (LP x Tp doubles),
+// stored tile-major: for each bound tile of 2^bs steps its LP rows of 2^bs
+// steps, so that a tile's rows are one contiguous run (written and read in
+// whole cache lines, whichever tiles a chunk is in transition for).
+__device__ __forceinline__ int64_t prow(int i, int j, int lp, int bs)
+{
+    return ((int64_t)((j >> bs) * lp + i) << bs) + (j & ((1 << bs) - 1));
+}
+
 // First index in [lo, hi) whose predicate is false, for a predicate that is
 // true on a prefix of the range.  Wave-cooperative 64-ary search: each lane
 // tests one probe, the ballot's count of true probes narrows the range to one
@@ -654,10 +663,10 @@ __global__ __launch_bounds__(256) void ecg_fused_kernel(
 #pragma unroll
             for (int i = 0; i < LP; ++i) acc[i] = __builtin_fma(g[i], apd, acc[i]);
         }
-        double* out = partial + R.p_off + ((int64_t)c * LP) * R.Tp + j;
+        double* out = partial + R.p_off + ((int64_t)c * LP) * R.Tp;
 #pragma unroll
         for (int i = 0; i < LP; ++i)
-            if (i < R.L) out[(int64_t)i * R.Tp] = acc[i];
+            if (i < R.L) out[prow(i, j, LP, R.bt_shift)] = acc[i];
     }
 }
 
@@ -1078,7 +1087,7 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
                 x += __shfl_xor(x, 8, 64);
                 const int i = 4 * lg + (lane >> 4);
                 if ((lane & 12) == 0 && i < R.L)
-                    out[(int64_t)i * R.Tp + j] = __builtin_fma(xyj.y, cp[2 * LP + i], __builtin_fma(xyj.x, cp[LP + i], x));
+                    out[prow(i, j, LP, BT_SHIFT_MFMA)] = __builtin_fma(xyj.y, cp[2 * LP + i], __builtin_fma(xyj.x, cp[LP + i], x));
             }
         }
     }
@@ -1121,7 +1130,7 @@ __global__ void ecg_reduce_kernel(const HeartDev* __restrict__ hearts, const Req
                 s = __builtin_fma(xyj.y, ab[LP + i], s);
                 s = __builtin_fma(lc[R.lay_off + l - 1].k0, pre_ab[(slot + nbt) * 2 * LP + i], s);
                 const int2 rg = pre_rng[slot + tile];
-                const double* p = partial + R.p_off + (int64_t)i * R.Tp + j;
+                const double* p = partial + R.p_off + prow(i, j, LP, R.bt_shift);
                 const int64_t st = (int64_t)LP * R.Tp;
                 // eight independent loads in flight, added in chunk order
                 int c = rg.x;
