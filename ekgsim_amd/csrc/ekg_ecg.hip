@@ -1104,23 +1104,30 @@ __global__ __launch_bounds__(1024) void ecg_chunk_kernel(
 // loop (simulator.cpp:519) receive no dipole.
 template <int LP>
 __global__ void ecg_reduce_kernel(const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs,
-                                  int B, const int64_t* __restrict__ out_start, int64_t n_out,
+                                  int B, const int64_t* __restrict__ red_start, int64_t n_red,
                                   const LayerConst* __restrict__ lc, const double2* __restrict__ xy,
                                   const int2* __restrict__ pre_rng, const double* __restrict__ pre_ab,
                                   const double* __restrict__ partial, double* __restrict__ ecg)
 {
-    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n_out;
+    // Items per request in the partial rows' order: bound tile, lead, step
+    // of the tile, so that consecutive threads read consecutive doubles of a
+    // tile's rows and share the tile's chunk range.
+    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n_red;
          idx += (int64_t)gridDim.x * blockDim.x) {
-        const int r = find_req(out_start, B, idx);
+        const int r = find_req(red_start, B, idx);
         const ReqDev R = reqs[r];
         const HeartDev& H = hearts[R.heart];
-        const int64_t loc = idx - out_start[r];
-        const int i = (int)(loc / R.T);
-        const int j = (int)(loc % R.T);
+        const int64_t loc = idx - red_start[r];
+        const int bs = R.bt_shift;
+        const int64_t blk = (int64_t)R.L << bs;
+        const int tile = (int)(loc / blk);
+        const int rem = (int)(loc - tile * blk);
+        const int i = rem >> bs;
+        const int j = (tile << bs) + (rem & ((1 << bs) - 1));
+        if (j >= R.T) continue;
         double s = 0.0;
         if (j < R.Tn) {
-            const int tile = j >> R.bt_shift;
-            const int nbt = R.Tp >> R.bt_shift;
+            const int nbt = R.Tp >> bs;
             const int nl = R.nl < H.max_layer ? R.nl : H.max_layer;
             for (int l = 1; l <= nl; ++l) {
                 const int64_t slot = R.pre_off + (int64_t)(l - 1) * (nbt + 1);
@@ -1130,7 +1137,7 @@ __global__ void ecg_reduce_kernel(const HeartDev* __restrict__ hearts, const Req
                 s = __builtin_fma(xyj.y, ab[LP + i], s);
                 s = __builtin_fma(lc[R.lay_off + l - 1].k0, pre_ab[(slot + nbt) * 2 * LP + i], s);
                 const int2 rg = pre_rng[slot + tile];
-                const double* p = partial + R.p_off + prow(i, j, LP, R.bt_shift);
+                const double* p = partial + R.p_off + prow(i, j, LP, bs);
                 const int64_t st = (int64_t)LP * R.Tp;
                 // eight independent loads in flight, added in chunk order
                 int c = rg.x;
@@ -1144,7 +1151,7 @@ __global__ void ecg_reduce_kernel(const HeartDev* __restrict__ hearts, const Req
                 for (; c < rg.y; ++c) s += p[(int64_t)c * st];
             }
         }
-        ecg[R.ecg_off + loc] = s;
+        ecg[R.ecg_off + (int64_t)i * R.T + j] = s;
     }
 }
 
@@ -1713,12 +1720,12 @@ void prime_kernels()
 
 hipError_t launch_ecg_reduce(const GroupArgs& a, hipStream_t s)
 {
-    if (a.n_out == 0) return hipSuccess;
+    if (a.n_red == 0) return hipSuccess;
     switch (a.LP) {
 #define EKG_REDUCE(LPV)                                                                                 \
     case LPV:                                                                                           \
-        hipLaunchKernelGGL(ecg_reduce_kernel<LPV>, dim3(grid_for(a.n_out, 256)), dim3(256), 0, s,      \
-                           a.hearts, a.reqs, a.B, a.out_start, a.n_out, a.lc, a.xy, a.pre_rng,          \
+        hipLaunchKernelGGL(ecg_reduce_kernel<LPV>, dim3(grid_for(a.n_red, 256)), dim3(256), 0, s,      \
+                           a.hearts, a.reqs, a.B, a.red_start, a.n_red, a.lc, a.xy, a.pre_rng,          \
                            a.pre_ab, a.partial, a.ecg);                                                 \
         break;
         EKG_REDUCE(4)

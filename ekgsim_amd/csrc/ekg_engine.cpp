@@ -860,6 +860,7 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     ws.h_tgrid.clear();
     ws.h_thl.clear();
     std::vector<std::pair<std::pair<double, double>, std::pair<int64_t, int64_t>>> grids;  // (start, dt) -> (Tp, off)
+    std::vector<int64_t> st_red(B + 1, 0);
     std::vector<int64_t> st_lf(B + 1, 0), st_ch(B + 1, 0), st_xy(B + 1, 0), st_lay(B + 1, 0), st_out(B + 1, 0),
         st_cc(B + 1, 0), st_cs(B + 1, 0), st_cl(B + 1, 0);
     int64_t k_off = 0, pos_off = 0, rows = 0, c_off = 0, p_off = 0, pre_off = 0;
@@ -937,6 +938,10 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
         st_xy[r + 1] = st_xy[r] + (int64_t)q.nl * q.Tp;
         st_lay[r + 1] = st_lay[r] + q.nl;
         st_out[r + 1] = st_out[r] + (int64_t)q.L * q.T;
+        {
+            const int64_t bt = (int64_t)1 << R.bt_shift;
+            st_red[r + 1] = st_red[r] + (int64_t)q.L * ((q.T + bt - 1) / bt) * bt;
+        }
         st_cs[r + 1] = st_cs[r] + z.classes;
         st_cl[r + 1] = st_cl[r] + (z.classes > 0 ? q.n_tiles : 0);
     }
@@ -979,7 +984,7 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     const size_t o_req = tables.size();
     append(tables, rd.data(), rd.size());
     const size_t o_st = tables.size();
-    for (auto* st : {&st_lf, &st_ch, &st_xy, &st_lay, &st_out, &st_cc, &st_cs, &st_cl})
+    for (auto* st : {&st_lf, &st_ch, &st_xy, &st_lay, &st_out, &st_cc, &st_cs, &st_cl, &st_red})
         append(tables, st->data(), st->size());
     const size_t o_rst = tables.size();
     append(tables, st_run.data(), st_run.size());
@@ -1036,6 +1041,7 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     a.out_start = stp + 4 * (B + 1);
     a.cs_start = stp + 6 * (B + 1);
     a.cl_start = stp + 7 * (B + 1);
+    a.red_start = stp + 8 * (B + 1);
     a.n_cc = st_cc[B];
     a.n_cs = st_cs[B];
     a.n_cl = st_cl[B];
@@ -1056,6 +1062,7 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     a.n_xy = st_xy[B];
     a.n_lay = st_lay[B];
     a.n_out = st_out[B];
+    a.n_red = st_red[B];
     a.layer_k = d_k;
     a.lead_pos = d_pos;
     a.tgrid = ws.tgrid.as<double>();

@@ -126,7 +126,8 @@ struct GroupArgs {
     const int64_t* out_start;  // L * T per request
     const int64_t* cs_start;   // activation classes per request
     const int64_t* cl_start;   // 64-step tiles of the class kernel per request (0 without classes)
-    int64_t n_lf, n_ch, n_xy, n_lay, n_out, n_cc, n_cs, n_cl;  // totals
+    const int64_t* red_start;  // reduction items per request: L x (T rounded up to a bound tile)
+    int64_t n_lf, n_ch, n_xy, n_lay, n_out, n_cc, n_cs, n_cl, n_red;  // totals
     int ch_uniform;            // chunks per request when all requests have the same count, else 0
     int one_heart;             // every request on one heart (tables hold it as heart 0)
     // class-window blocks: runs of consecutive requests of one heart
