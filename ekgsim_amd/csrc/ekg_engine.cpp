@@ -829,7 +829,7 @@ size_t group_device_bytes(const GroupPlan& g)
              (8 + 2 * LP * 8);
         s += (size_t)nmix * LP * r.Tp * 8;
         s += (size_t)r.L * r.T * 8;
-        s += sizeof(ekgk::ReqDev) + sizeof(ekgk::HeartDev) + 8 * 8;
+        s += sizeof(ekgk::ReqDev) + sizeof(ekgk::HeartDev) + 9 * 8;  // record, heart, nine start tables
     }
     return s;
 }
