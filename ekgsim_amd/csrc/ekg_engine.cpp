@@ -581,8 +581,9 @@ void ensure_layout(Heart& h, bool mfma)
 // group's missing entries (past the window's last) repeat its first entry
 // with coefficients 0, so their weights are finite and add nothing.
 //
-// For parallelism a window's groups are cut into pieces of at most
-// CLASS_PIECE groups, one block per (piece, request); each writes its sums
+// For parallelism a window's groups are cut into the fewest pieces of at
+// most CLASS_PIECE groups, of equal size up to one group (so that no block
+// is left with a short remainder), one block per (piece, request); each writes its sums
 // for all CLASS_WINDOW classes, and class_sums_kernel adds a class's pieces
 // in order.  The cut depends on the heart only, so a request's ECG does not
 // depend on the batch it runs in.
@@ -735,9 +736,11 @@ void ensure_cells(Heart& h, int nhood)
         }
         const int g_end = (int)crow.size();
         // at least one piece per window, so every class has sums (0 without entries)
+        const int n_pc = std::max(1, (g_end - g_begin + CLASS_PIECE - 1) / CLASS_PIECE);
+        const int pc_size = std::max(1, (g_end - g_begin + n_pc - 1) / n_pc);
         int g = g_begin;
         do {
-            const int ge = std::min(g_end, g + CLASS_PIECE);
+            const int ge = std::min(g_end, g + pc_size);
             pieces.push_back(int4{w, g, ge, 0});
             g = ge;
         } while (g < g_end);
