@@ -1336,25 +1336,28 @@ __global__ __launch_bounds__(CC_THREADS) void ecg_class_window_kernel(
 // pieces added in order.
 template <int LP>
 __global__ void class_sums_kernel(const HeartDev* __restrict__ hearts, const ReqDev* __restrict__ reqs, int B,
-                                  const int64_t* __restrict__ cs_start, int64_t n_cs,
-                                  const double* __restrict__ layer_k, const double* __restrict__ sums,
-                                  double* __restrict__ csums)
+                                  const int64_t* __restrict__ cs_start, const double* __restrict__ layer_k,
+                                  const double* __restrict__ sums, double* __restrict__ csums)
 {
-    const int64_t n = n_cs * 3 * LP;
-    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n;
-         idx += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t item = idx / (3 * LP);
-        const int v = (int)(idx - item * (3 * LP));
-        const int r = find_req(cs_start, B, item);
-        const ReqDev R = reqs[r];
+    // grid rows over the requests (the request is block-uniform), columns
+    // over a request's (class, column) items
+    for (int r = blockIdx.y; r < B; r += gridDim.y) {
+        const int64_t c0 = cs_start[r];
+        const int64_t n = (cs_start[r + 1] - c0) * 3 * LP;
+        if (n == 0) continue;
+        const ReqDev& R = reqs[r];
         const HeartDev& H = hearts[R.heart];
-        const int k = (int)(item - cs_start[r]);
-        const int wi = k / CLASS_WINDOW, i = k - wi * CLASS_WINDOW;
-        double s = 0.0;
-        for (int p = H.cwin[wi]; p < H.cwin[wi + 1]; ++p)
-            s += sums[(R.c_off + H.n_chunks + (int64_t)p * CLASS_WINDOW + i) * 3 * LP + v];
-        if (v < LP) s *= layer_k[R.k_off + 9 * (H.classes[k].x - 1)];
-        csums[item * 3 * LP + v] = s;
+        for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n;
+             idx += (int64_t)gridDim.x * blockDim.x) {
+            const int k = (int)(idx / (3 * LP));
+            const int v = (int)(idx - (int64_t)k * (3 * LP));
+            const int wi = k / CLASS_WINDOW, i = k - wi * CLASS_WINDOW;
+            double s = 0.0;
+            for (int p = H.cwin[wi]; p < H.cwin[wi + 1]; ++p)
+                s += sums[(R.c_off + H.n_chunks + (int64_t)p * CLASS_WINDOW + i) * 3 * LP + v];
+            if (v < LP) s *= layer_k[R.k_off + 9 * (H.classes[k].x - 1)];
+            csums[(c0 + k) * 3 * LP + v] = s;
+        }
     }
 }
 
@@ -1648,8 +1651,9 @@ hipError_t launch_class_sums(const GroupArgs& a, hipStream_t s)
     switch (a.LP) {
 #define EKG_CSUMS(LPV)                                                                                  \
     case LPV:                                                                                           \
-        hipLaunchKernelGGL(class_sums_kernel<LPV>, dim3(grid_for(a.n_cs * 3 * LPV, 256)), dim3(256), 0, s, \
-                           a.hearts, a.reqs, a.B, a.cs_start, a.n_cs, a.layer_k, a.sums, a.csums);       \
+        hipLaunchKernelGGL(class_sums_kernel<LPV>,                                                     \
+                           dim3(grid_for((int64_t)a.cs_max * 3 * LPV, 256), std::min(a.B, 65535)), dim3(256), \
+                           0, s, a.hearts, a.reqs, a.B, a.cs_start, a.layer_k, a.sums, a.csums);         \
         break;
         EKG_CSUMS(4)
         EKG_CSUMS(8)

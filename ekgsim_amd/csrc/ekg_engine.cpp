@@ -1047,6 +1047,8 @@ void enqueue_group(Ctx& ctx, GroupPlan& g, Workspace& ws, hipStream_t stream, co
     a.red_start = stp + 8 * (B + 1);
     a.n_cc = st_cc[B];
     a.n_cs = st_cs[B];
+    a.cs_max = 0;
+    for (int r = 0; r < B; ++r) a.cs_max = std::max<int>(a.cs_max, (int)(st_cs[r + 1] - st_cs[r]));
     a.n_cl = st_cl[B];
     a.n_lf = st_lf[B];
     a.n_ch = st_ch[B];

@@ -123,6 +123,7 @@ struct GroupArgs {
     int B, LP, nhood, mfma, mfma_waves;
     int lf_phase;              // leads per lead-field phase (2 or 4)
     int chunk_max;             // largest chunk target of the group's hearts (chunk kernel LDS rows)
+    int cs_max;                // most activation classes of one request
     const int64_t* lf_start;   // lead-field blocks (bricks) per request
     const int64_t* ch_start;   // chunks per request
     const int64_t* xy_start;   // nl * Tp per request
