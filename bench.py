@@ -358,8 +358,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(requests would shard across GPUs with no exchange; see DESIGN.md).", file=sys.stderr)
+        print("bench.py runs on one GPU only: there is no multi-GPU path "
+              "(requests would shard across GPUs by request, with no exchange between them).", file=sys.stderr)
         sys.exit(2)
     if args.stress:
         stress(args)
