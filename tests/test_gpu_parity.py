@@ -248,13 +248,19 @@ def test_pipeline_matches_reference_golden(ctx):
     assert rel_l2(got, g["ecg"]) <= TOL
 
 
-def test_mixed_shapes_one_group_equal_alone(ctx):
+@pytest.mark.parametrize("class_min", [0, -1])
+def test_mixed_shapes_one_group_equal_alone(ctx, class_min):
     """Requests with different lead counts, durations, start times and time
     steps in one launch group equal the single-request calls bitwise and
-    the FP64 oracle within 1e-5."""
+    the FP64 oracle within 1e-5; with activation classes (default threshold)
+    and without them (every voxel in a chunk, so every request's ECG comes
+    from the partial rows of the reduction, with lead counts that are not a
+    multiple of 4 and step counts that are not a multiple of the 8-step
+    bound tile)."""
     h = S.make_heart(28, seed=3)
     g = _heart(ctx, h)
     g.excite_delay(h.delay)
+    g.set_tune(class_min=class_min)
     act = O.excite_int(h.layer, h.seed, h.delay).astype(np.float64)
     rng = np.random.default_rng(5)
     reqs = []
