@@ -747,6 +747,22 @@ struct VoxelField {
             const float r = __builtin_amdgcn_rsqf((float)lo[w2]);
             g[w2] = (double)(r * r * r) * (e[w2][0] * sd0 + e[w2][1] * sd1 + e[w2][2] * sd2);
         }
+        // Corner neighbourhood (3D4): the eight dipole sums are
+        // d0 ((e0 + d0 d1 e1) + d0 d2 e2), four values and their negations
+        // (IEEE sums are odd in their operands, so these are the bits of the
+        // term-by-term sums below), formed once per lead ahead of the
+        // per-neighbour branches.
+        double s4[CH_LEADS][4];
+        if constexpr (NH == 2) {
+#pragma unroll
+            for (int w2 = 0; w2 < CH_LEADS; ++w2) {
+                const double p = e[w2][0] + e[w2][1], m = e[w2][0] - e[w2][1];
+                s4[w2][0] = p + e[w2][2];
+                s4[w2][1] = p - e[w2][2];
+                s4[w2][2] = m + e[w2][2];
+                s4[w2][3] = m - e[w2][2];
+            }
+        }
 #pragma unroll
         for (int qn = 0; qn < N::n; ++qn) {
             if ((mk >> cube_bit(N::d[qn][0], N::d[qn][1], N::d[qn][2])) & 1u) {
@@ -756,9 +772,14 @@ struct VoxelField {
                 for (int w2 = 0; w2 < CH_LEADS; ++w2) {
                     // sigma = e(u) . d (d has entries -1, 0, 1)
                     double sg = 0.0;
-                    if (d0) sg = d0 > 0 ? e[w2][0] : -e[w2][0];
-                    if (d1) sg = d1 > 0 ? sg + e[w2][1] : sg - e[w2][1];
-                    if (d2) sg = d2 > 0 ? sg + e[w2][2] : sg - e[w2][2];
+                    if constexpr (NH == 2) {
+                        const int q4 = (d0 * d1 > 0 ? 0 : 2) + (d0 * d2 > 0 ? 0 : 1);
+                        sg = d0 > 0 ? s4[w2][q4] : -s4[w2][q4];
+                    } else {
+                        if (d0) sg = d0 > 0 ? e[w2][0] : -e[w2][0];
+                        if (d1) sg = d1 > 0 ? sg + e[w2][1] : sg - e[w2][1];
+                        if (d2) sg = d2 > 0 ? sg + e[w2][2] : sg - e[w2][2];
+                    }
                     const double l = __builtin_fma(-2.0, sg, lo[w2] + dd);
                     const float r = __builtin_amdgcn_rsqf((float)l);
                     g[w2] = __builtin_fma((double)(r * r * r), sg - dd, g[w2]);
