@@ -100,7 +100,7 @@ NAIVE_LANE_CYCLES, LANE_PEAK = 38, 256 * 64 * 2.4e9
 # 256 MiB calibration copy: with activation classes
 # (profiles/ecg_class_window_kernel_128cubed_batch256_hbm_traffic.md) and
 # without (profiles/ecg_chunk_kernel_128cubed_batch256_hbm_traffic.md).
-TRAFFIC_BYTES = {"ecg_class_window_kernel": 436566943, "ecg_chunk_kernel": 1132166547}
+TRAFFIC_BYTES = {"ecg_class_window_kernel": 436566943, "ecg_chunk_kernel": 1131874216}
 # Issue-pipe occupancy of the dominant kernel on this workload, from one
 # rocprofv3 --pmc pass (SQ_VALU_MFMA_BUSY_CYCLES, SQ_INSTS_VALU minus the MFMA
 # instructions at 4 cycles each, SQ_VALU_MFMA_COEXEC_CYCLES, GRBM_GUI_ACTIVE)
@@ -109,7 +109,7 @@ TRAFFIC_BYTES = {"ecg_class_window_kernel": 436566943, "ecg_chunk_kernel": 11321
 # and the vector ALU never overlap (COEXEC 0), so their sum is the fraction of
 # the kernel's cycles spent issuing.
 PIPES = {"ecg_class_window_kernel": {"mfma_busy": 0.563, "valu_issue": 0.294, "coexec": 0.0},
-         "ecg_chunk_kernel": {"mfma_busy": 0.114, "valu_issue": 0.824, "coexec": 0.0}}
+         "ecg_chunk_kernel": {"mfma_busy": 0.116, "valu_issue": 0.806, "coexec": 0.0}}
 
 
 def log(*a):
